@@ -1,0 +1,183 @@
+/*
+ * tp_gpu.h -- C ABI of the MI355X test-particle tracking library
+ * (libtpgpu.so, HIP kernels for gfx950).
+ *
+ * Drop-in point: the reference tracer's tracking loop
+ *   void particle_tracking_hybrid(int mpi_rank, int nptl, double dt,
+ *        int nbins, int nt_out, int bc_flag, int nsteps_output, double pmass,
+ *        int *ntraj_accum, int tracking_method, int traj_diagnose,
+ *        particles *ptl_init, int *nsteps_ptl_tracking, particles *ptl,
+ *        particles *ptl_time);
+ * (src/tracking.c:659-699, declared include/tracking.h:78-82).  The
+ * reference passes the field, domain, charge/mass and spectrum settings
+ * through file-scope globals; here they are explicit: a context holds the
+ * field source (tp_gpu_set_field_*), and every call takes a
+ * tp_track_params.  No function of this library calls exit(), reads or
+ * writes files, or uses MPI; all return TP_OK (0) or a negative TP_E_*
+ * code, and tp_gpu_strerror() names it.
+ *
+ * Threading: one context per host thread; calls on one context are
+ * serialised by the caller.  Host buffers are read/written synchronously
+ * (every call has finished with them when it returns).
+ */
+#ifndef TP_GPU_H
+#define TP_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One particle; byte-for-byte the reference's struct particles
+ * (include/particle_info.h:6-12): 11 doubles, 88 bytes. */
+typedef struct tp_particle {
+    double x, y, z, qm;
+    double ux, uy, uz, t;     /* u = gamma * v / c */
+    double xshift, yshift, zshift;
+} tp_particle;
+
+/* error codes */
+#define TP_OK            0
+#define TP_E_ARG        -1   /* bad argument or inconsistent parameters */
+#define TP_E_NOMEM      -2   /* device or host allocation failed */
+#define TP_E_HIP        -3   /* a HIP runtime call failed */
+#define TP_E_NONFINITE  -4   /* NaN/Inf in particle state after tracking */
+#define TP_E_STATE      -5   /* call out of order (no field, no particles) */
+#define TP_E_NODEV      -6   /* no usable GPU */
+
+/* pushers (the reference's tracking_wirz, src/tracking.c:749-892, and a
+ * relativistic Boris push, which the reference does not have) */
+#define TP_PUSH_WIRZ   0
+#define TP_PUSH_BORIS  1
+
+/* arithmetic of the push and field evaluation */
+#define TP_FP64        0   /* everything in FP64 */
+#define TP_FP32_MIXED  1   /* FP32 push/field, FP64 positions and diagnostics */
+
+/* particle boundary conditions, src/tracking.c:1172-1221 */
+#define TP_BC_PERIODIC 0
+#define TP_BC_OPEN     1
+#define TP_BC_NONE     2
+
+/* trilinear gather conventions for gridded fields */
+#define TP_GATHER_REF_COMPAT 0  /* src/interpolation.c:141-181 weights (x and
+                                   z fractions exchanged), float rounding after
+                                   each corner (src/magnetic_field.c:603-608),
+                                   wrap cell of src/interpolation.c:61-90 */
+#define TP_GATHER_TEXTBOOK   1  /* standard trilinear weights, FP64
+                                   accumulation, periodic wrap */
+
+/* tp_track_params.flags */
+#define TP_FLAG_FIX_STALE_DCOEFF 1  /* zero the per-particle D terms each step
+                                       (the reference keeps stale dxx/dyy/dzz/
+                                       daa/dee, src/tracking.c:334-336) */
+#define TP_FLAG_CHECK_FINITE     2  /* return TP_E_NONFINITE on NaN/Inf state */
+
+typedef struct tp_track_params {
+    int pusher;            /* TP_PUSH_* */
+    int precision;         /* TP_FP64 / TP_FP32_MIXED */
+    int bc_flag;           /* TP_BC_* (src/tracking.c:61-65) */
+    int nbins;             /* energy bins (include/Global.h:22) */
+    int nt_out;            /* spectrum frames */
+    int sstep, estep;      /* steps sstep..estep inclusive (src/tracking.c:333);
+                              the reference uses sstep = 1 and
+                              estep = (int)(tmax/dt) (src/tracking.c:103) */
+    int einterval_t;       /* spectrum cadence; reference: estep/nt_out
+                              (src/tracking.c:104) */
+    int dcoeff_interval;   /* D cadence; reference: 100 (src/tracking.c:34) */
+    int flags;             /* TP_FLAG_* */
+    double dt;             /* time step */
+    double qm;             /* charge/mass: the reference's global charge_mass
+                              (src/tracking.c:787), not ptl->qm */
+    double xmin, xmax, ymin, ymax, zmin, zmax;   /* simul_domain */
+    double logemin, logde; /* src/diagnostics.c:629-631 */
+} tp_track_params;
+
+typedef struct tp_ctx tp_ctx;
+
+/* --- context ------------------------------------------------------------ */
+int  tp_gpu_init(int device, tp_ctx **ctx);
+void tp_gpu_finalize(tp_ctx *ctx);
+const char *tp_gpu_strerror(int code);
+/* device name and compute-unit count of the context's GPU */
+int  tp_gpu_device_info(tp_ctx *ctx, char *name, int name_len, int *num_cu);
+
+/* --- field sources (replace get_emf's switch, src/emfields.c:32-60) ------ */
+/* system 0: B = z-hat, E = 0 (getemf_test, src/emfields.c:210-218) */
+int tp_gpu_set_field_test(tp_ctx *ctx);
+/* system 2: ABC force-free B = B0 (A sin lz + C cos ly, B sin lx + A cos lz,
+ * C sin ly + B cos lx) (getb_ff, src/force_free.c:138-150); E = B x v from
+ * the velocity grid set by tp_gpu_set_velocity_grid, or 0 without one
+ * (gete_ff, src/force_free.c:78-86). */
+int tp_gpu_set_field_abc(tp_ctx *ctx, double A, double B, double C,
+                         double lambda, double B0);
+/* nodes (nx, ny, nz) over [lo, hi] endpoint-inclusive (dx = (hi-lo)/(n-1),
+ * src/domain.c:548-550), x fastest, AoS double3 velocity
+ * (get_double_velocity_at_point, src/velocity_field.c:555-584).
+ * v == NULL removes it. */
+int tp_gpu_set_velocity_grid(tp_ctx *ctx, int nx, int ny, int nz,
+                             const double lo[3], const double hi[3],
+                             const double *v);
+/* system 4: PIC grids, AoS float3 E and B on the same nodes, x fastest
+ * (getemf_pic_test_particle, src/emfields.c:153-166). */
+int tp_gpu_set_field_grid(tp_ctx *ctx, int nx, int ny, int nz,
+                          const double lo[3], const double hi[3],
+                          const float *efield, const float *bfield,
+                          int gather_mode);
+/* system 1: nwlcs wire-loop current systems; rows of 14 doubles in the
+ * column order of the WLCS configuration file read by read_wlcs
+ * (src/wlcs.c:97-179): cur_wr x_wr y_wr z_wr theta phi t0 cur_lp r_lp
+ * x_lp y_lp z_lp alpha beta; plus iBessel, omega and normI
+ * (src/wlcs.c:181-212).  mode TP_GATHER_REF_COMPAT evaluates the complete
+ * elliptic integrals as the reference's ellint does (src/wlcs.c:460-809,
+ * which is wrong for k^2 >= 0.9); TP_GATHER_TEXTBOOK evaluates them
+ * exactly. */
+int tp_gpu_set_field_wlcs(tp_ctx *ctx, int nwlcs, const double *rows,
+                          int ibessel, double omega, double norm_i, int mode);
+/* multi-mode turbulence (no counterpart in the reference): B = b0z z-hat +
+ * sum_n a_n cos(k_n . x + phi_n), E = 0; rows of 7 doubles
+ * (kx, ky, kz, ax, ay, az, phi). */
+int tp_gpu_set_field_modes(tp_ctx *ctx, int nmodes, const double *rows,
+                           double b0z);
+
+/* --- fixed-step tracking (particle_tracking_fixed_time_step,
+ *     src/tracking.c:90-190) ---------------------------------------------- */
+/* One call does what the reference's loop does for traj_diagnose = 0:
+ *   ptl[n]        in/out particles (reference: ptl)
+ *   ptl_init[n]   initial particles for the diffusion terms (ptl_init)
+ *   nsteps[n]     out: steps each particle was pushed, starting from 1
+ *                 (nsteps_ptl_tracking, src/tracking.c:127-129)
+ *   espect        out: [nt_out][nbins] particle counts inside the box
+ *   espect_escape out: [nt_out][nbins] counts of escaping particles (bc 1)
+ *   dcoeff        out: [8][nd] sums drr dxx dyy dzz dpp duu daa dee with
+ *                 nd = (estep - sstep + 1)/dcoeff_interval + 1 rows for
+ *                 sstep = 1; row k holds step k*dcoeff_interval, row 0 is 0
+ *   nptl_remain   out: [nd] remaining particles (nptl_remain_time)
+ * Any of espect/espect_escape/dcoeff/nptl_remain/nsteps may be NULL. */
+int tp_gpu_track_fixed(tp_ctx *ctx, const tp_track_params *p, int n,
+                       tp_particle *ptl, const tp_particle *ptl_init,
+                       int *nsteps, double *espect, double *espect_escape,
+                       double *dcoeff, int *nptl_remain);
+
+/* The same split into steps, for callers that keep particles resident in
+ * HBM across calls (and for timing the tracking alone): */
+int tp_gpu_load_particles(tp_ctx *ctx, int n, const tp_particle *ptl,
+                          const tp_particle *ptl_init);
+int tp_gpu_run_fixed(tp_ctx *ctx, const tp_track_params *p);
+int tp_gpu_store_particles(tp_ctx *ctx, tp_particle *ptl, int *nsteps);
+int tp_gpu_read_diagnostics(tp_ctx *ctx, double *espect,
+                            double *espect_escape, double *dcoeff,
+                            int *nptl_remain);
+/* Time of the tracking kernels of the last run, measured with HIP events on
+ * the stream they run on: total milliseconds and number of launches. */
+int tp_gpu_last_kernel_time(tp_ctx *ctx, double *ms, int *launches);
+
+/* --- field probe -------------------------------------------------------- */
+/* get_emf (src/emfields.c:32-60) at n points pts[4*i] = (x, y, z, t);
+ * out[6*i] = (Bx, By, Bz, Ex, Ey, Ez).  precision as in tp_track_params. */
+int tp_gpu_eval_field(tp_ctx *ctx, int n, const double *pts, double *out,
+                      int precision);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TP_GPU_H */

@@ -1,0 +1,347 @@
+/*
+ * ref_harness.c -- test infrastructure only.
+ *
+ * A small driver that links the reference tracer's own hot-path objects
+ * (compiled from src/tracking.c, src/emfields.c, src/force_free.c,
+ * src/interpolation.c, src/{magnetic,electric,velocity}_field.c, src/wlcs.c,
+ * src/diagnostics.c of xiaocanli/test_particle, see oracle/Makefile) and calls
+ * them on seeded inputs handed in as files.  Its outputs are the golden
+ * vectors the project's CPU oracle (oracle/tpref.c) is pinned against.
+ *
+ * The reference keeps all of its state in file-scope globals that are set by
+ * its setup routines; this driver sets the same globals from a "key value"
+ * spec file instead of parsing init.dat, because the reference's own
+ * test_particle() cannot run system 2 (src/test_particle.c:51 dereferences the
+ * WLCS table) and seeds from time(NULL) (src/particle_info.c:318).
+ *
+ * Modes (all binary files are native-endian):
+ *   fields  SPEC PTS OUT     PTS: n*(x,y,z,t) f64 ; OUT: n*(Bx,By,Bz,Ex,Ey,Ez)
+ *                            via get_emf (src/emfields.c:32)
+ *   wirz    SPEC PTL K OUT   K steps of tracking_wirz + particle_bc
+ *                            (src/tracking.c:749, :1172); OUT: final particles
+ *   track   SPEC PTL OUTDIR  the fixed-step loop of src/tracking.c:297 driven
+ *                            like src/tracking.c:90-190, dumping the raw
+ *                            spectrum counts, D sums and remaining counts
+ *   hybrid  SPEC PTL         particle_tracking_hybrid (src/tracking.c:659) with
+ *                            the t=0 spectrum of src/diagnostics.c:50; writes
+ *                            data/espectrum.dat and data/diff_coeffs.dat
+ *   diff    SPEC PTL0 PTL DT OUT  calc_diff_coeff (src/diagnostics.c:659)
+ *   ellint  SPEC KS OUT      ellint (src/wlcs.c:460); KS: n f64 moduli k,
+ *                            OUT: n*(E, K)
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#include <mpi.h>
+
+/* Layouts of the reference's structs (include/particle_info.h:6-12,
+ * include/domain.h:5-24, include/emfields.h:7-10, include/wlcs.h:24-37,
+ * include/force_free.h:30-32).  Redeclared here so that this driver needs
+ * only the reference's objects, not its headers' macro environment. */
+typedef struct { double x, y, z, qm, ux, uy, uz, t, xshift, yshift, zshift; } particles;
+typedef struct {
+    double xmin_ptl, xmax_ptl, ymin_ptl, ymax_ptl, zmin_ptl, zmax_ptl;
+    double xmin, xmax, ymin, ymax, zmin, zmax, tmin, tmax;
+} domain;
+typedef struct { int nx, ny, nz, nt; double dx, dy, dz, dt; } grids;
+typedef struct { double Bx, By, Bz, Ex, Ey, Ez; } emfields;
+typedef struct {
+    double x_wr, y_wr, z_wr;
+    double costheta_wr, cosphi_wr, sintheta_wr, sinphi_wr;
+    double cur_wr, t0;
+    double x_lp, y_lp, z_lp, r_lp;
+    double cosalpha_lp, cosbeta_lp, sinalpha_lp, sinbeta_lp;
+    double cur_lp, omega;
+} wlcs;
+typedef struct { double B0, lambda0; } param_ff;
+typedef struct { float x, y, z; } f3;
+typedef struct { double x, y, z; } d3;
+
+/* reference entry points */
+void set_variables_tracking(grids *sgrid, domain *sdomain, int stype, int mt);
+void set_ptl_params_tracking(int pbc, double qm);
+void set_variables_emfields(int stype);
+void set_variables_ff(grids *sgrid, domain *sdomain);
+void set_variabls_interpolation(grids *sgrid, domain *sdomain, int mt);
+void set_variables_bfield(grids *sgrid, domain *sdomain, double B0_field, int mt, int dtype);
+void set_variables_efield(grids *sgrid, domain *sdomain, double E0_field, int mt, int dtype);
+void set_variables_velocity(grids *sgrid, domain *sdomain, double v0_field, int mt, int dtype);
+void initialize_bfield(void);
+void initialize_efield(void);
+void initialize_vfield(void);
+void get_emf(double x, double y, double z, double t, emfields *emf);
+void tracking_wirz(particles *ptl, double dt);
+void particle_bc(particles *ptl, int *iescape);
+void calc_diff_coeff(particles *ptl_init, particles *ptl, double dt,
+        double *drr, double *dxx, double *dyy, double *dzz, double *dpp,
+        double *duu, double *daa, double *dee);
+void calc_energy_spectrum(int mpi_rank, int nptl, particles *ptl,
+        int nbins, int nt_out, double pmass, int bc_flag);
+void particle_tracking_multi_steps_openmp(int nptl, double dt, int tid, int sstep,
+        int estep, int nbins, int nt_out, int einterval_t, int traj_diagnose,
+        int nsteps_output, double pmass, int *ntraj_accum, particles *ptl_init,
+        int *ntraj_diagnostics_points_array, particles *ptl,
+        int *nsteps_ptl_tracking, double espect_private[][1],
+        double espect_escape_private[][1], particles *ptl_time,
+        int *nptl_remain_time, double *drr, double *dxx, double *dyy, double *dzz,
+        double *dpp, double *duu, double *daa, double *dee);
+void ellint(double k, double *elle, double *ellf);
+void particle_tracking_hybrid(int mpi_rank, int nptl, double dt, int nbins,
+        int nt_out, int bc_flag, int nsteps_output, double pmass,
+        int *ntraj_accum, int tracking_method, int traj_diagnose,
+        particles *ptl_init, int *nsteps_ptl_tracking, particles *ptl,
+        particles *ptl_time);
+
+/* reference globals this driver sets directly */
+extern param_ff param;                          /* src/force_free.c:33 */
+extern f3 *magc_float;                          /* src/magnetic_field.c:35 */
+extern f3 *efc_float;                           /* src/electric_field.c:35 */
+extern d3 *vc_double;                           /* src/velocity_field.c:34 */
+extern wlcs *config;                            /* src/wlcs.c:12 */
+extern int nwlcs, iBessel;                      /* src/wlcs.c:13 */
+extern double normI;                            /* src/wlcs.c:14 */
+extern double emin, emax, logemin, logemax, logde; /* src/diagnostics.c:32 */
+
+typedef struct {
+    int system, bc, nbins, nt_out, nx, ny, nz, nwlcs, ibessel, nsteps_output;
+    double lo[3], hi[3], tmax, qm, dt, pmass, emin, emax, ff_b0, ff_lambda;
+    double wlcs_omega, wlcs_normi;
+    char grid[512], vgrid[512], wlcs[512];
+} spec_t;
+
+static void die(const char *m) { fprintf(stderr, "ref_harness: %s\n", m); exit(2); }
+
+static void *slurp(const char *path, size_t *n)
+{
+    FILE *f = fopen(path, "rb");
+    if (!f) die(path);
+    fseek(f, 0, SEEK_END);
+    long sz = ftell(f);
+    fseek(f, 0, SEEK_SET);
+    void *p = malloc(sz > 0 ? (size_t)sz : 1);
+    if (sz > 0 && fread(p, 1, (size_t)sz, f) != (size_t)sz) die("short read");
+    fclose(f);
+    *n = (size_t)sz;
+    return p;
+}
+
+static void spit(const char *path, const void *p, size_t n)
+{
+    FILE *f = fopen(path, "wb");
+    if (!f) die(path);
+    if (n && fwrite(p, 1, n, f) != n) die("short write");
+    fclose(f);
+}
+
+static void read_spec(const char *path, spec_t *s)
+{
+    memset(s, 0, sizeof(*s));
+    s->nbins = 102; s->nt_out = 1; s->qm = 1.0; s->pmass = 1.0; s->ff_b0 = 1.0;
+    s->ff_lambda = 1.0; s->emin = 1e-3; s->emax = 1e1; s->nsteps_output = 1;
+    s->wlcs_normi = 1.0; s->wlcs_omega = 0.001;
+    FILE *f = fopen(path, "r");
+    if (!f) die(path);
+    char key[64], val[512];
+    while (fscanf(f, "%63s %511s", key, val) == 2) {
+#define I(k) if (!strcmp(key, #k)) { s->k = atoi(val); continue; }
+#define D(k) if (!strcmp(key, #k)) { s->k = strtod(val, NULL); continue; }
+#define S(k) if (!strcmp(key, #k)) { strcpy(s->k, val); continue; }
+        I(system) I(bc) I(nbins) I(nt_out) I(nx) I(ny) I(nz) I(nwlcs) I(ibessel)
+        I(nsteps_output)
+        D(tmax) D(qm) D(dt) D(pmass) D(emin) D(emax) D(ff_b0) D(ff_lambda)
+        D(wlcs_omega) D(wlcs_normi)
+        S(grid) S(vgrid) S(wlcs)
+        if (!strcmp(key, "xmin")) { s->lo[0] = strtod(val, NULL); continue; }
+        if (!strcmp(key, "ymin")) { s->lo[1] = strtod(val, NULL); continue; }
+        if (!strcmp(key, "zmin")) { s->lo[2] = strtod(val, NULL); continue; }
+        if (!strcmp(key, "xmax")) { s->hi[0] = strtod(val, NULL); continue; }
+        if (!strcmp(key, "ymax")) { s->hi[1] = strtod(val, NULL); continue; }
+        if (!strcmp(key, "zmax")) { s->hi[2] = strtod(val, NULL); continue; }
+        fprintf(stderr, "unknown key %s\n", key);
+        die("bad spec");
+#undef I
+#undef D
+#undef S
+    }
+    fclose(f);
+}
+
+/* Mirrors the setup src/domain.c:314-409 performs per system type, with
+ * the field arrays filled from files instead of HDF5/binary readers. */
+static void setup(const spec_t *s)
+{
+    domain dom;
+    grids g;
+    memset(&dom, 0, sizeof(dom));
+    memset(&g, 0, sizeof(g));
+    dom.xmin = s->lo[0]; dom.ymin = s->lo[1]; dom.zmin = s->lo[2];
+    dom.xmax = s->hi[0]; dom.ymax = s->hi[1]; dom.zmax = s->hi[2];
+    dom.xmin_ptl = dom.xmin; dom.xmax_ptl = dom.xmax;
+    dom.ymin_ptl = dom.ymin; dom.ymax_ptl = dom.ymax;
+    dom.zmin_ptl = dom.zmin; dom.zmax_ptl = dom.zmax;
+    dom.tmin = 0.0; dom.tmax = s->tmax;
+    if (s->nx > 1) {
+        g.nx = s->nx; g.ny = s->ny; g.nz = s->nz; g.nt = 1;
+        g.dx = (dom.xmax - dom.xmin) / (g.nx - 1.0);   /* src/domain.c:548 */
+        g.dy = (dom.ymax - dom.ymin) / (g.ny - 1.0);
+        g.dz = (dom.zmax - dom.zmin) / (g.nz - 1.0);
+        g.dt = 0.0;
+    }
+    set_variables_tracking(&g, &dom, s->system, 0);
+    set_ptl_params_tracking(s->bc, s->qm);
+    set_variables_emfields(s->system);
+    set_variabls_interpolation(&g, &dom, 0);
+    size_t nn = (size_t)g.nx * g.ny * g.nz, n;
+    if (s->system == 2) {
+        set_variables_ff(&g, &dom);
+        param.B0 = s->ff_b0;
+        param.lambda0 = s->ff_lambda;
+        set_variables_velocity(&g, &dom, 1.0, 0, (int)sizeof(double));
+        initialize_vfield();
+        if (s->vgrid[0]) {
+            void *v = slurp(s->vgrid, &n);
+            if (n != nn * sizeof(d3)) die("vgrid size");
+            memcpy(vc_double, v, n);
+            free(v);
+        } else {
+            memset(vc_double, 0, nn * sizeof(d3));
+        }
+    } else if (s->system == 4) {
+        set_variables_bfield(&g, &dom, 1.0, 0, (int)sizeof(float));
+        set_variables_efield(&g, &dom, 1.0, 0, (int)sizeof(float));
+        initialize_bfield();
+        initialize_efield();
+        float *v = slurp(s->grid, &n);     /* E (float3 AoS) then B */
+        if (n != 2 * nn * sizeof(f3)) die("grid size");
+        memcpy(efc_float, v, nn * sizeof(f3));
+        memcpy(magc_float, (char *)v + nn * sizeof(f3), nn * sizeof(f3));
+        free(v);
+    } else if (s->system == 1) {
+        /* same table read_wlcs (src/wlcs.c:53-222) builds */
+        double *v = slurp(s->wlcs, &n);
+        if (n != (size_t)s->nwlcs * 14 * sizeof(double)) die("wlcs size");
+        nwlcs = s->nwlcs;
+        iBessel = s->ibessel;
+        normI = s->wlcs_normi;
+        config = malloc(sizeof(wlcs) * nwlcs);
+        for (int i = 0; i < nwlcs; i++) {
+            const double *r = v + 14 * i;
+            config[i].cur_wr = r[0]; config[i].x_wr = r[1];
+            config[i].y_wr = r[2]; config[i].z_wr = r[3];
+            config[i].costheta_wr = cos(r[4]); config[i].sintheta_wr = sin(r[4]);
+            config[i].cosphi_wr = cos(r[5]); config[i].sinphi_wr = sin(r[5]);
+            config[i].t0 = r[6]; config[i].cur_lp = r[7]; config[i].r_lp = r[8];
+            config[i].x_lp = r[9]; config[i].y_lp = r[10]; config[i].z_lp = r[11];
+            config[i].cosalpha_lp = cos(r[12]); config[i].sinalpha_lp = sin(r[12]);
+            config[i].cosbeta_lp = cos(r[13]); config[i].sinbeta_lp = sin(r[13]);
+            config[i].omega = s->wlcs_omega;
+            config[i].cur_wr *= normI;
+            config[i].cur_lp *= normI;
+        }
+        free(v);
+    }
+    /* spectrum parameters as src/diagnostics.c:629-631 derives them */
+    emin = s->emin;
+    emax = s->emax;
+    logemin = log10(emin);
+    logemax = log10(emax);
+    logde = (logemax - logemin) / (s->nbins - 2);
+}
+
+int main(int argc, char **argv)
+{
+    if (argc < 3) die("usage: ref_harness MODE SPEC ...");
+    MPI_Init(&argc, &argv);
+    spec_t s;
+    read_spec(argv[2], &s);
+    setup(&s);
+    size_t n;
+    const char *mode = argv[1];
+    if (!strcmp(mode, "fields")) {
+        double *pts = slurp(argv[3], &n);
+        size_t np = n / (4 * sizeof(double));
+        emfields *out = malloc(np * sizeof(emfields) + 1);
+        for (size_t i = 0; i < np; i++)
+            get_emf(pts[4 * i], pts[4 * i + 1], pts[4 * i + 2], pts[4 * i + 3], &out[i]);
+        spit(argv[4], out, np * sizeof(emfields));
+    } else if (!strcmp(mode, "wirz")) {
+        particles *p = slurp(argv[3], &n);
+        size_t np = n / sizeof(particles);
+        int k = atoi(argv[4]);
+        for (size_t i = 0; i < np; i++) {
+            int esc = 0;
+            for (int j = 0; j < k; j++) {
+                tracking_wirz(&p[i], s.dt);
+                particle_bc(&p[i], &esc);
+            }
+        }
+        spit(argv[5], p, n);
+    } else if (!strcmp(mode, "diff")) {
+        particles *p0 = slurp(argv[3], &n);
+        size_t np = n / sizeof(particles);
+        particles *p = slurp(argv[4], &n);
+        double dt = strtod(argv[5], NULL);
+        double *out = malloc(np * 8 * sizeof(double) + 1);
+        for (size_t i = 0; i < np; i++) {
+            double *o = out + 8 * i;
+            calc_diff_coeff(&p0[i], &p[i], dt, o, o + 1, o + 2, o + 3, o + 4,
+                            o + 5, o + 6, o + 7);
+        }
+        spit(argv[6], out, np * 8 * sizeof(double));
+    } else if (!strcmp(mode, "track")) {
+        /* src/tracking.c:90-190 with one thread, keeping the raw arrays */
+        particles *p = slurp(argv[3], &n);
+        int np = (int)(n / sizeof(particles));
+        particles *p0 = malloc(n + 1);
+        memcpy(p0, p, n);
+        int estep = s.tmax / s.dt;
+        int einterval = estep / s.nt_out;
+        int nd = estep / 100 + 1;
+        int nb = s.nbins, nt = s.nt_out;
+        double *dsum = calloc((size_t)8 * nd, sizeof(double));
+        int *remain = malloc(sizeof(int) * nd);
+        for (int i = 0; i < nd; i++) remain[i] = np;
+        int *nsteps = malloc(sizeof(int) * (np + 1));
+        for (int i = 0; i < np; i++) nsteps[i] = 1;
+        double *esp = calloc((size_t)nb * nt + 1, sizeof(double));
+        double *esc = calloc((size_t)nb * nt + 1, sizeof(double));
+        particle_tracking_multi_steps_openmp(np, s.dt, 0, 1, estep, nb, nt,
+            einterval, 0, s.nsteps_output, s.pmass, NULL, p0, NULL, p, nsteps,
+            (double (*)[1])esp, (double (*)[1])esc, NULL, remain,
+            dsum, dsum + nd, dsum + 2 * nd, dsum + 3 * nd, dsum + 4 * nd,
+            dsum + 5 * nd, dsum + 6 * nd, dsum + 7 * nd);
+        char path[1024];
+        snprintf(path, sizeof(path), "%s/ptl.bin", argv[4]);
+        spit(path, p, n);
+        snprintf(path, sizeof(path), "%s/nsteps.bin", argv[4]);
+        spit(path, nsteps, sizeof(int) * np);
+        snprintf(path, sizeof(path), "%s/espect.bin", argv[4]);
+        spit(path, esp, sizeof(double) * nb * nt);
+        snprintf(path, sizeof(path), "%s/espect_escape.bin", argv[4]);
+        spit(path, esc, sizeof(double) * nb * nt);
+        snprintf(path, sizeof(path), "%s/dcoeff.bin", argv[4]);
+        spit(path, dsum, sizeof(double) * 8 * nd);
+        snprintf(path, sizeof(path), "%s/remain.bin", argv[4]);
+        spit(path, remain, sizeof(int) * nd);
+    } else if (!strcmp(mode, "ellint")) {
+        double *ks = slurp(argv[3], &n);
+        size_t nk = n / sizeof(double);
+        double *out = malloc(nk * 2 * sizeof(double) + 1);
+        for (size_t i = 0; i < nk; i++) ellint(ks[i], &out[2 * i], &out[2 * i + 1]);
+        spit(argv[4], out, nk * 2 * sizeof(double));
+    } else if (!strcmp(mode, "hybrid")) {
+        particles *p = slurp(argv[3], &n);
+        int np = (int)(n / sizeof(particles));
+        particles *p0 = malloc(n + 1);
+        memcpy(p0, p, n);
+        int *nsteps = malloc(sizeof(int) * (np + 1));
+        calc_energy_spectrum(0, np, p, s.nbins, s.nt_out, s.pmass, s.bc);
+        particle_tracking_hybrid(0, np, s.dt, s.nbins, s.nt_out, s.bc,
+            s.nsteps_output, s.pmass, NULL, 0, 0, p0, nsteps, p, NULL);
+    } else {
+        die("unknown mode");
+    }
+    MPI_Finalize();
+    return 0;
+}

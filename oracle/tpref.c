@@ -1,0 +1,783 @@
+/*
+ * tpref.c -- the project's FP64 CPU oracle.  TEST INFRASTRUCTURE ONLY:
+ * imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg as the checker; never part of the product path.
+ *
+ * A restatement, in plain C, of the reference tracer's hot path
+ * (xiaocanli/test_particle):
+ *   fields     get_emf src/emfields.c:32-60, getemf_test :210-218,
+ *              getb_ff/gete_ff src/force_free.c:78-86, :138-150,
+ *              trilinear gather src/interpolation.c:36-181 with the float
+ *              gathers src/magnetic_field.c:593-622,
+ *              src/electric_field.c:408-437 and the double velocity gather
+ *              src/velocity_field.c:555-584,
+ *              WLCS src/wlcs.c:281-449, :814-903
+ *   pusher     tracking_wirz src/tracking.c:749-892 (+ helpers :1066-1140)
+ *   bc         particle_bc src/tracking.c:1172-1221
+ *   loop       src/tracking.c:90-190, :297-462
+ *   diagnostics ptl_energy_fixed src/diagnostics.c:187-205,
+ *              calc_diff_coeff :659-705
+ * plus, for what the reference does not have, a relativistic Boris push
+ * and a multi-mode turbulence field.
+ *
+ * Arithmetic follows the reference's operation order, without
+ * contraction (compiled -ffp-contract=off), so that with libm sin/cos
+ * (tpref_set_math(1)) it reproduces the reference binary bit for bit, and
+ * with the shared tp_sincos (tpref_set_math(0)) the GPU kernels.
+ *
+ * Pins (tests/): golden vectors written by the reference binary through
+ * oracle/ref_harness.c (fields of systems 0/2/4, Wirz trajectories,
+ * calc_diff_coeff, spectrum/D sums of the full loop); closed forms (Wirz is
+ * exact in uniform B; Boris rotation angle 2 atan(Omega dt / 2 gamma));
+ * scipy ellipk/ellipe for the elliptic integrals; brute-force mode sums.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <omp.h>
+#include "tpref.h"
+#include "tp_math.h"
+
+static int g_use_libm = 1;
+static int g_carry_stale = 0;
+
+void tpref_set_math(int use_libm) { g_use_libm = use_libm; }
+
+/* 1: the stale per-particle D terms (dxx dyy dzz daa dee) carry over from
+ * one particle to the next in loop order, as the reference's loop-private
+ * variables do with one thread (src/tracking.c:306-312, 334-336); 0: each
+ * particle starts from zero (the semantics the GPU kernels implement). */
+void tpref_set_carry_stale(int on) { g_carry_stale = on; }
+
+void tpref_sincos(double x, double *s, double *c) { tp_sincos(x, s, c); }
+
+static void ellint_ref(double k, double *elle, double *ellk);
+void tpref_ellint_ref(double k, double *elle, double *ellk) { ellint_ref(k, elle, ellk); }
+
+static inline double o_sin(double x) { return g_use_libm ? sin(x) : tp_sin(x); }
+static inline double o_cos(double x) { return g_use_libm ? cos(x) : tp_cos(x); }
+
+typedef struct { double Bx, By, Bz, Ex, Ey, Ez; } emf_t;
+
+/* ---------------------------------------------------------------------- */
+/* trilinear gather                                                        */
+/* ---------------------------------------------------------------------- */
+
+/* grid_indices + get_trilinar_parameters, src/interpolation.c:36-105,
+ * 141-181.  Corner order and weights as there (the x and z fractions are
+ * exchanged relative to the corner they belong to).  The flat corner
+ * index is clamped into the array; the reference reads past a row when a
+ * coordinate equals the upper bound exactly. */
+static void trilinear_ref(int nx, int ny, int nz, const double lo[3],
+                          const double hi[3], double x, double y, double z,
+                          long corners[8], double w[8])
+{
+    double ddx = (hi[0] - lo[0]) / (nx - 1.0);
+    double ddy = (hi[1] - lo[1]) / (ny - 1.0);
+    double ddz = (hi[2] - lo[2]) / (nz - 1.0);
+    double fx, fy, fz, vx, vy, vz;
+    int ix1, iy1, iz1, ix2, iy2, iz2;
+    vx = floor((x - lo[0]) / ddx);
+    vy = floor((y - lo[1]) / ddy);
+    vz = floor((z - lo[2]) / ddz);
+    /* (int) of a value outside int range is INT_MIN on x86-64 */
+    ix1 = (vx > -2147483649.0 && vx < 2147483648.0) ? (int)vx : (-2147483647 - 1);
+    iy1 = (vy > -2147483649.0 && vy < 2147483648.0) ? (int)vy : (-2147483647 - 1);
+    iz1 = (vz > -2147483649.0 && vz < 2147483648.0) ? (int)vz : (-2147483647 - 1);
+    ix2 = ix1 + 1;
+    iy2 = iy1 + 1;
+    iz2 = iz1 + 1;
+    fx = (x - lo[0]) / ddx - ix1;
+    fy = (y - lo[1]) / ddy - iy1;
+    fz = (z - lo[2]) / ddz - iz1;
+    if (x > hi[0]) { ix1 = nx - 1; ix2 = 0; fx = x / ddx - (nx - 1); }
+    if (y > hi[1]) { iy1 = ny - 1; iy2 = 0; fy = y / ddy - (ny - 1); }
+    if (z > hi[2]) { iz1 = nz - 1; iz2 = 0; fz = z / ddz - (nz - 1); }
+    if (x < lo[0]) { ix1 = nx - 1; ix2 = 0; fx = x / ddx + 1.0; }
+    if (y < lo[1]) { iy1 = ny - 1; iy2 = 0; fy = y / ddy + 1.0; }
+    if (z < lo[2]) { iz1 = nz - 1; iz2 = 0; fz = z / ddz + 1.0; }
+    long total = (long)nx * ny * nz;
+#define FLAT(a, b, c) ((long)(c) + (long)nx * ((long)(b) + (long)ny * (long)(a)))
+    long cc[8] = {
+        FLAT(iz1, iy1, ix1), FLAT(iz2, iy1, ix1), FLAT(iz1, iy2, ix1),
+        FLAT(iz1, iy1, ix2), FLAT(iz1, iy2, ix2), FLAT(iz2, iy1, ix2),
+        FLAT(iz2, iy2, ix1), FLAT(iz2, iy2, ix2)};
+#undef FLAT
+    for (int i = 0; i < 8; i++) {
+        long c = cc[i];
+        corners[i] = c < 0 ? 0 : (c >= total ? total - 1 : c);
+    }
+    w[0] = (1.0 - fx) * (1.0 - fy) * (1.0 - fz);
+    w[1] = fx * (1.0 - fy) * (1.0 - fz);
+    w[2] = (1.0 - fx) * fy * (1.0 - fz);
+    w[3] = (1.0 - fx) * (1.0 - fy) * fz;
+    w[4] = (1.0 - fx) * fy * fz;
+    w[5] = fx * (1.0 - fy) * fz;
+    w[6] = fx * fy * (1.0 - fz);
+    w[7] = fx * fy * fz;
+}
+
+/* Textbook trilinear on the same endpoint-inclusive nodes: the position is
+ * folded into [lo, hi) by the period hi - lo, corner ix+1 gets weight fx. */
+static void trilinear_textbook(int nx, int ny, int nz, const double lo[3],
+                               const double hi[3], double x, double y,
+                               double z, long corners[8], double w[8])
+{
+    double p[3] = {x, y, z};
+    int n[3] = {nx, ny, nz};
+    int i1[3], i2[3];
+    double f[3];
+    for (int a = 0; a < 3; a++) {
+        double L = hi[a] - lo[a];
+        double d = L / (n[a] - 1.0);
+        double q = p[a] - lo[a];
+        q = q - L * floor(q / L);
+        double v = floor(q / d);
+        int i = (v >= 0.0 && v <= n[a] - 2.0) ? (int)v : (v < 0.0 ? 0 : n[a] - 2);
+        if (!(v == v)) i = 0;
+        i1[a] = i;
+        i2[a] = i + 1;
+        f[a] = q / d - i;
+    }
+    /* same corner order as the reference */
+#define FLAT(a, b, c) ((long)(c) + (long)nx * ((long)(b) + (long)ny * (long)(a)))
+    corners[0] = FLAT(i1[2], i1[1], i1[0]);
+    corners[1] = FLAT(i2[2], i1[1], i1[0]);
+    corners[2] = FLAT(i1[2], i2[1], i1[0]);
+    corners[3] = FLAT(i1[2], i1[1], i2[0]);
+    corners[4] = FLAT(i1[2], i2[1], i2[0]);
+    corners[5] = FLAT(i2[2], i1[1], i2[0]);
+    corners[6] = FLAT(i2[2], i2[1], i1[0]);
+    corners[7] = FLAT(i2[2], i2[1], i2[0]);
+#undef FLAT
+    double fx = f[0], fy = f[1], fz = f[2];
+    w[0] = (1.0 - fx) * (1.0 - fy) * (1.0 - fz);
+    w[1] = (1.0 - fx) * (1.0 - fy) * fz;
+    w[2] = (1.0 - fx) * fy * (1.0 - fz);
+    w[3] = fx * (1.0 - fy) * (1.0 - fz);
+    w[4] = fx * fy * (1.0 - fz);
+    w[5] = fx * (1.0 - fy) * fz;
+    w[6] = (1.0 - fx) * fy * fz;
+    w[7] = fx * fy * fz;
+}
+
+/* get_float_{e,b}field_at_point: float accumulator, rounded after every
+ * corner (src/magnetic_field.c:603-608) */
+static void gather_float_ref(const float *g, const long c[8], const double w[8],
+                             float out[3])
+{
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    for (int i = 0; i < 8; i++) {
+        ax = (float)((double)ax + (double)g[3 * c[i]] * w[i]);
+        ay = (float)((double)ay + (double)g[3 * c[i] + 1] * w[i]);
+        az = (float)((double)az + (double)g[3 * c[i] + 2] * w[i]);
+    }
+    out[0] = ax; out[1] = ay; out[2] = az;
+}
+
+static void gather_double(const float *gf, const double *gd, const long c[8],
+                          const double w[8], double out[3])
+{
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    for (int i = 0; i < 8; i++) {
+        double vx = gf ? (double)gf[3 * c[i]] : gd[3 * c[i]];
+        double vy = gf ? (double)gf[3 * c[i] + 1] : gd[3 * c[i] + 1];
+        double vz = gf ? (double)gf[3 * c[i] + 2] : gd[3 * c[i] + 2];
+        ax = ax + vx * w[i];
+        ay = ay + vy * w[i];
+        az = az + vz * w[i];
+    }
+    out[0] = ax; out[1] = ay; out[2] = az;
+}
+
+/* ---------------------------------------------------------------------- */
+/* WLCS                                                                    */
+/* ---------------------------------------------------------------------- */
+#define WL_L0 6.96E8     /* include/constants.h:14 */
+#define WL_C0 3.0E10     /* include/constants.h:4 */
+#define WL_I0 4.0E8      /* include/wlcs.h:41 */
+
+/* Complete elliptic integrals K(k), E(k) by the arithmetic-geometric mean
+ * (Abramowitz & Stegun 17.6.1-17.6.4).  The reference evaluates the same
+ * functions with Fukushima's piecewise series (src/wlcs.c:460-809); both
+ * are accurate to about 1e-15. */
+void tpref_ellint(double k, double *elle, double *ellk)
+{
+    double a = 1.0, b = sqrt(1.0 - k * k), c = k;
+    double sum = 0.5 * c * c, pw = 0.5;
+    for (int i = 0; i < 40; i++) {
+        double an = 0.5 * (a + b);
+        double bn = sqrt(a * b);
+        c = 0.5 * (a - b);
+        pw *= 2.0;
+        sum += pw * c * c;
+        a = an;
+        b = bn;
+        if (fabs(c) < 1e-17 * a) break;
+    }
+    double K = M_PI / (2.0 * a);
+    *ellk = K;
+    *elle = K * (1.0 - sum);
+}
+
+/* ellint as the reference evaluates it.  For m = k^2 < 0.9 its Taylor
+ * series are accurate and equal K, E.  For m >= 0.9 (src/wlcs.c:741-806) it
+ * uses the nome relation K = -ln q(1-m) K'/pi and Legendre's relation
+ * E = (pi/2 + K H')/K', but evaluates the K' series about 0.05 at offset
+ * m - 0.95 instead of 0.95 - m, i.e. it takes K' = K(m - 0.9) where
+ * K(1 - m) belongs.  Restated: K = K(m) K(m-0.9) / K(1-m),
+ * E = (pi/2 + K (K(1-m) - E(1-m))) / K(m-0.9). */
+static void ellint_ref(double k, double *elle, double *ellk)
+{
+    double m = k * k;
+    if (!(m >= 0.9)) {
+        tpref_ellint(k, elle, ellk);
+        return;
+    }
+    double Ec, Kc, Em, Km, Ew, Kw;
+    tpref_ellint(sqrt(1.0 - m), &Ec, &Kc);
+    tpref_ellint(k, &Em, &Km);
+    tpref_ellint(sqrt(m - 0.9), &Ew, &Kw);
+    double Kref = (M_PI * Km / Kc) * (Kw / M_PI);
+    *ellk = Kref;
+    *elle = 1.0 / Kw * (M_PI / 2.0 + Kref * (Kc - Ec));
+}
+
+/* transform / reverse_tran, src/wlcs.c:871-903 */
+static void wl_transform(double *ax, double *ay, double *az, double c1,
+                         double s1, double c2, double s2)
+{
+    double ax1 = c2 * (*ax) + s2 * (*ay);
+    double ay1 = -s2 * (*ax) + c2 * (*ay);
+    double az1 = *az;
+    *ax = c1 * ax1 - s1 * az1;
+    *ay = ay1;
+    *az = s1 * ax1 + c1 * az1;
+}
+
+static void wl_reverse(double *ax, double *ay, double *az, double c1,
+                       double s1, double c2, double s2)
+{
+    double ax1 = c1 * (*ax) + s1 * (*az);
+    double ay1 = *ay;
+    double az1 = -s1 * (*ax) + c1 * (*az);
+    *ax = c2 * ax1 - s2 * ay1;
+    *ay = s2 * ax1 + c2 * ay1;
+    *az = az1;
+}
+
+typedef struct {
+    double cur_wr, x_wr, y_wr, z_wr, ct, st, cp, sp, t0;
+    double cur_lp, r_lp, x_lp, y_lp, z_lp, ca, sa, cb, sb, omega;
+} wl_sys;
+
+/* the table read_wlcs builds (src/wlcs.c:166-212) */
+static void wl_setup(const tpref_field *f, int i, wl_sys *s)
+{
+    const double *r = f->wlcs_rows + 14 * i;
+    s->cur_wr = r[0] * f->norm_i;
+    s->x_wr = r[1]; s->y_wr = r[2]; s->z_wr = r[3];
+    s->ct = o_cos(r[4]); s->st = o_sin(r[4]);
+    s->cp = o_cos(r[5]); s->sp = o_sin(r[5]);
+    s->t0 = r[6];
+    s->cur_lp = r[7] * f->norm_i;
+    s->r_lp = r[8]; s->x_lp = r[9]; s->y_lp = r[10]; s->z_lp = r[11];
+    s->ca = o_cos(r[12]); s->sa = o_sin(r[12]);
+    s->cb = o_cos(r[13]); s->sb = o_sin(r[13]);
+    s->omega = f->omega;
+}
+
+/* getWireB, src/wlcs.c:361-401 */
+static void wl_wire_static(const wl_sys *s, double x, double y, double z,
+                           double b[3])
+{
+    double xp = x - s->x_wr, yp = y - s->y_wr, zp = z - s->z_wr;
+    wl_transform(&xp, &yp, &zp, s->ct, s->st, s->cp, s->sp);
+    double rho = sqrt(xp * xp + yp * yp);
+    if (rho < 0.1) rho = 0.1;
+    double cphi = xp / rho;
+    double phi0 = (yp >= 0.0) ? acos(cphi) : 2.0 * M_PI - acos(cphi);
+    double bphi = 1.0 / 5.0 * (s->cur_wr * WL_I0) / (rho * WL_L0);
+    double bx = bphi * o_cos(phi0 + M_PI / 2.0);
+    double by = bphi * o_sin(phi0 + M_PI / 2.0);
+    double bz = 0.0;
+    wl_reverse(&bx, &by, &bz, s->ct, s->st, s->cp, s->sp);
+    b[0] = bx; b[1] = by; b[2] = bz;
+}
+
+/* getWire_Bessel, src/wlcs.c:281-355 (float Bessel functions as there) */
+static void wl_wire_bessel(const wl_sys *s, double x, double y, double z,
+                           double t, double b[3], double e[3])
+{
+    double xp = x - s->x_wr, yp = y - s->y_wr, zp = z - s->z_wr;
+    double t1 = s->t0 + t;
+    double om = s->omega;
+    wl_transform(&xp, &yp, &zp, s->ct, s->st, s->cp, s->sp);
+    double rho = sqrt(xp * xp + yp * yp);
+    double x0 = om * rho * WL_L0 / WL_C0;
+    double bj0 = j0f(x0), bj1 = j1f(x0), by0 = y0f(x0), by1 = y1f(x0);
+    double const0 = (s->cur_wr * WL_I0) / (5.0 * rho * WL_L0) * M_PI / 2.0;
+    double wb = const0 * x0 * (-by1 * o_cos(om * t1) + bj1 * o_sin(om * t1));
+    double phi1;
+    if (rho == 0.0) phi1 = M_PI / 2.0;
+    else if (yp >= 0.0) phi1 = acos(xp / rho) + M_PI / 2.0;
+    else phi1 = 2.0 * M_PI - acos(xp / rho) + M_PI / 2.0;
+    double bx = wb * o_cos(phi1), by = wb * o_sin(phi1), bz = 0.0;
+    wl_reverse(&bx, &by, &bz, s->ct, s->st, s->cp, s->sp);
+    double we = -const0 * x0 * (by0 * o_sin(om * t1) + bj0 * o_cos(om * t1));
+    double ex = 0.0, ey = 0.0, ez = we;
+    wl_reverse(&ex, &ey, &ez, s->ct, s->st, s->cp, s->sp);
+    b[0] = bx; b[1] = by; b[2] = bz;
+    e[0] = ex; e[1] = ey; e[2] = ez;
+}
+
+/* getLoopB, src/wlcs.c:406-449 */
+static void wl_loop(const wl_sys *s, int compat, double x, double y, double z,
+                    double b[3])
+{
+    double xp = (x - s->x_lp) * WL_L0;
+    double yp = (y - s->y_lp) * WL_L0;
+    double zp = (z - s->z_lp) * WL_L0;
+    double al = s->r_lp * WL_L0;
+    double curI = s->cur_lp * WL_I0;
+    wl_transform(&xp, &yp, &zp, s->cb, s->sb, s->ca, s->sa);
+    double Irho = 0.2 * curI;
+    double rho_xy = sqrt(xp * xp + yp * yp);
+    double al2 = al * al;
+    double r2 = rho_xy * rho_xy + zp * zp;
+    double irho1 = 1 / sqrt((al + rho_xy) * (al + rho_xy) + zp * zp);
+    double irho2 = 1 / ((al - rho_xy) * (al - rho_xy) + zp * zp);
+    double k = sqrt(4.0 * al * rho_xy) * irho1;
+    double elle, ellk;
+    if (compat) ellint_ref(k, &elle, &ellk);
+    else tpref_ellint(k, &elle, &ellk);
+    double Brho = Irho * zp * irho1 * (-ellk + (al2 + r2) * elle * irho2) / rho_xy;
+    if (rho_xy == 0.0) Brho = 0.0;
+    double Bz = Irho * irho1 * (ellk - (r2 - al2) * elle * irho2);
+    Brho /= rho_xy;
+    double bx = Brho * xp, by = Brho * yp, bz = Bz;
+    wl_reverse(&bx, &by, &bz, s->cb, s->sb, s->ca, s->sa);
+    b[0] = bx; b[1] = by; b[2] = bz;
+}
+
+/* getemf_wlcs, src/wlcs.c:814-865 */
+static void emf_wlcs(const tpref_field *f, double x, double y, double z,
+                     double t, emf_t *o)
+{
+    double Bx = 0.0, By = 0.0, Bz = 0.0, Ex = 0.0, Ey = 0.0, Ez = 0.0;
+    for (int i = 0; i < f->nwlcs; i++) {
+        wl_sys s;
+        double b[3], e[3];
+        wl_setup(f, i, &s);
+        if (f->ibessel == 0) {
+            wl_wire_static(&s, x, y, z, b);
+            Bx += b[0]; By += b[1]; Bz += b[2];
+        } else {
+            wl_wire_bessel(&s, x, y, z, t, b, e);
+            Bx += b[0]; Ex += e[0];
+            By += b[1]; Ey += e[1];
+            Bz += b[2]; Ez += e[2];
+        }
+        wl_loop(&s, f->wlcs_mode == TP_GATHER_REF_COMPAT, x, y, z, b);
+        Bx += b[0]; By += b[1]; Bz += b[2];
+    }
+    o->Bx = Bx; o->By = By; o->Bz = Bz + 0.0;
+    o->Ex = Ex; o->Ey = Ey; o->Ez = Ez;
+}
+
+/* ---------------------------------------------------------------------- */
+/* get_emf                                                                 */
+/* ---------------------------------------------------------------------- */
+static void get_emf(const tpref_field *f, double x, double y, double z,
+                    double t, emf_t *o)
+{
+    o->Bx = o->By = o->Bz = o->Ex = o->Ey = o->Ez = 0.0;
+    switch (f->kind) {
+    case TPREF_FIELD_TEST:
+        o->Bz = 1.0;
+        break;
+    case TPREF_FIELD_ABC: {
+        /* getb_ff, src/force_free.c:138-150 */
+        double A = f->abc[0], B = f->abc[1], C = f->abc[2];
+        double lam = f->abc[3], B0 = f->abc[4];
+        double bx = A * o_sin(lam * z) + C * o_cos(lam * y);
+        double by = B * o_sin(lam * x) + A * o_cos(lam * z);
+        double bz = C * o_sin(lam * y) + B * o_cos(lam * x);
+        bx *= B0; by *= B0; bz *= B0;
+        o->Bx = bx; o->By = by; o->Bz = bz;
+        if (f->vgrid) {
+            /* gete_ff, src/force_free.c:78-86 */
+            long c[8];
+            double w[8], v[3];
+            trilinear_ref(f->vnx, f->vny, f->vnz, f->vlo, f->vhi, x, y, z, c, w);
+            gather_double(NULL, f->vgrid, c, w, v);
+            o->Ex = by * v[2] - bz * v[1];
+            o->Ey = bz * v[0] - bx * v[2];
+            o->Ez = bx * v[1] - by * v[0];
+        } else {
+            o->Ex = o->Ey = o->Ez = 0.0;
+        }
+        break;
+    }
+    case TPREF_FIELD_GRID: {
+        /* getemf_pic_test_particle, src/emfields.c:153-166 */
+        long c[8];
+        double w[8];
+        if (f->gather_mode == TP_GATHER_REF_COMPAT) {
+            float e[3], b[3];
+            trilinear_ref(f->nx, f->ny, f->nz, f->lo, f->hi, x, y, z, c, w);
+            gather_float_ref(f->egrid, c, w, e);
+            gather_float_ref(f->bgrid, c, w, b);
+            o->Ex = e[0]; o->Ey = e[1]; o->Ez = e[2];
+            o->Bx = b[0]; o->By = b[1]; o->Bz = b[2];
+        } else {
+            double e[3], b[3];
+            trilinear_textbook(f->nx, f->ny, f->nz, f->lo, f->hi, x, y, z, c, w);
+            gather_double(f->egrid, NULL, c, w, e);
+            gather_double(f->bgrid, NULL, c, w, b);
+            o->Ex = e[0]; o->Ey = e[1]; o->Ez = e[2];
+            o->Bx = b[0]; o->By = b[1]; o->Bz = b[2];
+        }
+        break;
+    }
+    case TPREF_FIELD_MODES: {
+        /* B = b0z z + sum_n a_n cos(k_n.x + phi_n); the phase is built
+         * as fma(kz, z, fma(ky, y, fma(kx, x, phi))) and the sum uses fma */
+        double bx = 0.0, by = 0.0, bz = 0.0;
+        for (int n = 0; n < f->nmodes; n++) {
+            const double *m = f->modes + 7 * n;
+            double ph = fma(m[2], z, fma(m[1], y, fma(m[0], x, m[6])));
+            double cs = g_use_libm ? cos(ph) : tp_cos(ph);
+            bx = fma(m[3], cs, bx);
+            by = fma(m[4], cs, by);
+            bz = fma(m[5], cs, bz);
+        }
+        o->Bx = bx; o->By = by; o->Bz = f->b0z + bz;
+        break;
+    }
+    case TPREF_FIELD_WLCS:
+        emf_wlcs(f, x, y, z, t, o);
+        break;
+    default:
+        break;
+    }
+}
+
+void tpref_eval_field(const tpref_field *f, int n, const double *pts,
+                      double *out)
+{
+    for (int i = 0; i < n; i++) {
+        emf_t e;
+        get_emf(f, pts[4 * i], pts[4 * i + 1], pts[4 * i + 2], pts[4 * i + 3], &e);
+        double *o = out + 6 * i;
+        o[0] = e.Bx; o[1] = e.By; o[2] = e.Bz;
+        o[3] = e.Ex; o[4] = e.Ey; o[5] = e.Ez;
+    }
+}
+
+/* ---------------------------------------------------------------------- */
+/* pushers                                                                 */
+/* ---------------------------------------------------------------------- */
+static inline double modulus(double x, double y, double z)
+{
+    return sqrt(x * x + y * y + z * z);
+}
+
+/* tracking_wirz, src/tracking.c:749-892, same operation order */
+void tpref_wirz_step(const tpref_field *f, tp_particle *p, double dt, double qm)
+{
+    double vminus[3], vplus[3], delta[3], delta_m[3], hpara[3], hperp[3], hr[3];
+    double r0[3] = {p->x, p->y, p->z};
+    double t0 = p->t;
+    emf_t emf;
+    delta[0] = p->ux; delta[1] = p->uy; delta[2] = p->uz;
+    double gama, igama, Btot, iBtot, vdothp, vperp, gyroR, omegac, dtheta;
+    double sd, cd, tmp1, tmp2, tmp3;
+    int i;
+
+    get_emf(f, r0[0], r0[1], r0[2], t0, &emf);
+    Btot = sqrt(emf.Bx * emf.Bx + emf.By * emf.By + emf.Bz * emf.Bz);
+    iBtot = 1.0 / Btot;
+    double dth = dt * 0.5;
+    delta_m[0] = delta[0] + qm * dth * emf.Ex;
+    delta_m[1] = delta[1] + qm * dth * emf.Ey;
+    delta_m[2] = delta[2] + qm * dth * emf.Ez;
+    gama = sqrt(1.0 + delta_m[0] * delta_m[0] + delta_m[1] * delta_m[1] +
+                delta_m[2] * delta_m[2]);
+    igama = 1.0 / gama;
+    for (i = 0; i < 3; i++) vminus[i] = delta_m[i] * igama;
+    hpara[0] = emf.Bx * iBtot;
+    hpara[1] = emf.By * iBtot;
+    hpara[2] = emf.Bz * iBtot;
+    vdothp = 0.0;
+    for (i = 0; i < 3; i++) vdothp += vminus[i] * hpara[i];
+    vperp = modulus(vminus[0] - vdothp * hpara[0], vminus[1] - vdothp * hpara[1],
+                    vminus[2] - vdothp * hpara[2]);
+    for (i = 0; i < 3; i++) hperp[i] = (vminus[i] - vdothp * hpara[i]) / vperp;
+    hr[0] = hpara[1] * hperp[2] - hpara[2] * hperp[1];
+    hr[1] = hpara[2] * hperp[0] - hpara[0] * hperp[2];
+    hr[2] = hpara[0] * hperp[1] - hpara[1] * hperp[0];
+    gyroR = vperp * gama * 1.0 * iBtot / qm;
+    omegac = qm * Btot * igama;
+    dtheta = omegac * dt;
+    sd = o_sin(dtheta * 0.5);
+    cd = o_cos(dtheta * 0.5);
+    tmp1 = dth * vdothp * 1.0;
+    tmp2 = gyroR * sd;
+    tmp3 = -gyroR * (1.0 - cd);
+    double pre[3];
+    for (i = 0; i < 3; i++)
+        pre[i] = r0[i] + hpara[i] * tmp1 + hperp[i] * tmp2 + hr[i] * tmp3;
+
+    get_emf(f, pre[0], pre[1], pre[2], t0 + dth, &emf);
+    Btot = sqrt(emf.Bx * emf.Bx + emf.By * emf.By + emf.Bz * emf.Bz);
+    iBtot = 1.0 / Btot;
+    hpara[0] = emf.Bx * iBtot;
+    hpara[1] = emf.By * iBtot;
+    hpara[2] = emf.Bz * iBtot;
+    vdothp = 0.0;
+    for (i = 0; i < 3; i++) vdothp += vminus[i] * hpara[i];
+    vperp = modulus(vminus[0] - vdothp * hpara[0], vminus[1] - vdothp * hpara[1],
+                    vminus[2] - vdothp * hpara[2]);
+    for (i = 0; i < 3; i++) hperp[i] = (vminus[i] - vdothp * hpara[i]) / vperp;
+    hr[0] = hpara[1] * hperp[2] - hpara[2] * hperp[1];
+    hr[1] = hpara[2] * hperp[0] - hpara[0] * hperp[2];
+    hr[2] = hpara[0] * hperp[1] - hpara[1] * hperp[0];
+    omegac = qm * Btot * igama;
+    dtheta = omegac * dt;
+    sd = o_sin(dtheta);
+    cd = o_cos(dtheta);
+    for (i = 0; i < 3; i++)
+        vplus[i] = hpara[i] * vdothp + vperp * (cd * hperp[i] - sd * hr[i]);
+    /* get_delta_vec, src/tracking.c:1066-1078 */
+    double bm = 0.0;
+    for (i = 0; i < 3; i++) bm += vplus[i] * vplus[i];
+    double g2 = 1.0 / sqrt(1.0 - bm);
+    for (i = 0; i < 3; i++) delta[i] = g2 * vplus[i];
+    p->ux = delta[0] + qm * dth * emf.Ex;
+    p->uy = delta[1] + qm * dth * emf.Ey;
+    p->uz = delta[2] + qm * dth * emf.Ez;
+    vdothp = 0.0;
+    for (i = 0; i < 3; i++) vdothp += vplus[i] * hpara[i];
+    vperp = modulus(vplus[0] - vdothp * hpara[0], vplus[1] - vdothp * hpara[1],
+                    vplus[2] - vdothp * hpara[2]);
+    gyroR = vperp * gama * 1.0 * iBtot / qm;
+    tmp1 = dt * vdothp * 1.0;
+    tmp2 = gyroR * sd;
+    tmp3 = -gyroR * (1.0 - cd);
+    double d[3];
+    for (i = 0; i < 3; i++) d[i] = hpara[i] * tmp1 + hperp[i] * tmp2 + hr[i] * tmp3;
+    p->x += d[0];
+    p->y += d[1];
+    p->z += d[2];
+    p->t += dt;
+}
+
+/* Relativistic Boris push (not in the reference; it occupies the call slot
+ * of src/tracking.c:429).  u = gamma v; field at (x, t):
+ *   h = qm dt/2;  u- = u + h E;  g- = sqrt(1 + |u-|^2);  tv = (h/g-) B;
+ *   sv = 2 tv / (1 + |tv|^2);  u' = u- + u- x tv;  u+ = u- + u' x sv;
+ *   u = u+ + h E;  x += u (dt / sqrt(1 + |u|^2));  t += dt */
+void tpref_boris_step(const tpref_field *f, tp_particle *p, double dt, double qm)
+{
+    emf_t e;
+    get_emf(f, p->x, p->y, p->z, p->t, &e);
+    double h = qm * (0.5 * dt);
+    double umx = p->ux + h * e.Ex;
+    double umy = p->uy + h * e.Ey;
+    double umz = p->uz + h * e.Ez;
+    double gm = sqrt(1.0 + umx * umx + umy * umy + umz * umz);
+    double fq = h / gm;
+    double tx = fq * e.Bx, ty = fq * e.By, tz = fq * e.Bz;
+    double t2 = tx * tx + ty * ty + tz * tz;
+    double sf = 2.0 / (1.0 + t2);
+    double sx = sf * tx, sy = sf * ty, sz = sf * tz;
+    double upx = umx + (umy * tz - umz * ty);
+    double upy = umy + (umz * tx - umx * tz);
+    double upz = umz + (umx * ty - umy * tx);
+    double uqx = umx + (upy * sz - upz * sy);
+    double uqy = umy + (upz * sx - upx * sz);
+    double uqz = umz + (upx * sy - upy * sx);
+    double ux = uqx + h * e.Ex;
+    double uy = uqy + h * e.Ey;
+    double uz = uqz + h * e.Ez;
+    double g = sqrt(1.0 + ux * ux + uy * uy + uz * uz);
+    double ig = dt / g;
+    p->ux = ux; p->uy = uy; p->uz = uz;
+    p->x += ux * ig;
+    p->y += uy * ig;
+    p->z += uz * ig;
+    p->t += dt;
+}
+
+/* particle_bc, src/tracking.c:1172-1221 */
+void tpref_particle_bc(const tp_track_params *q, tp_particle *p, int *iescape)
+{
+    double xd = q->xmax - q->xmin, yd = q->ymax - q->ymin, zd = q->zmax - q->zmin;
+    if (q->bc_flag == 0) {
+        if (p->x > q->xmax) { p->x -= xd; p->xshift += xd; }
+        if (p->y > q->ymax) { p->y -= yd; p->yshift += yd; }
+        if (p->z > q->zmax) { p->z -= zd; p->zshift += zd; }
+        if (p->x < q->xmin) { p->x += xd; p->xshift -= xd; }
+        if (p->y < q->ymin) { p->y += yd; p->yshift -= yd; }
+        if (p->z < q->zmin) { p->z += zd; p->zshift -= zd; }
+    } else if (q->bc_flag == 1) {
+        if (p->x > q->xmax || p->y > q->ymax || p->z > q->zmax ||
+            p->x < q->xmin || p->y < q->ymin || p->z < q->zmin)
+            *iescape = 1;
+    } else if (q->bc_flag == 2) {
+        *iescape = 0;
+    }
+}
+
+/* ptl_energy_fixed, src/diagnostics.c:187-205.  The int conversion of
+ * floor() follows x86-64: out-of-range and NaN give INT_MIN, hence bin 0. */
+int tpref_energy_bin(const tp_particle *p, int nbins, double logemin, double logde)
+{
+    double gama = sqrt(1.0 + p->ux * p->ux + p->uy * p->uy + p->uz * p->uz);
+    double ene = gama - 1.0;
+    double v = floor((log10(ene) - logemin) / logde);
+    if (!(v > -2147483649.0 && v < 2147483648.0)) return 0;
+    int ibin = (int)v;
+    if (ibin < 0) ibin = 0;
+    else if (ibin > nbins - 1) ibin = nbins - 1;
+    return ibin;
+}
+
+/* calc_diff_coeff, src/diagnostics.c:659-705 */
+void tpref_diff_coeff(const tpref_field *f, const tp_particle *p0,
+                      const tp_particle *p, double dt, double o[8])
+{
+    double t = p->t * 2 + dt;
+    double a;
+    a = p0->x - (p->x + p->xshift); double dxx = a * a / t;
+    a = p0->y - (p->y + p->yshift); double dyy = a * a / t;
+    a = p0->z - (p->z + p->zshift); double dzz = a * a / t;
+    double drr = dxx + dyy + dzz;
+    double ax = p0->ux - p->ux, ay = p0->uy - p->uy, az = p0->uz - p->uz;
+    double dpp = (ax * ax + ay * ay + az * az) / t;
+    double g = sqrt(1.0 + p->ux * p->ux + p->uy * p->uy + p->uz * p->uz);
+    double g0 = sqrt(1.0 + p0->ux * p0->ux + p0->uy * p0->uy + p0->uz * p0->uz);
+    a = g - g0; double dee = a * a / t;
+    emf_t e, e0;
+    get_emf(f, p->x, p->y, p->z, p->t, &e);
+    get_emf(f, p0->x, p0->y, p0->z, p0->t, &e0);
+    double ab = sqrt(e.Bx * e.Bx + e.By * e.By + e.Bz * e.Bz);
+    double au = sqrt(p->ux * p->ux + p->uy * p->uy + p->uz * p->uz);
+    double mu = (e.Bx * p->ux + e.By * p->uy + e.Bz * p->uz) / (ab * au);
+    ab = sqrt(e0.Bx * e0.Bx + e0.By * e0.By + e0.Bz * e0.Bz);
+    au = sqrt(p0->ux * p0->ux + p0->uy * p0->uy + p0->uz * p0->uz);
+    double mu0 = (e0.Bx * p0->ux + e0.By * p0->uy + e0.Bz * p0->uz) / (ab * au);
+    a = mu - mu0; double duu = a * a / t;
+    double th = acos(mu) * 180 / M_PI;
+    double th0 = acos(mu0) * 180 / M_PI;
+    a = th - th0; double daa = a * a / t;
+    o[0] = drr; o[1] = dxx; o[2] = dyy; o[3] = dzz;
+    o[4] = dpp; o[5] = duu; o[6] = daa; o[7] = dee;
+}
+
+/* ---------------------------------------------------------------------- */
+/* fixed-step loop, src/tracking.c:90-190 and :297-462                     */
+/* ---------------------------------------------------------------------- */
+int tpref_track_fixed(const tpref_field *f, const tp_track_params *q,
+                      int n, tp_particle *ptl, const tp_particle *ptl_init,
+                      int *nsteps, double *espect, double *espect_escape,
+                      double *dcoeff, int *nptl_remain, int nthreads,
+                      double *seconds)
+{
+    if (n < 0 || q->einterval_t <= 0 || q->dcoeff_interval <= 0 ||
+        q->estep < q->sstep || q->nbins < 1 || q->nt_out < 1)
+        return TP_E_ARG;
+    const int nb = q->nbins, nt = q->nt_out, di = q->dcoeff_interval;
+    const int nd = q->estep / di + 1;
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    /* per-thread accumulators, merged in thread order at the end */
+    double *dsum = calloc((size_t)nthreads * 8 * nd, sizeof(double));
+    int *rem = calloc((size_t)nthreads * nd, sizeof(int));
+    double *esp = calloc((size_t)nthreads * nb * nt, sizeof(double));
+    double *ese = calloc((size_t)nthreads * nb * nt, sizeof(double));
+    if (!dsum || !rem || !esp || !ese) {
+        free(dsum); free(rem); free(esp); free(ese);
+        return TP_E_NOMEM;
+    }
+    double t1 = omp_get_wtime();
+#pragma omp parallel num_threads(nthreads)
+    {
+        int tid = omp_get_thread_num();
+        double *ds = dsum + (size_t)tid * 8 * nd;
+        int *rm = rem + (size_t)tid * nd;
+        double *sp = esp + (size_t)tid * nb * nt;
+        double *se = ese + (size_t)tid * nb * nt;
+        double dcarry[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma omp for schedule(static)
+        for (int i = 0; i < n; i++) {
+            tp_particle *p = &ptl[i];
+            int esc_pre = 0, esc_after;
+            if (nsteps) nsteps[i] = 1;
+            tpref_particle_bc(q, p, &esc_pre);
+            esc_after = esc_pre;
+            int ntp = (int)floor(q->sstep / (double)q->einterval_t);
+            int tstep = 1;
+            /* the reference's stale per-particle terms start undefined;
+             * here they start at zero */
+            double dv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            if (g_carry_stale) memcpy(dv, dcarry, sizeof(dv));
+            if (esc_pre && 1 < nd) rm[1]--;
+            for (int j = q->sstep; j <= q->estep; j++) {
+                dv[0] = 0.0; dv[4] = 0.0; dv[5] = 0.0;  /* drr dpp duu */
+                if (q->flags & TP_FLAG_FIX_STALE_DCOEFF)
+                    for (int k = 0; k < 8; k++) dv[k] = 0.0;
+                if (esc_pre == 0 || q->bc_flag == 0) {
+                    if (q->pusher == TP_PUSH_WIRZ) tpref_wirz_step(f, p, q->dt, q->qm);
+                    else tpref_boris_step(f, p, q->dt, q->qm);
+                    tpref_particle_bc(q, p, &esc_after);
+                    if (nsteps) nsteps[i] += 1;
+                    if (esc_after == 0 && j % di == 0)
+                        tpref_diff_coeff(f, &ptl_init[i], p, q->dt, dv);
+                }
+                if (esc_pre == 0 && esc_after == 1 && q->bc_flag == 1 && ntp < nt)
+                    se[ntp * nb + tpref_energy_bin(p, nb, q->logemin, q->logde)] += 1.0;
+                if (j % q->einterval_t == 0) {
+                    if (esc_after == 0 && ntp < nt)
+                        sp[ntp * nb + tpref_energy_bin(p, nb, q->logemin, q->logde)] += 1.0;
+                    ntp++;
+                }
+                if (j % di == 0) {
+                    if (tstep < nd)
+                        for (int k = 0; k < 8; k++) ds[k * nd + tstep] += dv[k];
+                    tstep++;
+                }
+                if (esc_after && j % di == 0 && tstep < nd) rm[tstep]--;
+                esc_pre = esc_after;
+            }
+            memcpy(dcarry, dv, sizeof(dv));
+        }
+    }
+    double t2 = omp_get_wtime();
+    if (seconds) *seconds = t2 - t1;
+    if (dcoeff) {
+        for (int k = 0; k < 8 * nd; k++) {
+            double s = 0.0;
+            for (int th = 0; th < nthreads; th++) s += dsum[(size_t)th * 8 * nd + k];
+            dcoeff[k] = s;
+        }
+    }
+    if (nptl_remain) {
+        for (int k = 0; k < nd; k++) {
+            int s = n;
+            for (int th = 0; th < nthreads; th++) s += rem[(size_t)th * nd + k];
+            nptl_remain[k] = s;
+        }
+    }
+    for (int k = 0; k < nb * nt; k++) {
+        double a = 0.0, b = 0.0;
+        for (int th = 0; th < nthreads; th++) {
+            a += esp[(size_t)th * nb * nt + k];
+            b += ese[(size_t)th * nb * nt + k];
+        }
+        if (espect) espect[k] = a;
+        if (espect_escape) espect_escape[k] = b;
+    }
+    free(dsum); free(rem); free(esp); free(ese);
+    return TP_OK;
+}

@@ -1,0 +1,83 @@
+/*
+ * tpref.h -- the project's FP64 CPU oracle (TEST INFRASTRUCTURE ONLY).
+ *
+ * A plain-C restatement of the reference tracer's hot path, used by tests/
+ * as the parity checker, by __graft_entry__.smoke() and by bench.py's
+ * cpu_baseline leg.  The product (test_particle_amd/, libtpgpu.so) never
+ * links, loads or calls it.  Each function names the reference lines it
+ * follows; see the pins listed in oracle/tpref.c.
+ */
+#ifndef TPREF_H
+#define TPREF_H
+
+#include "../include/tp_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TPREF_FIELD_TEST  0
+#define TPREF_FIELD_WLCS  1
+#define TPREF_FIELD_ABC   2
+#define TPREF_FIELD_GRID  4
+#define TPREF_FIELD_MODES 5
+
+typedef struct tpref_field {
+    int kind;
+    /* ABC: A, B, C, lambda, B0 */
+    double abc[5];
+    /* optional velocity grid for E = B x v (ABC) */
+    int vnx, vny, vnz;
+    double vlo[3], vhi[3];
+    const double *vgrid;
+    /* PIC grid */
+    int nx, ny, nz;
+    double lo[3], hi[3];
+    const float *egrid, *bgrid;
+    int gather_mode;
+    /* multi-mode */
+    int nmodes;
+    const double *modes;
+    double b0z;
+    /* WLCS */
+    int nwlcs;
+    const double *wlcs_rows;
+    int ibessel;
+    double omega, norm_i;
+    int wlcs_mode;          /* TP_GATHER_REF_COMPAT: the reference's ellint */
+} tpref_field;
+
+/* 1: take sin/cos from libm (as the reference does); 0: tp_sincos, the
+ * routine the GPU kernels use. */
+void tpref_set_math(int use_libm);
+/* reproduce the reference's single-thread carry-over of stale D terms */
+void tpref_set_carry_stale(int on);
+
+void tpref_eval_field(const tpref_field *f, int n, const double *pts,
+                      double *out);
+void tpref_wirz_step(const tpref_field *f, tp_particle *p, double dt,
+                     double qm);
+void tpref_boris_step(const tpref_field *f, tp_particle *p, double dt,
+                      double qm);
+void tpref_particle_bc(const tp_track_params *prm, tp_particle *p,
+                       int *iescape);
+int  tpref_energy_bin(const tp_particle *p, int nbins, double logemin,
+                      double logde);
+void tpref_diff_coeff(const tpref_field *f, const tp_particle *p0,
+                      const tp_particle *p, double dt, double out[8]);
+/* Returns 0 or TP_E_ARG.  nthreads <= 0 uses the OpenMP default.
+ * *seconds (if given) receives the wall time of the step loop. */
+int  tpref_track_fixed(const tpref_field *f, const tp_track_params *prm,
+                       int n, tp_particle *ptl, const tp_particle *ptl_init,
+                       int *nsteps, double *espect, double *espect_escape,
+                       double *dcoeff, int *nptl_remain, int nthreads,
+                       double *seconds);
+/* exported for the tests of the shared routines */
+void tpref_sincos(double x, double *s, double *c);
+void tpref_ellint(double k, double *elle, double *ellk);
+void tpref_ellint_ref(double k, double *elle, double *ellk);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,237 @@
+"""ctypes wrapper of the CPU oracle libtpref.so -- TEST INFRASTRUCTURE ONLY.
+
+Imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+as the checker; never by the product package.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from test_particle_amd import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libtpref.so")
+REF_DIR = os.path.join(HERE, "_ref")
+
+FIELD_TEST, FIELD_WLCS, FIELD_ABC, FIELD_GRID, FIELD_MODES = 0, 1, 2, 4, 5
+
+
+class TprefField(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int),
+        ("abc", ctypes.c_double * 5),
+        ("vnx", ctypes.c_int), ("vny", ctypes.c_int), ("vnz", ctypes.c_int),
+        ("vlo", ctypes.c_double * 3), ("vhi", ctypes.c_double * 3),
+        ("vgrid", ctypes.POINTER(ctypes.c_double)),
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("lo", ctypes.c_double * 3), ("hi", ctypes.c_double * 3),
+        ("egrid", ctypes.POINTER(ctypes.c_float)),
+        ("bgrid", ctypes.POINTER(ctypes.c_float)),
+        ("gather_mode", ctypes.c_int),
+        ("nmodes", ctypes.c_int),
+        ("modes", ctypes.POINTER(ctypes.c_double)),
+        ("b0z", ctypes.c_double),
+        ("nwlcs", ctypes.c_int),
+        ("wlcs_rows", ctypes.POINTER(ctypes.c_double)),
+        ("ibessel", ctypes.c_int),
+        ("omega", ctypes.c_double),
+        ("norm_i", ctypes.c_double),
+        ("wlcs_mode", ctypes.c_int),
+    ]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        P = ctypes.POINTER
+        d, i = ctypes.c_double, ctypes.c_int
+        L.tpref_set_math.argtypes = [i]
+        L.tpref_set_carry_stale.argtypes = [i]
+        L.tpref_eval_field.argtypes = [P(TprefField), i, P(d), P(d)]
+        L.tpref_wirz_step.argtypes = [P(TprefField), P(d), d, d]
+        L.tpref_boris_step.argtypes = [P(TprefField), P(d), d, d]
+        L.tpref_diff_coeff.argtypes = [P(TprefField), P(d), P(d), d, P(d)]
+        L.tpref_energy_bin.argtypes = [P(d), i, d, d]
+        L.tpref_energy_bin.restype = i
+        L.tpref_track_fixed.argtypes = [P(TprefField), P(abi.TpTrackParams), i,
+                                        P(d), P(d), P(i), P(d), P(d), P(d),
+                                        P(i), i, P(d)]
+        L.tpref_track_fixed.restype = i
+        L.tpref_sincos.argtypes = [d, P(d), P(d)]
+        L.tpref_ellint.argtypes = [d, P(d), P(d)]
+        L.tpref_ellint_ref.argtypes = [d, P(d), P(d)]
+        _lib = L
+    return _lib
+
+
+class Field:
+    """Keeps the numpy buffers a TprefField points into alive."""
+
+    def __init__(self, kind):
+        self.s = TprefField()
+        self.s.kind = kind
+        self._keep = []
+
+    def _hold(self, a, dtype):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        self._keep.append(a)
+        return a
+
+    @classmethod
+    def test(cls):
+        return cls(FIELD_TEST)
+
+    @classmethod
+    def abc(cls, A=1.0, B=np.sqrt(2.0 / 3.0), C=np.sqrt(1.0 / 3.0), lam=1.0,
+            B0=1.0):
+        f = cls(FIELD_ABC)
+        for k, v in enumerate((A, B, C, lam, B0)):
+            f.s.abc[k] = v
+        return f
+
+    def with_velocity_grid(self, v, lo, hi):
+        """v: (nz, ny, nx, 3) float64 velocity nodes."""
+        v = self._hold(v, np.float64)
+        self.s.vnz, self.s.vny, self.s.vnx = v.shape[:3]
+        for k in range(3):
+            self.s.vlo[k] = lo[k]
+            self.s.vhi[k] = hi[k]
+        self.s.vgrid = abi.dptr(v)
+        return self
+
+    @classmethod
+    def grid(cls, e, b, lo, hi, mode=abi.TP_GATHER_REF_COMPAT):
+        """e, b: (nz, ny, nx, 3) float32 nodes (x fastest)."""
+        f = cls(FIELD_GRID)
+        e = f._hold(e, np.float32)
+        b = f._hold(b, np.float32)
+        f.s.nz, f.s.ny, f.s.nx = e.shape[:3]
+        for k in range(3):
+            f.s.lo[k] = lo[k]
+            f.s.hi[k] = hi[k]
+        f.s.egrid = abi.fptr(e)
+        f.s.bgrid = abi.fptr(b)
+        f.s.gather_mode = mode
+        return f
+
+    @classmethod
+    def modes(cls, rows, b0z):
+        f = cls(FIELD_MODES)
+        rows = f._hold(rows, np.float64)
+        f.s.nmodes = rows.shape[0]
+        f.s.modes = abi.dptr(rows)
+        f.s.b0z = b0z
+        return f
+
+    @classmethod
+    def wlcs(cls, rows, ibessel=0, omega=0.001, norm_i=1.0,
+             mode=abi.TP_GATHER_REF_COMPAT):
+        f = cls(FIELD_WLCS)
+        rows = f._hold(rows, np.float64)
+        f.s.nwlcs = rows.shape[0]
+        f.s.wlcs_rows = abi.dptr(rows)
+        f.s.ibessel = ibessel
+        f.s.omega = omega
+        f.s.norm_i = norm_i
+        f.s.wlcs_mode = mode
+        return f
+
+
+def set_carry_stale(on):
+    lib().tpref_set_carry_stale(1 if on else 0)
+
+
+def set_math(use_libm):
+    lib().tpref_set_math(1 if use_libm else 0)
+
+
+def eval_field(field, pts):
+    pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 4)
+    out = np.zeros((pts.shape[0], 6))
+    lib().tpref_eval_field(ctypes.byref(field.s), pts.shape[0], abi.dptr(pts),
+                           abi.dptr(out))
+    return out
+
+
+def steps(field, ptl, nsteps, dt, qm=1.0, pusher=abi.TP_PUSH_WIRZ,
+          params=None):
+    """nsteps pushes (+ particle_bc when params is given) per particle."""
+    ptl = abi.as_particles(ptl).copy()
+    L = lib()
+    fn = L.tpref_wirz_step if pusher == abi.TP_PUSH_WIRZ else L.tpref_boris_step
+    row = (ctypes.c_double * abi.NFIELD)
+    for i in range(ptl.shape[0]):
+        buf = row(*ptl[i])
+        for _ in range(nsteps):
+            fn(ctypes.byref(field.s), buf, dt, qm)
+            if params is not None:
+                esc = ctypes.c_int(0)
+                L.tpref_particle_bc(ctypes.byref(params), buf, ctypes.byref(esc))
+        ptl[i] = np.frombuffer(buf, dtype=np.float64)
+    return ptl
+
+
+def diff_coeff(field, p0, p, dt):
+    p0 = abi.as_particles(p0)
+    p = abi.as_particles(p)
+    out = np.zeros((p.shape[0], 8))
+    L = lib()
+    for i in range(p.shape[0]):
+        L.tpref_diff_coeff(ctypes.byref(field.s), abi.dptr(p0[i:i + 1]),
+                           abi.dptr(p[i:i + 1]), dt, abi.dptr(out[i:i + 1]))
+    return out
+
+
+def track_fixed(field, params, ptl, ptl_init=None, nthreads=1):
+    """Returns dict(ptl, nsteps, espect, espect_escape, dcoeff, remain,
+    seconds) -- the raw sums of the reference's fixed-step loop."""
+    ptl = abi.as_particles(ptl).copy()
+    ptl_init = ptl.copy() if ptl_init is None else abi.as_particles(ptl_init)
+    n = ptl.shape[0]
+    nd = abi.n_dcoeff_rows(params)
+    nsteps = np.zeros(n, dtype=np.int32)
+    esp = np.zeros((params.nt_out, params.nbins))
+    ese = np.zeros((params.nt_out, params.nbins))
+    dco = np.zeros((8, nd))
+    rem = np.zeros(nd, dtype=np.int32)
+    secs = ctypes.c_double(0.0)
+    rc = lib().tpref_track_fixed(ctypes.byref(field.s), ctypes.byref(params), n,
+                                 abi.dptr(ptl), abi.dptr(ptl_init),
+                                 abi.iptr(nsteps), abi.dptr(esp), abi.dptr(ese),
+                                 abi.dptr(dco), abi.iptr(rem), nthreads,
+                                 ctypes.byref(secs))
+    if rc != 0:
+        raise RuntimeError(f"tpref_track_fixed failed: {rc}")
+    return dict(ptl=ptl, nsteps=nsteps, espect=esp, espect_escape=ese,
+                dcoeff=dco, remain=rem, seconds=secs.value)
+
+
+def sincos(x):
+    L = lib()
+    x = np.asarray(x, dtype=np.float64).ravel()
+    s = np.empty_like(x)
+    c = np.empty_like(x)
+    a, b = ctypes.c_double(), ctypes.c_double()
+    for i, v in enumerate(x):
+        L.tpref_sincos(float(v), ctypes.byref(a), ctypes.byref(b))
+        s[i], c[i] = a.value, b.value
+    return s, c
+
+
+def ellint(k, ref_compat=False):
+    a, b = ctypes.c_double(), ctypes.c_double()
+    fn = lib().tpref_ellint_ref if ref_compat else lib().tpref_ellint
+    fn(float(k), ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value   # (E, K)

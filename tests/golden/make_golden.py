@@ -1,0 +1,215 @@
+"""Write the golden vectors of tests/golden/ from the reference tracer.
+
+Builds oracle/_ref (the reference's own hot-path sources compiled from
+/root/reference/src by oracle/Makefile, linked to oracle/ref_harness.c) and
+runs it on seeded inputs.  Every fixture is an .npz of inputs and the
+reference's outputs; nothing of the reference's source is stored.
+
+    python tests/golden/make_golden.py      # in the build container only
+"""
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.abspath(os.path.join(HERE, "..", ".."))
+sys.path.insert(0, ROOT)
+
+from test_particle_amd.particles import init_particles  # noqa: E402
+
+REF = os.path.join(ROOT, "oracle", "_ref")
+TWO_PI = 2.0 * np.pi
+
+
+def build_ref():
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"],
+                   check=True)
+
+
+def write_spec(path, **kw):
+    with open(path, "w") as f:
+        for k, v in kw.items():
+            if isinstance(v, (float, np.floating)):
+                f.write(f"{k} {float(v)!r}\n")
+            else:
+                f.write(f"{k} {v}\n")
+
+
+def run(binary, *args, cwd):
+    subprocess.run([os.path.join(REF, binary), *map(str, args)], cwd=cwd,
+                   check=True, stdout=subprocess.DEVNULL)
+
+
+def box(lo, hi):
+    return dict(xmin=float(lo[0]), ymin=float(lo[1]), zmin=float(lo[2]),
+                xmax=float(hi[0]), ymax=float(hi[1]), zmax=float(hi[2]))
+
+
+def fields_case(name, binary, spec, pts, extra=None, tmp=None):
+    sp = os.path.join(tmp, "spec.txt")
+    write_spec(sp, **spec)
+    pts.astype(np.float64).tofile(os.path.join(tmp, "pts.bin"))
+    run(binary, "fields", sp, "pts.bin", "out.bin", cwd=tmp)
+    out = np.fromfile(os.path.join(tmp, "out.bin")).reshape(-1, 6)
+    np.savez(os.path.join(HERE, name + ".npz"), pts=pts, out=out,
+             **(extra or {}))
+
+
+def wirz_case(name, binary, spec, ptl, ks, extra=None, tmp=None):
+    sp = os.path.join(tmp, "spec.txt")
+    write_spec(sp, **spec)
+    ptl.tofile(os.path.join(tmp, "ptl.bin"))
+    res = {}
+    for k in ks:
+        run(binary, "wirz", sp, "ptl.bin", k, "out.bin", cwd=tmp)
+        res[f"out_{k}"] = np.fromfile(os.path.join(tmp, "out.bin")).reshape(-1, 11)
+    np.savez(os.path.join(HERE, name + ".npz"), ptl=ptl, ks=np.array(ks),
+             **res, **(extra or {}))
+
+
+def track_case(name, binary, spec, ptl, extra=None, tmp=None):
+    sp = os.path.join(tmp, "spec.txt")
+    write_spec(sp, **spec)
+    ptl.tofile(os.path.join(tmp, "ptl.bin"))
+    od = os.path.join(tmp, "o")
+    os.makedirs(od, exist_ok=True)
+    run(binary, "track", sp, "ptl.bin", od, cwd=tmp)
+    rd = lambda f, dt=np.float64: np.fromfile(os.path.join(od, f), dtype=dt)
+    nb, nt = spec["nbins"], spec["nt_out"]
+    np.savez(os.path.join(HERE, name + ".npz"), ptl_in=ptl,
+             ptl_out=rd("ptl.bin").reshape(-1, 11),
+             nsteps=rd("nsteps.bin", np.int32),
+             espect=rd("espect.bin").reshape(nt, nb),
+             espect_escape=rd("espect_escape.bin").reshape(nt, nb),
+             dcoeff=rd("dcoeff.bin").reshape(8, -1),
+             remain=rd("remain.bin", np.int32),
+             spec=np.array(json.dumps({k: float(v) if isinstance(v, np.floating) else v
+                                       for k, v in spec.items()})),
+             **(extra or {}))
+
+
+def main():
+    build_ref()
+    rng = np.random.default_rng(7)
+    lo2pi, hi2pi = (0.0, 0.0, 0.0), (TWO_PI, TWO_PI, TWO_PI)
+    with tempfile.TemporaryDirectory() as tmp:
+        os.makedirs(os.path.join(tmp, "data"))
+        # --- ABC field, reference and unit coefficients ------------------
+        pts = np.concatenate([rng.uniform(-1.0, 8.0, (200, 4)),
+                              np.zeros((1, 4))])
+        base = dict(system=2, nx=5, ny=5, nz=5, tmax=1.0, dt=0.01, **box(lo2pi, hi2pi))
+        fields_case("fields_abc_ref", "ref_harness", base, pts, tmp=tmp)
+        fields_case("fields_abc1", "ref_harness_abc1",
+                    dict(base, ff_b0=1.3, ff_lambda=0.7), pts,
+                    extra=dict(b0=1.3, lam=0.7), tmp=tmp)
+        # --- ABC + E = B x v from a random double velocity grid -----------
+        vlo, vhi = (-0.5, 0.25, 1.0), (3.5, 2.25, 4.0)
+        vg = rng.normal(0.0, 0.1, (4, 6, 7, 3))      # (nz, ny, nx, 3)
+        vg.tofile(os.path.join(tmp, "vgrid.bin"))
+        vpts = np.column_stack([rng.uniform(-0.8, 3.8, 300), rng.uniform(0.0, 2.5, 300),
+                                rng.uniform(0.8, 4.2, 300), np.zeros(300)])
+        fields_case("fields_abc_vgrid", "ref_harness",
+                    dict(system=2, nx=7, ny=6, nz=4, tmax=1.0, dt=0.01,
+                         vgrid=os.path.join(tmp, "vgrid.bin"), **box(vlo, vhi)),
+                    vpts, extra=dict(vgrid=vg, lo=vlo, hi=vhi), tmp=tmp)
+        # --- PIC float grid (system 4), x/z weight exchange and wraps ------
+        glo, ghi = (0.0, -1.0, 2.0), (5.0, 3.0, 5.0)
+        nz, ny, nx = 4, 5, 6
+        eg = rng.normal(0, 1, (nz, ny, nx, 3)).astype(np.float32)
+        bg = rng.normal(0, 1, (nz, ny, nx, 3)).astype(np.float32)
+        np.concatenate([eg.ravel(), bg.ravel()]).tofile(os.path.join(tmp, "grid.bin"))
+        gp = np.column_stack([rng.uniform(-0.3, 5.3, 400), rng.uniform(-1.2, 3.2, 400),
+                              rng.uniform(1.8, 5.2, 400), np.zeros(400)])
+        nodes = np.array([[1.0, 0.0, 3.0, 0.0], [0.0, -1.0, 2.0, 0.0],
+                          [2.5, 1.5, 3.5, 0.0]])
+        gp = np.concatenate([gp, nodes])
+        gspec = dict(system=4, nx=nx, ny=ny, nz=nz, tmax=1.0, dt=0.01,
+                     grid=os.path.join(tmp, "grid.bin"), **box(glo, ghi))
+        fields_case("fields_grid", "ref_harness", gspec, gp,
+                    extra=dict(egrid=eg, bgrid=bg, lo=glo, hi=ghi), tmp=tmp)
+        # --- WLCS (static wire and Bessel wire) ---------------------------
+        rows = np.array([
+            [1.0, 0.5, 0.5, 0.5, 0.0, 0.0, 0.0, 0.0, 1.0, 0.5, 0.5, 0.5, 0.0, 0.0],
+            [0.7, 0.2, 0.9, 0.4, 0.3, 1.1, 10.0, 0.6, 0.3, 0.8, 0.1, 0.6, 0.5, 2.0],
+            [-0.4, 0.8, 0.1, 0.7, 2.0, 4.0, 5.0, 1.2, 0.7, 0.3, 0.6, 0.2, 1.3, 0.4]])
+        rows.tofile(os.path.join(tmp, "wlcs.bin"))
+        wp = np.column_stack([rng.uniform(-0.5, 1.5, (300, 3)), rng.uniform(0, 50, 300)])
+        for ib in (0, 1):
+            fields_case(f"fields_wlcs_b{ib}", "ref_harness",
+                        dict(system=1, nwlcs=3, ibessel=ib, wlcs_omega=0.001,
+                             wlcs_normi=0.125, wlcs=os.path.join(tmp, "wlcs.bin"),
+                             tmax=1.0, dt=0.01, **box((-1, -1, -1), (2, 2, 2))),
+                        wp, extra=dict(rows=rows, ibessel=ib, omega=0.001,
+                                       norm_i=0.125), tmp=tmp)
+        # --- Wirz trajectories ---------------------------------------------
+        ptl = init_particles(64, lo2pi, hi2pi, 0.7, seed=11)
+        wirz_case("wirz_test", "ref_harness",
+                  dict(system=0, tmax=1.0, dt=0.05, bc=0, **box(lo2pi, hi2pi)),
+                  ptl, [1, 1000], extra=dict(dt=0.05), tmp=tmp)
+        dt_c1 = 0.01 * TWO_PI * (1.0 / np.sqrt(1.0 - 0.49)) / np.sqrt(3.0)
+        wirz_case("wirz_abc1", "ref_harness_abc1",
+                  dict(system=2, nx=5, ny=5, nz=5, tmax=1.0, dt=dt_c1, bc=0,
+                       **box(lo2pi, hi2pi)),
+                  ptl, [1, 10, 100, 1000], extra=dict(dt=dt_c1), tmp=tmp)
+        gptl = init_particles(32, glo, ghi, 0.3, seed=12)
+        wirz_case("wirz_grid", "ref_harness", dict(gspec, bc=0), gptl, [1, 50, 200],
+                  extra=dict(dt=0.01, egrid=eg, bgrid=bg, lo=glo, hi=ghi), tmp=tmp)
+        # --- calc_diff_coeff ------------------------------------------------
+        p0 = init_particles(64, lo2pi, hi2pi, 0.7, seed=13)
+        p1 = p0.copy()
+        p1[:, :3] += rng.normal(0, 0.5, (64, 3))
+        p1[:, 4:7] += rng.normal(0, 0.1, (64, 3))
+        p1[:, 7] = 3.7
+        p1[:, 8:11] = rng.choice([-TWO_PI, 0.0, TWO_PI], (64, 3))
+        sp = os.path.join(tmp, "spec.txt")
+        write_spec(sp, system=2, nx=5, ny=5, nz=5, tmax=1.0, dt=0.01, **box(lo2pi, hi2pi))
+        p0.tofile(os.path.join(tmp, "p0.bin"))
+        p1.tofile(os.path.join(tmp, "p1.bin"))
+        run("ref_harness_abc1", "diff", sp, "p0.bin", "p1.bin", 0.05, "d.bin", cwd=tmp)
+        np.savez(os.path.join(HERE, "diff_abc1.npz"), p0=p0, p1=p1, dt=0.05,
+                 out=np.fromfile(os.path.join(tmp, "d.bin")).reshape(-1, 8))
+        # --- full fixed-step loop ------------------------------------------
+        tptl = init_particles(256, lo2pi, hi2pi, 0.7, seed=14)
+        track_case("track_abc1", "ref_harness_abc1",
+                   dict(system=2, nx=5, ny=5, nz=5, tmax=1000 * dt_c1 + 1e-9,
+                        dt=dt_c1, bc=0, nbins=102, nt_out=10, emin=1e-3, emax=10.0,
+                        **box(lo2pi, hi2pi)), tptl, tmp=tmp)
+        # open boundaries in the uniform test field: particles stream out
+        olo, ohi = (0.0, 0.0, 0.0), (4.0, 4.0, 4.0)
+        optl = init_particles(256, olo, ohi, 0.5, seed=15)
+        optl[-8:, 2] = 4.5         # a few start outside the box
+        optl[0, :3] = 2.0          # particle 0 stays inside until its first
+        optl[0, 6] = 0.0           # D sample (the reference reads stale terms)
+        track_case("track_open", "ref_harness",
+                   dict(system=0, tmax=600 * 0.05 + 1e-9, dt=0.05, bc=1,
+                        nbins=30, nt_out=6, emin=0.01, emax=1.0,
+                        **box(olo, ohi)), optl, tmp=tmp)
+        # PIC grid loop
+        track_case("track_grid", "ref_harness",
+                   dict(gspec, tmax=300 * 0.01 + 1e-9, bc=0, nbins=50, nt_out=3,
+                        emin=1e-3, emax=1.0), gptl,
+                   extra=dict(egrid=eg, bgrid=bg, lo=glo, hi=ghi, dt=0.01), tmp=tmp)
+        # --- the reference's own output files (format) ---------------------
+        sp = os.path.join(tmp, "spec.txt")
+        write_spec(sp, system=2, nx=5, ny=5, nz=5, tmax=300 * dt_c1 + 1e-9,
+                   dt=dt_c1, bc=0, nbins=102, nt_out=3, emin=1e-3, emax=10.0,
+                   **box(lo2pi, hi2pi))
+        hptl = init_particles(64, lo2pi, hi2pi, 0.7, seed=16)
+        hptl.tofile(os.path.join(tmp, "h.bin"))
+        run("ref_harness_abc1", "hybrid", sp, "h.bin", cwd=tmp)
+        with open(os.path.join(tmp, "data", "espectrum.dat")) as f:
+            espectrum_dat = f.read()
+        with open(os.path.join(tmp, "data", "diff_coeffs.dat")) as f:
+            diff_dat = f.read()
+        np.savez(os.path.join(HERE, "hybrid_files_abc1.npz"), ptl=hptl,
+                 dt=dt_c1, espectrum_dat=np.array(espectrum_dat),
+                 diff_coeffs_dat=np.array(diff_dat))
+    print("golden fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()
