@@ -1,0 +1,204 @@
+"""Benchmark of the tracking hot path on one MI355X.
+
+Workload (the PIC-grid configuration): a 256^3 periodic turbulent E/B grid
+(stored on 257^3 endpoint-inclusive nodes, FP32, 6 components), B = z-hat +
+dB with a k^-5/3 spectrum and rms |dB| = 1, E = -u x B with rms |u| = 0.05c;
+1,048,576 protons (q/m = 1, c = 1) at |v| = 0.7c, isotropic; FP64 Boris push
+with the reference's trilinear gather; dt = 0.01 gyroperiods (T_g = 2 pi
+gamma0 / B0); spectrum in 102 bins and diffusion coefficients every 100
+steps, as the reference's loop computes them.
+
+A "step" is one time step of every particle.  Inputs are resident in HBM
+before timing starts; the library call is synchronous, so the wall clock
+around it brackets exactly the K timed steps.
+
+    python bench.py [--gpus 1] [--steps 5000] [--warmup 200]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-steps/s on 1 MI355X, chosen vs plain form, at D_xx err <=1%"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_PSTEP = 192.0      # 8 corners x 6 FP32 components per field evaluation
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def host_cores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit():
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--particles", type=int, default=1 << 20)
+    ap.add_argument("--grid", type=int, default=256)
+    ap.add_argument("--seed", type=int, default=20261202)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 form")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "particles shard without communication, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    if int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: one process per run; WORLD_SIZE > 1 is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    from test_particle_amd import abi
+    from test_particle_amd.fields import gyroperiod, pic_turbulence_grid
+    from test_particle_amd.gpu import Tracker
+    from test_particle_amd.particles import init_particles
+
+    t0 = time.perf_counter()
+    log(f"[bench] building {args.grid}^3 turbulence grid (seed {args.seed})")
+    e, b, lo, hi = pic_turbulence_grid(args.grid, seed=args.seed)
+    log(f"[bench] grid ready in {time.perf_counter() - t0:.1f}s")
+    n = args.particles
+    vthe = 0.7
+    dt = 0.01 * gyroperiod(vthe, 1.0)
+    ptl = init_particles(n, lo, hi, vthe, seed=args.seed)
+
+    def params(steps, precision=abi.TP_FP64):
+        return abi.make_params(dt=dt, estep=steps, lo=lo, hi=hi, pusher=abi.TP_PUSH_BORIS,
+                               precision=precision, nbins=102,
+                               nt_out=max(1, min(100, steps // 100)), emin=1e-4, emax=1.0)
+
+    tr = Tracker(0)
+    gpu_name, ncu = tr.device_info()
+    log(f"[bench] {gpu_name}, {ncu} CUs")
+    tr.set_field_grid(e, b, lo, hi, abi.TP_GATHER_REF_COMPAT)
+    tr.load(ptl)
+    if args.warmup > 0:
+        log(f"[bench] warmup {args.warmup} steps")
+        tr.run(params(args.warmup))
+    K = args.steps
+    prm = params(K)
+    log(f"[bench] timed {K} steps x {n} particles (FP64 Boris, reference gather)")
+    t1 = time.perf_counter()
+    tr.run(prm)                       # synchronous: returns after the last kernel
+    wall = time.perf_counter() - t1
+    kms, launches = tr.last_kernel_time()
+    value = n * K / wall
+    diag = tr.diagnostics(prm)
+    dxx = diag["dcoeff"][1, 1:] / np.maximum(diag["remain"][1:], 1)
+    log(f"[bench] {value:.4e} particle-steps/s ({wall:.3f}s wall, kernels {kms:.1f} ms "
+        f"in {launches} launches)")
+
+    achieved = BYTES_PER_PSTEP * n * K / (kms * 1e-3) / 1e9 if kms > 0 else None
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "k_track_grid_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
+                "kernel": "k_track<BORIS, GRID_REF, FP64>",
+                "kernel_ms_per_launch": kms / max(launches, 1),
+                "steps_per_launch": K / max(launches, 1)}
+
+    forms = {"plain_fp64": {"value": value, "dtype": "f64"}}
+    if not args.no_fp32:
+        tr.load(ptl)
+        prm32 = params(K, abi.TP_FP32_MIXED)
+        t2 = time.perf_counter()
+        tr.run(prm32)
+        wall32 = time.perf_counter() - t2
+        d32 = tr.diagnostics(prm32)
+        dxx32 = d32["dcoeff"][1, 1:] / np.maximum(d32["remain"][1:], 1)
+        tr.load(ptl)
+        tr.run(prm)
+        d64 = tr.diagnostics(prm)
+        dxx64 = d64["dcoeff"][1, 1:] / np.maximum(d64["remain"][1:], 1)
+        err = float(np.max(np.abs(dxx32 / dxx64 - 1.0)))
+        forms["fp32_mixed"] = {"value": n * K / wall32, "dtype": "f32+f64 accumulators",
+                               "dxx_max_rel_err_vs_fp64": err}
+        log(f"[bench] FP32 form {n * K / wall32:.4e} particle-steps/s, D_xx err {err:.2e}")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(e, b, lo, hi, ptl, params, tr)
+    tr.close()
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": K, "warmup": args.warmup, "ms_per_step": wall * 1e3 / K,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "PIC-grid fields: 256^3 periodic turbulent FP32 E,B "
+                               "(257^3 endpoint-inclusive nodes), 1M protons |v|=0.7c, "
+                               "FP64 Boris, reference trilinear gather, dt=0.01 T_g",
+                   "particles": n, "grid_nodes": [args.grid + 1] * 3, "pusher": "boris",
+                   "gather": "ref_compat", "seed": args.seed, "gpu": gpu_name,
+                   "dxx_last": float(dxx[-1]) if dxx.size else None},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "forms": forms,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(e, b, lo, hi, ptl, params, tr):
+    """The CPU oracle (FP64, OpenMP) on this host's cores, on a bounded sample
+    of the same workload: the first particles of the same set for a number of
+    steps chosen from a short pilot so the timed sample is about 10-20 s.  The
+    GPU runs the same sample, and the two are compared."""
+    from oracle import tpref
+    from test_particle_amd import abi
+    cores = host_cores()
+    tpref.set_math(False)
+    f = tpref.Field.grid(e, b, lo, hi, abi.TP_GATHER_REF_COMPAT)
+    n_s = min(ptl.shape[0], 65536)
+    sample = ptl[:n_s]
+    pilot = tpref.track_fixed(f, params(20), sample[:8192], nthreads=cores)
+    rate = 8192 * 20 / max(pilot["seconds"], 1e-6)
+    steps = int(np.clip(12.0 * rate / n_s, 100, 2000)) // 100 * 100
+    log(f"[bench] CPU baseline: {n_s} particles x {steps} steps on {cores} threads "
+        f"(pilot {rate:.3e}/s)")
+    ref = tpref.track_fixed(f, params(steps), sample, nthreads=cores)
+    cpu_rate = n_s * steps / ref["seconds"]
+    g = tr.track_fixed(params(steps), sample)
+    same = bool(np.array_equal(g["ptl"], ref["ptl"]))
+    dx_ref = ref["dcoeff"][1, 1:]
+    dx_gpu = g["dcoeff"][1, 1:]
+    derr = float(np.max(np.abs(dx_gpu - dx_ref) / np.abs(dx_ref)))
+    log(f"[bench] CPU {cpu_rate:.3e} particle-steps/s; GPU sample bitwise={same}, "
+        f"D_xx rel err {derr:.1e}")
+    return {"value": cpu_rate, "unit": "particle-steps/s", "cores": cores, "kind": "port",
+            "sample": f"first {n_s} of the same particles, {steps} steps, same grid and "
+                      f"params, FP64 oracle with OpenMP ({cpu_model()})",
+            "gpu_sample_bitwise_equal": same, "gpu_sample_dxx_rel_err": derr}
+
+
+if __name__ == "__main__":
+    main()

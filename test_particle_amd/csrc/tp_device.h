@@ -1,0 +1,612 @@
+/*
+ * tp_device.h -- device-side field sources and particle pushers (gfx950).
+ *
+ * FP64 paths perform the operations of the reference's routines in the
+ * reference's order (the library is compiled with -ffp-contract=off, and
+ * sin/cos come from tp_math.h), so that results are bitwise those of the
+ * CPU oracle.  FP32 paths (TP_FP32_MIXED) evaluate fields and push in
+ * float and are compared against FP64 by tolerance.
+ */
+#ifndef TP_DEVICE_H
+#define TP_DEVICE_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "tp_math.h"
+
+#define TP_PI 3.14159265358979323846
+
+/* field kinds, one kernel instantiation each */
+enum {
+    FK_TEST = 0,      /* B = z-hat */
+    FK_ABC = 1,       /* ABC, E = 0 */
+    FK_ABCV = 2,      /* ABC, E = B x v(grid) */
+    FK_GRID_REF = 3,  /* PIC grid, reference gather */
+    FK_GRID_TB = 4,   /* PIC grid, textbook gather */
+    FK_MODES = 5,     /* multi-mode turbulence */
+    FK_WLCS = 6,      /* wire-loop current systems */
+};
+
+/* One WLCS system with the trigonometric values read_wlcs precomputes
+ * (src/wlcs.c:166-173) and currents scaled by normI (:209-212). */
+struct WlcsSys {
+    double cur_wr, x_wr, y_wr, z_wr, ct, st, cp, sp, t0;
+    double cur_lp, r_lp, x_lp, y_lp, z_lp, ca, sa, cb, sb, omega;
+};
+
+struct FieldDev {
+    int kind;
+    /* ABC */
+    double A, B, C, lam, B0;
+    /* velocity grid (E = B x v) */
+    int vnx, vny, vnz;
+    double vlo[3], vhi[3], vd[3];
+    const double *vgrid;            /* AoS double3 */
+    /* PIC grid, nodes of 8 floats: ex ey ez bx | by bz 0 0 */
+    int nx, ny, nz;
+    double lo[3], hi[3], d[3];
+    const float4 *grid;
+    /* modes: rows kx ky kz ax ay az phi */
+    int nmodes;
+    const double *modes;
+    double b0z;
+    /* WLCS */
+    int nwlcs, ibessel, wlmode;
+    const WlcsSys *wl;
+};
+
+struct Emf {
+    double bx, by, bz, ex, ey, ez;
+};
+
+struct EmfF {
+    float bx, by, bz, ex, ey, ez;
+};
+
+/* ------------------------------------------------------------------ */
+/* trilinear parameters                                                */
+/* ------------------------------------------------------------------ */
+
+__device__ __forceinline__ int tp_floor_to_int(double v)
+{
+    /* (int) of an out-of-range value is INT_MIN on the reference's x86-64 */
+    return (v > -2147483649.0 && v < 2147483648.0) ? (int)v : (int)0x80000000;
+}
+
+/* grid_indices + get_trilinar_parameters (src/interpolation.c:36-181):
+ * corner order and x/z-exchanged weights of the reference; flat corner
+ * index clamped into the array. */
+__device__ __forceinline__ void trilinear_ref(int nx, int ny, int nz,
+        const double *lo, const double *hi, const double *dd,
+        double x, double y, double z, long *c, double *w)
+{
+    double qx = (x - lo[0]) / dd[0];
+    double qy = (y - lo[1]) / dd[1];
+    double qz = (z - lo[2]) / dd[2];
+    int ix1 = tp_floor_to_int(floor(qx));
+    int iy1 = tp_floor_to_int(floor(qy));
+    int iz1 = tp_floor_to_int(floor(qz));
+    int ix2 = ix1 + 1, iy2 = iy1 + 1, iz2 = iz1 + 1;
+    double fx = qx - ix1, fy = qy - iy1, fz = qz - iz1;
+    if (x > hi[0]) { ix1 = nx - 1; ix2 = 0; fx = x / dd[0] - (nx - 1); }
+    if (y > hi[1]) { iy1 = ny - 1; iy2 = 0; fy = y / dd[1] - (ny - 1); }
+    if (z > hi[2]) { iz1 = nz - 1; iz2 = 0; fz = z / dd[2] - (nz - 1); }
+    if (x < lo[0]) { ix1 = nx - 1; ix2 = 0; fx = x / dd[0] + 1.0; }
+    if (y < lo[1]) { iy1 = ny - 1; iy2 = 0; fy = y / dd[1] + 1.0; }
+    if (z < lo[2]) { iz1 = nz - 1; iz2 = 0; fz = z / dd[2] + 1.0; }
+    long total = (long)nx * ny * nz;
+    long sx = nx, sxy = (long)nx * ny;
+    long a[8];
+    a[0] = ix1 + sx * iy1 + sxy * iz1;
+    a[1] = ix1 + sx * iy1 + sxy * iz2;
+    a[2] = ix1 + sx * iy2 + sxy * iz1;
+    a[3] = ix2 + sx * iy1 + sxy * iz1;
+    a[4] = ix2 + sx * iy2 + sxy * iz1;
+    a[5] = ix2 + sx * iy1 + sxy * iz2;
+    a[6] = ix1 + sx * iy2 + sxy * iz2;
+    a[7] = ix2 + sx * iy2 + sxy * iz2;
+#pragma unroll
+    for (int k = 0; k < 8; k++) c[k] = a[k] < 0 ? 0 : (a[k] >= total ? total - 1 : a[k]);
+    double gx = 1.0 - fx, gy = 1.0 - fy, gz = 1.0 - fz;
+    w[0] = gx * gy * gz;
+    w[1] = fx * gy * gz;
+    w[2] = gx * fy * gz;
+    w[3] = gx * gy * fz;
+    w[4] = gx * fy * fz;
+    w[5] = fx * gy * fz;
+    w[6] = fx * fy * gz;
+    w[7] = fx * fy * fz;
+}
+
+/* textbook trilinear: fold by the period, corner ix+1 weighted by fx */
+__device__ __forceinline__ void trilinear_tb(int nx, int ny, int nz,
+        const double *lo, const double *hi, const double *dd,
+        double x, double y, double z, long *c, double *w)
+{
+    double p[3] = {x, y, z};
+    int n[3] = {nx, ny, nz};
+    int i1[3];
+    double f[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        double L = hi[a] - lo[a];
+        double q = p[a] - lo[a];
+        q = q - L * floor(q / L);
+        double v = floor(q / dd[a]);
+        int i = (v >= 0.0 && v <= n[a] - 2.0) ? (int)v : (v < 0.0 ? 0 : n[a] - 2);
+        if (!(v == v)) i = 0;
+        i1[a] = i;
+        f[a] = q / dd[a] - i;
+    }
+    long sx = nx, sxy = (long)nx * ny;
+    long b = i1[0] + sx * i1[1] + sxy * i1[2];
+    c[0] = b;
+    c[1] = b + sxy;
+    c[2] = b + sx;
+    c[3] = b + 1;
+    c[4] = b + 1 + sx;
+    c[5] = b + 1 + sxy;
+    c[6] = b + sx + sxy;
+    c[7] = b + 1 + sx + sxy;
+    double fx = f[0], fy = f[1], fz = f[2];
+    double gx = 1.0 - fx, gy = 1.0 - fy, gz = 1.0 - fz;
+    w[0] = gx * gy * gz;
+    w[1] = gx * gy * fz;
+    w[2] = gx * fy * gz;
+    w[3] = fx * gy * gz;
+    w[4] = fx * fy * gz;
+    w[5] = fx * gy * fz;
+    w[6] = gx * fy * fz;
+    w[7] = fx * fy * fz;
+}
+
+/* ------------------------------------------------------------------ */
+/* WLCS (src/wlcs.c:281-449, 814-903)                                  */
+/* ------------------------------------------------------------------ */
+#define TP_WL_L0 6.96E8
+#define TP_WL_C0 3.0E10
+#define TP_WL_I0 4.0E8
+
+/* complete elliptic integrals by the arithmetic-geometric mean */
+__device__ __forceinline__ void tp_ellint_agm(double k, double *E, double *K)
+{
+    double a = 1.0, b = sqrt(1.0 - k * k), c = k;
+    double sum = 0.5 * c * c, pw = 0.5;
+    for (int i = 0; i < 40; i++) {
+        double an = 0.5 * (a + b);
+        double bn = sqrt(a * b);
+        c = 0.5 * (a - b);
+        pw *= 2.0;
+        sum += pw * c * c;
+        a = an;
+        b = bn;
+        if (fabs(c) < 1e-17 * a) break;
+    }
+    double KK = TP_PI / (2.0 * a);
+    *K = KK;
+    *E = KK * (1.0 - sum);
+}
+
+/* the reference's ellint, whose m >= 0.9 branch takes K(m-0.9) for
+ * K(1-m) (src/wlcs.c:741-806) */
+__device__ __forceinline__ void tp_ellint_ref(double k, double *E, double *K)
+{
+    double m = k * k;
+    if (!(m >= 0.9)) {
+        tp_ellint_agm(k, E, K);
+        return;
+    }
+    double Ec, Kc, Em, Km, Ew, Kw;
+    tp_ellint_agm(sqrt(1.0 - m), &Ec, &Kc);
+    tp_ellint_agm(k, &Em, &Km);
+    tp_ellint_agm(sqrt(m - 0.9), &Ew, &Kw);
+    double Kref = (TP_PI * Km / Kc) * (Kw / TP_PI);
+    *K = Kref;
+    *E = 1.0 / Kw * (TP_PI / 2.0 + Kref * (Kc - Ec));
+}
+
+__device__ __forceinline__ void wl_transform(double *ax, double *ay, double *az,
+        double c1, double s1, double c2, double s2)
+{
+    double ax1 = c2 * (*ax) + s2 * (*ay);
+    double ay1 = -s2 * (*ax) + c2 * (*ay);
+    double az1 = *az;
+    *ax = c1 * ax1 - s1 * az1;
+    *ay = ay1;
+    *az = s1 * ax1 + c1 * az1;
+}
+
+__device__ __forceinline__ void wl_reverse(double *ax, double *ay, double *az,
+        double c1, double s1, double c2, double s2)
+{
+    double ax1 = c1 * (*ax) + s1 * (*az);
+    double ay1 = *ay;
+    double az1 = -s1 * (*ax) + c1 * (*az);
+    *ax = c2 * ax1 - s2 * ay1;
+    *ay = s2 * ax1 + c2 * ay1;
+    *az = az1;
+}
+
+__device__ __noinline__ void emf_wlcs(const FieldDev &f, double x, double y,
+                                      double z, double t, Emf &o)
+{
+    double Bx = 0.0, By = 0.0, Bz = 0.0, Ex = 0.0, Ey = 0.0, Ez = 0.0;
+    for (int i = 0; i < f.nwlcs; i++) {
+        const WlcsSys s = f.wl[i];
+        double xp = x - s.x_wr, yp = y - s.y_wr, zp = z - s.z_wr;
+        wl_transform(&xp, &yp, &zp, s.ct, s.st, s.cp, s.sp);
+        if (f.ibessel == 0) {
+            /* getWireB, src/wlcs.c:361-401 */
+            double rho = sqrt(xp * xp + yp * yp);
+            if (rho < 0.1) rho = 0.1;
+            double cphi = xp / rho;
+            double phi0 = (yp >= 0.0) ? acos(cphi) : 2.0 * TP_PI - acos(cphi);
+            double bphi = 1.0 / 5.0 * (s.cur_wr * TP_WL_I0) / (rho * TP_WL_L0);
+            double sn, cs;
+            tp_sincos(phi0 + TP_PI / 2.0, &sn, &cs);
+            double bx = bphi * cs, by = bphi * sn, bz = 0.0;
+            wl_reverse(&bx, &by, &bz, s.ct, s.st, s.cp, s.sp);
+            Bx += bx; By += by; Bz += bz;
+        } else {
+            /* getWire_Bessel, src/wlcs.c:281-355 (float Bessel functions) */
+            double t1 = s.t0 + t, om = s.omega;
+            double rho = sqrt(xp * xp + yp * yp);
+            double x0 = om * rho * TP_WL_L0 / TP_WL_C0;
+            float x0f = (float)x0;
+            double bj0 = j0f(x0f), bj1 = j1f(x0f), by0 = y0f(x0f), by1 = y1f(x0f);
+            double const0 = (s.cur_wr * TP_WL_I0) / (5.0 * rho * TP_WL_L0) * TP_PI / 2.0;
+            double sw, cw;
+            tp_sincos(om * t1, &sw, &cw);
+            double wb = const0 * x0 * (-by1 * cw + bj1 * sw);
+            double phi1;
+            if (rho == 0.0) phi1 = TP_PI / 2.0;
+            else if (yp >= 0.0) phi1 = acos(xp / rho) + TP_PI / 2.0;
+            else phi1 = 2.0 * TP_PI - acos(xp / rho) + TP_PI / 2.0;
+            double sp1, cp1;
+            tp_sincos(phi1, &sp1, &cp1);
+            double bx = wb * cp1, by = wb * sp1, bz = 0.0;
+            wl_reverse(&bx, &by, &bz, s.ct, s.st, s.cp, s.sp);
+            double we = -const0 * x0 * (by0 * sw + bj0 * cw);
+            double ex = 0.0, ey = 0.0, ez = we;
+            wl_reverse(&ex, &ey, &ez, s.ct, s.st, s.cp, s.sp);
+            Bx += bx; Ex += ex;
+            By += by; Ey += ey;
+            Bz += bz; Ez += ez;
+        }
+        /* getLoopB, src/wlcs.c:406-449 */
+        double lx = (x - s.x_lp) * TP_WL_L0;
+        double ly = (y - s.y_lp) * TP_WL_L0;
+        double lz = (z - s.z_lp) * TP_WL_L0;
+        double al = s.r_lp * TP_WL_L0;
+        double curI = s.cur_lp * TP_WL_I0;
+        wl_transform(&lx, &ly, &lz, s.cb, s.sb, s.ca, s.sa);
+        double Irho = 0.2 * curI;
+        double rxy = sqrt(lx * lx + ly * ly);
+        double al2 = al * al;
+        double r2 = rxy * rxy + lz * lz;
+        double irho1 = 1 / sqrt((al + rxy) * (al + rxy) + lz * lz);
+        double irho2 = 1 / ((al - rxy) * (al - rxy) + lz * lz);
+        double k = sqrt(4.0 * al * rxy) * irho1;
+        double elle, ellk;
+        if (f.wlmode == 0) tp_ellint_ref(k, &elle, &ellk);
+        else tp_ellint_agm(k, &elle, &ellk);
+        double Brho = Irho * lz * irho1 * (-ellk + (al2 + r2) * elle * irho2) / rxy;
+        if (rxy == 0.0) Brho = 0.0;
+        double Bzl = Irho * irho1 * (ellk - (r2 - al2) * elle * irho2);
+        Brho /= rxy;
+        double bx = Brho * lx, by = Brho * ly, bz = Bzl;
+        wl_reverse(&bx, &by, &bz, s.cb, s.sb, s.ca, s.sa);
+        Bx += bx; By += by; Bz += bz;
+    }
+    o.bx = Bx; o.by = By; o.bz = Bz + 0.0;
+    o.ex = Ex; o.ey = Ey; o.ez = Ez;
+}
+
+/* ------------------------------------------------------------------ */
+/* get_emf, FP64 (src/emfields.c:32-60)                                */
+/* ------------------------------------------------------------------ */
+template <int FK>
+__device__ __forceinline__ void get_emf64(const FieldDev &f, double x, double y,
+                                          double z, double t, Emf &o)
+{
+    if (FK == FK_TEST) {
+        o.bx = 0.0; o.by = 0.0; o.bz = 1.0;
+        o.ex = 0.0; o.ey = 0.0; o.ez = 0.0;
+    } else if (FK == FK_ABC || FK == FK_ABCV) {
+        /* getb_ff, src/force_free.c:138-150 */
+        double sx, cx, sy, cy, sz, cz;
+        tp_sincos(f.lam * x, &sx, &cx);
+        tp_sincos(f.lam * y, &sy, &cy);
+        tp_sincos(f.lam * z, &sz, &cz);
+        double bx = f.A * sz + f.C * cy;
+        double by = f.B * sx + f.A * cz;
+        double bz = f.C * sy + f.B * cx;
+        bx *= f.B0; by *= f.B0; bz *= f.B0;
+        o.bx = bx; o.by = by; o.bz = bz;
+        if (FK == FK_ABCV) {
+            /* gete_ff, src/force_free.c:78-86, with the double velocity
+             * gather of src/velocity_field.c:555-584 */
+            long c[8];
+            double w[8];
+            trilinear_ref(f.vnx, f.vny, f.vnz, f.vlo, f.vhi, f.vd, x, y, z, c, w);
+            double vx = 0.0, vy = 0.0, vz = 0.0;
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const double *v = f.vgrid + 3 * c[k];
+                vx = vx + v[0] * w[k];
+                vy = vy + v[1] * w[k];
+                vz = vz + v[2] * w[k];
+            }
+            o.ex = by * vz - bz * vy;
+            o.ey = bz * vx - bx * vz;
+            o.ez = bx * vy - by * vx;
+        } else {
+            o.ex = 0.0; o.ey = 0.0; o.ez = 0.0;
+        }
+    } else if (FK == FK_GRID_REF) {
+        /* getemf_pic_test_particle, src/emfields.c:153-166: float
+         * accumulators rounded after every corner */
+        long c[8];
+        double w[8];
+        trilinear_ref(f.nx, f.ny, f.nz, f.lo, f.hi, f.d, x, y, z, c, w);
+        float ax = 0.f, ay = 0.f, az = 0.f, bx = 0.f, by = 0.f, bz = 0.f;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            float4 p = f.grid[2 * c[k]];
+            float4 q = f.grid[2 * c[k] + 1];
+            ax = (float)((double)ax + (double)p.x * w[k]);
+            ay = (float)((double)ay + (double)p.y * w[k]);
+            az = (float)((double)az + (double)p.z * w[k]);
+            bx = (float)((double)bx + (double)p.w * w[k]);
+            by = (float)((double)by + (double)q.x * w[k]);
+            bz = (float)((double)bz + (double)q.y * w[k]);
+        }
+        o.ex = ax; o.ey = ay; o.ez = az;
+        o.bx = bx; o.by = by; o.bz = bz;
+    } else if (FK == FK_GRID_TB) {
+        long c[8];
+        double w[8];
+        trilinear_tb(f.nx, f.ny, f.nz, f.lo, f.hi, f.d, x, y, z, c, w);
+        double ax = 0.0, ay = 0.0, az = 0.0, bx = 0.0, by = 0.0, bz = 0.0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            float4 p = f.grid[2 * c[k]];
+            float4 q = f.grid[2 * c[k] + 1];
+            ax = ax + (double)p.x * w[k];
+            ay = ay + (double)p.y * w[k];
+            az = az + (double)p.z * w[k];
+            bx = bx + (double)p.w * w[k];
+            by = by + (double)q.x * w[k];
+            bz = bz + (double)q.y * w[k];
+        }
+        o.ex = ax; o.ey = ay; o.ez = az;
+        o.bx = bx; o.by = by; o.bz = bz;
+    } else if (FK == FK_MODES) {
+        double bx = 0.0, by = 0.0, bz = 0.0;
+        for (int n = 0; n < f.nmodes; n++) {
+            const double *m = f.modes + 7 * n;
+            double ph = fma(m[2], z, fma(m[1], y, fma(m[0], x, m[6])));
+            double cs = tp_cos(ph);
+            bx = fma(m[3], cs, bx);
+            by = fma(m[4], cs, by);
+            bz = fma(m[5], cs, bz);
+        }
+        o.bx = bx; o.by = by; o.bz = f.b0z + bz;
+        o.ex = 0.0; o.ey = 0.0; o.ez = 0.0;
+    } else if (FK == FK_WLCS) {
+        emf_wlcs(f, x, y, z, t, o);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* get_emf, FP32 (precision form)                                      */
+/* ------------------------------------------------------------------ */
+template <int FK>
+__device__ __forceinline__ void get_emf32(const FieldDev &f, double x, double y,
+                                          double z, double t, EmfF &o)
+{
+    if (FK == FK_TEST) {
+        o.bx = 0.f; o.by = 0.f; o.bz = 1.f;
+        o.ex = 0.f; o.ey = 0.f; o.ez = 0.f;
+    } else if (FK == FK_ABC) {
+        float lam = (float)f.lam;
+        float sx, cx, sy, cy, sz, cz;
+        sincosf(lam * (float)x, &sx, &cx);
+        sincosf(lam * (float)y, &sy, &cy);
+        sincosf(lam * (float)z, &sz, &cz);
+        float A = (float)f.A, B = (float)f.B, C = (float)f.C, B0 = (float)f.B0;
+        o.bx = (A * sz + C * cy) * B0;
+        o.by = (B * sx + A * cz) * B0;
+        o.bz = (C * sy + B * cx) * B0;
+        o.ex = 0.f; o.ey = 0.f; o.ez = 0.f;
+    } else if (FK == FK_GRID_TB || FK == FK_GRID_REF) {
+        /* float weights and accumulation; the weight convention follows
+         * the field's gather mode */
+        long c[8];
+        double w[8];
+        if (FK == FK_GRID_REF)
+            trilinear_ref(f.nx, f.ny, f.nz, f.lo, f.hi, f.d, x, y, z, c, w);
+        else
+            trilinear_tb(f.nx, f.ny, f.nz, f.lo, f.hi, f.d, x, y, z, c, w);
+        float ax = 0.f, ay = 0.f, az = 0.f, bx = 0.f, by = 0.f, bz = 0.f;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            float4 p = f.grid[2 * c[k]];
+            float4 q = f.grid[2 * c[k] + 1];
+            float wk = (float)w[k];
+            ax = fmaf(p.x, wk, ax); ay = fmaf(p.y, wk, ay); az = fmaf(p.z, wk, az);
+            bx = fmaf(p.w, wk, bx); by = fmaf(q.x, wk, by); bz = fmaf(q.y, wk, bz);
+        }
+        o.ex = ax; o.ey = ay; o.ez = az;
+        o.bx = bx; o.by = by; o.bz = bz;
+    } else if (FK == FK_MODES) {
+        float bx = 0.f, by = 0.f, bz = 0.f;
+        float xf = (float)x, yf = (float)y, zf = (float)z;
+        for (int n = 0; n < f.nmodes; n++) {
+            const double *m = f.modes + 7 * n;
+            float ph = fmaf((float)m[2], zf, fmaf((float)m[1], yf, fmaf((float)m[0], xf, (float)m[6])));
+            float cs = cosf(ph);
+            bx = fmaf((float)m[3], cs, bx);
+            by = fmaf((float)m[4], cs, by);
+            bz = fmaf((float)m[5], cs, bz);
+        }
+        o.bx = bx; o.by = by; o.bz = (float)f.b0z + bz;
+        o.ex = 0.f; o.ey = 0.f; o.ez = 0.f;
+    } else {
+        Emf e;
+        get_emf64<FK>(f, x, y, z, t, e);
+        o.bx = (float)e.bx; o.by = (float)e.by; o.bz = (float)e.bz;
+        o.ex = (float)e.ex; o.ey = (float)e.ey; o.ez = (float)e.ez;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* pushers                                                             */
+/* ------------------------------------------------------------------ */
+struct Ptl {
+    double x, y, z, ux, uy, uz, t;
+};
+
+/* tracking_wirz, src/tracking.c:749-892, in the reference's order */
+template <int FK>
+__device__ __forceinline__ void wirz_step64(const FieldDev &f, Ptl &p, double dt,
+                                            double qm)
+{
+    Emf e;
+    get_emf64<FK>(f, p.x, p.y, p.z, p.t, e);
+    double Btot = sqrt(e.bx * e.bx + e.by * e.by + e.bz * e.bz);
+    double iB = 1.0 / Btot;
+    double dth = dt * 0.5;
+    double qd = qm * dth;
+    double dmx = p.ux + qd * e.ex;
+    double dmy = p.uy + qd * e.ey;
+    double dmz = p.uz + qd * e.ez;
+    double gama = sqrt(1.0 + dmx * dmx + dmy * dmy + dmz * dmz);
+    double ig = 1.0 / gama;
+    double vmx = dmx * ig, vmy = dmy * ig, vmz = dmz * ig;
+    double hx = e.bx * iB, hy = e.by * iB, hz = e.bz * iB;
+    double vd = 0.0;
+    vd += vmx * hx; vd += vmy * hy; vd += vmz * hz;
+    double ax = vmx - vd * hx, ay = vmy - vd * hy, az = vmz - vd * hz;
+    double vp = sqrt(ax * ax + ay * ay + az * az);
+    double px = (vmx - vd * hx) / vp, py = (vmy - vd * hy) / vp, pz = (vmz - vd * hz) / vp;
+    double rx = hy * pz - hz * py, ry = hz * px - hx * pz, rz = hx * py - hy * px;
+    double gR = vp * gama * 1.0 * iB / qm;
+    double om = qm * Btot * ig;
+    double dthe = om * dt;
+    double sd, cd;
+    tp_sincos(dthe * 0.5, &sd, &cd);
+    double t1 = dth * vd * 1.0;
+    double t2 = gR * sd;
+    double t3 = -gR * (1.0 - cd);
+    double prx = p.x + hx * t1 + px * t2 + rx * t3;
+    double pry = p.y + hy * t1 + py * t2 + ry * t3;
+    double prz = p.z + hz * t1 + pz * t2 + rz * t3;
+
+    get_emf64<FK>(f, prx, pry, prz, p.t + dth, e);
+    Btot = sqrt(e.bx * e.bx + e.by * e.by + e.bz * e.bz);
+    iB = 1.0 / Btot;
+    hx = e.bx * iB; hy = e.by * iB; hz = e.bz * iB;
+    vd = 0.0;
+    vd += vmx * hx; vd += vmy * hy; vd += vmz * hz;
+    ax = vmx - vd * hx; ay = vmy - vd * hy; az = vmz - vd * hz;
+    vp = sqrt(ax * ax + ay * ay + az * az);
+    px = (vmx - vd * hx) / vp; py = (vmy - vd * hy) / vp; pz = (vmz - vd * hz) / vp;
+    rx = hy * pz - hz * py; ry = hz * px - hx * pz; rz = hx * py - hy * px;
+    om = qm * Btot * ig;
+    dthe = om * dt;
+    tp_sincos(dthe, &sd, &cd);
+    double wx = hx * vd + vp * (cd * px - sd * rx);
+    double wy = hy * vd + vp * (cd * py - sd * ry);
+    double wz = hz * vd + vp * (cd * pz - sd * rz);
+    /* get_delta_vec, src/tracking.c:1066-1078 */
+    double bm = 0.0;
+    bm += wx * wx; bm += wy * wy; bm += wz * wz;
+    double g2 = 1.0 / sqrt(1.0 - bm);
+    p.ux = g2 * wx + qd * e.ex;
+    p.uy = g2 * wy + qd * e.ey;
+    p.uz = g2 * wz + qd * e.ez;
+    vd = 0.0;
+    vd += wx * hx; vd += wy * hy; vd += wz * hz;
+    ax = wx - vd * hx; ay = wy - vd * hy; az = wz - vd * hz;
+    vp = sqrt(ax * ax + ay * ay + az * az);
+    gR = vp * gama * 1.0 * iB / qm;
+    t1 = dt * vd * 1.0;
+    t2 = gR * sd;
+    t3 = -gR * (1.0 - cd);
+    p.x += hx * t1 + px * t2 + rx * t3;
+    p.y += hy * t1 + py * t2 + ry * t3;
+    p.z += hz * t1 + pz * t2 + rz * t3;
+    p.t += dt;
+}
+
+/* relativistic Boris (same operations as the oracle's tpref_boris_step) */
+template <int FK>
+__device__ __forceinline__ void boris_step64(const FieldDev &f, Ptl &p, double dt,
+                                             double qm)
+{
+    Emf e;
+    get_emf64<FK>(f, p.x, p.y, p.z, p.t, e);
+    double h = qm * (0.5 * dt);
+    double umx = p.ux + h * e.ex;
+    double umy = p.uy + h * e.ey;
+    double umz = p.uz + h * e.ez;
+    double gm = sqrt(1.0 + umx * umx + umy * umy + umz * umz);
+    double fq = h / gm;
+    double tx = fq * e.bx, ty = fq * e.by, tz = fq * e.bz;
+    double t2 = tx * tx + ty * ty + tz * tz;
+    double sf = 2.0 / (1.0 + t2);
+    double sx = sf * tx, sy = sf * ty, sz = sf * tz;
+    double upx = umx + (umy * tz - umz * ty);
+    double upy = umy + (umz * tx - umx * tz);
+    double upz = umz + (umx * ty - umy * tx);
+    double uqx = umx + (upy * sz - upz * sy);
+    double uqy = umy + (upz * sx - upx * sz);
+    double uqz = umz + (upx * sy - upy * sx);
+    double ux = uqx + h * e.ex;
+    double uy = uqy + h * e.ey;
+    double uz = uqz + h * e.ez;
+    double g = sqrt(1.0 + ux * ux + uy * uy + uz * uz);
+    double ig = dt / g;
+    p.ux = ux; p.uy = uy; p.uz = uz;
+    p.x += ux * ig;
+    p.y += uy * ig;
+    p.z += uz * ig;
+    p.t += dt;
+}
+
+/* Boris in FP32 with FP64 position and time accumulators */
+template <int FK>
+__device__ __forceinline__ void boris_step32(const FieldDev &f, Ptl &p,
+                                             float &ux, float &uy, float &uz,
+                                             double dt, float dtf, float hf)
+{
+    EmfF e;
+    get_emf32<FK>(f, p.x, p.y, p.z, p.t, e);
+    float umx = fmaf(hf, e.ex, ux);
+    float umy = fmaf(hf, e.ey, uy);
+    float umz = fmaf(hf, e.ez, uz);
+    float gm2 = fmaf(umx, umx, fmaf(umy, umy, fmaf(umz, umz, 1.0f)));
+    float fq = hf * __frsqrt_rn(gm2);
+    float tx = fq * e.bx, ty = fq * e.by, tz = fq * e.bz;
+    float t2 = fmaf(tx, tx, fmaf(ty, ty, tz * tz));
+    float sf = 2.0f / (1.0f + t2);
+    float sx = sf * tx, sy = sf * ty, sz = sf * tz;
+    float upx = umx + (umy * tz - umz * ty);
+    float upy = umy + (umz * tx - umx * tz);
+    float upz = umz + (umx * ty - umy * tx);
+    float uqx = umx + (upy * sz - upz * sy);
+    float uqy = umy + (upz * sx - upx * sz);
+    float uqz = umz + (upx * sy - upy * sx);
+    ux = fmaf(hf, e.ex, uqx);
+    uy = fmaf(hf, e.ey, uqy);
+    uz = fmaf(hf, e.ez, uqz);
+    float g2 = fmaf(ux, ux, fmaf(uy, uy, fmaf(uz, uz, 1.0f)));
+    float ig = dtf * __frsqrt_rn(g2);
+    p.x += (double)(ux * ig);
+    p.y += (double)(uy * ig);
+    p.z += (double)(uz * ig);
+    p.t += dt;
+}
+
+#endif /* TP_DEVICE_H */

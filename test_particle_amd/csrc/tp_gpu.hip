@@ -1,0 +1,970 @@
+/*
+ * tp_gpu.hip -- HIP kernels (gfx950) and the C ABI of include/tp_gpu.h.
+ *
+ * The reference's hot loop (src/tracking.c:297-462) runs one OpenMP
+ * iteration per particle with the time loop inside.  Here one lane owns
+ * one particle; a launch advances every particle through a chunk of steps
+ * with the state in registers (SoA arrays in HBM between chunks).
+ * Diagnostics are fused into the step loop:
+ *   - energy spectrum (ptl_energy_fixed, src/diagnostics.c:187-205): an LDS
+ *     histogram per block per frame, flushed with 32-bit integer atomics
+ *     (exact, order-independent);
+ *   - diffusion sums (calc_diff_coeff, src/diagnostics.c:659-705, summed at
+ *     src/tracking.c:347-365): a fixed-order wave/LDS reduction per block
+ *     into per-block partials, then a fixed-order reduction over blocks
+ *     (deterministic run to run; the reference uses omp atomics);
+ *   - remaining-particle counts: integer atomics.
+ */
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <new>
+
+#include "../../include/tp_gpu.h"
+#include "tp_device.h"
+
+enum { PREC64 = 0, PREC32 = 1 };
+
+/* ------------------------------------------------------------------ */
+/* context                                                             */
+/* ------------------------------------------------------------------ */
+struct State {
+    int cap = 0, n = 0;
+    double *buf = nullptr;   /* all double arrays, 26 * cap */
+    int *ibuf = nullptr;     /* esc, nsteps: 2 * cap */
+    tp_particle *stage = nullptr;
+    double *x, *y, *z, *ux, *uy, *uz, *t, *xs, *ys, *zs;
+    double *x0, *y0, *z0, *ux0, *uy0, *uz0, *t0, *g0, *mu0, *th0;
+    double *st;              /* 5 * cap stale D terms */
+    int *esc, *nsteps;
+    bool derived = false;    /* g0/mu0/th0 valid for current field */
+};
+
+struct tp_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    FieldDev f;
+    bool have_field = false;
+    /* owned field buffers */
+    float4 *grid = nullptr;
+    double *vgrid = nullptr, *modes = nullptr;
+    WlcsSys *wl = nullptr;
+    State s;
+    /* diagnostics of the last run */
+    int nb = 0, nt = 0, nd = 0;
+    unsigned *espect = nullptr, *espect_esc = nullptr;
+    double *dco = nullptr, *partials = nullptr;
+    int *rdec = nullptr;
+    size_t partials_cap = 0, diag_cap = 0, dco_cap = 0;
+    int last_n = 0;
+    double last_ms = 0.0;
+    int last_launches = 0;
+    int num_cu = 0;
+};
+
+#define HIPCHK(x)                                                        \
+    do {                                                                 \
+        hipError_t e_ = (x);                                             \
+        if (e_ != hipSuccess) {                                          \
+            fprintf(stderr, "tp_gpu: %s failed: %s (%s:%d)\n", #x,       \
+                    hipGetErrorString(e_), __FILE__, __LINE__);          \
+            return TP_E_HIP;                                             \
+        }                                                                \
+    } while (0)
+
+/* ------------------------------------------------------------------ */
+/* device helpers                                                      */
+/* ------------------------------------------------------------------ */
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+/* ptl_energy_fixed binning, src/diagnostics.c:193-203 */
+__device__ __forceinline__ int energy_bin(double ux, double uy, double uz,
+                                          int nbins, double logemin, double logde)
+{
+    double gama = sqrt(1.0 + ux * ux + uy * uy + uz * uz);
+    double ene = gama - 1.0;
+    double v = floor((log10(ene) - logemin) / logde);
+    if (!(v > -2147483649.0 && v < 2147483648.0)) return 0;
+    int ib = (int)v;
+    if (ib < 0) ib = 0;
+    else if (ib > nbins - 1) ib = nbins - 1;
+    return ib;
+}
+
+struct TrackArgs {
+    FieldDev f;
+    int n, bc, nbins, nt_out, einterval, di, fix_stale, first;
+    int j0, j1, ntp0, tstep0, nd;
+    double dt, qm, lo[3], hi[3], logemin, logde;
+    double *x, *y, *z, *ux, *uy, *uz, *t, *xs, *ys, *zs;
+    const double *x0, *y0, *z0, *ux0, *uy0, *uz0, *g0, *mu0, *th0;
+    double *st;
+    int *esc, *nsteps;
+    unsigned *espect, *espect_esc;
+    double *partials;
+    int *rdec;
+};
+
+/* particle_bc, src/tracking.c:1172-1221 */
+__device__ __forceinline__ void particle_bc(const TrackArgs &a, Ptl &p, double &xs,
+                                            double &ys, double &zs, int &esc)
+{
+    if (a.bc == 0) {
+        double xd = a.hi[0] - a.lo[0], yd = a.hi[1] - a.lo[1], zd = a.hi[2] - a.lo[2];
+        if (p.x > a.hi[0]) { p.x -= xd; xs += xd; }
+        if (p.y > a.hi[1]) { p.y -= yd; ys += yd; }
+        if (p.z > a.hi[2]) { p.z -= zd; zs += zd; }
+        if (p.x < a.lo[0]) { p.x += xd; xs -= xd; }
+        if (p.y < a.lo[1]) { p.y += yd; ys -= yd; }
+        if (p.z < a.lo[2]) { p.z += zd; zs -= zd; }
+    } else if (a.bc == 1) {
+        if (p.x > a.hi[0] || p.y > a.hi[1] || p.z > a.hi[2] ||
+            p.x < a.lo[0] || p.y < a.lo[1] || p.z < a.lo[2])
+            esc = 1;
+    } else if (a.bc == 2) {
+        esc = 0;
+    }
+}
+
+/* calc_diff_coeff, src/diagnostics.c:659-705.  The initial-particle terms
+ * (gamma_init, mu_init, theta_init) were evaluated once by k_derived with
+ * the same operations. */
+template <int FK>
+__device__ __forceinline__ void diff_terms(const TrackArgs &a, int i, const Ptl &p,
+        double xs, double ys, double zs, double &drr, double &dxx, double &dyy,
+        double &dzz, double &dpp, double &duu, double &daa, double &dee)
+{
+    double t = p.t * 2 + a.dt;
+    double q;
+    q = a.x0[i] - (p.x + xs); dxx = q * q / t;
+    q = a.y0[i] - (p.y + ys); dyy = q * q / t;
+    q = a.z0[i] - (p.z + zs); dzz = q * q / t;
+    drr = dxx + dyy + dzz;
+    double ax = a.ux0[i] - p.ux, ay = a.uy0[i] - p.uy, az = a.uz0[i] - p.uz;
+    dpp = (ax * ax + ay * ay + az * az) / t;
+    double g = sqrt(1.0 + p.ux * p.ux + p.uy * p.uy + p.uz * p.uz);
+    q = g - a.g0[i]; dee = q * q / t;
+    Emf e;
+    get_emf64<FK>(a.f, p.x, p.y, p.z, p.t, e);
+    double ab = sqrt(e.bx * e.bx + e.by * e.by + e.bz * e.bz);
+    double au = sqrt(p.ux * p.ux + p.uy * p.uy + p.uz * p.uz);
+    double mu = (e.bx * p.ux + e.by * p.uy + e.bz * p.uz) / (ab * au);
+    q = mu - a.mu0[i]; duu = q * q / t;
+    double th = acos(mu) * 180 / TP_PI;
+    q = th - a.th0[i]; daa = q * q / t;
+}
+
+/* ------------------------------------------------------------------ */
+/* tracking kernel                                                     */
+/* ------------------------------------------------------------------ */
+template <int PUSH, int FK, int PREC>
+__global__ void __launch_bounds__(256) k_track(TrackArgs a)
+{
+    extern __shared__ double smem_d[];
+    double *wsum = smem_d;                                /* [nwaves][8] */
+    unsigned *hist = (unsigned *)(smem_d + 8 * (blockDim.x / 64));
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
+    const int i = blockIdx.x * blockDim.x + tid;
+    const bool live = i < a.n;
+    for (int b = tid; b < a.nbins; b += blockDim.x) hist[b] = 0u;
+
+    Ptl p;
+    double xs = 0.0, ys = 0.0, zs = 0.0;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;  /* dxx dyy dzz daa dee */
+    int esc = 1, nst = 0;
+    if (live) {
+        p.x = a.x[i]; p.y = a.y[i]; p.z = a.z[i];
+        p.ux = a.ux[i]; p.uy = a.uy[i]; p.uz = a.uz[i]; p.t = a.t[i];
+        xs = a.xs[i]; ys = a.ys[i]; zs = a.zs[i];
+        s0 = a.st[i]; s1 = a.st[a.n + i]; s2 = a.st[2 * a.n + i];
+        s3 = a.st[3 * a.n + i]; s4 = a.st[4 * a.n + i];
+        esc = a.esc[i];
+        nst = a.nsteps[i];
+        if (a.first) {
+            /* initial wrap and escape check, src/tracking.c:314-332 */
+            esc = 0;
+            nst = 1;
+            particle_bc(a, p, xs, ys, zs, esc);
+            if (esc && 1 < a.nd) atomicAdd(&a.rdec[1], 1);
+        }
+    } else {
+        p.x = p.y = p.z = p.ux = p.uy = p.uz = p.t = 0.0;
+    }
+    float fux = (float)p.ux, fuy = (float)p.uy, fuz = (float)p.uz;
+    const float dtf = (float)a.dt;
+    const float hf = (float)(a.qm * (0.5 * a.dt));
+    __syncthreads();
+
+    int ntp = a.ntp0, tstep = a.tstep0;
+    for (int j = a.j0; j <= a.j1; j++) {
+        double d0 = 0.0, d4 = 0.0, d5 = 0.0;     /* drr dpp duu */
+        if (a.fix_stale) { s0 = s1 = s2 = s3 = s4 = 0.0; }
+        int esc_after = esc;
+        const bool do_d = (j % a.di) == 0;
+        if (live && (esc == 0 || a.bc == 0)) {
+            if (PREC == PREC64) {
+                if (PUSH == TP_PUSH_WIRZ) wirz_step64<FK>(a.f, p, a.dt, a.qm);
+                else boris_step64<FK>(a.f, p, a.dt, a.qm);
+            } else {
+                boris_step32<FK>(a.f, p, fux, fuy, fuz, a.dt, dtf, hf);
+                p.ux = fux; p.uy = fuy; p.uz = fuz;
+            }
+            particle_bc(a, p, xs, ys, zs, esc_after);
+            nst++;
+            if (esc_after == 0 && do_d)
+                diff_terms<FK>(a, i, p, xs, ys, zs, d0, s0, s1, s2, d4, d5, s3, s4);
+        }
+        if (live && esc == 0 && esc_after == 1 && a.bc == 1 && ntp < a.nt_out)
+            atomicAdd(&a.espect_esc[ntp * a.nbins +
+                      energy_bin(p.ux, p.uy, p.uz, a.nbins, a.logemin, a.logde)], 1u);
+        if (j % a.einterval == 0) {
+            if (live && esc_after == 0 && ntp < a.nt_out)
+                atomicAdd(&hist[energy_bin(p.ux, p.uy, p.uz, a.nbins, a.logemin, a.logde)], 1u);
+            __syncthreads();
+            if (ntp < a.nt_out) {
+                for (int b = tid; b < a.nbins; b += blockDim.x) {
+                    unsigned h = hist[b];
+                    if (h) {
+                        atomicAdd(&a.espect[ntp * a.nbins + b], h);
+                        hist[b] = 0u;
+                    }
+                }
+            }
+            __syncthreads();
+            ntp++;
+        }
+        if (do_d) {
+            double v[8];
+            v[0] = live ? d0 : 0.0; v[1] = live ? s0 : 0.0; v[2] = live ? s1 : 0.0;
+            v[3] = live ? s2 : 0.0; v[4] = live ? d4 : 0.0; v[5] = live ? d5 : 0.0;
+            v[6] = live ? s3 : 0.0; v[7] = live ? s4 : 0.0;
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                double r = wave_sum(v[k]);
+                if (lane == 0) wsum[wave * 8 + k] = r;
+            }
+            __syncthreads();
+            if (tid < 8) {
+                double r = 0.0;
+                for (int w = 0; w < nwaves; w++) r += wsum[w * 8 + tid];
+                a.partials[((size_t)(tstep - a.tstep0) * gridDim.x + blockIdx.x) * 8 + tid] = r;
+            }
+            __syncthreads();
+            tstep++;
+            unsigned long long m = __ballot(live && esc_after);
+            if (lane == 0 && m && tstep < a.nd) atomicAdd(&a.rdec[tstep], __popcll(m));
+        }
+        esc = esc_after;
+    }
+    if (live) {
+        a.x[i] = p.x; a.y[i] = p.y; a.z[i] = p.z;
+        a.ux[i] = p.ux; a.uy[i] = p.uy; a.uz[i] = p.uz; a.t[i] = p.t;
+        a.xs[i] = xs; a.ys[i] = ys; a.zs[i] = zs;
+        a.st[i] = s0; a.st[a.n + i] = s1; a.st[2 * a.n + i] = s2;
+        a.st[3 * a.n + i] = s3; a.st[4 * a.n + i] = s4;
+        a.esc[i] = esc;
+        a.nsteps[i] = nst;
+    }
+}
+
+/* Sum the per-block partials of `rows` D rows in a fixed order:
+ * block (row, component), strided per-thread sums then an LDS tree. */
+__global__ void __launch_bounds__(256) k_reduce_rows(const double *partials,
+        int nblocks, int tstep0, int nd, double *dco)
+{
+    __shared__ double sh[256];
+    const int r = blockIdx.x >> 3, k = blockIdx.x & 7;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += 256)
+        s += partials[((size_t)r * nblocks + b) * 8 + k];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && tstep0 + r < nd) dco[(size_t)k * nd + tstep0 + r] = sh[0];
+}
+
+/* AoS (struct particles) -> SoA, for both the current and the initial
+ * particles; the initial ones also get gamma_init, mu_init and theta_init
+ * as calc_diff_coeff evaluates them (src/diagnostics.c:680-703). */
+template <int FK>
+__global__ void __launch_bounds__(256) k_load(FieldDev f, int n, const tp_particle *cur,
+        const tp_particle *ini, double *x, double *y, double *z, double *ux,
+        double *uy, double *uz, double *t, double *xs, double *ys, double *zs,
+        double *x0, double *y0, double *z0, double *ux0, double *uy0, double *uz0,
+        double *t0, double *g0, double *mu0, double *th0, double *st, int *esc,
+        int *nsteps)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    tp_particle c = cur[i], p = ini[i];
+    x[i] = c.x; y[i] = c.y; z[i] = c.z; ux[i] = c.ux; uy[i] = c.uy; uz[i] = c.uz;
+    t[i] = c.t; xs[i] = c.xshift; ys[i] = c.yshift; zs[i] = c.zshift;
+    x0[i] = p.x; y0[i] = p.y; z0[i] = p.z; ux0[i] = p.ux; uy0[i] = p.uy; uz0[i] = p.uz;
+    t0[i] = p.t;
+    g0[i] = sqrt(1.0 + p.ux * p.ux + p.uy * p.uy + p.uz * p.uz);
+    Emf e;
+    get_emf64<FK>(f, p.x, p.y, p.z, p.t, e);
+    double ab = sqrt(e.bx * e.bx + e.by * e.by + e.bz * e.bz);
+    double au = sqrt(p.ux * p.ux + p.uy * p.uy + p.uz * p.uz);
+    double m0 = (e.bx * p.ux + e.by * p.uy + e.bz * p.uz) / (ab * au);
+    mu0[i] = m0;
+    th0[i] = acos(m0) * 180 / TP_PI;
+    for (int k = 0; k < 5; k++) st[(size_t)k * n + i] = 0.0;
+    esc[i] = 0;
+    nsteps[i] = 1;
+}
+
+__global__ void __launch_bounds__(256) k_store(int n, const double *x, const double *y,
+        const double *z, const double *ux, const double *uy, const double *uz,
+        const double *t, const double *xs, const double *ys, const double *zs,
+        tp_particle *out)
+{
+    /* out[] still holds the particles as uploaded; qm is not tracked */
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    tp_particle c;
+    c.x = x[i]; c.y = y[i]; c.z = z[i]; c.qm = out[i].qm;
+    c.ux = ux[i]; c.uy = uy[i]; c.uz = uz[i]; c.t = t[i];
+    c.xshift = xs[i]; c.yshift = ys[i]; c.zshift = zs[i];
+    out[i] = c;
+}
+
+template <int FK, int PREC>
+__global__ void __launch_bounds__(256) k_eval(FieldDev f, int n, const double *pts,
+                                              double *out)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double x = pts[4 * i], y = pts[4 * i + 1], z = pts[4 * i + 2], t = pts[4 * i + 3];
+    double *o = out + 6 * i;
+    if (PREC == PREC64) {
+        Emf e;
+        get_emf64<FK>(f, x, y, z, t, e);
+        o[0] = e.bx; o[1] = e.by; o[2] = e.bz; o[3] = e.ex; o[4] = e.ey; o[5] = e.ez;
+    } else {
+        EmfF e;
+        get_emf32<FK>(f, x, y, z, t, e);
+        o[0] = e.bx; o[1] = e.by; o[2] = e.bz; o[3] = e.ex; o[4] = e.ey; o[5] = e.ez;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* dispatch                                                            */
+/* ------------------------------------------------------------------ */
+static int field_kind(const tp_ctx *c) { return c->f.kind; }
+
+typedef void (*track_fn)(dim3, dim3, size_t, hipStream_t, const TrackArgs &);
+
+template <int PUSH, int FK, int PREC>
+static void launch_track(dim3 g, dim3 b, size_t sh, hipStream_t s, const TrackArgs &a)
+{
+    hipLaunchKernelGGL((k_track<PUSH, FK, PREC>), g, b, sh, s, a);
+}
+
+template <int FK>
+static track_fn pick_track_fk(int push, int prec)
+{
+    if (prec == TP_FP64) {
+        if (push == TP_PUSH_WIRZ) return launch_track<TP_PUSH_WIRZ, FK, PREC64>;
+        return launch_track<TP_PUSH_BORIS, FK, PREC64>;
+    }
+    if (push == TP_PUSH_BORIS) return launch_track<TP_PUSH_BORIS, FK, PREC32>;
+    return nullptr;
+}
+
+static track_fn pick_track(int fk, int push, int prec)
+{
+    switch (fk) {
+    case FK_TEST: return pick_track_fk<FK_TEST>(push, prec);
+    case FK_ABC: return pick_track_fk<FK_ABC>(push, prec);
+    case FK_ABCV: return pick_track_fk<FK_ABCV>(push, prec);
+    case FK_GRID_REF: return pick_track_fk<FK_GRID_REF>(push, prec);
+    case FK_GRID_TB: return pick_track_fk<FK_GRID_TB>(push, prec);
+    case FK_MODES: return pick_track_fk<FK_MODES>(push, prec);
+    case FK_WLCS: return pick_track_fk<FK_WLCS>(push, prec);
+    }
+    return nullptr;
+}
+
+template <int FK>
+static void launch_load(dim3 g, dim3 b, hipStream_t s, const FieldDev &f, int n,
+                        const tp_particle *cur, const tp_particle *ini, State &S)
+{
+    hipLaunchKernelGGL((k_load<FK>), g, b, 0, s, f, n, cur, ini, S.x, S.y, S.z, S.ux,
+                       S.uy, S.uz, S.t, S.xs, S.ys, S.zs, S.x0, S.y0, S.z0, S.ux0,
+                       S.uy0, S.uz0, S.t0, S.g0, S.mu0, S.th0, S.st, S.esc, S.nsteps);
+}
+
+static void dispatch_load(int fk, dim3 g, dim3 b, hipStream_t s, const FieldDev &f,
+                          int n, const tp_particle *cur, const tp_particle *ini,
+                          State &S)
+{
+    switch (fk) {
+    case FK_TEST: launch_load<FK_TEST>(g, b, s, f, n, cur, ini, S); break;
+    case FK_ABC: launch_load<FK_ABC>(g, b, s, f, n, cur, ini, S); break;
+    case FK_ABCV: launch_load<FK_ABCV>(g, b, s, f, n, cur, ini, S); break;
+    case FK_GRID_REF: launch_load<FK_GRID_REF>(g, b, s, f, n, cur, ini, S); break;
+    case FK_GRID_TB: launch_load<FK_GRID_TB>(g, b, s, f, n, cur, ini, S); break;
+    case FK_MODES: launch_load<FK_MODES>(g, b, s, f, n, cur, ini, S); break;
+    case FK_WLCS: launch_load<FK_WLCS>(g, b, s, f, n, cur, ini, S); break;
+    }
+}
+
+template <int FK>
+static void launch_eval(dim3 g, dim3 b, hipStream_t s, const FieldDev &f, int n,
+                        const double *pts, double *out, int prec)
+{
+    if (prec == TP_FP64) hipLaunchKernelGGL((k_eval<FK, PREC64>), g, b, 0, s, f, n, pts, out);
+    else hipLaunchKernelGGL((k_eval<FK, PREC32>), g, b, 0, s, f, n, pts, out);
+}
+
+static void dispatch_eval(int fk, dim3 g, dim3 b, hipStream_t s, const FieldDev &f,
+                          int n, const double *pts, double *out, int prec)
+{
+    switch (fk) {
+    case FK_TEST: launch_eval<FK_TEST>(g, b, s, f, n, pts, out, prec); break;
+    case FK_ABC: launch_eval<FK_ABC>(g, b, s, f, n, pts, out, prec); break;
+    case FK_ABCV: launch_eval<FK_ABCV>(g, b, s, f, n, pts, out, prec); break;
+    case FK_GRID_REF: launch_eval<FK_GRID_REF>(g, b, s, f, n, pts, out, prec); break;
+    case FK_GRID_TB: launch_eval<FK_GRID_TB>(g, b, s, f, n, pts, out, prec); break;
+    case FK_MODES: launch_eval<FK_MODES>(g, b, s, f, n, pts, out, prec); break;
+    case FK_WLCS: launch_eval<FK_WLCS>(g, b, s, f, n, pts, out, prec); break;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* C ABI                                                               */
+/* ------------------------------------------------------------------ */
+extern "C" {
+
+const char *tp_gpu_strerror(int code)
+{
+    switch (code) {
+    case TP_OK: return "ok";
+    case TP_E_ARG: return "invalid argument";
+    case TP_E_NOMEM: return "out of memory";
+    case TP_E_HIP: return "HIP runtime error";
+    case TP_E_NONFINITE: return "non-finite particle state";
+    case TP_E_STATE: return "call out of order";
+    case TP_E_NODEV: return "no usable GPU";
+    }
+    return "unknown error";
+}
+
+int tp_gpu_init(int device, tp_ctx **out)
+{
+    if (!out) return TP_E_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return TP_E_NODEV;
+    if (device < 0 || device >= ndev) return TP_E_ARG;
+    tp_ctx *c = new (std::nothrow) tp_ctx();
+    if (!c) return TP_E_NOMEM;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return TP_E_HIP;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cu = prop.multiProcessorCount;
+    memset(&c->f, 0, sizeof(c->f));
+    *out = c;
+    return TP_OK;
+}
+
+static void free_field(tp_ctx *c)
+{
+    hipFree(c->grid); c->grid = nullptr;
+    hipFree(c->vgrid); c->vgrid = nullptr;
+    hipFree(c->modes); c->modes = nullptr;
+    hipFree(c->wl); c->wl = nullptr;
+    c->have_field = false;
+    c->s.derived = false;
+}
+
+void tp_gpu_finalize(tp_ctx *c)
+{
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    free_field(c);
+    hipFree(c->s.buf);
+    hipFree(c->s.ibuf);
+    hipFree(c->s.stage);
+    hipFree(c->espect);
+    hipFree(c->espect_esc);
+    hipFree(c->dco);
+    hipFree(c->partials);
+    hipFree(c->rdec);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int tp_gpu_device_info(tp_ctx *c, char *name, int name_len, int *num_cu)
+{
+    if (!c) return TP_E_ARG;
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, c->device));
+    if (name && name_len > 0) {
+        snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    if (num_cu) *num_cu = prop.multiProcessorCount;
+    return TP_OK;
+}
+
+static void set_box(double *dst_lo, double *dst_hi, double *dst_d, const double *lo,
+                    const double *hi, int nx, int ny, int nz)
+{
+    int n[3] = {nx, ny, nz};
+    for (int a = 0; a < 3; a++) {
+        dst_lo[a] = lo[a];
+        dst_hi[a] = hi[a];
+        dst_d[a] = (hi[a] - lo[a]) / (n[a] - 1.0);   /* src/domain.c:548-550 */
+    }
+}
+
+int tp_gpu_set_field_test(tp_ctx *c)
+{
+    if (!c) return TP_E_ARG;
+    HIPCHK(hipSetDevice(c->device));
+    free_field(c);
+    memset(&c->f, 0, sizeof(c->f));
+    c->f.kind = FK_TEST;
+    c->have_field = true;
+    return TP_OK;
+}
+
+int tp_gpu_set_field_abc(tp_ctx *c, double A, double B, double C, double lambda, double B0)
+{
+    if (!c) return TP_E_ARG;
+    HIPCHK(hipSetDevice(c->device));
+    /* keep a velocity grid set before */
+    double *vg = c->vgrid;
+    c->vgrid = nullptr;
+    FieldDev keep = c->f;
+    free_field(c);
+    c->vgrid = vg;
+    memset(&c->f, 0, sizeof(c->f));
+    c->f.A = A; c->f.B = B; c->f.C = C; c->f.lam = lambda; c->f.B0 = B0;
+    if (vg) {
+        c->f.vnx = keep.vnx; c->f.vny = keep.vny; c->f.vnz = keep.vnz;
+        memcpy(c->f.vlo, keep.vlo, sizeof(keep.vlo));
+        memcpy(c->f.vhi, keep.vhi, sizeof(keep.vhi));
+        memcpy(c->f.vd, keep.vd, sizeof(keep.vd));
+        c->f.vgrid = vg;
+        c->f.kind = FK_ABCV;
+    } else {
+        c->f.kind = FK_ABC;
+    }
+    c->have_field = true;
+    return TP_OK;
+}
+
+int tp_gpu_set_velocity_grid(tp_ctx *c, int nx, int ny, int nz, const double lo[3],
+                             const double hi[3], const double *v)
+{
+    if (!c) return TP_E_ARG;
+    HIPCHK(hipSetDevice(c->device));
+    hipFree(c->vgrid);
+    c->vgrid = nullptr;
+    c->f.vgrid = nullptr;
+    if (c->f.kind == FK_ABCV) c->f.kind = FK_ABC;
+    c->s.derived = false;
+    if (!v) return TP_OK;
+    if (nx < 2 || ny < 2 || nz < 2 || !lo || !hi) return TP_E_ARG;
+    size_t bytes = (size_t)nx * ny * nz * 3 * sizeof(double);
+    if (hipMalloc(&c->vgrid, bytes) != hipSuccess) { c->vgrid = nullptr; return TP_E_NOMEM; }
+    HIPCHK(hipMemcpy(c->vgrid, v, bytes, hipMemcpyHostToDevice));
+    c->f.vnx = nx; c->f.vny = ny; c->f.vnz = nz;
+    set_box(c->f.vlo, c->f.vhi, c->f.vd, lo, hi, nx, ny, nz);
+    c->f.vgrid = c->vgrid;
+    if (c->f.kind == FK_ABC) c->f.kind = FK_ABCV;
+    return TP_OK;
+}
+
+int tp_gpu_set_field_grid(tp_ctx *c, int nx, int ny, int nz, const double lo[3],
+                          const double hi[3], const float *ef, const float *bf, int mode)
+{
+    if (!c || nx < 2 || ny < 2 || nz < 2 || !lo || !hi || !ef || !bf) return TP_E_ARG;
+    if (mode != TP_GATHER_REF_COMPAT && mode != TP_GATHER_TEXTBOOK) return TP_E_ARG;
+    HIPCHK(hipSetDevice(c->device));
+    free_field(c);
+    memset(&c->f, 0, sizeof(c->f));
+    size_t nn = (size_t)nx * ny * nz;
+    /* interleave: node k -> float4 {ex,ey,ez,bx}, float4 {by,bz,0,0} */
+    float *h = (float *)malloc(nn * 8 * sizeof(float));
+    if (!h) return TP_E_NOMEM;
+    for (size_t k = 0; k < nn; k++) {
+        float *o = h + 8 * k;
+        o[0] = ef[3 * k]; o[1] = ef[3 * k + 1]; o[2] = ef[3 * k + 2];
+        o[3] = bf[3 * k]; o[4] = bf[3 * k + 1]; o[5] = bf[3 * k + 2];
+        o[6] = 0.f; o[7] = 0.f;
+    }
+    hipError_t e = hipMalloc(&c->grid, nn * 8 * sizeof(float));
+    if (e != hipSuccess) { free(h); c->grid = nullptr; return TP_E_NOMEM; }
+    e = hipMemcpy(c->grid, h, nn * 8 * sizeof(float), hipMemcpyHostToDevice);
+    free(h);
+    if (e != hipSuccess) return TP_E_HIP;
+    c->f.nx = nx; c->f.ny = ny; c->f.nz = nz;
+    set_box(c->f.lo, c->f.hi, c->f.d, lo, hi, nx, ny, nz);
+    c->f.grid = c->grid;
+    c->f.kind = mode == TP_GATHER_REF_COMPAT ? FK_GRID_REF : FK_GRID_TB;
+    c->have_field = true;
+    return TP_OK;
+}
+
+int tp_gpu_set_field_wlcs(tp_ctx *c, int nwlcs, const double *rows, int ibessel,
+                          double omega, double norm_i, int mode)
+{
+    if (!c || nwlcs < 1 || !rows) return TP_E_ARG;
+    if (mode != TP_GATHER_REF_COMPAT && mode != TP_GATHER_TEXTBOOK) return TP_E_ARG;
+    HIPCHK(hipSetDevice(c->device));
+    free_field(c);
+    memset(&c->f, 0, sizeof(c->f));
+    WlcsSys *h = (WlcsSys *)malloc(sizeof(WlcsSys) * nwlcs);
+    if (!h) return TP_E_NOMEM;
+    for (int i = 0; i < nwlcs; i++) {
+        const double *r = rows + 14 * i;
+        WlcsSys &s = h[i];
+        s.cur_wr = r[0] * norm_i;
+        s.x_wr = r[1]; s.y_wr = r[2]; s.z_wr = r[3];
+        tp_sincos(r[4], &s.st, &s.ct);
+        tp_sincos(r[5], &s.sp, &s.cp);
+        s.t0 = r[6];
+        s.cur_lp = r[7] * norm_i;
+        s.r_lp = r[8]; s.x_lp = r[9]; s.y_lp = r[10]; s.z_lp = r[11];
+        tp_sincos(r[12], &s.sa, &s.ca);
+        tp_sincos(r[13], &s.sb, &s.cb);
+        s.omega = omega;
+    }
+    hipError_t e = hipMalloc(&c->wl, sizeof(WlcsSys) * nwlcs);
+    if (e != hipSuccess) { free(h); c->wl = nullptr; return TP_E_NOMEM; }
+    e = hipMemcpy(c->wl, h, sizeof(WlcsSys) * nwlcs, hipMemcpyHostToDevice);
+    free(h);
+    if (e != hipSuccess) return TP_E_HIP;
+    c->f.nwlcs = nwlcs;
+    c->f.ibessel = ibessel;
+    c->f.wlmode = mode == TP_GATHER_REF_COMPAT ? 0 : 1;
+    c->f.wl = c->wl;
+    c->f.kind = FK_WLCS;
+    c->have_field = true;
+    return TP_OK;
+}
+
+int tp_gpu_set_field_modes(tp_ctx *c, int nmodes, const double *rows, double b0z)
+{
+    if (!c || nmodes < 0 || (nmodes > 0 && !rows)) return TP_E_ARG;
+    HIPCHK(hipSetDevice(c->device));
+    free_field(c);
+    memset(&c->f, 0, sizeof(c->f));
+    if (nmodes > 0) {
+        size_t bytes = sizeof(double) * 7 * nmodes;
+        if (hipMalloc(&c->modes, bytes) != hipSuccess) { c->modes = nullptr; return TP_E_NOMEM; }
+        HIPCHK(hipMemcpy(c->modes, rows, bytes, hipMemcpyHostToDevice));
+    }
+    c->f.nmodes = nmodes;
+    c->f.modes = c->modes;
+    c->f.b0z = b0z;
+    c->f.kind = FK_MODES;
+    c->have_field = true;
+    return TP_OK;
+}
+
+int tp_gpu_eval_field(tp_ctx *c, int n, const double *pts, double *out, int prec)
+{
+    if (!c || n < 0 || (n > 0 && (!pts || !out))) return TP_E_ARG;
+    if (!c->have_field) return TP_E_STATE;
+    if (n == 0) return TP_OK;
+    HIPCHK(hipSetDevice(c->device));
+    double *dp = nullptr, *dout = nullptr;
+    if (hipMalloc(&dp, sizeof(double) * 4 * n) != hipSuccess) return TP_E_NOMEM;
+    if (hipMalloc(&dout, sizeof(double) * 6 * n) != hipSuccess) { hipFree(dp); return TP_E_NOMEM; }
+    int rc = TP_OK;
+    if (hipMemcpy(dp, pts, sizeof(double) * 4 * n, hipMemcpyHostToDevice) != hipSuccess) rc = TP_E_HIP;
+    if (rc == TP_OK) {
+        dispatch_eval(field_kind(c), dim3((n + 255) / 256), dim3(256), c->stream, c->f, n, dp,
+                      dout, prec);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess)
+            rc = TP_E_HIP;
+    }
+    if (rc == TP_OK &&
+        hipMemcpy(out, dout, sizeof(double) * 6 * n, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = TP_E_HIP;
+    hipFree(dp);
+    hipFree(dout);
+    return rc;
+}
+
+static int ensure_state(tp_ctx *c, int n)
+{
+    State &S = c->s;
+    if (n <= S.cap) { S.n = n; return TP_OK; }
+    hipFree(S.buf); hipFree(S.ibuf); hipFree(S.stage);
+    S.buf = nullptr; S.ibuf = nullptr; S.stage = nullptr; S.cap = 0;
+    size_t cap = (size_t)n;
+    if (hipMalloc(&S.buf, sizeof(double) * 25 * cap) != hipSuccess) { S.buf = nullptr; return TP_E_NOMEM; }
+    if (hipMalloc(&S.ibuf, sizeof(int) * 2 * cap) != hipSuccess) { S.ibuf = nullptr; return TP_E_NOMEM; }
+    if (hipMalloc(&S.stage, sizeof(tp_particle) * 2 * cap) != hipSuccess) { S.stage = nullptr; return TP_E_NOMEM; }
+    double *b = S.buf;
+    double **arr[20] = {&S.x, &S.y, &S.z, &S.ux, &S.uy, &S.uz, &S.t, &S.xs, &S.ys, &S.zs,
+                        &S.x0, &S.y0, &S.z0, &S.ux0, &S.uy0, &S.uz0, &S.t0, &S.g0, &S.mu0, &S.th0};
+    for (int k = 0; k < 20; k++) *arr[k] = b + (size_t)k * cap;
+    S.st = b + 20 * cap;   /* 5 * cap */
+    S.esc = S.ibuf;
+    S.nsteps = S.ibuf + cap;
+    S.cap = n;
+    S.n = n;
+    return TP_OK;
+}
+
+int tp_gpu_load_particles(tp_ctx *c, int n, const tp_particle *ptl, const tp_particle *ptl_init)
+{
+    if (!c || n < 0 || (n > 0 && (!ptl || !ptl_init))) return TP_E_ARG;
+    if (!c->have_field) return TP_E_STATE;
+    HIPCHK(hipSetDevice(c->device));
+    int rc = ensure_state(c, n);
+    if (rc) return rc;
+    c->s.derived = false;
+    if (n == 0) return TP_OK;
+    HIPCHK(hipMemcpyAsync(c->s.stage, ptl, sizeof(tp_particle) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->s.stage + n, ptl_init, sizeof(tp_particle) * n, hipMemcpyHostToDevice,
+                          c->stream));
+    dispatch_load(field_kind(c), dim3((n + 255) / 256), dim3(256), c->stream, c->f, n,
+                  c->s.stage, c->s.stage + n, c->s);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->s.derived = true;
+    return TP_OK;
+}
+
+static int block_size_for(const tp_ctx *c, int n)
+{
+    /* 256 lanes per block unless that leaves compute units idle */
+    int cu = c->num_cu > 0 ? c->num_cu : 256;
+    if ((n + 255) / 256 >= 2 * cu) return 256;
+    if ((n + 127) / 128 >= 2 * cu) return 128;
+    return 64;
+}
+
+int tp_gpu_run_fixed(tp_ctx *c, const tp_track_params *p)
+{
+    if (!c || !p) return TP_E_ARG;
+    if (!c->have_field || !c->s.derived) return TP_E_STATE;
+    if (p->einterval_t <= 0 || p->dcoeff_interval <= 0 || p->estep < p->sstep ||
+        p->nbins < 1 || p->nt_out < 1 || p->sstep < 0 || !(p->dt > 0.0) ||
+        p->bc_flag < 0 || p->bc_flag > 2)
+        return TP_E_ARG;
+    if (p->pusher != TP_PUSH_WIRZ && p->pusher != TP_PUSH_BORIS) return TP_E_ARG;
+    if (p->precision != TP_FP64 && p->precision != TP_FP32_MIXED) return TP_E_ARG;
+    track_fn fn = pick_track(field_kind(c), p->pusher, p->precision);
+    if (!fn) return TP_E_ARG;
+    HIPCHK(hipSetDevice(c->device));
+    const int n = c->s.n;
+    const int nb = p->nbins, nt = p->nt_out, di = p->dcoeff_interval;
+    const int nd = p->estep / di + 1;
+    /* diagnostics buffers */
+    size_t need = (size_t)nb * nt;
+    if (need > c->diag_cap) {
+        hipFree(c->espect); hipFree(c->espect_esc);
+        c->espect = nullptr; c->espect_esc = nullptr; c->diag_cap = 0;
+        if (hipMalloc(&c->espect, need * sizeof(unsigned)) != hipSuccess ||
+            hipMalloc(&c->espect_esc, need * sizeof(unsigned)) != hipSuccess)
+            return TP_E_NOMEM;
+        c->diag_cap = need;
+    }
+    if ((size_t)nd > c->dco_cap) {
+        hipFree(c->dco); hipFree(c->rdec);
+        c->dco = nullptr; c->rdec = nullptr; c->dco_cap = 0;
+        if (hipMalloc(&c->dco, sizeof(double) * 8 * nd) != hipSuccess ||
+            hipMalloc(&c->rdec, sizeof(int) * (nd + 1)) != hipSuccess)
+            return TP_E_NOMEM;
+        c->dco_cap = nd;
+    }
+    HIPCHK(hipMemsetAsync(c->espect, 0, need * sizeof(unsigned), c->stream));
+    HIPCHK(hipMemsetAsync(c->espect_esc, 0, need * sizeof(unsigned), c->stream));
+    HIPCHK(hipMemsetAsync(c->dco, 0, sizeof(double) * 8 * nd, c->stream));
+    HIPCHK(hipMemsetAsync(c->rdec, 0, sizeof(int) * (nd + 1), c->stream));
+    c->nb = nb; c->nt = nt; c->nd = nd; c->last_n = n;
+    c->last_ms = 0.0;
+    c->last_launches = 0;
+
+    const int bs = block_size_for(c, n);
+    const int nblocks = n > 0 ? (n + bs - 1) / bs : 0;
+    /* chunk of steps per launch: a multiple of the D cadence, bounded so
+     * the per-block partials stay under 64 MiB */
+    long rows_cap = nblocks > 0 ? (64L << 20) / ((long)nblocks * 64) : 1000;
+    if (rows_cap < 2) rows_cap = 2;
+    long chunk = (rows_cap - 1) * (long)di;
+    if (chunk > 2000) chunk = 2000;
+    if (chunk < 1) chunk = 1;
+    size_t pneed = (size_t)(chunk / di + 2) * (size_t)(nblocks > 0 ? nblocks : 1) * 8;
+    if (pneed > c->partials_cap) {
+        hipFree(c->partials);
+        c->partials = nullptr; c->partials_cap = 0;
+        if (hipMalloc(&c->partials, pneed * sizeof(double)) != hipSuccess) return TP_E_NOMEM;
+        c->partials_cap = pneed;
+    }
+    size_t shmem = sizeof(double) * 8 * (bs / 64) + sizeof(unsigned) * nb;
+    if (shmem > 64 * 1024) return TP_E_ARG;
+
+    TrackArgs a;
+    memset(&a, 0, sizeof(a));
+    a.f = c->f;
+    a.n = n; a.bc = p->bc_flag; a.nbins = nb; a.nt_out = nt; a.einterval = p->einterval_t;
+    a.di = di; a.fix_stale = (p->flags & TP_FLAG_FIX_STALE_DCOEFF) ? 1 : 0;
+    a.nd = nd;
+    a.dt = p->dt; a.qm = p->qm;
+    a.lo[0] = p->xmin; a.lo[1] = p->ymin; a.lo[2] = p->zmin;
+    a.hi[0] = p->xmax; a.hi[1] = p->ymax; a.hi[2] = p->zmax;
+    a.logemin = p->logemin; a.logde = p->logde;
+    State &S = c->s;
+    a.x = S.x; a.y = S.y; a.z = S.z; a.ux = S.ux; a.uy = S.uy; a.uz = S.uz; a.t = S.t;
+    a.xs = S.xs; a.ys = S.ys; a.zs = S.zs;
+    a.x0 = S.x0; a.y0 = S.y0; a.z0 = S.z0; a.ux0 = S.ux0; a.uy0 = S.uy0; a.uz0 = S.uz0;
+    a.g0 = S.g0; a.mu0 = S.mu0; a.th0 = S.th0;
+    a.st = S.st; a.esc = S.esc; a.nsteps = S.nsteps;
+    a.espect = c->espect; a.espect_esc = c->espect_esc;
+    a.partials = c->partials; a.rdec = c->rdec;
+
+    /* frame and D-row counters at step sstep (src/tracking.c:317, 328) */
+    int ntp = (int)floor(p->sstep / (double)p->einterval_t);
+    int tstep = 1;
+    float ms_total = 0.f;
+    for (long j0 = p->sstep; j0 <= p->estep && nblocks > 0; j0 += chunk) {
+        long j1 = j0 + chunk - 1;
+        if (j1 > p->estep) j1 = p->estep;
+        a.j0 = (int)j0; a.j1 = (int)j1;
+        a.first = (j0 == p->sstep) ? 1 : 0;
+        a.ntp0 = ntp; a.tstep0 = tstep;
+        HIPCHK(hipEventRecord(c->ev0, c->stream));
+        fn(dim3(nblocks), dim3(bs), shmem, c->stream, a);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(c->ev1, c->stream));
+        /* rows produced by this chunk */
+        int rows = 0;
+        for (long j = j0; j <= j1; j++) {
+            if (j % p->einterval_t == 0) ntp++;
+            if (j % di == 0) rows++;
+        }
+        if (rows > 0) {
+            hipLaunchKernelGGL(k_reduce_rows, dim3(rows * 8), dim3(256), 0, c->stream,
+                               c->partials, nblocks, tstep, nd, c->dco);
+            HIPCHK(hipGetLastError());
+        }
+        tstep += rows;
+        HIPCHK(hipEventSynchronize(c->ev1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        ms_total += ms;
+        c->last_launches++;
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->last_ms = ms_total;
+    return TP_OK;
+}
+
+int tp_gpu_last_kernel_time(tp_ctx *c, double *ms, int *launches)
+{
+    if (!c) return TP_E_ARG;
+    if (ms) *ms = c->last_ms;
+    if (launches) *launches = c->last_launches;
+    return TP_OK;
+}
+
+int tp_gpu_store_particles(tp_ctx *c, tp_particle *ptl, int *nsteps)
+{
+    if (!c) return TP_E_ARG;
+    const int n = c->s.n;
+    if (n == 0) return TP_OK;
+    if (!c->s.buf) return TP_E_STATE;
+    HIPCHK(hipSetDevice(c->device));
+    if (ptl) {
+        State &S = c->s;
+        hipLaunchKernelGGL(k_store, dim3((n + 255) / 256), dim3(256), 0, c->stream, n, S.x, S.y,
+                           S.z, S.ux, S.uy, S.uz, S.t, S.xs, S.ys, S.zs, S.stage);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(ptl, S.stage, sizeof(tp_particle) * n, hipMemcpyDeviceToHost,
+                              c->stream));
+    }
+    if (nsteps)
+        HIPCHK(hipMemcpyAsync(nsteps, c->s.nsteps, sizeof(int) * n, hipMemcpyDeviceToHost,
+                              c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return TP_OK;
+}
+
+int tp_gpu_read_diagnostics(tp_ctx *c, double *espect, double *espect_escape, double *dcoeff,
+                            int *nptl_remain)
+{
+    if (!c) return TP_E_ARG;
+    if (!c->espect || !c->dco) return TP_E_STATE;
+    HIPCHK(hipSetDevice(c->device));
+    const size_t nbt = (size_t)c->nb * c->nt;
+    unsigned *h = (unsigned *)malloc(sizeof(unsigned) * (nbt > 0 ? nbt : 1));
+    int *r = (int *)malloc(sizeof(int) * (c->nd + 1));
+    if (!h || !r) { free(h); free(r); return TP_E_NOMEM; }
+    int rc = TP_OK;
+    if (espect) {
+        if (hipMemcpy(h, c->espect, sizeof(unsigned) * nbt, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = TP_E_HIP;
+        for (size_t k = 0; rc == TP_OK && k < nbt; k++) espect[k] = (double)h[k];
+    }
+    if (rc == TP_OK && espect_escape) {
+        if (hipMemcpy(h, c->espect_esc, sizeof(unsigned) * nbt, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = TP_E_HIP;
+        for (size_t k = 0; rc == TP_OK && k < nbt; k++) espect_escape[k] = (double)h[k];
+    }
+    if (rc == TP_OK && dcoeff &&
+        hipMemcpy(dcoeff, c->dco, sizeof(double) * 8 * c->nd, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = TP_E_HIP;
+    if (rc == TP_OK && nptl_remain) {
+        if (hipMemcpy(r, c->rdec, sizeof(int) * c->nd, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = TP_E_HIP;
+        for (int k = 0; rc == TP_OK && k < c->nd; k++) nptl_remain[k] = c->last_n - r[k];
+    }
+    free(h);
+    free(r);
+    return rc;
+}
+
+int tp_gpu_track_fixed(tp_ctx *c, const tp_track_params *p, int n, tp_particle *ptl,
+                       const tp_particle *ptl_init, int *nsteps, double *espect,
+                       double *espect_escape, double *dcoeff, int *nptl_remain)
+{
+    if (!c || !p || n < 0 || (n > 0 && (!ptl || !ptl_init))) return TP_E_ARG;
+    int rc = tp_gpu_load_particles(c, n, ptl, ptl_init);
+    if (rc) return rc;
+    rc = tp_gpu_run_fixed(c, p);
+    if (rc) return rc;
+    rc = tp_gpu_store_particles(c, ptl, nsteps);
+    if (rc) return rc;
+    rc = tp_gpu_read_diagnostics(c, espect, espect_escape, dcoeff, nptl_remain);
+    if (rc) return rc;
+    if (p->flags & TP_FLAG_CHECK_FINITE) {
+        for (int i = 0; i < n; i++) {
+            const tp_particle &q = ptl[i];
+            if (!isfinite(q.x) || !isfinite(q.y) || !isfinite(q.z) || !isfinite(q.ux) ||
+                !isfinite(q.uy) || !isfinite(q.uz))
+                return TP_E_NONFINITE;
+        }
+    }
+    return TP_OK;
+}
+
+} /* extern "C" */

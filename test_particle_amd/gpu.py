@@ -1,0 +1,191 @@
+"""Python mirror of the C ABI (include/tp_gpu.h) over libtpgpu.so.
+
+Every call goes to the HIP library; there is no CPU fallback.  A missing
+library raises ImportError-like RuntimeError at first use.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import abi
+from .abi import dptr, fptr, iptr
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(abi.LIB_PATH):
+            raise RuntimeError(
+                f"{abi.LIB_PATH} is not built; run `python -m test_particle_amd.build`")
+        L = ctypes.CDLL(abi.LIB_PATH)
+        P = ctypes.POINTER
+        d, i, vp = ctypes.c_double, ctypes.c_int, ctypes.c_void_p
+        L.tp_gpu_init.argtypes = [i, P(vp)]
+        L.tp_gpu_finalize.argtypes = [vp]
+        L.tp_gpu_finalize.restype = None
+        L.tp_gpu_strerror.argtypes = [i]
+        L.tp_gpu_strerror.restype = ctypes.c_char_p
+        L.tp_gpu_device_info.argtypes = [vp, ctypes.c_char_p, i, P(i)]
+        L.tp_gpu_set_field_test.argtypes = [vp]
+        L.tp_gpu_set_field_abc.argtypes = [vp, d, d, d, d, d]
+        L.tp_gpu_set_velocity_grid.argtypes = [vp, i, i, i, P(d), P(d), P(d)]
+        L.tp_gpu_set_field_grid.argtypes = [vp, i, i, i, P(d), P(d), P(ctypes.c_float),
+                                            P(ctypes.c_float), i]
+        L.tp_gpu_set_field_wlcs.argtypes = [vp, i, P(d), i, d, d, i]
+        L.tp_gpu_set_field_modes.argtypes = [vp, i, P(d), d]
+        L.tp_gpu_eval_field.argtypes = [vp, i, P(d), P(d), i]
+        L.tp_gpu_track_fixed.argtypes = [vp, P(abi.TpTrackParams), i, P(d), P(d), P(i),
+                                         P(d), P(d), P(d), P(i)]
+        L.tp_gpu_load_particles.argtypes = [vp, i, P(d), P(d)]
+        L.tp_gpu_run_fixed.argtypes = [vp, P(abi.TpTrackParams)]
+        L.tp_gpu_store_particles.argtypes = [vp, P(d), P(i)]
+        L.tp_gpu_read_diagnostics.argtypes = [vp, P(d), P(d), P(d), P(i)]
+        L.tp_gpu_last_kernel_time.argtypes = [vp, P(d), P(i)]
+        _lib = L
+    return _lib
+
+
+class TPError(RuntimeError):
+    def __init__(self, code, what):
+        super().__init__(f"{what}: {lib().tp_gpu_strerror(code).decode()} ({code})")
+        self.code = code
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise TPError(rc, what)
+
+
+def _vec3(v):
+    return (ctypes.c_double * 3)(*[float(x) for x in v])
+
+
+class Tracker:
+    """One tp_ctx: a field source plus resident particles on one GPU."""
+
+    def __init__(self, device=0):
+        self._ctx = ctypes.c_void_p()
+        _check(lib().tp_gpu_init(device, ctypes.byref(self._ctx)), "tp_gpu_init")
+        self._n = 0
+
+    def close(self):
+        if self._ctx:
+            lib().tp_gpu_finalize(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def device_info(self):
+        buf = ctypes.create_string_buffer(256)
+        cu = ctypes.c_int()
+        _check(lib().tp_gpu_device_info(self._ctx, buf, 256, ctypes.byref(cu)), "device_info")
+        return buf.value.decode(), cu.value
+
+    # --- fields --------------------------------------------------------
+    def set_field_test(self):
+        _check(lib().tp_gpu_set_field_test(self._ctx), "set_field_test")
+
+    def set_field_abc(self, A=1.0, B=np.sqrt(2.0 / 3.0), C=np.sqrt(1.0 / 3.0), lam=1.0,
+                      B0=1.0):
+        _check(lib().tp_gpu_set_field_abc(self._ctx, A, B, C, lam, B0), "set_field_abc")
+
+    def set_velocity_grid(self, v, lo, hi):
+        if v is None:
+            _check(lib().tp_gpu_set_velocity_grid(self._ctx, 0, 0, 0, None, None, None),
+                   "set_velocity_grid")
+            return
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        nz, ny, nx = v.shape[:3]
+        _check(lib().tp_gpu_set_velocity_grid(self._ctx, nx, ny, nz, _vec3(lo), _vec3(hi),
+                                              dptr(v)), "set_velocity_grid")
+
+    def set_field_grid(self, e, b, lo, hi, mode=abi.TP_GATHER_REF_COMPAT):
+        e = np.ascontiguousarray(e, dtype=np.float32)
+        b = np.ascontiguousarray(b, dtype=np.float32)
+        nz, ny, nx = e.shape[:3]
+        _check(lib().tp_gpu_set_field_grid(self._ctx, nx, ny, nz, _vec3(lo), _vec3(hi),
+                                           fptr(e), fptr(b), mode), "set_field_grid")
+
+    def set_field_wlcs(self, rows, ibessel=0, omega=0.001, norm_i=1.0,
+                       mode=abi.TP_GATHER_REF_COMPAT):
+        rows = np.ascontiguousarray(rows, dtype=np.float64).reshape(-1, 14)
+        _check(lib().tp_gpu_set_field_wlcs(self._ctx, rows.shape[0], dptr(rows), ibessel,
+                                           omega, norm_i, mode), "set_field_wlcs")
+
+    def set_field_modes(self, rows, b0z):
+        rows = np.ascontiguousarray(rows, dtype=np.float64).reshape(-1, 7)
+        _check(lib().tp_gpu_set_field_modes(self._ctx, rows.shape[0], dptr(rows), b0z),
+               "set_field_modes")
+
+    def eval_field(self, pts, precision=abi.TP_FP64):
+        pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 4)
+        out = np.zeros((pts.shape[0], 6))
+        _check(lib().tp_gpu_eval_field(self._ctx, pts.shape[0], dptr(pts), dptr(out),
+                                       precision), "eval_field")
+        return out
+
+    # --- tracking --------------------------------------------------------
+    def track_fixed(self, params, ptl, ptl_init=None):
+        """The reference's fixed-step loop (src/tracking.c:90-190) on the
+        GPU; returns the same raw arrays as the oracle."""
+        ptl = abi.as_particles(ptl).copy()
+        ptl_init = ptl.copy() if ptl_init is None else abi.as_particles(ptl_init)
+        n = ptl.shape[0]
+        nd = abi.n_dcoeff_rows(params)
+        nsteps = np.zeros(n, dtype=np.int32)
+        esp = np.zeros((params.nt_out, params.nbins))
+        ese = np.zeros((params.nt_out, params.nbins))
+        dco = np.zeros((8, nd))
+        rem = np.zeros(nd, dtype=np.int32)
+        _check(lib().tp_gpu_track_fixed(self._ctx, ctypes.byref(params), n, dptr(ptl),
+                                        dptr(ptl_init), iptr(nsteps), dptr(esp), dptr(ese),
+                                        dptr(dco), iptr(rem)), "track_fixed")
+        return dict(ptl=ptl, nsteps=nsteps, espect=esp, espect_escape=ese, dcoeff=dco,
+                    remain=rem)
+
+    def load(self, ptl, ptl_init=None):
+        ptl = abi.as_particles(ptl)
+        ptl_init = ptl if ptl_init is None else abi.as_particles(ptl_init)
+        _check(lib().tp_gpu_load_particles(self._ctx, ptl.shape[0], dptr(ptl),
+                                           dptr(ptl_init)), "load_particles")
+        self._n = ptl.shape[0]
+
+    def run(self, params):
+        _check(lib().tp_gpu_run_fixed(self._ctx, ctypes.byref(params)), "run_fixed")
+
+    def store(self):
+        ptl = np.zeros((self._n, abi.NFIELD))
+        nsteps = np.zeros(self._n, dtype=np.int32)
+        _check(lib().tp_gpu_store_particles(self._ctx, dptr(ptl), iptr(nsteps)),
+               "store_particles")
+        return ptl, nsteps
+
+    def diagnostics(self, params):
+        nd = abi.n_dcoeff_rows(params)
+        esp = np.zeros((params.nt_out, params.nbins))
+        ese = np.zeros((params.nt_out, params.nbins))
+        dco = np.zeros((8, nd))
+        rem = np.zeros(nd, dtype=np.int32)
+        _check(lib().tp_gpu_read_diagnostics(self._ctx, dptr(esp), dptr(ese), dptr(dco),
+                                             iptr(rem)), "read_diagnostics")
+        return dict(espect=esp, espect_escape=ese, dcoeff=dco, remain=rem)
+
+    def last_kernel_time(self):
+        ms = ctypes.c_double()
+        k = ctypes.c_int()
+        _check(lib().tp_gpu_last_kernel_time(self._ctx, ctypes.byref(ms), ctypes.byref(k)),
+               "last_kernel_time")
+        return ms.value, k.value

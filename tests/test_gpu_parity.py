@@ -1,0 +1,248 @@
+"""Parity of the HIP kernels (through the C ABI of libtpgpu.so) with the CPU
+oracle on the same seeded inputs.
+
+The oracle runs with the shared tp_sincos (the GPU's sin/cos), so FP64
+trajectories, spectra and counts must agree bit for bit; the diffusion sums
+are reduced in a different order (per-block trees vs a sequential sum) and
+agree to 1e-12 relative."""
+import numpy as np
+import pytest
+
+from conftest import golden
+from test_particle_amd import abi
+from test_particle_amd.particles import init_particles
+
+pytestmark = pytest.mark.gpu
+
+TWO_PI = 2.0 * np.pi
+BOX = ((0.0, 0.0, 0.0), (TWO_PI, TWO_PI, TWO_PI))
+DT_C1 = 0.01 * TWO_PI * (1.0 / np.sqrt(1.0 - 0.49)) / np.sqrt(3.0)
+DCOEFF_RTOL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    from test_particle_amd.gpu import Tracker
+    t = Tracker(0)
+    name, cu = t.device_info()
+    print(f"[gpu] {name}, {cu} CUs", flush=True)
+    yield t
+    t.close()
+
+
+def bitwise(a, b, what=""):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    assert a.shape == b.shape, what
+    same = (a == b) | (np.isnan(a) & np.isnan(b))
+    if not same.all():
+        idx = np.argwhere(~same)
+        i = tuple(idx[0])
+        raise AssertionError(f"{what}: {len(idx)} differ, first {i}: {a[i]!r} vs {b[i]!r}")
+
+
+def close_rel(a, b, rtol, what=""):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    scale = np.maximum(np.abs(b), 1e-300)
+    err = np.abs(a - b) / scale
+    assert np.nanmax(err) <= rtol, f"{what}: max rel err {np.nanmax(err):.3e}"
+
+
+# ---------------------------------------------------------------- fields
+def _pts(n, lo, hi, seed):
+    rng = np.random.default_rng(seed)
+    lo = np.asarray(lo) - 0.3
+    hi = np.asarray(hi) + 0.3
+    p = np.column_stack([rng.uniform(lo[k], hi[k], n) for k in range(3)] + [np.zeros(n)])
+    return p
+
+
+def test_field_abc(gpu, oracle_tp):
+    pts = _pts(4096, *BOX, 1)
+    for coef in [(1.0, np.sqrt(2 / 3), np.sqrt(1 / 3), 1.0, 1.0), (1.0, 1.0, 1.0, 0.7, 1.3)]:
+        gpu.set_field_abc(*coef)
+        ref = oracle_tp.eval_field(oracle_tp.Field.abc(*coef), pts)
+        bitwise(gpu.eval_field(pts), ref, f"abc {coef}")
+
+
+def test_field_abc_velocity_grid(gpu, oracle_tp):
+    g = golden("fields_abc_vgrid")
+    gpu.set_field_abc()
+    gpu.set_velocity_grid(g["vgrid"], g["lo"], g["hi"])
+    f = oracle_tp.Field.abc().with_velocity_grid(g["vgrid"], g["lo"], g["hi"])
+    try:
+        bitwise(gpu.eval_field(g["pts"]), oracle_tp.eval_field(f, g["pts"]), "abc+v")
+    finally:
+        gpu.set_velocity_grid(None, None, None)
+
+
+@pytest.mark.parametrize("mode", [abi.TP_GATHER_REF_COMPAT, abi.TP_GATHER_TEXTBOOK])
+def test_field_grid(gpu, oracle_tp, mode):
+    g = golden("fields_grid")
+    gpu.set_field_grid(g["egrid"], g["bgrid"], g["lo"], g["hi"], mode)
+    f = oracle_tp.Field.grid(g["egrid"], g["bgrid"], g["lo"], g["hi"], mode)
+    pts = np.concatenate([g["pts"], _pts(2000, g["lo"], g["hi"], 2)])
+    bitwise(gpu.eval_field(pts), oracle_tp.eval_field(f, pts), f"grid mode {mode}")
+    if mode == abi.TP_GATHER_REF_COMPAT:
+        # and with the reference binary itself (sin/cos play no part here)
+        bitwise(gpu.eval_field(g["pts"]), g["out"], "grid vs reference")
+
+
+def test_field_modes(gpu, oracle_tp):
+    from test_particle_amd.fields import mode_table
+    rows = mode_table(64, kmin=0.5, kmax=50.0, rms=0.5, seed=3)
+    gpu.set_field_modes(rows, 1.0)
+    pts = _pts(2048, (-50, -50, -50), (50, 50, 50), 3)
+    bitwise(gpu.eval_field(pts), oracle_tp.eval_field(oracle_tp.Field.modes(rows, 1.0), pts),
+            "modes")
+
+
+@pytest.mark.parametrize("ib", [0, 1])
+@pytest.mark.parametrize("mode", [abi.TP_GATHER_REF_COMPAT, abi.TP_GATHER_TEXTBOOK])
+def test_field_wlcs(gpu, oracle_tp, ib, mode):
+    """acos and the float Bessel functions come from the device math library
+    on the GPU and from libm in the oracle: 1e-12 relative to the field scale
+    for the static wire, 2e-6 with the float Bessel functions."""
+    g = golden(f"fields_wlcs_b{ib}")
+    kw = dict(ibessel=ib, omega=float(g["omega"]), norm_i=float(g["norm_i"]), mode=mode)
+    gpu.set_field_wlcs(g["rows"], **kw)
+    out = gpu.eval_field(g["pts"])
+    ref = oracle_tp.eval_field(oracle_tp.Field.wlcs(g["rows"], **kw), g["pts"])
+    scale = np.abs(ref).max(axis=1, keepdims=True)
+    assert (np.isfinite(out) == np.isfinite(ref)).all()
+    ok = np.isfinite(ref)
+    err = (np.abs(out - ref) / scale)[ok]
+    assert err.max() < (1e-12 if ib == 0 else 2e-6)
+
+
+# ---------------------------------------------------------------- tracking
+def _params(estep, nt_out, pusher, lo=BOX[0], hi=BOX[1], dt=DT_C1, bc=0, nbins=102,
+            emin=1e-3, emax=10.0, precision=abi.TP_FP64, **kw):
+    return abi.make_params(dt=dt, estep=estep, lo=lo, hi=hi, pusher=pusher, bc_flag=bc,
+                           nbins=nbins, nt_out=nt_out, emin=emin, emax=emax,
+                           precision=precision, **kw)
+
+
+def _compare_track(r, ref, what):
+    bitwise(r["ptl"], ref["ptl"], what + " particles")
+    bitwise(r["nsteps"], ref["nsteps"], what + " nsteps")
+    bitwise(r["espect"], ref["espect"], what + " spectrum")
+    bitwise(r["espect_escape"], ref["espect_escape"], what + " escape spectrum")
+    bitwise(r["remain"], ref["remain"], what + " remaining")
+    close_rel(r["dcoeff"], ref["dcoeff"], DCOEFF_RTOL, what + " D sums")
+
+
+@pytest.mark.parametrize("pusher", [abi.TP_PUSH_WIRZ, abi.TP_PUSH_BORIS])
+def test_track_abc_c1_shape(gpu, oracle_tp, pusher):
+    """C1 geometry (ABC A=B=C=1, |v|=0.7c, dt=0.01 T_g) on 1000 particles,
+    2000 steps: chaotic orbits stay bitwise equal to the oracle."""
+    print(f"[case] abc pusher={pusher} n=1000 steps=2000", flush=True)
+    ptl = init_particles(1000, *BOX, 0.7, seed=21)
+    prm = _params(2000, 20, pusher)
+    gpu.set_field_abc(1.0, 1.0, 1.0)
+    r = gpu.track_fixed(prm, ptl)
+    ref = oracle_tp.track_fixed(oracle_tp.Field.abc(1.0, 1.0, 1.0), prm, ptl, nthreads=1)
+    _compare_track(r, ref, f"abc pusher {pusher}")
+    assert r["espect"].sum() == 20 * 1000
+
+
+def test_track_open_boundary(gpu, oracle_tp):
+    g = golden("track_open")
+    import json
+    spec = json.loads(str(g["spec"]))
+    prm = _params(int(spec["tmax"] / spec["dt"]), spec["nt_out"], abi.TP_PUSH_WIRZ,
+                  lo=(spec["xmin"], spec["ymin"], spec["zmin"]),
+                  hi=(spec["xmax"], spec["ymax"], spec["zmax"]), dt=spec["dt"], bc=1,
+                  nbins=spec["nbins"], emin=spec["emin"], emax=spec["emax"])
+    gpu.set_field_test()
+    r = gpu.track_fixed(prm, g["ptl_in"])
+    ref = oracle_tp.track_fixed(oracle_tp.Field.test(), prm, g["ptl_in"], nthreads=1)
+    _compare_track(r, ref, "open bc")
+    assert r["espect_escape"].sum() > 0
+
+
+@pytest.mark.parametrize("mode", [abi.TP_GATHER_REF_COMPAT, abi.TP_GATHER_TEXTBOOK])
+def test_track_grid(gpu, oracle_tp, mode):
+    g = golden("track_grid")
+    ptl = init_particles(700, g["lo"], g["hi"], 0.3, seed=22)   # ragged: 700 lanes
+    prm = _params(600, 3, abi.TP_PUSH_BORIS, lo=g["lo"], hi=g["hi"], dt=0.01, nbins=50,
+                  emin=1e-3, emax=1.0)
+    gpu.set_field_grid(g["egrid"], g["bgrid"], g["lo"], g["hi"], mode)
+    r = gpu.track_fixed(prm, ptl)
+    f = oracle_tp.Field.grid(g["egrid"], g["bgrid"], g["lo"], g["hi"], mode)
+    ref = oracle_tp.track_fixed(f, prm, ptl, nthreads=1)
+    _compare_track(r, ref, f"grid mode {mode}")
+
+
+def test_track_reference_golden_loop(gpu):
+    """The GPU run of the reference's own loop fixture (Wirz, ABC, 256
+    particles, 1000 steps).  GPU sin/cos differ from libm by < 2 ulp, so
+    orbits agree to 1e-9 over this chaotic horizon and the spectra exactly."""
+    g = golden("track_abc1")
+    import json
+    spec = json.loads(str(g["spec"]))
+    prm = _params(int(spec["tmax"] / spec["dt"]), spec["nt_out"], abi.TP_PUSH_WIRZ,
+                  dt=spec["dt"])
+    gpu.set_field_abc(1.0, 1.0, 1.0)
+    r = gpu.track_fixed(prm, g["ptl_in"])
+    pos = r["ptl"][:, :3] + r["ptl"][:, 8:11]
+    pos_ref = g["ptl_out"][:, :3] + g["ptl_out"][:, 8:11]
+    assert np.abs(pos - pos_ref).max() < 1e-9 * np.abs(pos_ref).max()
+    close_rel(r["dcoeff"][:, 1:], g["dcoeff"][:, 1:], 1e-8, "D vs reference")
+    assert (r["espect"].sum(axis=1) == g["espect"].sum(axis=1)).all()
+
+
+def test_track_edge_cases(gpu, oracle_tp):
+    gpu.set_field_abc(1.0, 1.0, 1.0)
+    f = oracle_tp.Field.abc(1.0, 1.0, 1.0)
+    # one particle; spectrum cadence of 1 (the reference starts at frame 1
+    # and would overflow its rows: frames past nt_out are dropped)
+    ptl = init_particles(1, *BOX, 0.7, seed=23)
+    prm = _params(150, 100, abi.TP_PUSH_BORIS, einterval_t=1)
+    _compare_track(gpu.track_fixed(prm, ptl),
+                   oracle_tp.track_fixed(f, prm, ptl, nthreads=1), "n=1")
+    # empty input
+    empty = np.zeros((0, 11))
+    r = gpu.track_fixed(_params(100, 1, abi.TP_PUSH_BORIS), empty)
+    assert r["ptl"].shape == (0, 11) and r["espect"].sum() == 0
+    # estep not a multiple of the D cadence or of nt_out; 65 lanes
+    ptl = init_particles(65, *BOX, 0.7, seed=24)
+    prm = _params(437, 7, abi.TP_PUSH_WIRZ)
+    _compare_track(gpu.track_fixed(prm, ptl),
+                   oracle_tp.track_fixed(f, prm, ptl, nthreads=1), "ragged steps")
+
+
+def test_track_resident_split_calls(gpu, oracle_tp):
+    """load / run / store / read_diagnostics give what one call gives."""
+    ptl = init_particles(300, *BOX, 0.7, seed=25)
+    prm = _params(500, 5, abi.TP_PUSH_BORIS)
+    gpu.set_field_abc(1.0, 1.0, 1.0)
+    one = gpu.track_fixed(prm, ptl)
+    gpu.load(ptl)
+    gpu.run(prm)
+    p2, ns = gpu.store()
+    d = gpu.diagnostics(prm)
+    bitwise(p2, one["ptl"], "resident particles")
+    bitwise(ns, one["nsteps"], "resident nsteps")
+    bitwise(d["espect"], one["espect"], "resident spectrum")
+    bitwise(d["dcoeff"], one["dcoeff"], "resident D")
+    ms, launches = gpu.last_kernel_time()
+    assert ms > 0 and launches >= 1
+
+
+def test_fp32_mixed_precision_form(gpu):
+    """FP32 Boris + FP32 ABC with FP64 positions against the FP64 GPU run on
+    identical inputs (itself bitwise the oracle): D_xx within 1 % and the
+    per-particle energy drift below 1e-4 over 2000 steps."""
+    ptl = init_particles(4096, *BOX, 0.7, seed=26)
+    gpu.set_field_abc(1.0, 1.0, 1.0)
+    r64 = gpu.track_fixed(_params(2000, 20, abi.TP_PUSH_BORIS), ptl)
+    r32 = gpu.track_fixed(_params(2000, 20, abi.TP_PUSH_BORIS, precision=abi.TP_FP32_MIXED),
+                          ptl)
+    dxx64 = r64["dcoeff"][1, 1:] / r64["remain"][1:]
+    dxx32 = r32["dcoeff"][1, 1:] / r32["remain"][1:]
+    assert np.abs(dxx32 / dxx64 - 1).max() < 0.01
+    g0 = np.sqrt(1 + (ptl[:, 4:7] ** 2).sum(1))
+    g32 = np.sqrt(1 + (r32["ptl"][:, 4:7] ** 2).sum(1))
+    assert np.abs(g32 / g0 - 1).max() < 1e-4
