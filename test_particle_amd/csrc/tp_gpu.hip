@@ -741,7 +741,10 @@ int tp_gpu_load_particles(tp_ctx *c, int n, const tp_particle *ptl, const tp_par
     int rc = ensure_state(c, n);
     if (rc) return rc;
     c->s.derived = false;
-    if (n == 0) return TP_OK;
+    if (n == 0) {
+        c->s.derived = true;
+        return TP_OK;
+    }
     HIPCHK(hipMemcpyAsync(c->s.stage, ptl, sizeof(tp_particle) * n, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->s.stage + n, ptl_init, sizeof(tp_particle) * n, hipMemcpyHostToDevice,
                           c->stream));
