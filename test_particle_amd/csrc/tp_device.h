@@ -78,7 +78,7 @@ __device__ __forceinline__ int tp_floor_to_int(double v)
  * index clamped into the array. */
 __device__ __forceinline__ void trilinear_ref(int nx, int ny, int nz,
         const double *lo, const double *hi, const double *dd,
-        double x, double y, double z, long *c, double *w)
+        double x, double y, double z, unsigned *c, double *w, bool *clamped = nullptr)
 {
     double qx = (x - lo[0]) / dd[0];
     double qy = (y - lo[1]) / dd[1];
@@ -105,8 +105,13 @@ __device__ __forceinline__ void trilinear_ref(int nx, int ny, int nz,
     a[5] = ix2 + sx * iy1 + sxy * iz2;
     a[6] = ix1 + sx * iy2 + sxy * iz2;
     a[7] = ix2 + sx * iy2 + sxy * iz2;
+    bool cl = false;
 #pragma unroll
-    for (int k = 0; k < 8; k++) c[k] = a[k] < 0 ? 0 : (a[k] >= total ? total - 1 : a[k]);
+    for (int k = 0; k < 8; k++) {
+        cl |= (a[k] < 0) | (a[k] >= total);
+        c[k] = (unsigned)(a[k] < 0 ? 0 : (a[k] >= total ? total - 1 : a[k]));
+    }
+    if (clamped) *clamped = cl;
     double gx = 1.0 - fx, gy = 1.0 - fy, gz = 1.0 - fz;
     w[0] = gx * gy * gz;
     w[1] = fx * gy * gz;
@@ -121,7 +126,7 @@ __device__ __forceinline__ void trilinear_ref(int nx, int ny, int nz,
 /* textbook trilinear: fold by the period, corner ix+1 weighted by fx */
 __device__ __forceinline__ void trilinear_tb(int nx, int ny, int nz,
         const double *lo, const double *hi, const double *dd,
-        double x, double y, double z, long *c, double *w)
+        double x, double y, double z, unsigned *c, double *w)
 {
     double p[3] = {x, y, z};
     int n[3] = {nx, ny, nz};
@@ -138,8 +143,8 @@ __device__ __forceinline__ void trilinear_tb(int nx, int ny, int nz,
         i1[a] = i;
         f[a] = q / dd[a] - i;
     }
-    long sx = nx, sxy = (long)nx * ny;
-    long b = i1[0] + sx * i1[1] + sxy * i1[2];
+    unsigned sx = nx, sxy = (unsigned)nx * ny;
+    unsigned b = i1[0] + sx * i1[1] + sxy * i1[2];
     c[0] = b;
     c[1] = b + sxy;
     c[2] = b + sx;
@@ -158,6 +163,103 @@ __device__ __forceinline__ void trilinear_tb(int nx, int ny, int nz,
     w[5] = fx * gy * fz;
     w[6] = gx * fy * fz;
     w[7] = fx * fy * fz;
+}
+
+/* ------------------------------------------------------------------ */
+/* register-resident cell cache for the grid gather                    */
+/* ------------------------------------------------------------------ */
+/* A particle moves ~0.06 cells per step, so it gathers from the same eight
+ * corners for ~10 steps in a row.  The eight corners' six components (48
+ * floats) stay in VGPRs and are reloaded only when the corner set changes.
+ * The corner set is identified by the flat indices of corners 0 and 7:
+ * c7 - c0 = dx + nx dy + nx ny dz with each d either 1 or 1 - n (a wrap
+ * cell), and these eight sums are distinct, so (c0, c7) fixes all eight
+ * corners.  Cells whose indices were clamped are never reused.  The
+ * gathered values are the same floats either way, so results are
+ * bit-identical to the uncached gather. */
+#define TP_GC_INVALID 0xFFFFFFFFu
+
+struct GridCache {
+    unsigned k0, k7;
+    float v[48];
+};
+
+__device__ __forceinline__ void gc_init(GridCache &gc) { gc.k0 = TP_GC_INVALID; gc.k7 = 0u; }
+
+__device__ __forceinline__ void gc_fill(const FieldDev &f, GridCache &gc, const unsigned *c,
+                                        bool clamped)
+{
+    if (c[0] != gc.k0 || c[7] != gc.k7) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const float4 p = f.grid[2 * (size_t)c[k]];
+            const float2 q = *(const float2 *)(f.grid + 2 * (size_t)c[k] + 1);
+            gc.v[6 * k + 0] = p.x; gc.v[6 * k + 1] = p.y; gc.v[6 * k + 2] = p.z;
+            gc.v[6 * k + 3] = p.w; gc.v[6 * k + 4] = q.x; gc.v[6 * k + 5] = q.y;
+        }
+        gc.k0 = clamped ? TP_GC_INVALID : c[0];
+        gc.k7 = c[7];
+    }
+}
+
+template <int FK>
+__device__ __forceinline__ void grid_emf64_cached(const FieldDev &f, GridCache &gc, double x,
+                                                  double y, double z, Emf &o)
+{
+    unsigned c[8];
+    double w[8];
+    bool cl = false;
+    if (FK == FK_GRID_REF) {
+        trilinear_ref(f.nx, f.ny, f.nz, f.lo, f.hi, f.d, x, y, z, c, w, &cl);
+        gc_fill(f, gc, c, cl);
+        /* float accumulators rounded after every corner (src/magnetic_field.c:603-608) */
+        float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f, a4 = 0.f, a5 = 0.f;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            a0 = (float)((double)a0 + (double)gc.v[6 * k + 0] * w[k]);
+            a1 = (float)((double)a1 + (double)gc.v[6 * k + 1] * w[k]);
+            a2 = (float)((double)a2 + (double)gc.v[6 * k + 2] * w[k]);
+            a3 = (float)((double)a3 + (double)gc.v[6 * k + 3] * w[k]);
+            a4 = (float)((double)a4 + (double)gc.v[6 * k + 4] * w[k]);
+            a5 = (float)((double)a5 + (double)gc.v[6 * k + 5] * w[k]);
+        }
+        o.ex = a0; o.ey = a1; o.ez = a2; o.bx = a3; o.by = a4; o.bz = a5;
+    } else {
+        trilinear_tb(f.nx, f.ny, f.nz, f.lo, f.hi, f.d, x, y, z, c, w);
+        gc_fill(f, gc, c, false);
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0, a5 = 0.0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            a0 = a0 + (double)gc.v[6 * k + 0] * w[k];
+            a1 = a1 + (double)gc.v[6 * k + 1] * w[k];
+            a2 = a2 + (double)gc.v[6 * k + 2] * w[k];
+            a3 = a3 + (double)gc.v[6 * k + 3] * w[k];
+            a4 = a4 + (double)gc.v[6 * k + 4] * w[k];
+            a5 = a5 + (double)gc.v[6 * k + 5] * w[k];
+        }
+        o.ex = a0; o.ey = a1; o.ez = a2; o.bx = a3; o.by = a4; o.bz = a5;
+    }
+}
+
+template <int FK>
+__device__ __forceinline__ void grid_emf32_cached(const FieldDev &f, GridCache &gc, double x,
+                                                  double y, double z, EmfF &o)
+{
+    unsigned c[8];
+    double w[8];
+    bool cl = false;
+    if (FK == FK_GRID_REF) trilinear_ref(f.nx, f.ny, f.nz, f.lo, f.hi, f.d, x, y, z, c, w, &cl);
+    else trilinear_tb(f.nx, f.ny, f.nz, f.lo, f.hi, f.d, x, y, z, c, w);
+    gc_fill(f, gc, c, cl);
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f, a4 = 0.f, a5 = 0.f;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        float wk = (float)w[k];
+        a0 = fmaf(gc.v[6 * k + 0], wk, a0); a1 = fmaf(gc.v[6 * k + 1], wk, a1);
+        a2 = fmaf(gc.v[6 * k + 2], wk, a2); a3 = fmaf(gc.v[6 * k + 3], wk, a3);
+        a4 = fmaf(gc.v[6 * k + 4], wk, a4); a5 = fmaf(gc.v[6 * k + 5], wk, a5);
+    }
+    o.ex = a0; o.ey = a1; o.ez = a2; o.bx = a3; o.by = a4; o.bz = a5;
 }
 
 /* ------------------------------------------------------------------ */
@@ -326,7 +428,7 @@ __device__ __forceinline__ void get_emf64(const FieldDev &f, double x, double y,
         if (FK == FK_ABCV) {
             /* gete_ff, src/force_free.c:78-86, with the double velocity
              * gather of src/velocity_field.c:555-584 */
-            long c[8];
+            unsigned c[8];
             double w[8];
             trilinear_ref(f.vnx, f.vny, f.vnz, f.vlo, f.vhi, f.vd, x, y, z, c, w);
             double vx = 0.0, vy = 0.0, vz = 0.0;
@@ -346,7 +448,7 @@ __device__ __forceinline__ void get_emf64(const FieldDev &f, double x, double y,
     } else if (FK == FK_GRID_REF) {
         /* getemf_pic_test_particle, src/emfields.c:153-166: float
          * accumulators rounded after every corner */
-        long c[8];
+        unsigned c[8];
         double w[8];
         trilinear_ref(f.nx, f.ny, f.nz, f.lo, f.hi, f.d, x, y, z, c, w);
         float ax = 0.f, ay = 0.f, az = 0.f, bx = 0.f, by = 0.f, bz = 0.f;
@@ -364,7 +466,7 @@ __device__ __forceinline__ void get_emf64(const FieldDev &f, double x, double y,
         o.ex = ax; o.ey = ay; o.ez = az;
         o.bx = bx; o.by = by; o.bz = bz;
     } else if (FK == FK_GRID_TB) {
-        long c[8];
+        unsigned c[8];
         double w[8];
         trilinear_tb(f.nx, f.ny, f.nz, f.lo, f.hi, f.d, x, y, z, c, w);
         double ax = 0.0, ay = 0.0, az = 0.0, bx = 0.0, by = 0.0, bz = 0.0;
@@ -422,7 +524,7 @@ __device__ __forceinline__ void get_emf32(const FieldDev &f, double x, double y,
     } else if (FK == FK_GRID_TB || FK == FK_GRID_REF) {
         /* float weights and accumulation; the weight convention follows
          * the field's gather mode */
-        long c[8];
+        unsigned c[8];
         double w[8];
         if (FK == FK_GRID_REF)
             trilinear_ref(f.nx, f.ny, f.nz, f.lo, f.hi, f.d, x, y, z, c, w);
@@ -460,6 +562,23 @@ __device__ __forceinline__ void get_emf32(const FieldDev &f, double x, double y,
     }
 }
 
+/* field evaluation inside the step loop: grids go through the cell cache */
+template <int FK>
+__device__ __forceinline__ void emf64_step(const FieldDev &f, GridCache &gc, double x,
+                                           double y, double z, double t, Emf &o)
+{
+    if (FK == FK_GRID_REF || FK == FK_GRID_TB) grid_emf64_cached<FK>(f, gc, x, y, z, o);
+    else get_emf64<FK>(f, x, y, z, t, o);
+}
+
+template <int FK>
+__device__ __forceinline__ void emf32_step(const FieldDev &f, GridCache &gc, double x,
+                                           double y, double z, double t, EmfF &o)
+{
+    if (FK == FK_GRID_REF || FK == FK_GRID_TB) grid_emf32_cached<FK>(f, gc, x, y, z, o);
+    else get_emf32<FK>(f, x, y, z, t, o);
+}
+
 /* ------------------------------------------------------------------ */
 /* pushers                                                             */
 /* ------------------------------------------------------------------ */
@@ -469,11 +588,11 @@ struct Ptl {
 
 /* tracking_wirz, src/tracking.c:749-892, in the reference's order */
 template <int FK>
-__device__ __forceinline__ void wirz_step64(const FieldDev &f, Ptl &p, double dt,
-                                            double qm)
+__device__ __forceinline__ void wirz_step64(const FieldDev &f, GridCache &gc, Ptl &p,
+                                            double dt, double qm)
 {
     Emf e;
-    get_emf64<FK>(f, p.x, p.y, p.z, p.t, e);
+    emf64_step<FK>(f, gc, p.x, p.y, p.z, p.t, e);
     double Btot = sqrt(e.bx * e.bx + e.by * e.by + e.bz * e.bz);
     double iB = 1.0 / Btot;
     double dth = dt * 0.5;
@@ -503,7 +622,7 @@ __device__ __forceinline__ void wirz_step64(const FieldDev &f, Ptl &p, double dt
     double pry = p.y + hy * t1 + py * t2 + ry * t3;
     double prz = p.z + hz * t1 + pz * t2 + rz * t3;
 
-    get_emf64<FK>(f, prx, pry, prz, p.t + dth, e);
+    emf64_step<FK>(f, gc, prx, pry, prz, p.t + dth, e);
     Btot = sqrt(e.bx * e.bx + e.by * e.by + e.bz * e.bz);
     iB = 1.0 / Btot;
     hx = e.bx * iB; hy = e.by * iB; hz = e.bz * iB;
@@ -542,11 +661,11 @@ __device__ __forceinline__ void wirz_step64(const FieldDev &f, Ptl &p, double dt
 
 /* relativistic Boris (same operations as the oracle's tpref_boris_step) */
 template <int FK>
-__device__ __forceinline__ void boris_step64(const FieldDev &f, Ptl &p, double dt,
-                                             double qm)
+__device__ __forceinline__ void boris_step64(const FieldDev &f, GridCache &gc, Ptl &p,
+                                             double dt, double qm)
 {
     Emf e;
-    get_emf64<FK>(f, p.x, p.y, p.z, p.t, e);
+    emf64_step<FK>(f, gc, p.x, p.y, p.z, p.t, e);
     double h = qm * (0.5 * dt);
     double umx = p.ux + h * e.ex;
     double umy = p.uy + h * e.ey;
@@ -577,12 +696,12 @@ __device__ __forceinline__ void boris_step64(const FieldDev &f, Ptl &p, double d
 
 /* Boris in FP32 with FP64 position and time accumulators */
 template <int FK>
-__device__ __forceinline__ void boris_step32(const FieldDev &f, Ptl &p,
+__device__ __forceinline__ void boris_step32(const FieldDev &f, GridCache &gc, Ptl &p,
                                              float &ux, float &uy, float &uz,
                                              double dt, float dtf, float hf)
 {
     EmfF e;
-    get_emf32<FK>(f, p.x, p.y, p.z, p.t, e);
+    emf32_step<FK>(f, gc, p.x, p.y, p.z, p.t, e);
     float umx = fmaf(hf, e.ex, ux);
     float umy = fmaf(hf, e.ey, uy);
     float umz = fmaf(hf, e.ez, uz);

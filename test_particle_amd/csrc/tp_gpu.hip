@@ -63,6 +63,8 @@ struct tp_ctx {
     double last_ms = 0.0;
     int last_launches = 0;
     int num_cu = 0;
+    hipEvent_t *evs = nullptr;   /* start/stop per step-kernel launch */
+    int nev = 0;
 };
 
 #define HIPCHK(x)                                                        \
@@ -78,15 +80,8 @@ struct tp_ctx {
 /* ------------------------------------------------------------------ */
 /* device helpers                                                      */
 /* ------------------------------------------------------------------ */
-__device__ __forceinline__ double wave_sum(double v)
-{
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
 /* ptl_energy_fixed binning, src/diagnostics.c:193-203 */
-__device__ __forceinline__ int energy_bin(double ux, double uy, double uz,
+__device__ __noinline__ int energy_bin(double ux, double uy, double uz,
                                           int nbins, double logemin, double logde)
 {
     double gama = sqrt(1.0 + ux * ux + uy * uy + uz * uz);
@@ -102,7 +97,7 @@ __device__ __forceinline__ int energy_bin(double ux, double uy, double uz,
 struct TrackArgs {
     FieldDev f;
     int n, bc, nbins, nt_out, einterval, di, fix_stale, first;
-    int j0, j1, ntp0, tstep0, nd;
+    int j0, j1, ntp0, nd;
     double dt, qm, lo[3], hi[3], logemin, logde;
     double *x, *y, *z, *ux, *uy, *uz, *t, *xs, *ys, *zs;
     const double *x0, *y0, *z0, *ux0, *uy0, *uz0, *g0, *mu0, *th0;
@@ -134,59 +129,29 @@ __device__ __forceinline__ void particle_bc(const TrackArgs &a, Ptl &p, double &
     }
 }
 
-/* calc_diff_coeff, src/diagnostics.c:659-705.  The initial-particle terms
- * (gamma_init, mu_init, theta_init) were evaluated once by k_derived with
- * the same operations. */
-template <int FK>
-__device__ __forceinline__ void diff_terms(const TrackArgs &a, int i, const Ptl &p,
-        double xs, double ys, double zs, double &drr, double &dxx, double &dyy,
-        double &dzz, double &dpp, double &duu, double &daa, double &dee)
-{
-    double t = p.t * 2 + a.dt;
-    double q;
-    q = a.x0[i] - (p.x + xs); dxx = q * q / t;
-    q = a.y0[i] - (p.y + ys); dyy = q * q / t;
-    q = a.z0[i] - (p.z + zs); dzz = q * q / t;
-    drr = dxx + dyy + dzz;
-    double ax = a.ux0[i] - p.ux, ay = a.uy0[i] - p.uy, az = a.uz0[i] - p.uz;
-    dpp = (ax * ax + ay * ay + az * az) / t;
-    double g = sqrt(1.0 + p.ux * p.ux + p.uy * p.uy + p.uz * p.uz);
-    q = g - a.g0[i]; dee = q * q / t;
-    Emf e;
-    get_emf64<FK>(a.f, p.x, p.y, p.z, p.t, e);
-    double ab = sqrt(e.bx * e.bx + e.by * e.by + e.bz * e.bz);
-    double au = sqrt(p.ux * p.ux + p.uy * p.uy + p.uz * p.uz);
-    double mu = (e.bx * p.ux + e.by * p.uy + e.bz * p.uz) / (ab * au);
-    q = mu - a.mu0[i]; duu = q * q / t;
-    double th = acos(mu) * 180 / TP_PI;
-    q = th - a.th0[i]; daa = q * q / t;
-}
-
 /* ------------------------------------------------------------------ */
-/* tracking kernel                                                     */
+/* step kernel: steps j0..j1 of every particle                         */
 /* ------------------------------------------------------------------ */
+/* One launch covers a run of steps that ends at a diffusion-coefficient
+ * step (or at estep); k_dcoeff then evaluates the D terms on the state
+ * after that step.  The spectrum (any cadence) and the escape spectrum are
+ * binned here. */
 template <int PUSH, int FK, int PREC>
 __global__ void __launch_bounds__(256) k_track(TrackArgs a)
 {
-    extern __shared__ double smem_d[];
-    double *wsum = smem_d;                                /* [nwaves][8] */
-    unsigned *hist = (unsigned *)(smem_d + 8 * (blockDim.x / 64));
+    extern __shared__ unsigned hist[];
     const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
     const int i = blockIdx.x * blockDim.x + tid;
     const bool live = i < a.n;
     for (int b = tid; b < a.nbins; b += blockDim.x) hist[b] = 0u;
 
     Ptl p;
     double xs = 0.0, ys = 0.0, zs = 0.0;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;  /* dxx dyy dzz daa dee */
     int esc = 1, nst = 0;
     if (live) {
         p.x = a.x[i]; p.y = a.y[i]; p.z = a.z[i];
         p.ux = a.ux[i]; p.uy = a.uy[i]; p.uz = a.uz[i]; p.t = a.t[i];
         xs = a.xs[i]; ys = a.ys[i]; zs = a.zs[i];
-        s0 = a.st[i]; s1 = a.st[a.n + i]; s2 = a.st[2 * a.n + i];
-        s3 = a.st[3 * a.n + i]; s4 = a.st[4 * a.n + i];
         esc = a.esc[i];
         nst = a.nsteps[i];
         if (a.first) {
@@ -202,26 +167,23 @@ __global__ void __launch_bounds__(256) k_track(TrackArgs a)
     float fux = (float)p.ux, fuy = (float)p.uy, fuz = (float)p.uz;
     const float dtf = (float)a.dt;
     const float hf = (float)(a.qm * (0.5 * a.dt));
+    GridCache gc;
+    gc_init(gc);
     __syncthreads();
 
-    int ntp = a.ntp0, tstep = a.tstep0;
+    int ntp = a.ntp0;
     for (int j = a.j0; j <= a.j1; j++) {
-        double d0 = 0.0, d4 = 0.0, d5 = 0.0;     /* drr dpp duu */
-        if (a.fix_stale) { s0 = s1 = s2 = s3 = s4 = 0.0; }
         int esc_after = esc;
-        const bool do_d = (j % a.di) == 0;
         if (live && (esc == 0 || a.bc == 0)) {
             if (PREC == PREC64) {
-                if (PUSH == TP_PUSH_WIRZ) wirz_step64<FK>(a.f, p, a.dt, a.qm);
-                else boris_step64<FK>(a.f, p, a.dt, a.qm);
+                if (PUSH == TP_PUSH_WIRZ) wirz_step64<FK>(a.f, gc, p, a.dt, a.qm);
+                else boris_step64<FK>(a.f, gc, p, a.dt, a.qm);
             } else {
-                boris_step32<FK>(a.f, p, fux, fuy, fuz, a.dt, dtf, hf);
+                boris_step32<FK>(a.f, gc, p, fux, fuy, fuz, a.dt, dtf, hf);
                 p.ux = fux; p.uy = fuy; p.uz = fuz;
             }
             particle_bc(a, p, xs, ys, zs, esc_after);
             nst++;
-            if (esc_after == 0 && do_d)
-                diff_terms<FK>(a, i, p, xs, ys, zs, d0, s0, s1, s2, d4, d5, s3, s4);
         }
         if (live && esc == 0 && esc_after == 1 && a.bc == 1 && ntp < a.nt_out)
             atomicAdd(&a.espect_esc[ntp * a.nbins +
@@ -242,57 +204,110 @@ __global__ void __launch_bounds__(256) k_track(TrackArgs a)
             __syncthreads();
             ntp++;
         }
-        if (do_d) {
-            double v[8];
-            v[0] = live ? d0 : 0.0; v[1] = live ? s0 : 0.0; v[2] = live ? s1 : 0.0;
-            v[3] = live ? s2 : 0.0; v[4] = live ? d4 : 0.0; v[5] = live ? d5 : 0.0;
-            v[6] = live ? s3 : 0.0; v[7] = live ? s4 : 0.0;
-#pragma unroll
-            for (int k = 0; k < 8; k++) {
-                double r = wave_sum(v[k]);
-                if (lane == 0) wsum[wave * 8 + k] = r;
-            }
-            __syncthreads();
-            if (tid < 8) {
-                double r = 0.0;
-                for (int w = 0; w < nwaves; w++) r += wsum[w * 8 + tid];
-                a.partials[((size_t)(tstep - a.tstep0) * gridDim.x + blockIdx.x) * 8 + tid] = r;
-            }
-            __syncthreads();
-            tstep++;
-            unsigned long long m = __ballot(live && esc_after);
-            if (lane == 0 && m && tstep < a.nd) atomicAdd(&a.rdec[tstep], __popcll(m));
-        }
         esc = esc_after;
     }
     if (live) {
         a.x[i] = p.x; a.y[i] = p.y; a.z[i] = p.z;
         a.ux[i] = p.ux; a.uy[i] = p.uy; a.uz[i] = p.uz; a.t[i] = p.t;
         a.xs[i] = xs; a.ys[i] = ys; a.zs[i] = zs;
-        a.st[i] = s0; a.st[a.n + i] = s1; a.st[2 * a.n + i] = s2;
-        a.st[3 * a.n + i] = s3; a.st[4 * a.n + i] = s4;
         a.esc[i] = esc;
         a.nsteps[i] = nst;
     }
 }
 
-/* Sum the per-block partials of `rows` D rows in a fixed order:
- * block (row, component), strided per-thread sums then an LDS tree. */
-__global__ void __launch_bounds__(256) k_reduce_rows(const double *partials,
-        int nblocks, int tstep0, int nd, double *dco)
+/* ------------------------------------------------------------------ */
+/* diffusion terms at a D step (calc_diff_coeff, src/diagnostics.c:659-705,
+ * summed as src/tracking.c:347-369 does)                              */
+/* ------------------------------------------------------------------ */
+/* Particles inside the box get fresh terms; escaped ones contribute
+ * drr = dpp = duu = 0 and their last (stale) dxx dyy dzz daa dee, as the
+ * reference's loop-private variables do (src/tracking.c:334-336).  The
+ * initial-particle terms gamma_init, mu_init, theta_init were evaluated by
+ * k_load with the same operations.  Each block writes one row of 8 partial
+ * sums (fixed-order wave and LDS reduction); escaped particles decrement
+ * the remaining count of the next row. */
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <int FK>
+__global__ void __launch_bounds__(256) k_dcoeff(TrackArgs a, int tstep)
+{
+    __shared__ double wsum[4][8];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
+    const int i = blockIdx.x * blockDim.x + tid;
+    const bool live = i < a.n;
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int escaped = 0;
+    if (live) {
+        const int n = a.n;
+        double dxx = a.st[i], dyy = a.st[n + i], dzz = a.st[2 * n + i];
+        double daa = a.st[3 * n + i], dee = a.st[4 * n + i];
+        double drr = 0.0, dpp = 0.0, duu = 0.0;
+        if (a.fix_stale) dxx = dyy = dzz = daa = dee = 0.0;
+        escaped = a.esc[i];
+        if (escaped == 0) {
+            double x = a.x[i], y = a.y[i], z = a.z[i];
+            double ux = a.ux[i], uy = a.uy[i], uz = a.uz[i], tt = a.t[i];
+            double t = tt * 2 + a.dt;
+            double q;
+            q = a.x0[i] - (x + a.xs[i]); dxx = q * q / t;
+            q = a.y0[i] - (y + a.ys[i]); dyy = q * q / t;
+            q = a.z0[i] - (z + a.zs[i]); dzz = q * q / t;
+            drr = dxx + dyy + dzz;
+            double ax = a.ux0[i] - ux, ay = a.uy0[i] - uy, az = a.uz0[i] - uz;
+            dpp = (ax * ax + ay * ay + az * az) / t;
+            double g = sqrt(1.0 + ux * ux + uy * uy + uz * uz);
+            q = g - a.g0[i]; dee = q * q / t;
+            Emf e;
+            get_emf64<FK>(a.f, x, y, z, tt, e);
+            double ab = sqrt(e.bx * e.bx + e.by * e.by + e.bz * e.bz);
+            double au = sqrt(ux * ux + uy * uy + uz * uz);
+            double mu = (e.bx * ux + e.by * uy + e.bz * uz) / (ab * au);
+            q = mu - a.mu0[i]; duu = q * q / t;
+            double th = acos(mu) * 180 / TP_PI;
+            q = th - a.th0[i]; daa = q * q / t;
+        }
+        a.st[i] = dxx; a.st[n + i] = dyy; a.st[2 * n + i] = dzz;
+        a.st[3 * n + i] = daa; a.st[4 * n + i] = dee;
+        v[0] = drr; v[1] = dxx; v[2] = dyy; v[3] = dzz;
+        v[4] = dpp; v[5] = duu; v[6] = daa; v[7] = dee;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        double r = wave_sum(v[k]);
+        if (lane == 0) wsum[wave][k] = r;
+    }
+    unsigned long long m = __ballot(live && escaped);
+    if (lane == 0 && m && tstep + 1 < a.nd) atomicAdd(&a.rdec[tstep + 1], __popcll(m));
+    __syncthreads();
+    if (tid < 8) {
+        double r = 0.0;
+        for (int w = 0; w < nwaves; w++) r += wsum[w][tid];
+        a.partials[(size_t)blockIdx.x * 8 + tid] = r;
+    }
+}
+
+/* Sum the per-block partials of one D row in a fixed order: one block per
+ * component, strided per-thread sums then an LDS tree. */
+__global__ void __launch_bounds__(256) k_reduce_row(const double *partials, int nblocks,
+                                                    int row, int nd, double *dco)
 {
     __shared__ double sh[256];
-    const int r = blockIdx.x >> 3, k = blockIdx.x & 7;
+    const int k = blockIdx.x;
     double s = 0.0;
-    for (int b = threadIdx.x; b < nblocks; b += 256)
-        s += partials[((size_t)r * nblocks + b) * 8 + k];
+    for (int b = threadIdx.x; b < nblocks; b += 256) s += partials[(size_t)b * 8 + k];
     sh[threadIdx.x] = s;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
         if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
         __syncthreads();
     }
-    if (threadIdx.x == 0 && tstep0 + r < nd) dco[(size_t)k * nd + tstep0 + r] = sh[0];
+    if (threadIdx.x == 0 && row < nd) dco[(size_t)k * nd + row] = sh[0];
 }
 
 /* AoS (struct particles) -> SoA, for both the current and the initial
@@ -396,6 +411,26 @@ static track_fn pick_track(int fk, int push, int prec)
     case FK_WLCS: return pick_track_fk<FK_WLCS>(push, prec);
     }
     return nullptr;
+}
+
+template <int FK>
+static void launch_dcoeff(dim3 g, dim3 b, hipStream_t s, const TrackArgs &a, int tstep)
+{
+    hipLaunchKernelGGL((k_dcoeff<FK>), g, b, 0, s, a, tstep);
+}
+
+static void dispatch_dcoeff(int fk, dim3 g, dim3 b, hipStream_t s, const TrackArgs &a,
+                            int tstep)
+{
+    switch (fk) {
+    case FK_TEST: launch_dcoeff<FK_TEST>(g, b, s, a, tstep); break;
+    case FK_ABC: launch_dcoeff<FK_ABC>(g, b, s, a, tstep); break;
+    case FK_ABCV: launch_dcoeff<FK_ABCV>(g, b, s, a, tstep); break;
+    case FK_GRID_REF: launch_dcoeff<FK_GRID_REF>(g, b, s, a, tstep); break;
+    case FK_GRID_TB: launch_dcoeff<FK_GRID_TB>(g, b, s, a, tstep); break;
+    case FK_MODES: launch_dcoeff<FK_MODES>(g, b, s, a, tstep); break;
+    case FK_WLCS: launch_dcoeff<FK_WLCS>(g, b, s, a, tstep); break;
+    }
 }
 
 template <int FK>
@@ -510,6 +545,8 @@ void tp_gpu_finalize(tp_ctx *c)
     hipFree(c->dco);
     hipFree(c->partials);
     hipFree(c->rdec);
+    for (int k = 0; k < c->nev; k++) hipEventDestroy(c->evs[k]);
+    free(c->evs);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -603,10 +640,11 @@ int tp_gpu_set_field_grid(tp_ctx *c, int nx, int ny, int nz, const double lo[3],
 {
     if (!c || nx < 2 || ny < 2 || nz < 2 || !lo || !hi || !ef || !bf) return TP_E_ARG;
     if (mode != TP_GATHER_REF_COMPAT && mode != TP_GATHER_TEXTBOOK) return TP_E_ARG;
+    size_t nn = (size_t)nx * ny * nz;
+    if (nn >= 0xFFFFFFFFull) return TP_E_ARG;    /* 32-bit corner indices */
     HIPCHK(hipSetDevice(c->device));
     free_field(c);
     memset(&c->f, 0, sizeof(c->f));
-    size_t nn = (size_t)nx * ny * nz;
     /* interleave: node k -> float4 {ex,ey,ez,bx}, float4 {by,bz,0,0} */
     float *h = (float *)malloc(nn * 8 * sizeof(float));
     if (!h) return TP_E_NOMEM;
@@ -809,22 +847,35 @@ int tp_gpu_run_fixed(tp_ctx *c, const tp_track_params *p)
 
     const int bs = block_size_for(c, n);
     const int nblocks = n > 0 ? (n + bs - 1) / bs : 0;
-    /* chunk of steps per launch: a multiple of the D cadence, bounded so
-     * the per-block partials stay under 64 MiB */
-    long rows_cap = nblocks > 0 ? (64L << 20) / ((long)nblocks * 64) : 1000;
-    if (rows_cap < 2) rows_cap = 2;
-    long chunk = (rows_cap - 1) * (long)di;
-    if (chunk > 2000) chunk = 2000;
-    if (chunk < 1) chunk = 1;
-    size_t pneed = (size_t)(chunk / di + 2) * (size_t)(nblocks > 0 ? nblocks : 1) * 8;
+    size_t pneed = (size_t)(nblocks > 0 ? nblocks : 1) * 8;
     if (pneed > c->partials_cap) {
         hipFree(c->partials);
         c->partials = nullptr; c->partials_cap = 0;
         if (hipMalloc(&c->partials, pneed * sizeof(double)) != hipSuccess) return TP_E_NOMEM;
         c->partials_cap = pneed;
     }
-    size_t shmem = sizeof(double) * 8 * (bs / 64) + sizeof(unsigned) * nb;
+    size_t shmem = sizeof(unsigned) * nb;
     if (shmem > 64 * 1024) return TP_E_ARG;
+    /* launches: each ends at the next D step (a multiple of di) or estep */
+    long nlaunch = 0;
+    for (long j0 = p->sstep; j0 <= p->estep; nlaunch++) {
+        long j1 = (j0 / di + 1) * (long)di;
+        if (j0 % di == 0) j1 = j0;
+        if (j1 > p->estep) j1 = p->estep;
+        j0 = j1 + 1;
+    }
+    if (nlaunch > c->nev) {
+        for (int k = 0; k < c->nev; k++) hipEventDestroy(c->evs[k]);
+        free(c->evs);
+        c->nev = 0;
+        c->evs = (hipEvent_t *)malloc(sizeof(hipEvent_t) * 2 * nlaunch);
+        if (!c->evs) return TP_E_NOMEM;
+        for (long k = 0; k < 2 * nlaunch; k++) {
+            HIPCHK(hipEventCreate(&c->evs[k]));
+            c->nev = (int)(k + 1);
+        }
+        c->nev = (int)(2 * nlaunch);
+    }
 
     TrackArgs a;
     memset(&a, 0, sizeof(a));
@@ -848,37 +899,39 @@ int tp_gpu_run_fixed(tp_ctx *c, const tp_track_params *p)
     /* frame and D-row counters at step sstep (src/tracking.c:317, 328) */
     int ntp = (int)floor(p->sstep / (double)p->einterval_t);
     int tstep = 1;
-    float ms_total = 0.f;
-    for (long j0 = p->sstep; j0 <= p->estep && nblocks > 0; j0 += chunk) {
-        long j1 = j0 + chunk - 1;
+    int launches = 0;
+    for (long j0 = p->sstep; j0 <= p->estep && nblocks > 0;) {
+        long j1 = (j0 % di == 0) ? j0 : (j0 / di + 1) * (long)di;
         if (j1 > p->estep) j1 = p->estep;
         a.j0 = (int)j0; a.j1 = (int)j1;
         a.first = (j0 == p->sstep) ? 1 : 0;
-        a.ntp0 = ntp; a.tstep0 = tstep;
-        HIPCHK(hipEventRecord(c->ev0, c->stream));
+        a.ntp0 = ntp;
+        HIPCHK(hipEventRecord(c->evs[2 * launches], c->stream));
         fn(dim3(nblocks), dim3(bs), shmem, c->stream, a);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipEventRecord(c->ev1, c->stream));
-        /* rows produced by this chunk */
-        int rows = 0;
-        for (long j = j0; j <= j1; j++) {
+        HIPCHK(hipEventRecord(c->evs[2 * launches + 1], c->stream));
+        launches++;
+        for (long j = j0; j <= j1; j++)
             if (j % p->einterval_t == 0) ntp++;
-            if (j % di == 0) rows++;
-        }
-        if (rows > 0) {
-            hipLaunchKernelGGL(k_reduce_rows, dim3(rows * 8), dim3(256), 0, c->stream,
-                               c->partials, nblocks, tstep, nd, c->dco);
+        if (j1 % di == 0) {
+            dispatch_dcoeff(field_kind(c), dim3(nblocks), dim3(bs), c->stream, a, tstep);
             HIPCHK(hipGetLastError());
+            hipLaunchKernelGGL(k_reduce_row, dim3(8), dim3(256), 0, c->stream, c->partials,
+                               nblocks, tstep, nd, c->dco);
+            HIPCHK(hipGetLastError());
+            tstep++;
         }
-        tstep += rows;
-        HIPCHK(hipEventSynchronize(c->ev1));
-        float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-        ms_total += ms;
-        c->last_launches++;
+        j0 = j1 + 1;
     }
     HIPCHK(hipStreamSynchronize(c->stream));
+    double ms_total = 0.0;
+    for (int k = 0; k < launches; k++) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, c->evs[2 * k], c->evs[2 * k + 1]));
+        ms_total += ms;
+    }
     c->last_ms = ms_total;
+    c->last_launches = launches;
     return TP_OK;
 }
 
