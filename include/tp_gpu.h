@@ -133,6 +133,13 @@ int tp_gpu_set_field_grid(tp_ctx *ctx, int nx, int ny, int nz,
  * exactly. */
 int tp_gpu_set_field_wlcs(tp_ctx *ctx, int nwlcs, const double *rows,
                           int ibessel, double omega, double norm_i, int mode);
+/* The same systems from the table read_wlcs keeps in memory: rows of 19
+ * doubles in the member order of the reference's struct wlcs
+ * (include/wlcs.h:24-37: x_wr y_wr z_wr costheta_wr cosphi_wr sintheta_wr
+ * sinphi_wr cur_wr t0 x_lp y_lp z_lp r_lp cosalpha_lp cosbeta_lp sinalpha_lp
+ * sinbeta_lp cur_lp omega), currents already scaled by normI. */
+int tp_gpu_set_field_wlcs_table(tp_ctx *ctx, int nwlcs, const double *table,
+                                int ibessel, int mode);
 /* multi-mode turbulence (no counterpart in the reference): B = b0z z-hat +
  * sum_n a_n cos(k_n . x + phi_n), E = 0; rows of 7 doubles
  * (kx, ky, kz, ax, ay, az, phi). */

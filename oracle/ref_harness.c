@@ -300,8 +300,10 @@ int main(int argc, char **argv)
         int nd = estep / 100 + 1;
         int nb = s.nbins, nt = s.nt_out;
         double *dsum = calloc((size_t)8 * nd, sizeof(double));
-        int *remain = malloc(sizeof(int) * nd);
-        for (int i = 0; i < nd; i++) remain[i] = np;
+        /* one spare entry: with open boundaries the reference decrements
+         * nptl_remain_time[nd] (src/tracking.c:366-369) */
+        int *remain = malloc(sizeof(int) * (nd + 1));
+        for (int i = 0; i <= nd; i++) remain[i] = np;
         int *nsteps = malloc(sizeof(int) * (np + 1));
         for (int i = 0; i < np; i++) nsteps[i] = 1;
         double *esp = calloc((size_t)nb * nt + 1, sizeof(double));

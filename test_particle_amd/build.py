@@ -15,21 +15,38 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -ffp-contract=off: FP64 kernels must perform the same IEEE operations as
 # the CPU oracle (no fused multiply-adds other than the explicit fma()).
 # -packed-fp32-ops: keep the compiler from emitting packed FP32 math.
-FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
          "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang",
          "-packed-fp32-ops", "-I", os.path.join(ROOT, "include"), "-I", CSRC]
 SOURCES = ["tp_gpu.hip"]
+# host-side C (the reference's language), compiled by gcc and linked in
+C_SOURCES = ["tp_host.c"]
+CC = os.environ.get("CC", "gcc")
+C_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-Wall", "-ffp-contract=off",
+           "-I", os.path.join(ROOT, "include")]
 
 
-def build(verbose=False):
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    cmd = [HIPCC, *FLAGS, "-o", OUT + ".tmp", *srcs]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building libtpgpu.so")
+        raise RuntimeError(f"build step failed: {cmd[0]}")
+
+
+def build(verbose=False):
+    objs = []
+    for c in C_SOURCES:
+        o = os.path.join(CSRC, os.path.splitext(c)[0] + ".o")
+        _run([CC, *C_FLAGS, "-c", os.path.join(CSRC, c), "-o", o], verbose)
+        objs.append(o)
+    for h in SOURCES:
+        o = os.path.join(CSRC, os.path.splitext(h)[0] + ".o")
+        _run([HIPCC, *FLAGS, "-c", os.path.join(CSRC, h), "-o", o], verbose)
+        objs.append(o)
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", OUT + ".tmp", *objs],
+         verbose)
     os.replace(OUT + ".tmp", OUT)
     return OUT
 

@@ -705,6 +705,40 @@ int tp_gpu_set_field_wlcs(tp_ctx *c, int nwlcs, const double *rows, int ibessel,
     return TP_OK;
 }
 
+int tp_gpu_set_field_wlcs_table(tp_ctx *c, int nwlcs, const double *tab, int ibessel,
+                                int mode)
+{
+    if (!c || nwlcs < 1 || !tab) return TP_E_ARG;
+    if (mode != TP_GATHER_REF_COMPAT && mode != TP_GATHER_TEXTBOOK) return TP_E_ARG;
+    HIPCHK(hipSetDevice(c->device));
+    free_field(c);
+    memset(&c->f, 0, sizeof(c->f));
+    WlcsSys *h = (WlcsSys *)malloc(sizeof(WlcsSys) * nwlcs);
+    if (!h) return TP_E_NOMEM;
+    for (int i = 0; i < nwlcs; i++) {
+        const double *r = tab + 19 * i;
+        WlcsSys &s = h[i];
+        s.x_wr = r[0]; s.y_wr = r[1]; s.z_wr = r[2];
+        s.ct = r[3]; s.cp = r[4]; s.st = r[5]; s.sp = r[6];
+        s.cur_wr = r[7]; s.t0 = r[8];
+        s.x_lp = r[9]; s.y_lp = r[10]; s.z_lp = r[11]; s.r_lp = r[12];
+        s.ca = r[13]; s.cb = r[14]; s.sa = r[15]; s.sb = r[16];
+        s.cur_lp = r[17]; s.omega = r[18];
+    }
+    hipError_t e = hipMalloc(&c->wl, sizeof(WlcsSys) * nwlcs);
+    if (e != hipSuccess) { free(h); c->wl = nullptr; return TP_E_NOMEM; }
+    e = hipMemcpy(c->wl, h, sizeof(WlcsSys) * nwlcs, hipMemcpyHostToDevice);
+    free(h);
+    if (e != hipSuccess) return TP_E_HIP;
+    c->f.nwlcs = nwlcs;
+    c->f.ibessel = ibessel;
+    c->f.wlmode = mode == TP_GATHER_REF_COMPAT ? 0 : 1;
+    c->f.wl = c->wl;
+    c->f.kind = FK_WLCS;
+    c->have_field = true;
+    return TP_OK;
+}
+
 int tp_gpu_set_field_modes(tp_ctx *c, int nmodes, const double *rows, double b0z)
 {
     if (!c || nmodes < 0 || (nmodes > 0 && !rows)) return TP_E_ARG;

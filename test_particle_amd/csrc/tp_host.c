@@ -1,0 +1,131 @@
+/*
+ * tp_host.c -- host-side output writers (include/tp_host.h).
+ *
+ * The reference writes its spectra and diffusion coefficients from the
+ * MPI-reduced arrays in src/diagnostics.c; these functions produce the same
+ * files from the raw arrays the GPU library returns, with the same formulas
+ * for the bin centres and widths and the same printf formats.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+
+#include "../../include/tp_host.h"
+
+int tp_host_spectrum_bins(int nbins, double emin, double emax, double *ebins,
+                          double *einterval)
+{
+    if (nbins < 3 || !(emin > 0.0) || !(emax > emin)) return TP_E_ARG;
+    double logemin = log10(emin), logemax = log10(emax);
+    double logde = (logemax - logemin) / (nbins - 2);
+    if (ebins) {
+        ebins[0] = 0.5 * emin;
+        for (int i = 1; i < nbins - 1; i++) ebins[i] = pow(10, logemin + logde * (i - 0.5));
+        ebins[nbins - 1] = pow(10, logemax + 0.5 * logde);
+    }
+    if (einterval) {
+        einterval[0] = emin;
+        for (int i = 1; i < nbins - 1; i++)
+            einterval[i] = pow(10, logemin) * (pow(10, logde * i) - pow(10, logde * (i - 1)));
+        einterval[nbins - 1] = pow(10, logemin) * (pow(10, logde * nbins) -
+                                                   pow(10, logde * (nbins - 1)));
+    }
+    return TP_OK;
+}
+
+int tp_host_spectrum_t0(int n, const tp_particle *ptl, int nbins, double emin,
+                        double emax, double *counts)
+{
+    if (n < 0 || (n > 0 && !ptl) || !counts || nbins < 3 || !(emin > 0.0) || !(emax > emin))
+        return TP_E_ARG;
+    double logemin = log10(emin);
+    double logde = (log10(emax) - logemin) / (nbins - 2);
+    for (int i = 0; i < nbins; i++) counts[i] = 0.0;
+    for (int i = 0; i < n; i++) {
+        const tp_particle *p = &ptl[i];
+        double gama = sqrt(1.0 + p->ux * p->ux + p->uy * p->uy + p->uz * p->uz);
+        double v = (log10(gama - 1.0) - logemin) / logde;
+        /* int conversion truncates; out-of-range and NaN give INT_MIN on
+         * x86-64, hence bin 0 */
+        int ibin = (v > -2147483649.0 && v < 2147483648.0) ? (int)v : 0;
+        if (ibin < 0) ibin = 0;
+        else if (ibin > nbins - 1) ibin = nbins - 1;
+        counts[ibin] += 1.0;
+    }
+    return TP_OK;
+}
+
+int tp_host_write_espectrum_t0(const char *fname, const char *fname_escape, int nbins,
+                               double emin, double emax, const double *counts_t0,
+                               int bc_flag)
+{
+    if (!fname || !counts_t0 || nbins < 3) return TP_E_ARG;
+    double *eb = malloc(sizeof(double) * nbins), *ei = malloc(sizeof(double) * nbins);
+    if (!eb || !ei || tp_host_spectrum_bins(nbins, emin, emax, eb, ei) != TP_OK) {
+        free(eb); free(ei);
+        return TP_E_ARG;
+    }
+    int rc = TP_OK;
+    FILE *fp = fopen(fname, "w");
+    if (!fp) rc = TP_E_ARG;
+    if (fp) {
+        for (int i = 0; i < nbins; i++) fprintf(fp, "%10.4e ", eb[i]);
+        fprintf(fp, "\n");
+        for (int i = 0; i < nbins; i++) fprintf(fp, "%10.4e ", counts_t0[i] / ei[i]);
+        fprintf(fp, "\n");
+        fclose(fp);
+    }
+    if (rc == TP_OK && bc_flag == 1 && fname_escape) {
+        fp = fopen(fname_escape, "w");
+        if (!fp) rc = TP_E_ARG;
+        if (fp) {
+            for (int i = 0; i < nbins; i++) fprintf(fp, "%10.4e ", eb[i]);
+            fprintf(fp, "\n");
+            for (int i = 0; i < nbins; i++) fprintf(fp, "%10.4e ", 0.0);
+            fprintf(fp, "\n");
+            fclose(fp);
+        }
+    }
+    free(eb);
+    free(ei);
+    return rc;
+}
+
+int tp_host_append_espectrum(const char *fname, int nbins, int nt_out, double emin,
+                             double emax, const double *counts)
+{
+    if (!fname || !counts || nbins < 3 || nt_out < 0) return TP_E_ARG;
+    double *ei = malloc(sizeof(double) * nbins);
+    if (!ei || tp_host_spectrum_bins(nbins, emin, emax, NULL, ei) != TP_OK) {
+        free(ei);
+        return TP_E_ARG;
+    }
+    FILE *fp = fopen(fname, "a");
+    if (!fp) {
+        free(ei);
+        return TP_E_ARG;
+    }
+    for (int i = 0; i < nt_out; i++) {
+        for (int j = 0; j < nbins; j++) fprintf(fp, "%10.4e ", counts[(size_t)i * nbins + j] / ei[j]);
+        fprintf(fp, "\n");
+    }
+    fclose(fp);
+    free(ei);
+    return TP_OK;
+}
+
+int tp_host_write_diff_coeffs(const char *fname, int nd, const double *d, const int *remain)
+{
+    if (!fname || nd < 0 || (nd > 0 && (!d || !remain))) return TP_E_ARG;
+    FILE *fp = fopen(fname, "w");
+    if (!fp) return TP_E_ARG;
+    fprintf(fp, "drr        dxx        dyy        dzz        dpp");
+    fprintf(fp, "        duu        daa        dee        nptl\n");
+    for (int i = 0; i < nd; i++) {
+        fprintf(fp, "%10.4e %10.4e %10.4e %10.4e %10.4e %10.4e %10.4e %10.4e %d\n",
+                d[i], d[nd + i], d[2 * nd + i], d[3 * nd + i], d[4 * nd + i],
+                d[5 * nd + i], d[6 * nd + i], d[7 * nd + i], remain[i]);
+    }
+    fclose(fp);
+    return TP_OK;
+}

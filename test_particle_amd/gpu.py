@@ -44,6 +44,12 @@ def lib():
         L.tp_gpu_store_particles.argtypes = [vp, P(d), P(i)]
         L.tp_gpu_read_diagnostics.argtypes = [vp, P(d), P(d), P(d), P(i)]
         L.tp_gpu_last_kernel_time.argtypes = [vp, P(d), P(i)]
+        cs = ctypes.c_char_p
+        L.tp_host_spectrum_bins.argtypes = [i, d, d, P(d), P(d)]
+        L.tp_host_spectrum_t0.argtypes = [i, P(d), i, d, d, P(d)]
+        L.tp_host_write_espectrum_t0.argtypes = [cs, cs, i, d, d, P(d), i]
+        L.tp_host_append_espectrum.argtypes = [cs, i, i, d, d, P(d)]
+        L.tp_host_write_diff_coeffs.argtypes = [cs, i, P(d), P(i)]
         _lib = L
     return _lib
 
@@ -189,3 +195,44 @@ class Tracker:
         _check(lib().tp_gpu_last_kernel_time(self._ctx, ctypes.byref(ms), ctypes.byref(k)),
                "last_kernel_time")
         return ms.value, k.value
+
+
+# --- host-side writers (include/tp_host.h) -----------------------------------
+def spectrum_bins(nbins, emin, emax):
+    eb = np.zeros(nbins)
+    ei = np.zeros(nbins)
+    _check(lib().tp_host_spectrum_bins(nbins, emin, emax, dptr(eb), dptr(ei)), "spectrum_bins")
+    return eb, ei
+
+
+def spectrum_t0(ptl, nbins, emin, emax):
+    ptl = abi.as_particles(ptl)
+    c = np.zeros(nbins)
+    _check(lib().tp_host_spectrum_t0(ptl.shape[0], dptr(ptl), nbins, emin, emax, dptr(c)),
+           "spectrum_t0")
+    return c
+
+
+def write_outputs(dirname, ptl_init, result, nbins, emin, emax, bc_flag=0):
+    """The files the reference writes for one run: data/espectrum.dat
+    (t = 0 spectrum then one row per frame), data/espect_escape.dat (open
+    boundaries) and data/diff_coeffs.dat."""
+    import os
+    os.makedirs(dirname, exist_ok=True)
+    L = lib()
+    f_sp = os.path.join(dirname, "espectrum.dat").encode()
+    f_es = os.path.join(dirname, "espect_escape.dat").encode()
+    t0 = spectrum_t0(ptl_init, nbins, emin, emax)
+    _check(L.tp_host_write_espectrum_t0(f_sp, f_es, nbins, emin, emax, dptr(t0), bc_flag),
+           "write_espectrum_t0")
+    esp = np.ascontiguousarray(result["espect"], dtype=np.float64)
+    _check(L.tp_host_append_espectrum(f_sp, nbins, esp.shape[0], emin, emax, dptr(esp)),
+           "append_espectrum")
+    if bc_flag == 1:
+        ese = np.ascontiguousarray(result["espect_escape"], dtype=np.float64)
+        _check(L.tp_host_append_espectrum(f_es, nbins, ese.shape[0], emin, emax, dptr(ese)),
+               "append_espectrum")
+    d = np.ascontiguousarray(result["dcoeff"], dtype=np.float64)
+    r = np.ascontiguousarray(result["remain"], dtype=np.int32)
+    _check(L.tp_host_write_diff_coeffs(os.path.join(dirname, "diff_coeffs.dat").encode(),
+                                       d.shape[1], dptr(d), iptr(r)), "write_diff_coeffs")

@@ -1,0 +1,144 @@
+/*
+ * tp_hybrid_shim.c -- GPU drop-in for the reference tracer's tracking loop.
+ *
+ * Build this file INTO THE REFERENCE PROGRAM (it includes the reference's
+ * headers and reads its globals) and link libtpgpu.so.  It defines
+ * particle_tracking_hybrid with exactly the reference's signature
+ * (src/tracking.c:659-663, include/tracking.h:78-82); compile the reference's
+ * src/tracking.c with -Dparticle_tracking_hybrid=particle_tracking_hybrid_cpu
+ * so the CPU loop stays available for what the GPU path does not cover
+ * (trajectory diagnostics, adaptive stepping, system 3, two time slices).
+ *
+ * The field source is taken from the reference's globals once per call;
+ * results go through the reference's own collect_espectrum and
+ * collect_diff_coeffs (src/diagnostics.c:221-248, 723-782), so the MPI
+ * reductions and output files are unchanged.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <omp.h>
+#include <mpi.h>
+#include "constants.h"
+#include "domain.h"
+#include "particle_info.h"
+#include "tracking.h"
+#include "diagnostics.h"
+#include "force_free.h"
+#include "magnetic_field.h"
+#include "electric_field.h"
+#include "velocity_field.h"
+#include "wlcs.h"
+#include "tp_gpu.h"
+
+void particle_tracking_hybrid_cpu(int mpi_rank, int nptl, double dt, int nbins,
+        int nt_out, int bc_flag, int nsteps_output, double pmass,
+        int *ntraj_accum, int tracking_method, int traj_diagnose,
+        particles *ptl_init, int *nsteps_ptl_tracking, particles *ptl,
+        particles *ptl_time);
+
+/* reference globals (src/tracking.c:35-40, src/force_free.c:33,
+ * src/{magnetic,electric,velocity}_field.c:34-39, src/wlcs.c:12-14,
+ * src/diagnostics.c:32) */
+extern struct domain simul_domain;
+extern struct grids simul_grid;
+extern int system_type, multi_tframe, data_type;
+extern double charge_mass;
+extern struct param_ff param;
+extern struct bfields_float *magc_float;
+extern struct efields_float *efc_float;
+extern struct vfields_double *vc_double;
+extern struct wlcs *config;
+extern int nwlcs, iBessel;
+extern double emin, logemin, logde;
+
+static tp_ctx *tp_shim_ctx;
+
+/* 0 on success; nonzero when the GPU path does not cover this setup */
+static int tp_shim_set_field(tp_ctx *c)
+{
+    double lo[3] = {simul_domain.xmin, simul_domain.ymin, simul_domain.zmin};
+    double hi[3] = {simul_domain.xmax, simul_domain.ymax, simul_domain.zmax};
+    switch (system_type) {
+    case 0:
+        return tp_gpu_set_field_test(c);
+    case 1:
+        return tp_gpu_set_field_wlcs_table(c, nwlcs, (const double *)config, iBessel,
+                                           TP_GATHER_REF_COMPAT);
+    case 2: {
+        int rc = tp_gpu_set_velocity_grid(c, simul_grid.nx, simul_grid.ny, simul_grid.nz,
+                                          lo, hi, (const double *)vc_double);
+        if (rc) return rc;
+        return tp_gpu_set_field_abc(c, ffA, ffB, ffC, param.lambda0, param.B0);
+    }
+    case 4:
+        if (multi_tframe != 0 || data_type != (int)sizeof(float)) return -1;
+        return tp_gpu_set_field_grid(c, simul_grid.nx, simul_grid.ny, simul_grid.nz, lo, hi,
+                                     (const float *)efc_float, (const float *)magc_float,
+                                     TP_GATHER_REF_COMPAT);
+    default:
+        return -1;   /* system 3 (MHD float grids) stays on the CPU */
+    }
+}
+
+void particle_tracking_hybrid(int mpi_rank, int nptl, double dt, int nbins,
+        int nt_out, int bc_flag, int nsteps_output, double pmass,
+        int *ntraj_accum, int tracking_method, int traj_diagnose,
+        particles *ptl_init, int *nsteps_ptl_tracking, particles *ptl,
+        particles *ptl_time)
+{
+    int rc = -1;
+    if (!tp_shim_ctx && tp_gpu_init(0, &tp_shim_ctx) != TP_OK) tp_shim_ctx = NULL;
+    if (tp_shim_ctx && tracking_method == 0 && traj_diagnose == 0)
+        rc = tp_shim_set_field(tp_shim_ctx);
+    if (rc != 0) {
+        particle_tracking_hybrid_cpu(mpi_rank, nptl, dt, nbins, nt_out, bc_flag,
+                nsteps_output, pmass, ntraj_accum, tracking_method, traj_diagnose,
+                ptl_init, nsteps_ptl_tracking, ptl, ptl_time);
+        return;
+    }
+    tp_track_params p;
+    p.pusher = TP_PUSH_WIRZ;
+    p.precision = TP_FP64;
+    p.bc_flag = bc_flag;
+    p.nbins = nbins;
+    p.nt_out = nt_out;
+    p.sstep = 1;
+    p.estep = simul_domain.tmax / dt;              /* src/tracking.c:103 */
+    p.einterval_t = p.estep / nt_out;              /* src/tracking.c:104 */
+    p.dcoeff_interval = 100;                       /* src/tracking.c:34 */
+    p.flags = 0;
+    p.dt = dt;
+    p.qm = charge_mass;
+    p.xmin = simul_domain.xmin; p.xmax = simul_domain.xmax;
+    p.ymin = simul_domain.ymin; p.ymax = simul_domain.ymax;
+    p.zmin = simul_domain.zmin; p.zmax = simul_domain.zmax;
+    p.logemin = logemin;
+    p.logde = logde;
+    int nd = p.estep / 100 + 1;                    /* src/tracking.c:109 */
+    double *espect = calloc((size_t)nt_out * nbins, sizeof(double));
+    double *espect_esc = calloc((size_t)nt_out * nbins, sizeof(double));
+    double *tot = calloc((size_t)nt_out * nbins, sizeof(double));
+    double *d = calloc((size_t)8 * nd, sizeof(double));
+    int *remain = calloc(nd, sizeof(int));
+    double t1 = omp_get_wtime();
+    rc = tp_gpu_track_fixed(tp_shim_ctx, &p, nptl, (tp_particle *)ptl,
+                            (const tp_particle *)ptl_init, nsteps_ptl_tracking, espect,
+                            espect_esc, d, remain);
+    if (rc != TP_OK) {
+        fprintf(stderr, "tp_gpu_track_fixed: %s\n", tp_gpu_strerror(rc));
+        MPI_Abort(MPI_COMM_WORLD, 1);
+    }
+    collect_diff_coeffs(mpi_rank, nd, d, d + nd, d + 2 * nd, d + 3 * nd, d + 4 * nd,
+                        d + 5 * nd, d + 6 * nd, d + 7 * nd, "data/diff_coeffs.dat", remain);
+    double t2 = omp_get_wtime();
+    if (mpi_rank == 0) printf("Elapsed time: %lf s\n", t2 - t1);
+    collect_espectrum(mpi_rank, nbins, nt_out, (double (*)[nbins])espect,
+                      (double (*)[nbins])tot, "data/espectrum.dat");
+    if (bc_flag == 1) {
+        for (size_t k = 0; k < (size_t)nt_out * nbins; k++) tot[k] = 0.0;
+        collect_espectrum(mpi_rank, nbins, nt_out, (double (*)[nbins])espect_esc,
+                          (double (*)[nbins])tot, "data/espect_escape.dat");
+    }
+    free(espect); free(espect_esc); free(tot); free(d); free(remain);
+}
