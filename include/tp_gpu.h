@@ -178,6 +178,53 @@ int tp_gpu_read_diagnostics(tp_ctx *ctx, double *espect,
  * the stream they run on: total milliseconds and number of launches. */
 int tp_gpu_last_kernel_time(tp_ctx *ctx, double *ms, int *launches);
 
+/* --- adaptive Bulirsch-Stoer tracking ------------------------------------ */
+/* The reference's adaptive stepper (src/unused/StepperBS.c:103-435, NR3
+ * ch. 17 without dense output: nseq = 2(k+1), polynomial extrapolation,
+ * controller constants STEPFAC1..4 = 0.65/0.94/0.02/4.0, KFAC1/2 = 0.8/0.9)
+ * integrates y = (x, u), u = gamma v, with
+ *   dx/dt = u/gamma,  du/dt = qm (E + (u/gamma) x B)
+ * (the reference's derivs, :443-471, is not this ODE) and keeps its
+ * controller state per particle (the reference keeps it in statics,
+ * :167-168).  Particles run in a persistent kernel that refills lanes from
+ * a work queue as particles finish. */
+typedef struct tp_bs_params {
+    double rtol, atol;      /* error control; the reference: 1e-14, 1e-5 */
+    double qm;              /* charge/mass */
+    double t_end;           /* end time, used when t_end_gyro <= 0 */
+    double t_end_gyro;      /* end time per particle in its gyroperiods
+                               T_g = 2 pi gamma0 / (|qm| b_ref), from its t */
+    double b_ref;           /* field scale of T_g */
+    double h0;              /* first trial step; <= 0: 0.01 T_g */
+    int kmax;               /* extrapolation depth KMAXX, 3..8 (reference 8) */
+    int max_attempts;       /* cap on step attempts per particle (0: 10^7);
+                               a particle reaching it is counted as failed */
+    int block;              /* lanes per block, 64/128/256 (0: tuned or 64) */
+    int waves_per_cu;       /* resident waves per compute unit (0: tuned or 8) */
+    int use_graph;          /* launch through a captured hipGraph */
+} tp_bs_params;
+
+typedef struct tp_bs_stats {
+    long long accepted, rejected, rhs_evals;
+    int failed;             /* particles stopped at max_attempts */
+    int launches;
+    double kernel_ms;       /* device time of the work-queue kernel */
+} tp_bs_stats;
+
+/* Resident particles (tp_gpu_load_particles) are advanced to their end
+ * times; tp_gpu_store_particles returns them, its nsteps = accepted steps. */
+int tp_gpu_run_adaptive(tp_ctx *ctx, const tp_bs_params *p, tp_bs_stats *stats);
+/* load + run + store in one call (nsteps may be NULL) */
+int tp_gpu_track_adaptive(tp_ctx *ctx, const tp_bs_params *p, int n, tp_particle *ptl,
+                          int *nsteps, tp_bs_stats *stats);
+/* Launch-shape autotuner: times the work-queue kernel on the first (up to
+ * 65,536) resident particles over a short horizon for each candidate
+ * (block, waves_per_cu), keeps the state unchanged, writes the choice to
+ * *block and *waves_per_cu and, if cache_path is given, caches it there
+ * keyed by field kind, mode count and tolerance (read back on later calls). */
+int tp_gpu_tune_adaptive(tp_ctx *ctx, const tp_bs_params *p, const char *cache_path,
+                         int *block, int *waves_per_cu);
+
 /* --- field probe -------------------------------------------------------- */
 /* get_emf (src/emfields.c:32-60) at n points pts[4*i] = (x, y, z, t);
  * out[6*i] = (Bx, By, Bz, Ex, Ey, Ez).  precision as in tp_track_params. */

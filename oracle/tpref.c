@@ -781,3 +781,290 @@ int tpref_track_fixed(const tpref_field *f, const tp_track_params *q,
     free(dsum); free(rem); free(esp); free(ese);
     return TP_OK;
 }
+
+
+/* ---------------------------------------------------------------------- */
+/* adaptive Bulirsch-Stoer (algorithm of src/unused/StepperBS.c:103-435,   */
+/* NR3 ch. 17, without dense output), per-particle controller state,       */
+/* ODE in (x, u) with du/dt = qm (E + u/gamma x B).                        */
+/* ---------------------------------------------------------------------- */
+#define BS_N 6
+#define BS_STEPFAC1 0.65
+#define BS_STEPFAC2 0.94
+#define BS_STEPFAC3 0.02
+#define BS_STEPFAC4 4.0
+#define BS_KFAC1 0.8
+#define BS_KFAC2 0.9
+
+typedef struct {
+    int kmax, ktarg0;
+    int nseq[9], cost[9];
+    double coeff[9][9];
+} bs_consts;
+
+/* StepperBase, src/unused/StepperBS.c:103-131 (non-dense sequence) */
+static void bs_setup(int kmax, double rtol, bs_consts *c)
+{
+    c->kmax = kmax;
+    for (int i = 0; i <= kmax; i++) c->nseq[i] = 2 * (i + 1);
+    c->cost[0] = c->nseq[0] + 1;
+    for (int k = 0; k < kmax; k++) c->cost[k + 1] = c->cost[k] + c->nseq[k + 1];
+    double logfact = -log10(rtol > 1.0e-12 ? rtol : 1.0e-12) * 0.6 + 0.5;
+    int kt = (int)logfact;
+    if (kt > kmax - 1) kt = kmax - 1;
+    if (kt < 1) kt = 1;
+    c->ktarg0 = kt;
+    for (int k = 0; k <= kmax; k++)
+        for (int l = 0; l < k; l++) {
+            double ratio = (double)c->nseq[k] / c->nseq[l];
+            c->coeff[k][l] = 1.0 / (ratio * ratio - 1.0);
+        }
+}
+
+static void bs_rhs(const tpref_field *f, double qm, double t, const double *y, double *dy)
+{
+    emf_t e;
+    get_emf(f, y[0], y[1], y[2], t, &e);
+    double g = sqrt(1.0 + y[3] * y[3] + y[4] * y[4] + y[5] * y[5]);
+    double ig = 1.0 / g;
+    double vx = y[3] * ig, vy = y[4] * ig, vz = y[5] * ig;
+    dy[0] = vx;
+    dy[1] = vy;
+    dy[2] = vz;
+    dy[3] = qm * (e.Ex + (vy * e.Bz - vz * e.By));
+    dy[4] = qm * (e.Ey + (vz * e.Bx - vx * e.Bz));
+    dy[5] = qm * (e.Ez + (vx * e.By - vy * e.Bx));
+}
+
+typedef struct {
+    double t, hnext;
+    double dydx[BS_N];
+    int ktarg, first, last, reject, prev_reject;
+    long long rhs;
+    int acc, rej, attempts;
+} bs_state;
+
+/* updatey, src/unused/StepperBS.c:364-414 (non-dense part) */
+static void bs_midpoint(const tpref_field *f, double qm, double x, const double *y,
+                        const double *dydx, double htot, int nstep, double *yend,
+                        long long *rhs)
+{
+    double ym[BS_N], yn[BS_N];
+    double h = htot / nstep;
+    for (int i = 0; i < BS_N; i++) {
+        ym[i] = y[i];
+        yn[i] = y[i] + h * dydx[i];
+    }
+    double xnew = x + h;
+    bs_rhs(f, qm, xnew, yn, yend);
+    (*rhs)++;
+    double h2 = 2.0 * h;
+    for (int nn = 1; nn < nstep; nn++) {
+        for (int i = 0; i < BS_N; i++) {
+            double swap = ym[i] + h2 * yend[i];
+            ym[i] = yn[i];
+            yn[i] = swap;
+        }
+        xnew += h;
+        bs_rhs(f, qm, xnew, yn, yend);
+        (*rhs)++;
+    }
+    for (int i = 0; i < BS_N; i++) yend[i] = 0.5 * (ym[i] + yn[i] + h * yend[i]);
+}
+
+/* step, src/unused/StepperBS.c:159-354 (non-dense).  Returns 0, or 1 when
+ * the attempt cap was reached. */
+static int bs_step(const tpref_field *f, const tp_bs_params *prm, const bs_consts *c,
+                   bs_state *st, double *y, double htry, int cap)
+{
+    double table[8][BS_N], hopt[9], work[9];
+    double ysav[BS_N], yseq[BS_N];
+    double h = htry, hnew, err = 0.0;
+    int k = 0;
+    work[0] = 0.0;
+    for (int i = 0; i < BS_N; i++) ysav[i] = y[i];
+    if (h != st->hnext && !st->first) st->last = 1;
+    if (st->reject) {
+        st->prev_reject = 1;
+        st->last = 0;
+    }
+    st->reject = 0;
+    int firstk = 1;
+    hnew = fabs(h);
+    while (firstk || st->reject) {
+        h = hnew;
+        firstk = 0;
+        st->reject = 0;
+        if (++st->attempts > cap) {
+            for (int i = 0; i < BS_N; i++) y[i] = ysav[i];
+            return 1;
+        }
+        for (k = 0; k <= st->ktarg + 1; k++) {
+            bs_midpoint(f, prm->qm, st->t, ysav, st->dydx, h, c->nseq[k], yseq, &st->rhs);
+            if (k == 0) {
+                for (int i = 0; i < BS_N; i++) y[i] = yseq[i];
+                continue;
+            }
+            for (int i = 0; i < BS_N; i++) table[k - 1][i] = yseq[i];
+            /* polyextr, :419-435 */
+            for (int j = k - 1; j > 0; j--)
+                for (int i = 0; i < BS_N; i++)
+                    table[j - 1][i] = table[j][i] + c->coeff[k][j] * (table[j][i] - table[j - 1][i]);
+            for (int i = 0; i < BS_N; i++)
+                y[i] = table[0][i] + c->coeff[k][0] * (table[0][i] - y[i]);
+            err = 0.0;
+            for (int i = 0; i < BS_N; i++) {
+                double a = fabs(ysav[i]), b = fabs(y[i]);
+                double scale = prm->atol + prm->rtol * (a > b ? a : b);
+                double q = (y[i] - table[0][i]) / scale;
+                err += q * q;
+            }
+            err = sqrt(err / BS_N);
+            double expo = 1.0 / (2 * k + 1);
+            double facmin = tp_pow(BS_STEPFAC3, expo);
+            double fac;
+            if (err == 0.0) {
+                fac = 1.0 / facmin;
+            } else {
+                fac = BS_STEPFAC2 / tp_pow(err / BS_STEPFAC1, expo);
+                double lo = facmin / BS_STEPFAC4, hi = 1.0 / facmin;
+                fac = fac < hi ? fac : hi;
+                fac = lo > fac ? lo : fac;
+            }
+            hopt[k] = fabs(h * fac);
+            work[k] = c->cost[k] / hopt[k];
+            if ((st->first || st->last) && err <= 1.0) break;
+            if (k == st->ktarg - 1 && !st->prev_reject && !st->first && !st->last) {
+                if (err <= 1.0) break;
+                double r = (double)(c->nseq[st->ktarg] * c->nseq[st->ktarg + 1] /
+                                    (c->nseq[0] * c->nseq[0]));
+                if (err > r * r) {
+                    st->reject = 1;
+                    st->ktarg = k;
+                    if (st->ktarg > 1 && work[k - 1] < BS_KFAC1 * work[k]) st->ktarg--;
+                    hnew = hopt[st->ktarg];
+                    break;
+                }
+            }
+            if (k == st->ktarg) {
+                if (err <= 1.0) break;
+                double r = (double)c->nseq[k + 1] / c->nseq[0];
+                if (err > r * r) {
+                    st->reject = 1;
+                    if (st->ktarg > 1 && work[k - 1] < BS_KFAC1 * work[k]) st->ktarg--;
+                    hnew = hopt[st->ktarg];
+                    break;
+                }
+            }
+            if (k == st->ktarg + 1) {
+                if (err > 1.0) {
+                    st->reject = 1;
+                    if (st->ktarg > 1 && work[st->ktarg - 1] < BS_KFAC1 * work[st->ktarg])
+                        st->ktarg--;
+                    hnew = hopt[st->ktarg];
+                }
+                break;
+            }
+        }
+        if (st->reject) {
+            st->prev_reject = 1;
+            st->rej++;
+        }
+    }
+    double dnew[BS_N];
+    bs_rhs(f, prm->qm, st->t + h, y, dnew);
+    st->rhs++;
+    for (int i = 0; i < BS_N; i++) st->dydx[i] = dnew[i];
+    st->t += h;
+    st->first = 0;
+    st->acc++;
+    int kopt;
+    const int KM1 = c->kmax - 1;
+    if (k == 1) {
+        kopt = 2;
+    } else if (k <= st->ktarg) {
+        kopt = k;
+        if (work[k - 1] < BS_KFAC1 * work[k]) kopt = k - 1;
+        else if (work[k] < BS_KFAC2 * work[k - 1]) kopt = (k + 1 < KM1) ? k + 1 : KM1;
+    } else {
+        kopt = k - 1;
+        if (k > 2 && work[k - 2] < BS_KFAC1 * work[k - 1]) kopt = k - 2;
+        if (work[k] < BS_KFAC2 * work[kopt]) kopt = (k < KM1) ? k : KM1;
+    }
+    if (st->prev_reject) {
+        st->ktarg = kopt < k ? kopt : k;
+        double a = fabs(h), b = hopt[st->ktarg];
+        hnew = a < b ? a : b;
+        st->prev_reject = 0;
+    } else {
+        if (kopt <= k) {
+            hnew = hopt[kopt];
+        } else {
+            if (k < st->ktarg && work[k] < BS_KFAC2 * work[k - 1])
+                hnew = hopt[k] * c->cost[kopt + 1] / c->cost[k];
+            else
+                hnew = hopt[k] * c->cost[kopt] / c->cost[k];
+        }
+        st->ktarg = kopt;
+    }
+    st->hnext = hnew;
+    return 0;
+}
+
+/* End time of a particle: t_end, or t + t_end_gyro gyroperiods */
+static double bs_tend(const tp_bs_params *prm, const tp_particle *p, double *tg)
+{
+    double g0 = sqrt(1.0 + p->ux * p->ux + p->uy * p->uy + p->uz * p->uz);
+    double T = 2.0 * M_PI * g0 / (fabs(prm->qm) * prm->b_ref);
+    *tg = T;
+    return prm->t_end_gyro > 0.0 ? p->t + prm->t_end_gyro * T : prm->t_end;
+}
+
+int tpref_bs_track(const tpref_field *f, const tp_bs_params *prm, int n, tp_particle *ptl,
+                   int *nsteps, long long *totals, int nthreads, double *seconds)
+{
+    if (n < 0 || prm->kmax < 3 || prm->kmax > 8 || !(prm->rtol > 0.0)) return TP_E_ARG;
+    bs_consts c;
+    bs_setup(prm->kmax, prm->rtol, &c);
+    int cap = prm->max_attempts > 0 ? prm->max_attempts : 10000000;
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    long long acc = 0, rej = 0, rhs = 0, failed = 0;
+    double t1 = omp_get_wtime();
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads) reduction(+ : acc, rej, rhs, failed)
+    for (int i = 0; i < n; i++) {
+        tp_particle *p = &ptl[i];
+        double tg;
+        double tend = bs_tend(prm, p, &tg);
+        double y[BS_N] = {p->x, p->y, p->z, p->ux, p->uy, p->uz};
+        bs_state st;
+        memset(&st, 0, sizeof(st));
+        st.t = p->t;
+        st.hnext = prm->h0 > 0.0 ? prm->h0 : 0.01 * tg;
+        st.ktarg = c.ktarg0;
+        st.first = 1;
+        bs_rhs(f, prm->qm, st.t, y, st.dydx);
+        st.rhs = 1;
+        int fail = 0;
+        while (st.t < tend - 1e-13 * fabs(tend)) {
+            double h = st.hnext;
+            if (st.t + h > tend) h = tend - st.t;
+            if (bs_step(f, prm, &c, &st, y, h, cap)) {
+                fail = 1;
+                break;
+            }
+        }
+        p->x = y[0]; p->y = y[1]; p->z = y[2];
+        p->ux = y[3]; p->uy = y[4]; p->uz = y[5];
+        p->t = st.t;
+        if (nsteps) nsteps[i] = st.acc;
+        acc += st.acc;
+        rej += st.rej;
+        rhs += st.rhs;
+        failed += fail;
+    }
+    if (seconds) *seconds = omp_get_wtime() - t1;
+    if (totals) {
+        totals[0] = acc; totals[1] = rej; totals[2] = rhs; totals[3] = failed;
+    }
+    return TP_OK;
+}

@@ -72,6 +72,11 @@ int  tpref_track_fixed(const tpref_field *f, const tp_track_params *prm,
                        int *nsteps, double *espect, double *espect_escape,
                        double *dcoeff, int *nptl_remain, int nthreads,
                        double *seconds);
+/* adaptive Bulirsch-Stoer of every particle to its end time; nsteps gets
+ * accepted steps, totals = {accepted, rejected, rhs evaluations, failed} */
+int  tpref_bs_track(const tpref_field *f, const tp_bs_params *prm, int n,
+                    tp_particle *ptl, int *nsteps, long long *totals,
+                    int nthreads, double *seconds);
 /* exported for the tests of the shared routines */
 void tpref_sincos(double x, double *s, double *c);
 void tpref_ellint(double k, double *elle, double *ellk);

@@ -69,6 +69,9 @@ def lib():
                                         P(d), P(d), P(i), P(d), P(d), P(d),
                                         P(i), i, P(d)]
         L.tpref_track_fixed.restype = i
+        L.tpref_bs_track.argtypes = [P(TprefField), P(abi.TpBsParams), i, P(d), P(i),
+                                     P(ctypes.c_longlong), i, P(d)]
+        L.tpref_bs_track.restype = i
         L.tpref_sincos.argtypes = [d, P(d), P(d)]
         L.tpref_ellint.argtypes = [d, P(d), P(d)]
         L.tpref_ellint_ref.argtypes = [d, P(d), P(d)]
@@ -216,6 +219,21 @@ def track_fixed(field, params, ptl, ptl_init=None, nthreads=1):
         raise RuntimeError(f"tpref_track_fixed failed: {rc}")
     return dict(ptl=ptl, nsteps=nsteps, espect=esp, espect_escape=ese,
                 dcoeff=dco, remain=rem, seconds=secs.value)
+
+
+def bs_track(field, params, ptl, nthreads=1):
+    """Adaptive Bulirsch-Stoer of every particle to its end time."""
+    ptl = abi.as_particles(ptl).copy()
+    n = ptl.shape[0]
+    nsteps = np.zeros(n, dtype=np.int32)
+    tot = (ctypes.c_longlong * 4)()
+    secs = ctypes.c_double()
+    rc = lib().tpref_bs_track(ctypes.byref(field.s), ctypes.byref(params), n, abi.dptr(ptl),
+                              abi.iptr(nsteps), tot, nthreads, ctypes.byref(secs))
+    if rc != 0:
+        raise RuntimeError(f"tpref_bs_track failed: {rc}")
+    return dict(ptl=ptl, nsteps=nsteps, accepted=tot[0], rejected=tot[1], rhs=tot[2],
+                failed=tot[3], seconds=secs.value)
 
 
 def sincos(x):

@@ -63,6 +63,44 @@ class TpTrackParams(ctypes.Structure):
     ]
 
 
+class TpBsParams(ctypes.Structure):
+    _fields_ = [
+        ("rtol", ctypes.c_double),
+        ("atol", ctypes.c_double),
+        ("qm", ctypes.c_double),
+        ("t_end", ctypes.c_double),
+        ("t_end_gyro", ctypes.c_double),
+        ("b_ref", ctypes.c_double),
+        ("h0", ctypes.c_double),
+        ("kmax", ctypes.c_int),
+        ("max_attempts", ctypes.c_int),
+        ("block", ctypes.c_int),
+        ("waves_per_cu", ctypes.c_int),
+        ("use_graph", ctypes.c_int),
+    ]
+
+
+class TpBsStats(ctypes.Structure):
+    _fields_ = [
+        ("accepted", ctypes.c_longlong),
+        ("rejected", ctypes.c_longlong),
+        ("rhs_evals", ctypes.c_longlong),
+        ("failed", ctypes.c_int),
+        ("launches", ctypes.c_int),
+        ("kernel_ms", ctypes.c_double),
+    ]
+
+
+def make_bs_params(*, rtol=1e-8, atol=1e-10, qm=1.0, t_end=0.0, t_end_gyro=0.0, b_ref=1.0,
+                   h0=0.0, kmax=8, max_attempts=0, block=0, waves_per_cu=0, use_graph=1):
+    p = TpBsParams()
+    p.rtol, p.atol, p.qm = rtol, atol, qm
+    p.t_end, p.t_end_gyro, p.b_ref, p.h0 = t_end, t_end_gyro, b_ref, h0
+    p.kmax, p.max_attempts, p.block, p.waves_per_cu, p.use_graph = (
+        kmax, max_attempts, block, waves_per_cu, use_graph)
+    return p
+
+
 def make_params(*, dt, estep, lo, hi, qm=1.0, pusher=TP_PUSH_WIRZ,
                 precision=TP_FP64, bc_flag=TP_BC_PERIODIC, nbins=102,
                 nt_out=1, emin=1e-3, emax=1e1, sstep=1, einterval_t=None,

@@ -25,61 +25,8 @@
 #include "../../include/tp_gpu.h"
 #include "tp_device.h"
 
-enum { PREC64 = 0, PREC32 = 1 };
+#include "tp_ctx.h"
 
-/* ------------------------------------------------------------------ */
-/* context                                                             */
-/* ------------------------------------------------------------------ */
-struct State {
-    int cap = 0, n = 0;
-    double *buf = nullptr;   /* all double arrays, 26 * cap */
-    int *ibuf = nullptr;     /* esc, nsteps: 2 * cap */
-    tp_particle *stage = nullptr;
-    double *x, *y, *z, *ux, *uy, *uz, *t, *xs, *ys, *zs;
-    double *x0, *y0, *z0, *ux0, *uy0, *uz0, *t0, *g0, *mu0, *th0;
-    double *st;              /* 5 * cap stale D terms */
-    int *esc, *nsteps;
-    bool derived = false;    /* g0/mu0/th0 valid for current field */
-};
-
-struct tp_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    FieldDev f;
-    bool have_field = false;
-    /* owned field buffers */
-    float4 *grid = nullptr;
-    double *vgrid = nullptr, *modes = nullptr;
-    WlcsSys *wl = nullptr;
-    State s;
-    /* diagnostics of the last run */
-    int nb = 0, nt = 0, nd = 0;
-    unsigned *espect = nullptr, *espect_esc = nullptr;
-    double *dco = nullptr, *partials = nullptr;
-    int *rdec = nullptr;
-    size_t partials_cap = 0, diag_cap = 0, dco_cap = 0;
-    int last_n = 0;
-    double last_ms = 0.0;
-    int last_launches = 0;
-    int num_cu = 0;
-    hipEvent_t *evs = nullptr;   /* start/stop per step-kernel launch */
-    int nev = 0;
-};
-
-#define HIPCHK(x)                                                        \
-    do {                                                                 \
-        hipError_t e_ = (x);                                             \
-        if (e_ != hipSuccess) {                                          \
-            fprintf(stderr, "tp_gpu: %s failed: %s (%s:%d)\n", #x,       \
-                    hipGetErrorString(e_), __FILE__, __LINE__);          \
-            return TP_E_HIP;                                             \
-        }                                                                \
-    } while (0)
-
-/* ------------------------------------------------------------------ */
-/* device helpers                                                      */
-/* ------------------------------------------------------------------ */
 /* ptl_energy_fixed binning, src/diagnostics.c:193-203 */
 __device__ __noinline__ int energy_bin(double ux, double uy, double uz,
                                           int nbins, double logemin, double logde)
@@ -378,7 +325,7 @@ __global__ void __launch_bounds__(256) k_eval(FieldDev f, int n, const double *p
 /* ------------------------------------------------------------------ */
 /* dispatch                                                            */
 /* ------------------------------------------------------------------ */
-static int field_kind(const tp_ctx *c) { return c->f.kind; }
+
 
 typedef void (*track_fn)(dim3, dim3, size_t, hipStream_t, const TrackArgs &);
 
@@ -547,6 +494,8 @@ void tp_gpu_finalize(tp_ctx *c)
     hipFree(c->rdec);
     for (int k = 0; k < c->nev; k++) hipEventDestroy(c->evs[k]);
     free(c->evs);
+    tp_bs_release(c);
+    free(c->bs_tune_block);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
     if (c->stream) hipStreamDestroy(c->stream);

@@ -110,4 +110,65 @@ TP_HD double tp_cos(double x)
     return c;
 }
 
+/*
+ * Deterministic log2 / exp2 / pow for the adaptive stepper's controller
+ * (step-size factors err^(1/(2k+1))).  Only reproducibility between host and
+ * device matters there; accuracy is ~1e-15 relative.
+ *   log2: frexp into [sqrt(1/2), sqrt(2)), ln m = 2 atanh(s) with
+ *         s = (m-1)/(m+1), odd series to s^23;
+ *   exp2: 2^y = 2^n e^(f ln 2) with n = rint(y), Taylor series to r^14.
+ * Arguments: x > 0 finite (log2, pow); |y| < 1000 (exp2).
+ */
+TP_HD double tp_log2(double x)
+{
+    int e;
+    double m = frexp(x, &e);
+    if (m < 0x1.6a09e667f3bcdp-1) {   /* sqrt(1/2) */
+        m = 2.0 * m;
+        e = e - 1;
+    }
+    double s = (m - 1.0) / (m + 1.0);
+    double s2 = s * s;
+    double p = 0x1.642c8590b2164p-5;              /* 1/23 */
+    p = fma(p, s2, 0x1.8618618618618p-5);         /* 1/21 */
+    p = fma(p, s2, 0x1.af286bca1af28p-5);         /* 1/19 */
+    p = fma(p, s2, 0x1.e1e1e1e1e1e1ep-5);         /* 1/17 */
+    p = fma(p, s2, 0x1.1111111111111p-4);         /* 1/15 */
+    p = fma(p, s2, 0x1.3b13b13b13b14p-4);         /* 1/13 */
+    p = fma(p, s2, 0x1.745d1745d1746p-4);         /* 1/11 */
+    p = fma(p, s2, 0x1.c71c71c71c71cp-4);         /* 1/9 */
+    p = fma(p, s2, 0x1.2492492492492p-3);         /* 1/7 */
+    p = fma(p, s2, 0x1.999999999999ap-3);         /* 1/5 */
+    p = fma(p, s2, 0x1.5555555555555p-2);         /* 1/3 */
+    double lnm = 2.0 * s + 2.0 * s * s2 * p;
+    return (double)e + lnm * 0x1.71547652b82fep+0; /* 1/ln 2 */
+}
+
+TP_HD double tp_exp2(double y)
+{
+    double n = rint(y);
+    double r = (y - n) * 0x1.62e42fefa39efp-1;    /* ln 2 */
+    double p = 0x1.93974a8c07c9dp-37;              /* 1/14! */
+    p = fma(p, r, 0x1.6124613a86d09p-33);          /* 1/13! */
+    p = fma(p, r, 0x1.1eed8eff8d898p-29);          /* 1/12! */
+    p = fma(p, r, 0x1.ae64567f544e4p-26);          /* 1/11! */
+    p = fma(p, r, 0x1.27e4fb7789f5cp-22);          /* 1/10! */
+    p = fma(p, r, 0x1.71de3a556c734p-19);          /* 1/9! */
+    p = fma(p, r, 0x1.a01a01a01a01ap-16);          /* 1/8! */
+    p = fma(p, r, 0x1.a01a01a01a01ap-13);          /* 1/7! */
+    p = fma(p, r, 0x1.6c16c16c16c17p-10);          /* 1/6! */
+    p = fma(p, r, 0x1.1111111111111p-7);           /* 1/5! */
+    p = fma(p, r, 0x1.5555555555555p-5);           /* 1/4! */
+    p = fma(p, r, 0x1.5555555555555p-3);           /* 1/3! */
+    p = fma(p, r, 0.5);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    return ldexp(p, (int)n);
+}
+
+TP_HD double tp_pow(double x, double y)
+{
+    return tp_exp2(y * tp_log2(x));
+}
+
 #endif /* TP_MATH_H */

@@ -44,6 +44,10 @@ def lib():
         L.tp_gpu_store_particles.argtypes = [vp, P(d), P(i)]
         L.tp_gpu_read_diagnostics.argtypes = [vp, P(d), P(d), P(d), P(i)]
         L.tp_gpu_last_kernel_time.argtypes = [vp, P(d), P(i)]
+        L.tp_gpu_run_adaptive.argtypes = [vp, P(abi.TpBsParams), P(abi.TpBsStats)]
+        L.tp_gpu_track_adaptive.argtypes = [vp, P(abi.TpBsParams), i, P(d), P(i),
+                                            P(abi.TpBsStats)]
+        L.tp_gpu_tune_adaptive.argtypes = [vp, P(abi.TpBsParams), ctypes.c_char_p, P(i), P(i)]
         cs = ctypes.c_char_p
         L.tp_host_spectrum_bins.argtypes = [i, d, d, P(d), P(d)]
         L.tp_host_spectrum_t0.argtypes = [i, P(d), i, d, d, P(d)]
@@ -188,6 +192,32 @@ class Tracker:
         _check(lib().tp_gpu_read_diagnostics(self._ctx, dptr(esp), dptr(ese), dptr(dco),
                                              iptr(rem)), "read_diagnostics")
         return dict(espect=esp, espect_escape=ese, dcoeff=dco, remain=rem)
+
+    # --- adaptive Bulirsch-Stoer ------------------------------------------
+    def run_adaptive(self, params):
+        st = abi.TpBsStats()
+        _check(lib().tp_gpu_run_adaptive(self._ctx, ctypes.byref(params), ctypes.byref(st)),
+               "run_adaptive")
+        return dict(accepted=st.accepted, rejected=st.rejected, rhs=st.rhs_evals,
+                    failed=st.failed, kernel_ms=st.kernel_ms)
+
+    def track_adaptive(self, params, ptl):
+        ptl = abi.as_particles(ptl).copy()
+        nsteps = np.zeros(ptl.shape[0], dtype=np.int32)
+        st = abi.TpBsStats()
+        _check(lib().tp_gpu_track_adaptive(self._ctx, ctypes.byref(params), ptl.shape[0],
+                                           dptr(ptl), iptr(nsteps), ctypes.byref(st)),
+               "track_adaptive")
+        return dict(ptl=ptl, nsteps=nsteps, accepted=st.accepted, rejected=st.rejected,
+                    rhs=st.rhs_evals, failed=st.failed, kernel_ms=st.kernel_ms)
+
+    def tune_adaptive(self, params, cache_path=None):
+        b = ctypes.c_int()
+        w = ctypes.c_int()
+        cp = cache_path.encode() if cache_path else None
+        _check(lib().tp_gpu_tune_adaptive(self._ctx, ctypes.byref(params), cp, ctypes.byref(b),
+                                          ctypes.byref(w)), "tune_adaptive")
+        return b.value, w.value
 
     def last_kernel_time(self):
         ms = ctypes.c_double()
