@@ -447,7 +447,7 @@ static void get_emf(const tpref_field *f, double x, double y, double z,
         for (int n = 0; n < f->nmodes; n++) {
             const double *m = f->modes + 7 * n;
             double ph = fma(m[2], z, fma(m[1], y, fma(m[0], x, m[6])));
-            double cs = g_use_libm ? cos(ph) : tp_cos(ph);
+            double cs = g_use_libm ? cos(ph) : tp_cos_pi(ph);
             bx = fma(m[3], cs, bx);
             by = fma(m[4], cs, by);
             bz = fma(m[5], cs, bz);

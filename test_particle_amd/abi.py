@@ -11,7 +11,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libtpgpu.so")
+# TPGPU_LIB selects another build of the library (A/B timing of variants)
+LIB_PATH = os.environ.get("TPGPU_LIB") or os.path.join(HERE, "libtpgpu.so")
 
 PARTICLE_FIELDS = ("x", "y", "z", "qm", "ux", "uy", "uz", "t",
                    "xshift", "yshift", "zshift")

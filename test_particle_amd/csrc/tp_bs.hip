@@ -104,9 +104,11 @@ struct BsLane {
     int phase, k, nn, nstep, ktarg, attempts, acc, rej;
     int first, last, reject, prev_reject;
     double t, tend, h, hnew, hnext, hsub, xe;
-    double yv[BS_N], ysav[BS_N], ycur[BS_N], dydx[BS_N], ym[BS_N], yn[BS_N];
+    /* yv: the last accepted state, which is also the start of the step being
+     * attempted (the reference's ysav) */
+    double yv[BS_N], ycur[BS_N], dydx[BS_N], ym[BS_N], yn[BS_N];
     double tab[8][BS_N], hopt[9], work[9];
-    long long rhs, tot_acc, tot_rej, tot_rhs, tot_fail;
+    int rhs;
 };
 
 __device__ __forceinline__ void bs_finish(const BsArgs &a, BsLane &L, int failed)
@@ -117,10 +119,10 @@ __device__ __forceinline__ void bs_finish(const BsArgs &a, BsLane &L, int failed
     a.t[i] = L.t;
     a.nsteps[i] = L.acc;
     a.flag[i] = failed;
-    L.tot_acc += L.acc;
-    L.tot_rej += L.rej;
-    L.tot_rhs += L.rhs;
-    L.tot_fail += failed;
+    atomicAdd(&a.ctr[Q_ACC], (unsigned long long)L.acc);
+    atomicAdd(&a.ctr[Q_REJ], (unsigned long long)L.rej);
+    atomicAdd(&a.ctr[Q_RHS], (unsigned long long)L.rhs);
+    if (failed) atomicAdd(&a.ctr[Q_FAIL], 1ull);
     L.pi = -1;
 }
 
@@ -129,8 +131,8 @@ __device__ __forceinline__ void bs_begin_seq(const BsArgs &a, BsLane &L)
     L.nstep = a.c.nseq[L.k];
     L.hsub = L.h / L.nstep;
     for (int i = 0; i < BS_N; i++) {
-        L.ym[i] = L.ysav[i];
-        L.yn[i] = L.ysav[i] + L.hsub * L.dydx[i];
+        L.ym[i] = L.yv[i];
+        L.yn[i] = L.yv[i] + L.hsub * L.dydx[i];
     }
     L.xe = L.t + L.hsub;
     L.nn = 1;
@@ -159,7 +161,6 @@ __device__ __forceinline__ void bs_start_step(const BsArgs &a, BsLane &L)
     double h = L.hnext;
     if (L.t + h > L.tend) h = L.tend - L.t;
     L.work[0] = 0.0;
-    for (int i = 0; i < BS_N; i++) L.ysav[i] = L.yv[i];
     if (h != L.hnext && !L.first) L.last = 1;
     if (L.reject) {
         L.prev_reject = 1;
@@ -189,7 +190,7 @@ __device__ __forceinline__ void bs_end_seq(const BsArgs &a, BsLane &L, const dou
         L.ycur[i] = L.tab[0][i] + a.c.coeff[k][0] * (L.tab[0][i] - L.ycur[i]);
     double err = 0.0;
     for (int i = 0; i < BS_N; i++) {
-        double p = fabs(L.ysav[i]), q = fabs(L.ycur[i]);
+        double p = fabs(L.yv[i]), q = fabs(L.ycur[i]);
         double scale = a.atol + a.rtol * (p > q ? p : q);
         double r = (L.ycur[i] - L.tab[0][i]) / scale;
         err += r * r;
@@ -307,19 +308,12 @@ __device__ __forceinline__ void bs_accept(const BsArgs &a, BsLane &L, const doub
     bs_start_step(a, L);
 }
 
-__device__ __forceinline__ long long wave_sum_ll(long long v)
-{
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
 template <int FK>
 __global__ void __launch_bounds__(256) k_bs(BsArgs a)
 {
     const int lane = threadIdx.x & 63;
     BsLane L;
     L.pi = -1;
-    L.tot_acc = L.tot_rej = L.tot_rhs = L.tot_fail = 0;
     while (true) {
         /* refill idle lanes: one atomic per wavefront */
         unsigned long long need = __ballot(L.pi == -1);
@@ -358,13 +352,17 @@ __global__ void __launch_bounds__(256) k_bs(BsArgs a)
             }
         }
         if (__ballot(L.pi >= 0) == 0ull) break;
-        if (L.pi >= 0) {
-            /* the one right-hand-side evaluation of this iteration */
-            double ze[BS_N];
-            for (int i = 0; i < BS_N; i++)
-                ze[i] = L.phase == PH_INIT ? L.yv[i] : (L.phase == PH_MID ? L.yn[i] : L.ycur[i]);
-            double f[BS_N];
-            bs_rhs<FK>(a.f, a.qm, L.xe, ze, f);
+        /* the one right-hand-side evaluation of this iteration, executed by
+         * the whole wavefront (retired lanes evaluate a dummy point) so the
+         * field sum stays in uniform control flow and its uniform table
+         * reads become scalar loads */
+        const bool act = L.pi >= 0;
+        double ze[BS_N];
+        for (int i = 0; i < BS_N; i++)
+            ze[i] = !act ? 0.0 : (L.phase == PH_INIT ? L.yv[i] : (L.phase == PH_MID ? L.yn[i] : L.ycur[i]));
+        double f[BS_N];
+        bs_rhs<FK>(a.f, a.qm, act ? L.xe : 0.0, ze, f);
+        if (act) {
             L.rhs++;
             if (L.phase == PH_INIT) {
                 for (int i = 0; i < BS_N; i++) L.dydx[i] = f[i];
@@ -389,14 +387,6 @@ __global__ void __launch_bounds__(256) k_bs(BsArgs a)
                 bs_accept(a, L, f);
             }
         }
-    }
-    long long s0 = wave_sum_ll(L.tot_acc), s1 = wave_sum_ll(L.tot_rej);
-    long long s2 = wave_sum_ll(L.tot_rhs), s3 = wave_sum_ll(L.tot_fail);
-    if (lane == 0) {
-        atomicAdd(&a.ctr[Q_ACC], (unsigned long long)s0);
-        atomicAdd(&a.ctr[Q_REJ], (unsigned long long)s1);
-        atomicAdd(&a.ctr[Q_RHS], (unsigned long long)s2);
-        atomicAdd(&a.ctr[Q_FAIL], (unsigned long long)s3);
     }
 }
 

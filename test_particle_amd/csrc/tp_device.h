@@ -333,8 +333,11 @@ __device__ __noinline__ void emf_wlcs(const FieldDev &f, double x, double y,
                                       double z, double t, Emf &o)
 {
     double Bx = 0.0, By = 0.0, Bz = 0.0, Ex = 0.0, Ey = 0.0, Ez = 0.0;
+    /* read-only table, uniform index: constant address space (scalar loads) */
+    typedef const WlcsSys __attribute__((address_space(4))) *cwl;
+    const cwl wl = (cwl)f.wl;
     for (int i = 0; i < f.nwlcs; i++) {
-        const WlcsSys s = f.wl[i];
+        const WlcsSys s = wl[i];
         double xp = x - s.x_wr, yp = y - s.y_wr, zp = z - s.z_wr;
         wl_transform(&xp, &yp, &zp, s.ct, s.st, s.cp, s.sp);
         if (f.ibessel == 0) {
@@ -484,11 +487,28 @@ __device__ __forceinline__ void get_emf64(const FieldDev &f, double x, double y,
         o.ex = ax; o.ey = ay; o.ez = az;
         o.bx = bx; o.by = by; o.bz = bz;
     } else if (FK == FK_MODES) {
+        /* four independent cosines per iteration (instruction-level
+         * parallelism for the FP64 FMA chains), summed in mode order */
         double bx = 0.0, by = 0.0, bz = 0.0;
-        for (int n = 0; n < f.nmodes; n++) {
-            const double *m = f.modes + 7 * n;
-            double ph = fma(m[2], z, fma(m[1], y, fma(m[0], x, m[6])));
-            double cs = tp_cos(ph);
+        const int n4 = f.nmodes & ~3;
+        /* the table is read-only for the whole kernel: read it through the
+         * constant address space so uniform reads become scalar loads */
+        typedef const double __attribute__((address_space(4))) *ctab;
+        const ctab tab = (ctab)f.modes;
+        for (int n = 0; n < n4; n += 4) {
+            const ctab m = tab + 7 * n;
+            double c0 = tp_cos_pi(fma(m[2], z, fma(m[1], y, fma(m[0], x, m[6]))));
+            double c1 = tp_cos_pi(fma(m[9], z, fma(m[8], y, fma(m[7], x, m[13]))));
+            double c2 = tp_cos_pi(fma(m[16], z, fma(m[15], y, fma(m[14], x, m[20]))));
+            double c3 = tp_cos_pi(fma(m[23], z, fma(m[22], y, fma(m[21], x, m[27]))));
+            bx = fma(m[3], c0, bx); by = fma(m[4], c0, by); bz = fma(m[5], c0, bz);
+            bx = fma(m[10], c1, bx); by = fma(m[11], c1, by); bz = fma(m[12], c1, bz);
+            bx = fma(m[17], c2, bx); by = fma(m[18], c2, by); bz = fma(m[19], c2, bz);
+            bx = fma(m[24], c3, bx); by = fma(m[25], c3, by); bz = fma(m[26], c3, bz);
+        }
+        for (int n = n4; n < f.nmodes; n++) {
+            const ctab m = tab + 7 * n;
+            double cs = tp_cos_pi(fma(m[2], z, fma(m[1], y, fma(m[0], x, m[6]))));
             bx = fma(m[3], cs, bx);
             by = fma(m[4], cs, by);
             bz = fma(m[5], cs, bz);

@@ -111,6 +111,36 @@ TP_HD double tp_cos(double x)
 }
 
 /*
+ * cos(x) alone, for sums over many modes: reduction by pi instead of pi/2
+ * (k = rint(x/pi), r = x - k pi in [-pi/2, pi/2], three-part constant, the
+ * first product exact by the same fma argument as above), then one even
+ * Taylor polynomial to r^22 (truncation < 4e-20 on that interval) and the
+ * sign (-1)^k.  About 40 % fewer operations than tp_sincos.
+ */
+TP_HD double tp_cos_pi(double x)
+{
+    double k = rint(x * 0x1.45f306dc9c883p-2);             /* 1/pi */
+    double r = fma(-k, 0x1.921fb54442d18p+1, x);            /* exact */
+    r = fma(-k, 0x1.1a62633145c07p-53, r);
+    r = fma(-k, -0x1.f1976b7ed8fbcp-109, r);
+    double z = r * r;
+    double p = 0x1.0ce396db7f853p-70;                       /*  1/22! */
+    p = fma(p, z, -0x1.e542ba4020225p-62);                  /* -1/20! */
+    p = fma(p, z, 0x1.6827863b97d97p-53);                   /*  1/18! */
+    p = fma(p, z, -0x1.ae7f3e733b81fp-45);                  /* -1/16! */
+    p = fma(p, z, 0x1.93974a8c07c9dp-37);                   /*  1/14! */
+    p = fma(p, z, -0x1.1eed8eff8d898p-29);                  /* -1/12! */
+    p = fma(p, z, 0x1.27e4fb7789f5cp-22);                   /*  1/10! */
+    p = fma(p, z, -0x1.a01a01a01a01ap-16);                  /* -1/8!  */
+    p = fma(p, z, 0x1.6c16c16c16c17p-10);                   /*  1/6!  */
+    p = fma(p, z, -0x1.5555555555555p-5);                   /* -1/4!  */
+    p = fma(p, z, 0.5);
+    double c = fma(-z, p, 1.0);                             /* 1 - z/2 + z^2/24 - ... */
+    double half = k * 0.5;
+    return (half == floor(half)) ? c : -c;
+}
+
+/*
  * Deterministic log2 / exp2 / pow for the adaptive stepper's controller
  * (step-size factors err^(1/(2k+1))).  Only reproducibility between host and
  * device matters there; accuracy is ~1e-15 relative.

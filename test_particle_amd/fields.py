@@ -46,6 +46,16 @@ def mode_table(nmodes, kmin, kmax, rms, seed=0, index=-5.0 / 3.0):
     return np.ascontiguousarray(rows, dtype=np.float64)
 
 
+def turbulence_modes(nmodes=1000, decades=3.0, kmin_rg=0.1, vmax=0.9, rms=0.5, b0=1.0,
+                     seed=20261202, index=-5.0 / 3.0):
+    """The adaptive-stepper workload's field: nmodes modes with k log-spaced
+    over `decades` decades from kmin = kmin_rg / r_g(vmax), r_g = gamma v /
+    (|q/m| B0) with q/m = 1; rms |dB| = rms over the guide field b0."""
+    u = vmax / np.sqrt(1.0 - vmax * vmax)
+    kmin = kmin_rg / (u / b0)
+    return mode_table(nmodes, kmin, kmin * 10.0 ** decades, rms, seed=seed, index=index)
+
+
 def eval_modes_numpy(rows, pts, b0z):
     """Direct FP64 sum of the mode table at pts (n, 3): the brute-force
     check of the in-kernel evaluator."""

@@ -16,12 +16,17 @@ import numpy as np
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["grid", "abc", "copy"])
+    ap.add_argument("what", choices=["grid", "abc", "copy", "modes"])
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--particles", type=int, default=1 << 20)
     ap.add_argument("--pusher", default="boris", choices=["boris", "wirz"])
     ap.add_argument("--fp32", action="store_true")
     ap.add_argument("--mib", type=int, default=1024)
+    ap.add_argument("--nmodes", type=int, default=1000)
+    ap.add_argument("--tgyro", type=float, default=0.05, help="adaptive horizon (gyroperiods)")
+    ap.add_argument("--tune", action="store_true")
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--wpc", type=int, default=0)
     a = ap.parse_args()
     if a.what == "copy":
         import torch
@@ -38,6 +43,29 @@ def main():
     from .gpu import Tracker
     from .particles import init_particles
     tr = Tracker(0)
+    if a.what == "modes":
+        from .fields import turbulence_modes
+        rows = turbulence_modes(a.nmodes)
+        tr.set_field_modes(rows, 1.0)
+        ptl = init_particles(a.particles, (0, 0, 0), (100, 100, 100), 0.1, kind="loguniform",
+                             vmax=0.9)
+        tr.load(ptl)
+        prm = abi.make_bs_params(rtol=1e-8, atol=1e-10, t_end_gyro=a.tgyro, b_ref=1.0,
+                                 block=a.block, waves_per_cu=a.wpc)
+        if a.tune:
+            t0 = time.perf_counter()
+            b, w = tr.tune_adaptive(prm)
+            print(f"tuned: block {b}, waves/CU {w} ({time.perf_counter() - t0:.1f}s)", flush=True)
+            tr.load(ptl)
+        t0 = time.perf_counter()
+        st = tr.run_adaptive(prm)
+        wall = time.perf_counter() - t0
+        print(f"modes({a.nmodes}): {a.particles} particles to {a.tgyro} T_g: "
+              f"{st['accepted'] / wall:.4e} accepted steps/s, {st['rhs'] / wall:.4e} RHS/s, "
+              f"rejected {st['rejected']}, failed {st['failed']}, kernel {st['kernel_ms']:.1f} ms, "
+              f"wall {wall:.2f}s", flush=True)
+        tr.close()
+        return
     pusher = abi.TP_PUSH_BORIS if a.pusher == "boris" else abi.TP_PUSH_WIRZ
     prec = abi.TP_FP32_MIXED if a.fp32 else abi.TP_FP64
     if a.what == "grid":
