@@ -333,11 +333,8 @@ __device__ __noinline__ void emf_wlcs(const FieldDev &f, double x, double y,
                                       double z, double t, Emf &o)
 {
     double Bx = 0.0, By = 0.0, Bz = 0.0, Ex = 0.0, Ey = 0.0, Ez = 0.0;
-    /* read-only table, uniform index: constant address space (scalar loads) */
-    typedef const WlcsSys __attribute__((address_space(4))) *cwl;
-    const cwl wl = (cwl)f.wl;
     for (int i = 0; i < f.nwlcs; i++) {
-        const WlcsSys s = wl[i];
+        const WlcsSys s = f.wl[i];
         double xp = x - s.x_wr, yp = y - s.y_wr, zp = z - s.z_wr;
         wl_transform(&xp, &yp, &zp, s.ct, s.st, s.cp, s.sp);
         if (f.ibessel == 0) {
