@@ -231,6 +231,66 @@ int tp_gpu_tune_adaptive(tp_ctx *ctx, const tp_bs_params *p, const char *cache_p
 int tp_gpu_eval_field(tp_ctx *ctx, int n, const double *pts, double *out,
                       int precision);
 
+/* --- learned field surrogate ---------------------------------------------
+ * No counterpart in the reference (its closest slot is the field dispatch
+ * get_emf, src/emfields.c:32-60).  A coordinate network stands in for the
+ * multi-mode field of tp_gpu_set_field_modes:
+ *   f   = [sin 2 pi (q_j . x), cos 2 pi (q_j . x)]   j < 128 (256 features)
+ *   h1  = gelu(f W1 + b1), h2 = gelu(h1 W2 + b2), h3 = gelu(h2 W3 + b3)  (128 each)
+ *   B   = b0z z-hat + scale (h3 W4 + b4)
+ * with gelu(z) = z / (1 + exp(-2 sqrt(2/pi) (z + 0.044715 z^3))).  Layers
+ * 1-3 run on the matrix cores in bfloat16 with FP32 accumulation; layer 4,
+ * the biases and the frequencies are FP32.  The frequencies q_j = n_j / L
+ * with integer n_j make the network periodic in the box L.
+ * params: TP_MLP_NPARAMS floats, row index = input:
+ *   W1[256][128] b1[128] W2[128][128] b2[128] W3[128][128] b3[128] W4[128][3] b4[3] */
+#define TP_MLP_NFREQ    128
+#define TP_MLP_NPARAMS  66307
+
+typedef struct tp_mlp_train_params {
+    long long nsamples;   /* training set size (points drawn uniformly in [0, period)^3) */
+    int batch;            /* samples per optimiser step, a multiple of 256 */
+    int epochs;           /* passes over the training set */
+    double period;        /* L, the box of the samples */
+    double lr, lr_min;    /* Adam step size, cosine-annealed from lr to lr_min */
+    double beta1, beta2, eps;
+    unsigned long long seed;  /* sample positions: splitmix64 of (seed, index) */
+} tp_mlp_train_params;
+
+typedef struct tp_mlp_train_stats {
+    double first_loss;    /* mean loss of the first epoch's first step */
+    double last_loss;     /* mean loss over the last epoch */
+    long long steps;      /* optimiser steps taken */
+    double sample_ms;     /* time to generate the training set (exact FP64 field) */
+    double train_ms;      /* time of the optimiser steps */
+} tp_mlp_train_stats;
+
+/* Sets the network (host arrays: q[3*128] cycles per unit length,
+ * params[TP_MLP_NPARAMS]) and clears the optimiser state. */
+int tp_gpu_mlp_set(tp_ctx *ctx, const float *q, const float *params, double scale);
+/* Copies the current FP32 parameters out. */
+int tp_gpu_mlp_get(tp_ctx *ctx, float *params);
+/* Network outputs (before scale and guide field) at n points x[3*i]. */
+int tp_gpu_mlp_forward(tp_ctx *ctx, int n, const double *x, double *out);
+/* Mean-squared-error loss of the outputs against y[3*i] over n samples and
+ * its gradient with respect to the parameters (no update). */
+int tp_gpu_mlp_loss_grad(tp_ctx *ctx, int n, const float *x, const float *y, float *grad,
+                         double *loss);
+/* Trains the network on samples of the context's mode field (set by
+ * tp_gpu_set_field_modes; targets are the exact FP64 mode sums divided by
+ * scale, without the guide field) with the kernels above and Adam. */
+int tp_gpu_train_surrogate(tp_ctx *ctx, const tp_mlp_train_params *p,
+                           tp_mlp_train_stats *stats);
+/* Makes the network the field source of tp_gpu_run_fixed (FP32 Boris only),
+ * with the context's modes as the exact field.  Each particle compares the
+ * network with the exact FP32 mode sum at its start and every
+ * check_interval steps (0: start only); once |B_net - B_exact| / |B_exact|
+ * exceeds tol it uses the exact field for the rest of the run.  The D
+ * steps' pitch angles use the exact FP64 field. */
+int tp_gpu_set_field_mlp(tp_ctx *ctx, double tol, int check_interval);
+/* Particles that switched to the exact field during the last run. */
+int tp_gpu_mlp_fallback_count(tp_ctx *ctx, long long *count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -75,6 +75,12 @@ def lib():
         L.tpref_sincos.argtypes = [d, P(d), P(d)]
         L.tpref_ellint.argtypes = [d, P(d), P(d)]
         L.tpref_ellint_ref.argtypes = [d, P(d), P(d)]
+        L.tpref_mlp_nparams.restype = i
+        L.tpref_bf16_round.argtypes = [d]
+        L.tpref_bf16_round.restype = d
+        L.tpref_gelu.argtypes = [i, P(d), P(d), P(d)]
+        L.tpref_mlp_forward.argtypes = [P(d), P(d), i, P(d), P(d), i]
+        L.tpref_mlp_loss_grad.argtypes = [P(d), P(d), i, P(d), P(d), P(d), P(d), i]
         _lib = L
     return _lib
 
@@ -253,3 +259,46 @@ def ellint(k, ref_compat=False):
     fn = lib().tpref_ellint_ref if ref_compat else lib().tpref_ellint
     fn(float(k), ctypes.byref(a), ctypes.byref(b))
     return a.value, b.value   # (E, K)
+
+
+# --- learned field surrogate (oracle/mlpref.c) -----------------------------------
+
+def mlp_nparams():
+    return lib().tpref_mlp_nparams()
+
+
+def bf16_round(v):
+    return np.array([lib().tpref_bf16_round(float(x)) for x in np.ravel(v)])
+
+
+def gelu(z):
+    z = np.ascontiguousarray(z, dtype=np.float64).ravel()
+    g = np.empty_like(z)
+    gd = np.empty_like(z)
+    lib().tpref_gelu(z.size, abi.dptr(z), abi.dptr(g), abi.dptr(gd))
+    return g, gd
+
+
+def mlp_forward(params, q, x, emulate=False):
+    """Network outputs (n, 3) before the output scale and the guide field."""
+    params = np.ascontiguousarray(params, dtype=np.float64)
+    q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1, 3)
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+    out = np.empty((x.shape[0], 3))
+    lib().tpref_mlp_forward(abi.dptr(params), abi.dptr(q), x.shape[0], abi.dptr(x),
+                            abi.dptr(out), 1 if emulate else 0)
+    return out
+
+
+def mlp_loss_grad(params, q, x, y, emulate=False):
+    """(mean squared error, gradient over the flat parameter vector)."""
+    params = np.ascontiguousarray(params, dtype=np.float64)
+    q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1, 3)
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+    y = np.ascontiguousarray(y, dtype=np.float64).reshape(-1, 3)
+    g = np.empty(params.size)
+    loss = ctypes.c_double()
+    lib().tpref_mlp_loss_grad(abi.dptr(params), abi.dptr(q), x.shape[0], abi.dptr(x),
+                              abi.dptr(y), abi.dptr(g), ctypes.byref(loss),
+                              1 if emulate else 0)
+    return loss.value, g

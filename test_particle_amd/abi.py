@@ -92,6 +92,43 @@ class TpBsStats(ctypes.Structure):
     ]
 
 
+TP_MLP_NFREQ = 128
+TP_MLP_NPARAMS = 66307
+
+
+class TpMlpTrainParams(ctypes.Structure):
+    _fields_ = [
+        ("nsamples", ctypes.c_longlong),
+        ("batch", ctypes.c_int),
+        ("epochs", ctypes.c_int),
+        ("period", ctypes.c_double),
+        ("lr", ctypes.c_double),
+        ("lr_min", ctypes.c_double),
+        ("beta1", ctypes.c_double),
+        ("beta2", ctypes.c_double),
+        ("eps", ctypes.c_double),
+        ("seed", ctypes.c_ulonglong),
+    ]
+
+
+class TpMlpTrainStats(ctypes.Structure):
+    _fields_ = [
+        ("first_loss", ctypes.c_double),
+        ("last_loss", ctypes.c_double),
+        ("steps", ctypes.c_longlong),
+        ("sample_ms", ctypes.c_double),
+        ("train_ms", ctypes.c_double),
+    ]
+
+
+def make_mlp_train_params(*, nsamples, period, batch=16384, epochs=4, lr=2e-3, lr_min=2e-5,
+                          beta1=0.9, beta2=0.999, eps=1e-8, seed=1):
+    p = TpMlpTrainParams()
+    p.nsamples, p.batch, p.epochs, p.period = int(nsamples), int(batch), int(epochs), period
+    p.lr, p.lr_min, p.beta1, p.beta2, p.eps, p.seed = lr, lr_min, beta1, beta2, eps, seed
+    return p
+
+
 def make_bs_params(*, rtol=1e-8, atol=1e-10, qm=1.0, t_end=0.0, t_end_gyro=0.0, b_ref=1.0,
                    h0=0.0, kmax=8, max_attempts=0, block=0, waves_per_cu=0, use_graph=1):
     p = TpBsParams()

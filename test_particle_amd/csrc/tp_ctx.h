@@ -35,6 +35,7 @@ struct tp_ctx {
     /* owned field buffers */
     float4 *grid = nullptr;
     double *vgrid = nullptr, *modes = nullptr;
+    float *modes32 = nullptr;
     WlcsSys *wl = nullptr;
     State s;
     /* diagnostics of the last run */
@@ -54,9 +55,12 @@ struct tp_ctx {
     hipGraphExec_t bs_graph = nullptr;
     unsigned long long bs_graph_key = 0;
     int *bs_tune_block = nullptr;
+    /* learned surrogate: network, optimiser state and training buffers */
+    struct MlpState *mlp = nullptr;
 };
 
 void tp_bs_release(tp_ctx *c);
+void tp_mlp_release(tp_ctx *c);
 
 #define HIPCHK(x)                                                        \
     do {                                                                 \

@@ -25,6 +25,26 @@ enum {
     FK_GRID_TB = 4,   /* PIC grid, textbook gather */
     FK_MODES = 5,     /* multi-mode turbulence */
     FK_WLCS = 6,      /* wire-loop current systems */
+    FK_MLP = 7,       /* learned surrogate of the modes, exact modes as fallback */
+};
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+/* A surrogate network on the device (layout in tp_mlp.hip): bfloat16 weight
+ * fragments of layers 1-3 in the order the matrix-core kernels read them,
+ * FP32 biases, layer 4 and Fourier frequencies, the residual check and the
+ * per-particle fallback flags. */
+struct MlpNet {
+    const bf16x8 *w1f, *w2f, *w3f, *w2b, *w3b;
+    const float *b1, *b2, *b3, *w4, *b4;
+    const float *q;              /* 128 x 3 frequencies (cycles per unit length) */
+    float scale;                 /* output scale */
+    float tol;                   /* residual tolerance |dB| / |B| */
+    int check;                   /* residual check every `check` steps (0: start only) */
+    int *fb;                     /* per particle: 1 = uses the exact field */
+    unsigned *nfb;               /* particles switched to the exact field */
 };
 
 /* One WLCS system with the trigonometric values read_wlcs precomputes
@@ -50,6 +70,9 @@ struct FieldDev {
     int nmodes;
     const double *modes;
     double b0z;
+    /* the same table for FP32: (kx ky kz phi) / 2 pi, ax ay az 0 per mode */
+    const float *modes32;
+    MlpNet mlp;
     /* WLCS */
     int nwlcs, ibessel, wlmode;
     const WlcsSys *wl;
@@ -483,7 +506,7 @@ __device__ __forceinline__ void get_emf64(const FieldDev &f, double x, double y,
         }
         o.ex = ax; o.ey = ay; o.ez = az;
         o.bx = bx; o.by = by; o.bz = bz;
-    } else if (FK == FK_MODES) {
+    } else if (FK == FK_MODES || FK == FK_MLP) {
         /* four independent cosines per iteration (instruction-level
          * parallelism for the FP64 FMA chains), summed in mode order */
         double bx = 0.0, by = 0.0, bz = 0.0;
@@ -515,6 +538,42 @@ __device__ __forceinline__ void get_emf64(const FieldDev &f, double x, double y,
     } else if (FK == FK_WLCS) {
         emf_wlcs(f, x, y, z, t, o);
     }
+}
+
+/* FP32 sum over the modes: the phase is formed in revolutions from the
+ * table's (k, phi) / 2 pi, reduced to [-1/2, 1/2] and passed to the hardware
+ * cosine, whose argument is in revolutions (v_cos_f32: cos(2 pi r)).  Four
+ * modes per iteration for instruction-level parallelism; the table is
+ * uniform and read through the constant address space (scalar loads). */
+__device__ __forceinline__ void modes_sum32(const float *tab32, int nmodes, float x, float y,
+                                            float z, float &bxo, float &byo, float &bzo)
+{
+    typedef const float __attribute__((address_space(4))) *ctab;
+    const ctab t = (ctab)tab32;
+    float bx = 0.f, by = 0.f, bz = 0.f;
+    const int n4 = nmodes & ~3;
+    for (int n = 0; n < n4; n += 4) {
+        const ctab m = t + 8 * n;
+        float u0 = fmaf(m[2], z, fmaf(m[1], y, fmaf(m[0], x, m[3])));
+        float u1 = fmaf(m[10], z, fmaf(m[9], y, fmaf(m[8], x, m[11])));
+        float u2 = fmaf(m[18], z, fmaf(m[17], y, fmaf(m[16], x, m[19])));
+        float u3 = fmaf(m[26], z, fmaf(m[25], y, fmaf(m[24], x, m[27])));
+        float c0 = __builtin_amdgcn_cosf(u0 - rintf(u0));
+        float c1 = __builtin_amdgcn_cosf(u1 - rintf(u1));
+        float c2 = __builtin_amdgcn_cosf(u2 - rintf(u2));
+        float c3 = __builtin_amdgcn_cosf(u3 - rintf(u3));
+        bx = fmaf(m[4], c0, bx); by = fmaf(m[5], c0, by); bz = fmaf(m[6], c0, bz);
+        bx = fmaf(m[12], c1, bx); by = fmaf(m[13], c1, by); bz = fmaf(m[14], c1, bz);
+        bx = fmaf(m[20], c2, bx); by = fmaf(m[21], c2, by); bz = fmaf(m[22], c2, bz);
+        bx = fmaf(m[28], c3, bx); by = fmaf(m[29], c3, by); bz = fmaf(m[30], c3, bz);
+    }
+    for (int n = n4; n < nmodes; n++) {
+        const ctab m = t + 8 * n;
+        float u = fmaf(m[2], z, fmaf(m[1], y, fmaf(m[0], x, m[3])));
+        float c = __builtin_amdgcn_cosf(u - rintf(u));
+        bx = fmaf(m[4], c, bx); by = fmaf(m[5], c, by); bz = fmaf(m[6], c, bz);
+    }
+    bxo = bx; byo = by; bzo = bz;
 }
 
 /* ------------------------------------------------------------------ */
@@ -558,17 +617,9 @@ __device__ __forceinline__ void get_emf32(const FieldDev &f, double x, double y,
         }
         o.ex = ax; o.ey = ay; o.ez = az;
         o.bx = bx; o.by = by; o.bz = bz;
-    } else if (FK == FK_MODES) {
-        float bx = 0.f, by = 0.f, bz = 0.f;
-        float xf = (float)x, yf = (float)y, zf = (float)z;
-        for (int n = 0; n < f.nmodes; n++) {
-            const double *m = f.modes + 7 * n;
-            float ph = fmaf((float)m[2], zf, fmaf((float)m[1], yf, fmaf((float)m[0], xf, (float)m[6])));
-            float cs = cosf(ph);
-            bx = fmaf((float)m[3], cs, bx);
-            by = fmaf((float)m[4], cs, by);
-            bz = fmaf((float)m[5], cs, bz);
-        }
+    } else if (FK == FK_MODES || FK == FK_MLP) {
+        float bx, by, bz;
+        modes_sum32(f.modes32, f.nmodes, (float)x, (float)y, (float)z, bx, by, bz);
         o.bx = bx; o.by = by; o.bz = (float)f.b0z + bz;
         o.ex = 0.f; o.ey = 0.f; o.ez = 0.f;
     } else {
@@ -711,14 +762,11 @@ __device__ __forceinline__ void boris_step64(const FieldDev &f, GridCache &gc, P
     p.t += dt;
 }
 
-/* Boris in FP32 with FP64 position and time accumulators */
-template <int FK>
-__device__ __forceinline__ void boris_step32(const FieldDev &f, GridCache &gc, Ptl &p,
-                                             float &ux, float &uy, float &uz,
-                                             double dt, float dtf, float hf)
+/* Boris in FP32 with FP64 position and time accumulators, given the field
+ * at the particle */
+__device__ __forceinline__ void boris_push32(const EmfF &e, Ptl &p, float &ux, float &uy,
+                                             float &uz, double dt, float dtf, float hf)
 {
-    EmfF e;
-    emf32_step<FK>(f, gc, p.x, p.y, p.z, p.t, e);
     float umx = fmaf(hf, e.ex, ux);
     float umy = fmaf(hf, e.ey, uy);
     float umz = fmaf(hf, e.ez, uz);
@@ -743,6 +791,16 @@ __device__ __forceinline__ void boris_step32(const FieldDev &f, GridCache &gc, P
     p.y += (double)(uy * ig);
     p.z += (double)(uz * ig);
     p.t += dt;
+}
+
+template <int FK>
+__device__ __forceinline__ void boris_step32(const FieldDev &f, GridCache &gc, Ptl &p,
+                                             float &ux, float &uy, float &uz,
+                                             double dt, float dtf, float hf)
+{
+    EmfF e;
+    emf32_step<FK>(f, gc, p.x, p.y, p.z, p.t, e);
+    boris_push32(e, p, ux, uy, uz, dt, dtf, hf);
 }
 
 #endif /* TP_DEVICE_H */

@@ -27,54 +27,7 @@
 
 #include "tp_ctx.h"
 
-/* ptl_energy_fixed binning, src/diagnostics.c:193-203 */
-__device__ __noinline__ int energy_bin(double ux, double uy, double uz,
-                                          int nbins, double logemin, double logde)
-{
-    double gama = sqrt(1.0 + ux * ux + uy * uy + uz * uz);
-    double ene = gama - 1.0;
-    double v = floor((log10(ene) - logemin) / logde);
-    if (!(v > -2147483649.0 && v < 2147483648.0)) return 0;
-    int ib = (int)v;
-    if (ib < 0) ib = 0;
-    else if (ib > nbins - 1) ib = nbins - 1;
-    return ib;
-}
-
-struct TrackArgs {
-    FieldDev f;
-    int n, bc, nbins, nt_out, einterval, di, fix_stale, first;
-    int j0, j1, ntp0, nd;
-    double dt, qm, lo[3], hi[3], logemin, logde;
-    double *x, *y, *z, *ux, *uy, *uz, *t, *xs, *ys, *zs;
-    const double *x0, *y0, *z0, *ux0, *uy0, *uz0, *g0, *mu0, *th0;
-    double *st;
-    int *esc, *nsteps;
-    unsigned *espect, *espect_esc;
-    double *partials;
-    int *rdec;
-};
-
-/* particle_bc, src/tracking.c:1172-1221 */
-__device__ __forceinline__ void particle_bc(const TrackArgs &a, Ptl &p, double &xs,
-                                            double &ys, double &zs, int &esc)
-{
-    if (a.bc == 0) {
-        double xd = a.hi[0] - a.lo[0], yd = a.hi[1] - a.lo[1], zd = a.hi[2] - a.lo[2];
-        if (p.x > a.hi[0]) { p.x -= xd; xs += xd; }
-        if (p.y > a.hi[1]) { p.y -= yd; ys += yd; }
-        if (p.z > a.hi[2]) { p.z -= zd; zs += zd; }
-        if (p.x < a.lo[0]) { p.x += xd; xs -= xd; }
-        if (p.y < a.lo[1]) { p.y += yd; ys -= yd; }
-        if (p.z < a.lo[2]) { p.z += zd; zs -= zd; }
-    } else if (a.bc == 1) {
-        if (p.x > a.hi[0] || p.y > a.hi[1] || p.z > a.hi[2] ||
-            p.x < a.lo[0] || p.y < a.lo[1] || p.z < a.lo[2])
-            esc = 1;
-    } else if (a.bc == 2) {
-        esc = 0;
-    }
-}
+#include "tp_track.h"
 
 /* ------------------------------------------------------------------ */
 /* step kernel: steps j0..j1 of every particle                         */
@@ -377,6 +330,7 @@ static void dispatch_dcoeff(int fk, dim3 g, dim3 b, hipStream_t s, const TrackAr
     case FK_GRID_TB: launch_dcoeff<FK_GRID_TB>(g, b, s, a, tstep); break;
     case FK_MODES: launch_dcoeff<FK_MODES>(g, b, s, a, tstep); break;
     case FK_WLCS: launch_dcoeff<FK_WLCS>(g, b, s, a, tstep); break;
+    case FK_MLP: launch_dcoeff<FK_MODES>(g, b, s, a, tstep); break;
     }
 }
 
@@ -401,6 +355,7 @@ static void dispatch_load(int fk, dim3 g, dim3 b, hipStream_t s, const FieldDev 
     case FK_GRID_TB: launch_load<FK_GRID_TB>(g, b, s, f, n, cur, ini, S); break;
     case FK_MODES: launch_load<FK_MODES>(g, b, s, f, n, cur, ini, S); break;
     case FK_WLCS: launch_load<FK_WLCS>(g, b, s, f, n, cur, ini, S); break;
+    case FK_MLP: launch_load<FK_MODES>(g, b, s, f, n, cur, ini, S); break;
     }
 }
 
@@ -423,6 +378,7 @@ static void dispatch_eval(int fk, dim3 g, dim3 b, hipStream_t s, const FieldDev 
     case FK_GRID_TB: launch_eval<FK_GRID_TB>(g, b, s, f, n, pts, out, prec); break;
     case FK_MODES: launch_eval<FK_MODES>(g, b, s, f, n, pts, out, prec); break;
     case FK_WLCS: launch_eval<FK_WLCS>(g, b, s, f, n, pts, out, prec); break;
+    case FK_MLP: launch_eval<FK_MODES>(g, b, s, f, n, pts, out, prec); break;
     }
 }
 
@@ -473,6 +429,7 @@ static void free_field(tp_ctx *c)
     hipFree(c->grid); c->grid = nullptr;
     hipFree(c->vgrid); c->vgrid = nullptr;
     hipFree(c->modes); c->modes = nullptr;
+    hipFree(c->modes32); c->modes32 = nullptr;
     hipFree(c->wl); c->wl = nullptr;
     c->have_field = false;
     c->s.derived = false;
@@ -495,6 +452,7 @@ void tp_gpu_finalize(tp_ctx *c)
     for (int k = 0; k < c->nev; k++) hipEventDestroy(c->evs[k]);
     free(c->evs);
     tp_bs_release(c);
+    tp_mlp_release(c);
     free(c->bs_tune_block);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
@@ -698,9 +656,30 @@ int tp_gpu_set_field_modes(tp_ctx *c, int nmodes, const double *rows, double b0z
         size_t bytes = sizeof(double) * 7 * nmodes;
         if (hipMalloc(&c->modes, bytes) != hipSuccess) { c->modes = nullptr; return TP_E_NOMEM; }
         HIPCHK(hipMemcpy(c->modes, rows, bytes, hipMemcpyHostToDevice));
+        /* FP32 copy: (kx ky kz phi) / 2 pi, ax ay az 0 */
+        float *t32 = (float *)malloc(sizeof(float) * 8 * nmodes);
+        if (!t32) return TP_E_NOMEM;
+        const double inv2pi = 1.0 / (2.0 * TP_PI);
+        for (int m = 0; m < nmodes; m++) {
+            const double *r = rows + 7 * m;
+            float *o = t32 + 8 * m;
+            o[0] = (float)(r[0] * inv2pi); o[1] = (float)(r[1] * inv2pi);
+            o[2] = (float)(r[2] * inv2pi); o[3] = (float)(r[6] * inv2pi);
+            o[4] = (float)r[3]; o[5] = (float)r[4]; o[6] = (float)r[5]; o[7] = 0.f;
+        }
+        if (hipMalloc(&c->modes32, sizeof(float) * 8 * nmodes) != hipSuccess) {
+            c->modes32 = nullptr;
+            free(t32);
+            return TP_E_NOMEM;
+        }
+        hipError_t e = hipMemcpy(c->modes32, t32, sizeof(float) * 8 * nmodes,
+                                 hipMemcpyHostToDevice);
+        free(t32);
+        HIPCHK(e);
     }
     c->f.nmodes = nmodes;
     c->f.modes = c->modes;
+    c->f.modes32 = c->modes32;
     c->f.b0z = b0z;
     c->f.kind = FK_MODES;
     c->have_field = true;
@@ -796,10 +775,17 @@ int tp_gpu_run_fixed(tp_ctx *c, const tp_track_params *p)
         return TP_E_ARG;
     if (p->pusher != TP_PUSH_WIRZ && p->pusher != TP_PUSH_BORIS) return TP_E_ARG;
     if (p->precision != TP_FP64 && p->precision != TP_FP32_MIXED) return TP_E_ARG;
-    track_fn fn = pick_track(field_kind(c), p->pusher, p->precision);
+    const bool mlp = field_kind(c) == FK_MLP;
+    track_fn fn = mlp ? (p->pusher == TP_PUSH_BORIS && p->precision == TP_FP32_MIXED
+                             ? tp_mlp_launch_track : nullptr)
+                      : pick_track(field_kind(c), p->pusher, p->precision);
     if (!fn) return TP_E_ARG;
     HIPCHK(hipSetDevice(c->device));
     const int n = c->s.n;
+    if (mlp) {
+        int rc = tp_mlp_prepare_run(c, n);
+        if (rc) return rc;
+    }
     const int nb = p->nbins, nt = p->nt_out, di = p->dcoeff_interval;
     const int nd = p->estep / di + 1;
     /* diagnostics buffers */
@@ -828,7 +814,8 @@ int tp_gpu_run_fixed(tp_ctx *c, const tp_track_params *p)
     c->last_ms = 0.0;
     c->last_launches = 0;
 
-    const int bs = block_size_for(c, n);
+    /* the surrogate kernel keeps 4 wave tiles in LDS: full 256-lane blocks */
+    const int bs = mlp ? 256 : block_size_for(c, n);
     const int nblocks = n > 0 ? (n + bs - 1) / bs : 0;
     size_t pneed = (size_t)(nblocks > 0 ? nblocks : 1) * 8;
     if (pneed > c->partials_cap) {
@@ -847,7 +834,7 @@ int tp_gpu_run_fixed(tp_ctx *c, const tp_track_params *p)
         if (j1 > p->estep) j1 = p->estep;
         j0 = j1 + 1;
     }
-    if (nlaunch > c->nev) {
+    if (2 * nlaunch > c->nev) {      /* two events per launch */
         for (int k = 0; k < c->nev; k++) hipEventDestroy(c->evs[k]);
         free(c->evs);
         c->nev = 0;

@@ -48,6 +48,15 @@ def lib():
         L.tp_gpu_track_adaptive.argtypes = [vp, P(abi.TpBsParams), i, P(d), P(i),
                                             P(abi.TpBsStats)]
         L.tp_gpu_tune_adaptive.argtypes = [vp, P(abi.TpBsParams), ctypes.c_char_p, P(i), P(i)]
+        f = ctypes.c_float
+        L.tp_gpu_mlp_set.argtypes = [vp, P(f), P(f), d]
+        L.tp_gpu_mlp_get.argtypes = [vp, P(f)]
+        L.tp_gpu_mlp_forward.argtypes = [vp, i, P(d), P(d)]
+        L.tp_gpu_mlp_loss_grad.argtypes = [vp, i, P(f), P(f), P(f), P(d)]
+        L.tp_gpu_train_surrogate.argtypes = [vp, P(abi.TpMlpTrainParams),
+                                             P(abi.TpMlpTrainStats)]
+        L.tp_gpu_set_field_mlp.argtypes = [vp, d, i]
+        L.tp_gpu_mlp_fallback_count.argtypes = [vp, P(ctypes.c_longlong)]
         cs = ctypes.c_char_p
         L.tp_host_spectrum_bins.argtypes = [i, d, d, P(d), P(d)]
         L.tp_host_spectrum_t0.argtypes = [i, P(d), i, d, d, P(d)]
@@ -218,6 +227,52 @@ class Tracker:
         _check(lib().tp_gpu_tune_adaptive(self._ctx, ctypes.byref(params), cp, ctypes.byref(b),
                                           ctypes.byref(w)), "tune_adaptive")
         return b.value, w.value
+
+    # --- learned field surrogate ---------------------------------------------
+    def mlp_set(self, q, params, scale):
+        q = np.ascontiguousarray(q, dtype=np.float32).reshape(abi.TP_MLP_NFREQ, 3)
+        params = np.ascontiguousarray(params, dtype=np.float32).ravel()
+        if params.size != abi.TP_MLP_NPARAMS:
+            raise ValueError(f"expected {abi.TP_MLP_NPARAMS} parameters")
+        _check(lib().tp_gpu_mlp_set(self._ctx, fptr(q), fptr(params), float(scale)), "mlp_set")
+
+    def mlp_get(self):
+        out = np.zeros(abi.TP_MLP_NPARAMS, dtype=np.float32)
+        _check(lib().tp_gpu_mlp_get(self._ctx, fptr(out)), "mlp_get")
+        return out
+
+    def mlp_forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros_like(x)
+        _check(lib().tp_gpu_mlp_forward(self._ctx, x.shape[0], dptr(x), dptr(out)),
+               "mlp_forward")
+        return out
+
+    def mlp_loss_grad(self, x, y):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 3)
+        y = np.ascontiguousarray(y, dtype=np.float32).reshape(-1, 3)
+        g = np.zeros(abi.TP_MLP_NPARAMS, dtype=np.float32)
+        loss = ctypes.c_double()
+        _check(lib().tp_gpu_mlp_loss_grad(self._ctx, x.shape[0], fptr(x), fptr(y), fptr(g),
+                                          ctypes.byref(loss)), "mlp_loss_grad")
+        return loss.value, g
+
+    def train_surrogate(self, params):
+        st = abi.TpMlpTrainStats()
+        _check(lib().tp_gpu_train_surrogate(self._ctx, ctypes.byref(params), ctypes.byref(st)),
+               "train_surrogate")
+        return dict(first_loss=st.first_loss, last_loss=st.last_loss, steps=st.steps,
+                    sample_ms=st.sample_ms, train_ms=st.train_ms)
+
+    def set_field_mlp(self, tol=1e-3, check_interval=100):
+        _check(lib().tp_gpu_set_field_mlp(self._ctx, float(tol), int(check_interval)),
+               "set_field_mlp")
+
+    def mlp_fallback_count(self):
+        v = ctypes.c_longlong()
+        _check(lib().tp_gpu_mlp_fallback_count(self._ctx, ctypes.byref(v)),
+               "mlp_fallback_count")
+        return v.value
 
     def last_kernel_time(self):
         ms = ctypes.c_double()

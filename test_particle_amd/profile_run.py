@@ -16,7 +16,7 @@ import numpy as np
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["grid", "abc", "copy", "modes"])
+    ap.add_argument("what", choices=["grid", "abc", "copy", "modes", "surrogate"])
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--particles", type=int, default=1 << 20)
     ap.add_argument("--pusher", default="boris", choices=["boris", "wirz"])
@@ -27,6 +27,9 @@ def main():
     ap.add_argument("--tune", action="store_true")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--wpc", type=int, default=0)
+    ap.add_argument("--tol", type=float, default=1e-3, help="surrogate residual tolerance")
+    ap.add_argument("--epochs", type=int, default=0, help="surrogate training epochs")
+    ap.add_argument("--exact", action="store_true", help="surrogate workload, exact field")
     a = ap.parse_args()
     if a.what == "copy":
         import torch
@@ -64,6 +67,34 @@ def main():
               f"{st['accepted'] / wall:.4e} accepted steps/s, {st['rhs'] / wall:.4e} RHS/s, "
               f"rejected {st['rejected']}, failed {st['failed']}, kernel {st['kernel_ms']:.1f} ms, "
               f"wall {wall:.2f}s", flush=True)
+        tr.close()
+        return
+    if a.what == "surrogate":
+        from . import surrogate
+        rows, period = surrogate.surrogate_field(a.nmodes)
+        tr.set_field_modes(rows, 1.0)
+        q = surrogate.fourier_frequencies(period, sigma=3.0, seed=3)
+        tr.mlp_set(q, surrogate.init_params(3), surrogate.output_scale(rows))
+        if a.epochs > 0:
+            st = tr.train_surrogate(abi.make_mlp_train_params(
+                nsamples=1 << 24, period=period, batch=16384, epochs=a.epochs))
+            print(f"trained: {st}", flush=True)
+        if not a.exact:
+            tr.set_field_mlp(a.tol, 100)
+        lo, hi = (0.0, 0.0, 0.0), (period,) * 3
+        ptl = init_particles(a.particles, lo, hi, 0.7)
+        tr.load(ptl)
+        prm = abi.make_params(dt=0.01 * gyroperiod(0.7, 1.0), estep=a.steps, lo=lo, hi=hi,
+                              pusher=abi.TP_PUSH_BORIS, precision=abi.TP_FP32_MIXED,
+                              nbins=102, nt_out=max(1, a.steps // 100), emin=1e-4, emax=1.0)
+        t0 = time.perf_counter()
+        tr.run(prm)
+        wall = time.perf_counter() - t0
+        ms, k = tr.last_kernel_time()
+        nfb = tr.mlp_fallback_count() if not a.exact else 0
+        print(f"surrogate({'exact' if a.exact else 'net'}, tol {a.tol}): {a.particles} x "
+              f"{a.steps} steps, {a.particles * a.steps / wall:.4e} particle-steps/s, "
+              f"kernels {ms:.2f} ms / {k} launches, fallbacks {nfb}", flush=True)
         tr.close()
         return
     pusher = abi.TP_PUSH_BORIS if a.pusher == "boris" else abi.TP_PUSH_WIRZ

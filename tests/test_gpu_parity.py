@@ -271,3 +271,21 @@ def test_fp32_mixed_precision_form(gpu):
     g0 = np.sqrt(1 + (ptl[:, 4:7] ** 2).sum(1))
     g32 = np.sqrt(1 + (r32["ptl"][:, 4:7] ** 2).sum(1))
     assert np.abs(g32 / g0 - 1).max() < 1e-4
+
+
+def test_track_growing_launch_counts_one_context(gpu):
+    """One context, runs whose launch count grows (2 events per launch are
+    reallocated as needed): every run equals the same run on a fresh
+    context, bit for bit."""
+    from test_particle_amd.gpu import Tracker
+    from test_particle_amd.particles import init_particles
+    ptl = init_particles(256, BOX[0], BOX[1], 0.7, seed=9)
+    gpu.set_field_abc(1.0, 1.0, 1.0)
+    for steps in (100, 300, 500, 1200):
+        print(f"[case] growing launch counts: {steps} steps", flush=True)
+        prm = _params(steps, steps // 100, abi.TP_PUSH_BORIS)
+        r = gpu.track_fixed(prm, ptl)
+        with Tracker(0) as fresh:
+            fresh.set_field_abc(1.0, 1.0, 1.0)
+            ref = fresh.track_fixed(prm, ptl)
+        _compare_track(r, ref, f"{steps} steps")
