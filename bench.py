@@ -74,8 +74,13 @@ def main():
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 form")
-    ap.add_argument("--forms", default="abc,bs",
-                    help="extra workloads reported under 'forms' (abc, bs; '' for none)")
+    ap.add_argument("--forms", default="abc,bs,surrogate",
+                    help="extra workloads reported under 'forms' (abc, bs, surrogate; '' "
+                         "for none)")
+    ap.add_argument("--sur-epochs", type=int, default=4, help="surrogate training epochs")
+    ap.add_argument("--sur-sigma", type=float, default=4.0,
+                    help="spread of the surrogate's Fourier frequencies (lattice units)")
+    ap.add_argument("--sur-steps", type=int, default=5000, help="surrogate-workload steps")
     ap.add_argument("--bs-tgyro", type=float, default=2.0,
                     help="adaptive-stepper horizon in gyroperiods")
     args = ap.parse_args()
@@ -177,6 +182,8 @@ def main():
         forms.update(forms_abc(args))
     if "bs" in extra:
         forms.update(forms_bs(args))
+    if "surrogate" in extra:
+        forms.update(forms_surrogate(args))
 
     out = {
         "metric": METRIC, "value": chosen_value, "unit": "particle-steps/s", "n_gpus": 1,
@@ -312,6 +319,107 @@ def forms_bs(args):
         "horizon_gyroperiods": args.bs_tgyro, "rtol": 1e-8, "atol": 1e-10,
         "rhs_per_s": st["rhs"] / w, "accepted": st["accepted"], "rejected": st["rejected"],
         "failed": st["failed"], "kernel_ms": st["kernel_ms"], "wall_s": w}}
+
+
+def forms_surrogate(args):
+    """The learned-surrogate workload: the 1000-mode turbulence of the
+    adaptive workload with its wavevectors on the lattice of its own k_min
+    (periodic box L = 2 pi / k_min), the network trained on the GPU
+    (2^24 exact FP64 samples), then 4M particles at |v| = 0.7c, FP32 Boris,
+    dt = 0.01 T_g, periodic in L: the exact FP32 mode sum against the
+    network with the exact-field fallback at tolerance 1e-3 (checked every
+    100 steps) and, as the plain form, the FP64 push with the exact FP64 mode
+    sum (timed over 500 steps).  D_xx and the spectrum are judged against
+    the exact-field FP32 run."""
+    from test_particle_amd import abi, surrogate
+    from test_particle_amd.fields import gyroperiod
+    from test_particle_amd.gpu import Tracker
+    from test_particle_amd.particles import init_particles
+    rows, period = surrogate.surrogate_field(1000, seed=args.seed)
+    lo, hi = (0.0, 0.0, 0.0), (period,) * 3
+    n, K = 1 << 22, args.sur_steps
+    ptl = init_particles(n, lo, hi, 0.7, seed=args.seed)
+    dt = 0.01 * gyroperiod(0.7, 1.0)
+
+    def params(steps, prec):
+        return abi.make_params(dt=dt, estep=steps, lo=lo, hi=hi, pusher=abi.TP_PUSH_BORIS,
+                               precision=prec, nbins=102, nt_out=max(1, min(100, steps // 100)),
+                               emin=1e-4, emax=1.0)
+    out = {}
+    with Tracker(0) as tr:
+        tr.set_field_modes(rows, 1.0)
+        q = surrogate.fourier_frequencies(period, sigma=args.sur_sigma, seed=args.seed)
+        scale = surrogate.output_scale(rows)
+        tr.mlp_set(q, surrogate.init_params(args.seed), scale)
+        tp = abi.make_mlp_train_params(nsamples=1 << 24, period=period, batch=16384,
+                                       epochs=args.sur_epochs, lr=2e-3, lr_min=2e-5,
+                                       seed=args.seed)
+        log(f"[bench] surrogate: training {args.sur_epochs} epochs of 2^24 samples")
+        st = tr.train_surrogate(tp)
+        log(f"[bench] surrogate trained: loss {st['first_loss']:.3e} -> {st['last_loss']:.3e}, "
+            f"{st['steps']} steps in {st['train_ms'] / 1e3:.1f}s (samples {st['sample_ms']:.0f} ms)")
+        xs = np.random.default_rng(args.seed + 1).uniform(0.0, period, (1 << 16, 3))
+        pts = np.zeros((xs.shape[0], 4))
+        pts[:, :3] = xs
+        exact = tr.eval_field(pts)[:, :3]
+        bnet = scale * tr.mlp_forward(xs)
+        bnet[:, 2] += 1.0
+        res = np.linalg.norm(bnet - exact, axis=1) / np.linalg.norm(exact, axis=1)
+        # plain FP64 form (exact FP64 mode sum), a shorter run
+        k64 = min(K, 500)
+        tr.load(ptl)
+        t0 = time.perf_counter()
+        tr.run(params(k64, abi.TP_FP64))
+        v64 = n * k64 / (time.perf_counter() - t0)
+        # exact FP32 (the judge of the surrogate)
+        tr.load(ptl)
+        pe = params(K, abi.TP_FP32_MIXED)
+        t0 = time.perf_counter()
+        tr.run(pe)
+        vex = n * K / (time.perf_counter() - t0)
+        dex = tr.diagnostics(pe)
+        dxx_ex = dex["dcoeff"][1, 1:] / np.maximum(dex["remain"][1:], 1)
+        runs = {}
+        for tol in (1e-3, float(np.quantile(res, 0.99))):
+            tr.set_field_mlp(tol, 100)
+            tr.load(ptl)
+            t0 = time.perf_counter()
+            tr.run(pe)
+            v = n * K / (time.perf_counter() - t0)
+            d = tr.diagnostics(pe)
+            dxx = d["dcoeff"][1, 1:] / np.maximum(d["remain"][1:], 1)
+            runs[tol] = {"value": v, "fallback_fraction": tr.mlp_fallback_count() / n,
+                         "dxx_max_rel_err_vs_exact_fp32": float(np.max(np.abs(dxx / dxx_ex - 1))),
+                         "spectrum_max_rel_err_vs_exact_fp32": spectrum_err(d["espect"],
+                                                                            dex["espect"])}
+            tr.set_field_modes(rows, 1.0)
+            log(f"[bench] surrogate tol {tol:.2e}: {v:.4e} particle-steps/s, fallback "
+                f"{runs[tol]['fallback_fraction']:.3f}, D_xx err "
+                f"{runs[tol]['dxx_max_rel_err_vs_exact_fp32']:.2e}")
+    log(f"[bench] surrogate workload: FP64 {v64:.4e}, exact FP32 {vex:.4e} particle-steps/s")
+    tols = sorted(runs)
+    out["surrogate_modes1000_4M_plain_fp64"] = {
+        "value": v64, "dtype": "f64", "particles": n, "steps": k64,
+        "field": "exact FP64 mode sum (1000 modes)"}
+    out["surrogate_modes1000_4M_exact_fp32"] = {
+        "value": vex, "dtype": "f32 push and mode sum, f64 positions", "particles": n,
+        "steps": K, "field": "exact FP32 mode sum (hardware cosine)"}
+    for tol in tols:
+        r = runs[tol]
+        r.update({"dtype": "bf16 network (f32 accumulation), f32 push, f64 positions",
+                  "particles": n, "steps": K, "tolerance": tol, "check_interval": 100,
+                  "speedup_vs_exact_fp32": r["value"] / vex,
+                  "speedup_vs_plain_fp64": r["value"] / v64})
+        out[f"surrogate_modes1000_4M_net_tol{tol:.1e}"] = r
+    out["surrogate_training"] = {
+        "samples": 1 << 24, "batch": 16384, "epochs": args.sur_epochs,
+        "steps": st["steps"], "first_loss": st["first_loss"], "last_loss": st["last_loss"],
+        "train_s": st["train_ms"] / 1e3, "sample_s": st["sample_ms"] / 1e3,
+        "fourier_sigma_lattice_units": args.sur_sigma, "period": period,
+        "residual_median": float(np.median(res)), "residual_p90": float(np.quantile(res, 0.9)),
+        "residual_p99": float(np.quantile(res, 0.99)),
+        "fraction_residual_below_1e-3": float(np.mean(res < 1e-3))}
+    return out
 
 
 def cpu_baseline(e, b, lo, hi, ptl, params, tr):

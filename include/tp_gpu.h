@@ -282,11 +282,14 @@ int tp_gpu_mlp_loss_grad(tp_ctx *ctx, int n, const float *x, const float *y, flo
 int tp_gpu_train_surrogate(tp_ctx *ctx, const tp_mlp_train_params *p,
                            tp_mlp_train_stats *stats);
 /* Makes the network the field source of tp_gpu_run_fixed (FP32 Boris only),
- * with the context's modes as the exact field.  Each particle compares the
- * network with the exact FP32 mode sum at its start and every
- * check_interval steps (0: start only); once |B_net - B_exact| / |B_exact|
- * exceeds tol it uses the exact field for the rest of the run.  The D
- * steps' pitch angles use the exact FP64 field. */
+ * with the context's modes as the exact field.  Before the run's first
+ * launch, and before every launch that starts after a multiple of
+ * check_interval steps (0: the first only), each particle still on the
+ * network compares it with the exact FP32 mode sum at its position; once
+ * |B_net - B_exact| / |B_exact| exceeds tol it uses the exact field for the
+ * rest of the run.  Network and exact-field particles are run by separate
+ * kernels (no wave mixes them).  The D steps' pitch angles use the exact
+ * FP64 field. */
 int tp_gpu_set_field_mlp(tp_ctx *ctx, double tol, int check_interval);
 /* Particles that switched to the exact field during the last run. */
 int tp_gpu_mlp_fallback_count(tp_ctx *ctx, long long *count);

@@ -541,8 +541,9 @@ __device__ __forceinline__ void get_emf64(const FieldDev &f, double x, double y,
 }
 
 /* FP32 sum over the modes: the phase is formed in revolutions from the
- * table's (k, phi) / 2 pi, reduced to [-1/2, 1/2] and passed to the hardware
- * cosine, whose argument is in revolutions (v_cos_f32: cos(2 pi r)).  Four
+ * table's (k, phi) / 2 pi, reduced to [0, 1) by v_fract_f32 and passed to
+ * the hardware cosine, whose argument is in revolutions (v_cos_f32:
+ * cos(2 pi r)).  Four
  * modes per iteration for instruction-level parallelism; the table is
  * uniform and read through the constant address space (scalar loads). */
 __device__ __forceinline__ void modes_sum32(const float *tab32, int nmodes, float x, float y,
@@ -558,10 +559,10 @@ __device__ __forceinline__ void modes_sum32(const float *tab32, int nmodes, floa
         float u1 = fmaf(m[10], z, fmaf(m[9], y, fmaf(m[8], x, m[11])));
         float u2 = fmaf(m[18], z, fmaf(m[17], y, fmaf(m[16], x, m[19])));
         float u3 = fmaf(m[26], z, fmaf(m[25], y, fmaf(m[24], x, m[27])));
-        float c0 = __builtin_amdgcn_cosf(u0 - rintf(u0));
-        float c1 = __builtin_amdgcn_cosf(u1 - rintf(u1));
-        float c2 = __builtin_amdgcn_cosf(u2 - rintf(u2));
-        float c3 = __builtin_amdgcn_cosf(u3 - rintf(u3));
+        float c0 = __builtin_amdgcn_cosf(__builtin_amdgcn_fractf(u0));
+        float c1 = __builtin_amdgcn_cosf(__builtin_amdgcn_fractf(u1));
+        float c2 = __builtin_amdgcn_cosf(__builtin_amdgcn_fractf(u2));
+        float c3 = __builtin_amdgcn_cosf(__builtin_amdgcn_fractf(u3));
         bx = fmaf(m[4], c0, bx); by = fmaf(m[5], c0, by); bz = fmaf(m[6], c0, bz);
         bx = fmaf(m[12], c1, bx); by = fmaf(m[13], c1, by); bz = fmaf(m[14], c1, bz);
         bx = fmaf(m[20], c2, bx); by = fmaf(m[21], c2, by); bz = fmaf(m[22], c2, bz);
@@ -570,7 +571,7 @@ __device__ __forceinline__ void modes_sum32(const float *tab32, int nmodes, floa
     for (int n = n4; n < nmodes; n++) {
         const ctab m = t + 8 * n;
         float u = fmaf(m[2], z, fmaf(m[1], y, fmaf(m[0], x, m[3])));
-        float c = __builtin_amdgcn_cosf(u - rintf(u));
+        float c = __builtin_amdgcn_cosf(__builtin_amdgcn_fractf(u));
         bx = fmaf(m[4], c, bx); by = fmaf(m[5], c, by); bz = fmaf(m[6], c, bz);
     }
     bxo = bx; byo = by; bzo = bz;

@@ -41,8 +41,9 @@ __global__ void __launch_bounds__(256) k_track(TrackArgs a)
 {
     extern __shared__ unsigned hist[];
     const int tid = threadIdx.x;
-    const int i = blockIdx.x * blockDim.x + tid;
-    const bool live = i < a.n;
+    const int gi = blockIdx.x * blockDim.x + tid;
+    const bool live = gi < a.n;
+    const int i = live ? (a.idx ? a.idx[gi] : gi) : 0;
     for (int b = tid; b < a.nbins; b += blockDim.x) hist[b] = 0u;
 
     Ptl p;
@@ -280,7 +281,6 @@ __global__ void __launch_bounds__(256) k_eval(FieldDev f, int n, const double *p
 /* ------------------------------------------------------------------ */
 
 
-typedef void (*track_fn)(dim3, dim3, size_t, hipStream_t, const TrackArgs &);
 
 template <int PUSH, int FK, int PREC>
 static void launch_track(dim3 g, dim3 b, size_t sh, hipStream_t s, const TrackArgs &a)
@@ -776,9 +776,9 @@ int tp_gpu_run_fixed(tp_ctx *c, const tp_track_params *p)
     if (p->pusher != TP_PUSH_WIRZ && p->pusher != TP_PUSH_BORIS) return TP_E_ARG;
     if (p->precision != TP_FP64 && p->precision != TP_FP32_MIXED) return TP_E_ARG;
     const bool mlp = field_kind(c) == FK_MLP;
-    track_fn fn = mlp ? (p->pusher == TP_PUSH_BORIS && p->precision == TP_FP32_MIXED
-                             ? tp_mlp_launch_track : nullptr)
-                      : pick_track(field_kind(c), p->pusher, p->precision);
+    if (mlp && (p->pusher != TP_PUSH_BORIS || p->precision != TP_FP32_MIXED)) return TP_E_ARG;
+    /* surrogate field: the exact-field kernel runs the particles that fell back */
+    track_fn fn = pick_track(mlp ? FK_MODES : field_kind(c), p->pusher, p->precision);
     if (!fn) return TP_E_ARG;
     HIPCHK(hipSetDevice(c->device));
     const int n = c->s.n;
@@ -814,8 +814,7 @@ int tp_gpu_run_fixed(tp_ctx *c, const tp_track_params *p)
     c->last_ms = 0.0;
     c->last_launches = 0;
 
-    /* the surrogate kernel keeps 4 wave tiles in LDS: full 256-lane blocks */
-    const int bs = mlp ? 256 : block_size_for(c, n);
+    const int bs = block_size_for(c, n);
     const int nblocks = n > 0 ? (n + bs - 1) / bs : 0;
     size_t pneed = (size_t)(nblocks > 0 ? nblocks : 1) * 8;
     if (pneed > c->partials_cap) {
@@ -877,7 +876,16 @@ int tp_gpu_run_fixed(tp_ctx *c, const tp_track_params *p)
         a.first = (j0 == p->sstep) ? 1 : 0;
         a.ntp0 = ntp;
         HIPCHK(hipEventRecord(c->evs[2 * launches], c->stream));
-        fn(dim3(nblocks), dim3(bs), shmem, c->stream, a);
+        if (mlp) {
+            /* residual checks before the first launch and at launches that
+             * start after a multiple of the check interval */
+            const int ci = c->f.mlp.check;
+            const int chk = a.first || (ci > 0 && (j0 - 1) % ci == 0);
+            int rc = tp_mlp_launch(c, a, shmem, chk, bs, fn);
+            if (rc) return rc;
+        } else {
+            fn(dim3(nblocks), dim3(bs), shmem, c->stream, a);
+        }
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(c->evs[2 * launches + 1], c->stream));
         launches++;

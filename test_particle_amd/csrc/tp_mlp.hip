@@ -97,16 +97,17 @@ enum { MLP_EVAL = 0, MLP_TRAIN = 1 };
 #define GELU_C2 0.044715f
 #define LOG2E 1.4426950408889634f
 
-/* gelu(z) = z sigma(a), a = 2 sqrt(2/pi) (z + 0.044715 z^3); optionally
- * gelu'(z) = sigma + z sigma (1 - sigma) da/dz.  exp2 and the reciprocal are
- * the hardware ones (v_exp_f32, v_rcp_f32); for a -> -inf, sigma -> 0 and
- * both results go to 0 without NaN. */
+/* gelu(z) = z sigma(a), a = 2 sqrt(2/pi) (z + 0.044715 z^3) (the tanh form
+ * of Hendrycks & Gimpel); optionally gelu'(z) = sigma + z sigma (1 - sigma)
+ * da/dz.  exp2 and the reciprocal are the hardware ones (v_exp_f32,
+ * v_rcp_f32); for a -> -inf, sigma -> 0 and both results go to 0 without
+ * NaN. */
 template <bool DERIV>
 __device__ __forceinline__ float gelu_f(float z, float &gd)
 {
     float z2 = z * z;
-    float a = GELU_C1 * z * fmaf(GELU_C2, z2, 1.0f);
-    float e = __builtin_amdgcn_exp2f(-a * LOG2E);
+    /* exp(-a) = 2^(-a log2 e) */
+    float e = __builtin_amdgcn_exp2f((-GELU_C1 * LOG2E) * z * fmaf(GELU_C2, z2, 1.0f));
     float s = __builtin_amdgcn_rcpf(1.0f + e);
     if (DERIV) gd = fmaf(z * s * (1.0f - s) * GELU_C1, fmaf(3.0f * GELU_C2, z2, 1.0f), s);
     return z * s;
@@ -121,14 +122,13 @@ __device__ __forceinline__ f32x4 mfma(const bf16x8 &a, const bf16x8 &b, const f3
  * the tile (and in training, h and gelu' to the batch buffers); layer 3
  * keeps FP32 h in acc for the output layer. */
 template <int MODE, int LAYER>
-__device__ __forceinline__ void hidden_epilogue(f32x4 (&acc)[8][4], const float *bias,
-                                                __bf16 *tile, const TrainBufs &tb, long row0)
+__device__ __forceinline__ void hidden_epilogue(f32x4 (&acc)[8][4], __bf16 *tile,
+                                                const TrainBufs &tb, long row0)
 {
     const int lane = threadIdx.x & 63, g = lane >> 4, i16 = lane & 15;
 #pragma unroll
     for (int nb = 0; nb < 8; nb++) {
         const int n0 = nb * 16 + 4 * g;
-        const f32x4 b = *(const f32x4 *)(bias + n0);
 #pragma unroll
         for (int mb = 0; mb < 4; mb++) {
             const int m = mb * 16 + i16;
@@ -136,7 +136,7 @@ __device__ __forceinline__ void hidden_epilogue(f32x4 (&acc)[8][4], const float 
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 float d = 0.f;
-                h[r] = gelu_f<MODE == MLP_TRAIN>(acc[nb][mb][r] + b[r], d);
+                h[r] = gelu_f<MODE == MLP_TRAIN>(acc[nb][mb][r], d);
                 gd[r] = d;
             }
             if (LAYER < 3) {
@@ -161,28 +161,60 @@ __device__ __forceinline__ void hidden_epilogue(f32x4 (&acc)[8][4], const float 
     }
 }
 
-/* one 128 -> 128 layer: acc = W^T X^T from the wave's tile */
-__device__ __forceinline__ void layer128(f32x4 (&acc)[8][4], const bf16x8 *wf,
-                                         const __bf16 *tile)
+/* accumulators start at the layer's bias (the 4 outputs of this lane) */
+__device__ __forceinline__ void acc_bias(f32x4 (&acc)[8][4], const float *bias)
+{
+    const int g = (threadIdx.x & 63) >> 4;
+    /* not hoisted into the previous layer's epilogue, where acc is live */
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int nb = 0; nb < 8; nb++) {
+        const f32x4 b = *(const f32x4 *)(bias + nb * 16 + 4 * g);
+#pragma unroll
+        for (int mb = 0; mb < 4; mb++) acc[nb][mb] = b;
+    }
+}
+
+/* acc += W^T X^T over k-blocks kb0 .. kb0 + nkb - 1 of the layer (tile
+ * column kb * 32 holds k-block kb0 + kb of the input); the first half of
+ * the weight fragments of k-block kb + 1 is loaded while kb is multiplied,
+ * the second half at the start of its own k-block, behind the MFMAs of the
+ * first half (register budget of two waves per SIMD) */
+__device__ __forceinline__ void mfma_kblocks(f32x4 (&acc)[8][4], const bf16x8 *wf, int KB,
+                                             int kb0, int nkb, const __bf16 *tile)
 {
     const int lane = threadIdx.x & 63, g = lane >> 4, i16 = lane & 15;
+    bf16x8 an[4];
 #pragma unroll
-    for (int nb = 0; nb < 8; nb++)
-#pragma unroll
-        for (int mb = 0; mb < 4; mb++) acc[nb][mb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int nb = 0; nb < 4; nb++) an[nb] = wf[(nb * KB + kb0) * 64 + lane];
 #pragma unroll 1
-    for (int kb = 0; kb < 4; kb++) {
+    for (int kb = 0; kb < nkb; kb++) {
+        bf16x8 a[8];
+#pragma unroll
+        for (int nb = 0; nb < 4; nb++) a[nb] = an[nb];
+#pragma unroll
+        for (int nb = 4; nb < 8; nb++) a[nb] = wf[(nb * KB + kb0 + kb) * 64 + lane];
+        if (kb + 1 < nkb) {
+#pragma unroll
+            for (int nb = 0; nb < 4; nb++) an[nb] = wf[(nb * KB + kb0 + kb + 1) * 64 + lane];
+        }
         bf16x8 bx[4];
 #pragma unroll
         for (int mb = 0; mb < 4; mb++)
             bx[mb] = *(const bf16x8 *)(tile + (mb * 16 + i16) * TS + kb * 32 + 8 * g);
 #pragma unroll
-        for (int nb = 0; nb < 8; nb++) {
-            const bf16x8 a = wf[(nb * 4 + kb) * 64 + lane];
+        for (int nb = 0; nb < 8; nb++)
 #pragma unroll
-            for (int mb = 0; mb < 4; mb++) acc[nb][mb] = mfma(a, bx[mb], acc[nb][mb]);
-        }
+            for (int mb = 0; mb < 4; mb++) acc[nb][mb] = mfma(a[nb], bx[mb], acc[nb][mb]);
     }
+}
+
+/* one 128 -> 128 layer: acc = b + W^T X^T from the wave's tile */
+__device__ __forceinline__ void layer128(f32x4 (&acc)[8][4], const bf16x8 *wf,
+                                         const float *bias, const __bf16 *tile)
+{
+    acc_bias(acc, bias);
+    mfma_kblocks(acc, wf, 4, 0, 4, tile);
 }
 
 /* The network at the wave's 64 points (x, y, z of this lane's sample);
@@ -198,10 +230,7 @@ __device__ __forceinline__ void mlp_wave(const MlpNet &net, float x, float y, fl
     typedef const float __attribute__((address_space(4))) *cf;
     const cf q = (cf)net.q;
     f32x4 acc[8][4];
-#pragma unroll
-    for (int nb = 0; nb < 8; nb++)
-#pragma unroll
-        for (int mb = 0; mb < 4; mb++) acc[nb][mb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    acc_bias(acc, net.b1);
 
     /* layer 1 over the sin half, then the cos half of the features */
 #pragma unroll
@@ -212,33 +241,23 @@ __device__ __forceinline__ void mlp_wave(const MlpNet &net, float x, float y, fl
 #pragma unroll
             for (int t = 0; t < 8; t++) {
                 const int j = j8 * 8 + t;
+                /* phase in revolutions, reduced to [0, 1) (the hardware sine
+                 * and cosine take revolutions) */
                 float u = fmaf(q[3 * j + 2], z, fmaf(q[3 * j + 1], y, q[3 * j] * x));
-                float r = u - rintf(u);
+                float r = __builtin_amdgcn_fractf(u);
                 v[t] = (__bf16)(h ? __builtin_amdgcn_cosf(r) : __builtin_amdgcn_sinf(r));
             }
             *(bf16x8 *)(tile + lane * TS + j8 * 8) = v;
             if (MODE == MLP_TRAIN)
                 *(bf16x8 *)(tb.h0 + (row0 + lane) * NIN + h * NF + j8 * 8) = v;
         }
-#pragma unroll 1
-        for (int kb = 0; kb < 4; kb++) {
-            bf16x8 bx[4];
-#pragma unroll
-            for (int mb = 0; mb < 4; mb++)
-                bx[mb] = *(const bf16x8 *)(tile + (mb * 16 + i16) * TS + kb * 32 + 8 * g);
-#pragma unroll
-            for (int nb = 0; nb < 8; nb++) {
-                const bf16x8 a = net.w1f[(nb * 8 + h * 4 + kb) * 64 + lane];
-#pragma unroll
-                for (int mb = 0; mb < 4; mb++) acc[nb][mb] = mfma(a, bx[mb], acc[nb][mb]);
-            }
-        }
+        mfma_kblocks(acc, net.w1f, 8, h * 4, 4, tile);
     }
-    hidden_epilogue<MODE, 1>(acc, net.b1, tile, tb, row0);
-    layer128(acc, net.w2f, tile);
-    hidden_epilogue<MODE, 2>(acc, net.b2, tile, tb, row0);
-    layer128(acc, net.w3f, tile);
-    hidden_epilogue<MODE, 3>(acc, net.b3, tile, tb, row0);
+    hidden_epilogue<MODE, 1>(acc, tile, tb, row0);
+    layer128(acc, net.w2f, net.b2, tile);
+    hidden_epilogue<MODE, 2>(acc, tile, tb, row0);
+    layer128(acc, net.w3f, net.b3, tile);
+    hidden_epilogue<MODE, 3>(acc, tile, tb, row0);
 
     /* output layer in FP32: partial sums over this lane's 32 hidden units
      * for its 4 samples, completed across the 4 lane groups */
@@ -619,33 +638,106 @@ __global__ void __launch_bounds__(256) k_mlp_sample(FieldDev f, long n, double L
 /* ------------------------------------------------------------------ */
 /* step kernel with the surrogate field                                */
 /* ------------------------------------------------------------------ */
-/* The FP32 Boris step of k_track (tp_gpu.hip) with the field from the
- * network: every wave evaluates the network at its 64 particles each step
- * (skipped when all of them use the exact field); a particle compares it
- * with the exact FP32 mode sum at the start of the run and every `check`
- * steps, and switches to the exact field for the rest of the run once the
- * relative residual exceeds tol. */
+/* Residual check of every particle that still uses the network: network
+ * and exact FP32 mode sum at its position; a relative residual above tol
+ * switches it to the exact field for the rest of the run.  first: the
+ * run's first launch (all flags cleared, all particles checked). */
+__global__ void __launch_bounds__(256, 2) k_mlp_check(TrackArgs a, int first)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 tiles[4 * TILE];
+    const MlpNet &net = a.f.mlp;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = i < a.n;
+    int fb = 1;
+    bool moving = false;
+    float px = 0.f, py = 0.f, pz = 0.f;
+    if (live) {
+        fb = first ? 0 : net.fb[i];
+        moving = first || a.esc[i] == 0 || a.bc == 0;
+        px = (float)a.x[i]; py = (float)a.y[i]; pz = (float)a.z[i];
+    }
+    const bool chk = live && moving && fb == 0;
+    float o[3] = {0.f, 0.f, 0.f};
+    TrainBufs tb = {};
+    if (__any(chk))
+        mlp_wave<MLP_EVAL>(net, px, py, pz, tiles + (threadIdx.x >> 6) * TILE, o, tb, 0);
+    if (chk) {
+        const float b0z = (float)a.f.b0z;
+        float bx, by, bz;
+        modes_sum32(a.f.modes32, a.f.nmodes, px, py, pz, bx, by, bz);
+        bz += b0z;
+        const float dx = net.scale * o[0] - bx, dy = net.scale * o[1] - by;
+        const float dz = fmaf(net.scale, o[2], b0z) - bz;
+        const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        const float bb = fmaf(bx, bx, fmaf(by, by, bz * bz));
+        if (!(r2 <= net.tol * net.tol * bb)) {
+            fb = 1;
+            atomicAdd(net.nfb, 1u);
+        }
+    }
+    if (live) net.fb[i] = fb;
+}
+
+/* Splits the particles into the network list (fb == 0) and the exact-field
+ * list (fb == 1): ballot prefix sums within a wave, wave offsets within the
+ * block, one atomic per block and list for the block's base (the order of
+ * blocks within a list is arbitrary; every particle appears exactly once). */
+__global__ void __launch_bounds__(256) k_mlp_split(int n, const int *fb, int *idx_net,
+                                                   int *idx_ex, unsigned *cnt)
+{
+    __shared__ unsigned wn[4], we[4], base[2];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const bool live = i < n;
+    const int f = live ? fb[i] : 0;
+    const bool isn = live && f == 0, ise = live && f != 0;
+    const unsigned long long bn = __ballot(isn), be = __ballot(ise);
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const unsigned pn = (unsigned)__popcll(bn & below), pe = (unsigned)__popcll(be & below);
+    if (lane == 0) {
+        wn[w] = (unsigned)__popcll(bn);
+        we[w] = (unsigned)__popcll(be);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        base[0] = atomicAdd(&cnt[0], wn[0] + wn[1] + wn[2] + wn[3]);
+        base[1] = atomicAdd(&cnt[1], we[0] + we[1] + we[2] + we[3]);
+    }
+    __syncthreads();
+    unsigned on = base[0], oe = base[1];
+    for (int k = 0; k < w; k++) {
+        on += wn[k];
+        oe += we[k];
+    }
+    if (isn) idx_net[on + pn] = i;
+    if (ise) idx_ex[oe + pe] = i;
+}
+
+/* The FP32 Boris step of k_track (tp_gpu.hip) for the particles of the
+ * network list, the field from the network: every wave evaluates it at its
+ * 64 particles each step.  Particles that fell back run in the exact-field
+ * kernel instead, so no wave mixes the two. */
 __global__ void __launch_bounds__(256, 2) k_track_mlp(TrackArgs a)
 {
     extern __shared__ unsigned hist[];
     __shared__ __attribute__((aligned(16))) __bf16 tiles[4 * TILE];
     const int tid = threadIdx.x;
-    const int i = blockIdx.x * blockDim.x + tid;
-    const bool live = i < a.n;
+    const int gi = blockIdx.x * blockDim.x + tid;
+    const bool live = gi < a.n;
+    const int i = live ? a.idx[gi] : 0;
     const MlpNet &net = a.f.mlp;
     __bf16 *tile = tiles + (tid >> 6) * TILE;
     for (int b = tid; b < a.nbins; b += blockDim.x) hist[b] = 0u;
 
     Ptl p;
     double xs = 0.0, ys = 0.0, zs = 0.0;
-    int esc = 1, nst = 0, fb = 0;
+    int esc = 1, nst = 0;
     if (live) {
         p.x = a.x[i]; p.y = a.y[i]; p.z = a.z[i];
         p.ux = a.ux[i]; p.uy = a.uy[i]; p.uz = a.uz[i]; p.t = a.t[i];
         xs = a.xs[i]; ys = a.ys[i]; zs = a.zs[i];
         esc = a.esc[i];
         nst = a.nsteps[i];
-        fb = a.first ? 0 : net.fb[i];
         if (a.first) {
             esc = 0;
             nst = 1;
@@ -659,7 +751,6 @@ __global__ void __launch_bounds__(256, 2) k_track_mlp(TrackArgs a)
     const float dtf = (float)a.dt;
     const float hf = (float)(a.qm * (0.5 * a.dt));
     const float b0z = (float)a.f.b0z;
-    const float tol2 = net.tol * net.tol;
     TrainBufs tb = {};
     __syncthreads();
 
@@ -667,32 +758,13 @@ __global__ void __launch_bounds__(256, 2) k_track_mlp(TrackArgs a)
     for (int j = a.j0; j <= a.j1; j++) {
         const bool moving = live && (esc == 0 || a.bc == 0);
         float o[3] = {0.f, 0.f, 0.f};
-        if (__any(moving && fb == 0))
+        if (__any(moving))
             mlp_wave<MLP_EVAL>(net, (float)p.x, (float)p.y, (float)p.z, tile, o, tb, 0);
         EmfF e;
         e.ex = 0.f; e.ey = 0.f; e.ez = 0.f;
         e.bx = net.scale * o[0];
         e.by = net.scale * o[1];
         e.bz = fmaf(net.scale, o[2], b0z);
-        const bool chk = moving && fb == 0 &&
-                         ((a.first && j == a.j0) || (net.check > 0 && j % net.check == 0));
-        if (moving && (fb || chk)) {
-            float bx, by, bz;
-            modes_sum32(a.f.modes32, a.f.nmodes, (float)p.x, (float)p.y, (float)p.z, bx, by, bz);
-            bz += b0z;
-            if (chk) {
-                const float dx = e.bx - bx, dy = e.by - by, dz = e.bz - bz;
-                const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                const float bb = fmaf(bx, bx, fmaf(by, by, bz * bz));
-                if (!(r2 <= tol2 * bb)) {
-                    fb = 1;
-                    atomicAdd(net.nfb, 1u);
-                }
-            }
-            if (fb) {
-                e.bx = bx; e.by = by; e.bz = bz;
-            }
-        }
         int esc_after = esc;
         if (moving) {
             boris_push32(e, p, fux, fuy, fuz, a.dt, dtf, hf);
@@ -727,7 +799,6 @@ __global__ void __launch_bounds__(256, 2) k_track_mlp(TrackArgs a)
         a.xs[i] = xs; a.ys[i] = ys; a.zs[i] = zs;
         a.esc[i] = esc;
         a.nsteps[i] = nst;
-        net.fb[i] = fb;
     }
 }
 
@@ -744,6 +815,9 @@ struct MlpState {
     int *fb = nullptr;
     unsigned *nfb = nullptr;
     int fb_cap = 0;
+    int *idx = nullptr;          /* network list, then exact-field list (2 x fb_cap) */
+    unsigned *cnt = nullptr;     /* list lengths */
+    int n_net = 0, n_ex = 0;
     /* batch buffers */
     long rows_cap = 0;
     __bf16 *h0 = nullptr, *h1 = nullptr, *h2 = nullptr;
@@ -760,7 +834,8 @@ void tp_mlp_release(tp_ctx *c)
 {
     MlpState *m = c->mlp;
     if (!m) return;
-    void *ptrs[] = {m->params, m->am, m->av, m->q, m->frag, m->fb, m->nfb, m->h0, m->h1,
+    void *ptrs[] = {m->params, m->am, m->av, m->q, m->frag, m->fb, m->nfb, m->idx, m->cnt,
+                    m->h0, m->h1,
                     m->h2, m->dz1, m->dz2, m->dz3, m->g1, m->g2, m->g3, m->h3, m->dout,
                     m->lossp, m->lacc, m->part, m->grad, m->dsx, m->dsy};
     for (void *p : ptrs) hipFree(p);
@@ -894,6 +969,7 @@ int tp_gpu_mlp_set(tp_ctx *c, const float *q, const float *params, double scale)
         rc |= dalloc((void **)&m->q, 3 * NF * 4);
         rc |= dalloc((void **)&m->frag, FR_TOTAL * sizeof(bf16x8));
         rc |= dalloc((void **)&m->nfb, sizeof(unsigned));
+        rc |= dalloc((void **)&m->cnt, 2 * sizeof(unsigned));
         rc |= dalloc((void **)&m->lacc, sizeof(double));
         rc |= dalloc((void **)&m->grad, P_TOTAL * 4);
         if (rc) {
@@ -1088,24 +1164,68 @@ int tp_gpu_mlp_fallback_count(tp_ctx *c, long long *count)
 
 }  /* extern "C" */
 
-/* run_fixed's hooks for the surrogate field: per-particle flags for n
- * particles, the counter cleared, the network pointers refreshed */
+/* run_fixed's hooks for the surrogate field: per-particle flags and index
+ * lists for n particles, the fallback counter cleared, the network
+ * pointers refreshed */
 int tp_mlp_prepare_run(tp_ctx *c, int n)
 {
     MlpState *m = c->mlp;
     if (!m || !m->set) return TP_E_STATE;
     if (n > m->fb_cap) {
         hipFree(m->fb);
+        hipFree(m->idx);
+        m->fb = nullptr;
+        m->idx = nullptr;
         m->fb_cap = 0;
-        if (dalloc((void **)&m->fb, (size_t)(n > 0 ? n : 1) * sizeof(int))) return TP_E_NOMEM;
+        const size_t k = (size_t)(n > 0 ? n : 1);
+        if (dalloc((void **)&m->fb, k * sizeof(int)) ||
+            dalloc((void **)&m->idx, 2 * k * sizeof(int)))
+            return TP_E_NOMEM;
         m->fb_cap = n;
     }
     HIPCHK(hipMemsetAsync(m->nfb, 0, sizeof(unsigned), c->stream));
+    m->n_net = m->n_ex = 0;
     c->f.mlp = net_of(m);
     return TP_OK;
 }
 
-void tp_mlp_launch_track(dim3 g, dim3 b, size_t sh, hipStream_t s, const TrackArgs &a)
+/* one launch of the surrogate-field run: optional residual check and
+ * split (the list lengths are read back: one host synchronisation per
+ * check), then the network kernel over the network list and the
+ * exact-field kernel over the other */
+int tp_mlp_launch(tp_ctx *c, const TrackArgs &a, size_t shmem, int check, int block,
+                  track_fn exact_fn)
 {
-    hipLaunchKernelGGL(k_track_mlp, g, b, sh, s, a);
+    MlpState *m = c->mlp;
+    const int n = a.n;
+    if (check) {
+        const dim3 g((n + 255) / 256);
+        hipLaunchKernelGGL(k_mlp_check, g, dim3(256), 0, c->stream, a, a.first);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemsetAsync(m->cnt, 0, 2 * sizeof(unsigned), c->stream));
+        hipLaunchKernelGGL(k_mlp_split, g, dim3(256), 0, c->stream, n, (const int *)m->fb,
+                           m->idx, m->idx + m->fb_cap, m->cnt);
+        HIPCHK(hipGetLastError());
+        unsigned h[2] = {0u, 0u};
+        HIPCHK(hipMemcpyAsync(h, m->cnt, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if ((long)h[0] + (long)h[1] != (long)n) return TP_E_HIP;
+        m->n_net = (int)h[0];
+        m->n_ex = (int)h[1];
+    }
+    TrackArgs b = a;
+    if (m->n_net > 0) {
+        b.idx = m->idx;
+        b.n = m->n_net;
+        hipLaunchKernelGGL(k_track_mlp, dim3((m->n_net + 255) / 256), dim3(256), shmem,
+                           c->stream, b);
+        HIPCHK(hipGetLastError());
+    }
+    if (m->n_ex > 0) {
+        b.idx = m->idx + m->fb_cap;
+        b.n = m->n_ex;
+        exact_fn(dim3((m->n_ex + block - 1) / block), dim3(block), shmem, c->stream, b);
+        HIPCHK(hipGetLastError());
+    }
+    return TP_OK;
 }

@@ -34,6 +34,7 @@ struct TrackArgs {
     unsigned *espect, *espect_esc;
     double *partials;
     int *rdec;
+    const int *idx;      /* step kernels: particle g of the launch is idx[g] (null: g) */
 };
 
 /* particle_bc, src/tracking.c:1172-1221 */
@@ -57,9 +58,13 @@ static __device__ __forceinline__ void particle_bc(const TrackArgs &a, Ptl &p, d
     }
 }
 
-/* the surrogate-field step kernel (tp_mlp.hip) */
+/* the surrogate field (tp_mlp.hip): run set-up and one launch (residual
+ * check and split into network and exact-field particles when `check`,
+ * then the two step kernels; exact_fn launches the exact-field kernel) */
 struct tp_ctx;
+typedef void (*track_fn)(dim3, dim3, size_t, hipStream_t, const TrackArgs &);
 int tp_mlp_prepare_run(tp_ctx *c, int n);
-void tp_mlp_launch_track(dim3 g, dim3 b, size_t sh, hipStream_t s, const TrackArgs &a);
+int tp_mlp_launch(tp_ctx *c, const TrackArgs &a, size_t shmem, int check, int block,
+                  track_fn exact_fn);
 
 #endif /* TP_TRACK_H */

@@ -178,7 +178,38 @@ def test_trained_network_saves_exact_evaluations(gpu, net, trained):
     print(f"fallbacks: trained {nfb_tr}, untrained {nfb_un} of {n}")
     assert nfb_un > 0.95 * n
     assert nfb_tr < 0.5 * n
+    # every particle advanced once per step, whichever list it was in
+    assert np.array_equal(r_tr["nsteps"], ref["nsteps"])
+    assert np.array_equal(r_un["nsteps"], ref["nsteps"])
     dxx = lambda r: r["dcoeff"][1, 1:] / np.maximum(r["remain"][1:], 1)  # noqa: E731
     err = np.max(np.abs(dxx(r_tr) / dxx(ref) - 1.0))
     print(f"D_xx max rel err vs exact: trained {err:.3e}")
     assert err < 0.05
+
+
+def test_network_field_drives_the_push(gpu, net, trained):
+    """One step with no fallback (tolerance 1e30): the velocity change is
+    the Boris rotation in the network's field at the start positions (the
+    field from tp_gpu_mlp_forward, the push redone in FP64 on the host)."""
+    n = 1000
+    case(f"network-field push, {n} particles x 1 step")
+    ptl = init_particles(n, (0, 0, 0), (net["period"],) * 3, 0.7, seed=8)
+    r, nfb = _run(gpu, net, ptl, 1, mlp=(trained["params"], 1e30))
+    assert nfb == 0
+    b = net["scale"] * gpu.mlp_forward(ptl[:, :3])
+    b[:, 2] += 1.0
+    dt = 0.01 * gyroperiod(0.7, 1.0)
+    u = ptl[:, 4:7].astype(np.float32).astype(np.float64)
+    h = 0.5 * dt
+    gm = np.sqrt(1.0 + np.sum(u * u, axis=1, keepdims=True))
+    t = h * b / gm
+    sf = 2.0 / (1.0 + np.sum(t * t, axis=1, keepdims=True))
+    up = u + np.cross(u, t)
+    uq = u + np.cross(up, sf * t)
+    assert np.max(np.abs(r["ptl"][:, 4:7] - uq)) < 1e-5 * np.max(np.abs(uq))
+    g = np.sqrt(1.0 + np.sum(uq * uq, axis=1, keepdims=True))
+    x1 = ptl[:, :3] + dt * uq / g
+    x1 = np.mod(x1, net["period"])
+    d = np.abs(r["ptl"][:, :3] - x1)
+    d = np.minimum(d, net["period"] - d)
+    assert np.max(d) < 1e-5 * dt
