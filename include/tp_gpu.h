@@ -165,6 +165,22 @@ int tp_gpu_track_fixed(tp_ctx *ctx, const tp_track_params *p, int n,
                        int *nsteps, double *espect, double *espect_escape,
                        double *dcoeff, int *nptl_remain);
 
+/* The same with trajectory diagnostics, the reference's traj_diagnose = 1
+ * call (src/diagnostics.c:545-557; recording at src/tracking.c:440-447):
+ * particle i is copied into ptl_time[ntraj_accum[i-1] + k], k = 1, 2, ...
+ * (ntraj_accum[-1] read as 0), after every step j with
+ * j % nsteps_output == 0 at which it is pushed.  Slot ntraj_accum[i-1] + 0
+ * holds the caller's initial point, as the reference fills it; slots that
+ * are not written keep the caller's contents.  points[i] (nullable) gets
+ * the final k; points past the particle's ntraj_accum[i] - ntraj_accum[i-1]
+ * slots are counted but not written (the reference writes past them).
+ * Not available with the surrogate field. */
+int tp_gpu_track_fixed_traj(tp_ctx *ctx, const tp_track_params *p, int n,
+                            tp_particle *ptl, const tp_particle *ptl_init, int *nsteps,
+                            double *espect, double *espect_escape, double *dcoeff,
+                            int *nptl_remain, int nsteps_output, const int *ntraj_accum,
+                            tp_particle *ptl_time, int *points);
+
 /* The same split into steps, for callers that keep particles resident in
  * HBM across calls (and for timing the tracking alone): */
 int tp_gpu_load_particles(tp_ctx *ctx, int n, const tp_particle *ptl,

@@ -22,6 +22,9 @@
  *   track   SPEC PTL OUTDIR  the fixed-step loop of src/tracking.c:297 driven
  *                            like src/tracking.c:90-190, dumping the raw
  *                            spectrum counts, D sums and remaining counts
+ *   traj    SPEC PTL OUTDIR  the same loop with trajectory diagnostics
+ *                            (traj_diagnose = 1, src/tracking.c:440-447)
+ *   trajh   SPEC PTL OUTDIR  particle_tracking_hybrid with traj_diagnose = 1
  *   hybrid  SPEC PTL         particle_tracking_hybrid (src/tracking.c:659) with
  *                            the t=0 spectrum of src/diagnostics.c:50; writes
  *                            data/espectrum.dat and data/diff_coeffs.dat
@@ -326,12 +329,74 @@ int main(int argc, char **argv)
         spit(path, dsum, sizeof(double) * 8 * nd);
         snprintf(path, sizeof(path), "%s/remain.bin", argv[4]);
         spit(path, remain, sizeof(int) * nd);
+    } else if (!strcmp(mode, "traj")) {
+        /* src/tracking.c:297-462 with traj_diagnose = 1, one thread: every
+         * particle gets estep/nsteps_output + 1 trajectory slots, slot 0 its
+         * initial state (as src/diagnostics.c:545-552 fills it) */
+        particles *p = slurp(argv[3], &n);
+        int np = (int)(n / sizeof(particles));
+        particles *p0 = malloc(n + 1);
+        memcpy(p0, p, n);
+        int estep = s.tmax / s.dt;
+        int einterval = estep / s.nt_out;
+        int nd = estep / 100 + 1;
+        int nb = s.nbins, nt = s.nt_out;
+        int slots = estep / s.nsteps_output + 1;
+        int *accum = malloc(sizeof(int) * (np + 1));
+        for (int i = 0; i < np; i++) accum[i] = (i + 1) * slots;
+        int *points = malloc(sizeof(int) * (np + 1));
+        for (int i = 0; i < np; i++) points[i] = 1;
+        particles *traj = calloc((size_t)np * slots + 1, sizeof(particles));
+        for (int i = 0; i < np; i++) memcpy(&traj[(size_t)i * slots], &p[i], sizeof(particles));
+        double *dsum = calloc((size_t)8 * nd, sizeof(double));
+        int *remain = malloc(sizeof(int) * (nd + 1));
+        for (int i = 0; i <= nd; i++) remain[i] = np;
+        int *nsteps = malloc(sizeof(int) * (np + 1));
+        for (int i = 0; i < np; i++) nsteps[i] = 1;
+        double *esp = calloc((size_t)nb * nt + 1, sizeof(double));
+        double *esc = calloc((size_t)nb * nt + 1, sizeof(double));
+        particle_tracking_multi_steps_openmp(np, s.dt, 0, 1, estep, nb, nt,
+            einterval, 1, s.nsteps_output, s.pmass, accum, p0, points, p, nsteps,
+            (double (*)[1])esp, (double (*)[1])esc, traj, remain,
+            dsum, dsum + nd, dsum + 2 * nd, dsum + 3 * nd, dsum + 4 * nd,
+            dsum + 5 * nd, dsum + 6 * nd, dsum + 7 * nd);
+        char path[1024];
+        snprintf(path, sizeof(path), "%s/ptl.bin", argv[4]);
+        spit(path, p, n);
+        snprintf(path, sizeof(path), "%s/traj.bin", argv[4]);
+        spit(path, traj, sizeof(particles) * (size_t)np * slots);
+        snprintf(path, sizeof(path), "%s/points.bin", argv[4]);
+        spit(path, points, sizeof(int) * np);
+        snprintf(path, sizeof(path), "%s/espect.bin", argv[4]);
+        spit(path, esp, sizeof(double) * nb * nt);
     } else if (!strcmp(mode, "ellint")) {
         double *ks = slurp(argv[3], &n);
         size_t nk = n / sizeof(double);
         double *out = malloc(nk * 2 * sizeof(double) + 1);
         for (size_t i = 0; i < nk; i++) ellint(ks[i], &out[2 * i], &out[2 * i + 1]);
         spit(argv[4], out, nk * 2 * sizeof(double));
+    } else if (!strcmp(mode, "trajh")) {
+        /* particle_tracking_hybrid with traj_diagnose = 1 (the call of
+         * src/diagnostics.c:555-557), estep/nsteps_output + 1 slots per
+         * particle, slot 0 the initial state */
+        particles *p = slurp(argv[3], &n);
+        int np = (int)(n / sizeof(particles));
+        particles *p0 = malloc(n + 1);
+        memcpy(p0, p, n);
+        int estep = s.tmax / s.dt;
+        int slots = estep / s.nsteps_output + 1;
+        int *accum = malloc(sizeof(int) * (np + 1));
+        for (int i = 0; i < np; i++) accum[i] = (i + 1) * slots;
+        particles *traj = calloc((size_t)np * slots + 1, sizeof(particles));
+        for (int i = 0; i < np; i++) memcpy(&traj[(size_t)i * slots], &p[i], sizeof(particles));
+        int *nsteps = malloc(sizeof(int) * (np + 1));
+        particle_tracking_hybrid(0, np, s.dt, s.nbins, s.nt_out, s.bc,
+            s.nsteps_output, s.pmass, accum, 0, 1, p0, nsteps, p, traj);
+        char path[1024];
+        snprintf(path, sizeof(path), "%s/ptl.bin", argv[4]);
+        spit(path, p, n);
+        snprintf(path, sizeof(path), "%s/traj.bin", argv[4]);
+        spit(path, traj, sizeof(particles) * (size_t)np * slots);
     } else if (!strcmp(mode, "hybrid")) {
         particles *p = slurp(argv[3], &n);
         int np = (int)(n / sizeof(particles));

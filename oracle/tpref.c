@@ -679,6 +679,28 @@ void tpref_diff_coeff(const tpref_field *f, const tp_particle *p0,
 /* ---------------------------------------------------------------------- */
 /* fixed-step loop, src/tracking.c:90-190 and :297-462                     */
 /* ---------------------------------------------------------------------- */
+/* Trajectory diagnostics of the next tpref_track_fixed call (the
+ * reference's traj_diagnose = 1, src/tracking.c:440-447): particle i is
+ * copied into ptl_time[ntraj_accum[i-1] + k], k = 1, 2, ..., at every step
+ * j with j % nsteps_output == 0 at which it is tracked; points[i] receives
+ * the final k (starting at 1: slot 0 is the caller's initial point).  The
+ * reference does not bound k; here a point past the particle's slots
+ * (ntraj_accum[i] - ntraj_accum[i-1]) is counted but not written.
+ * nsteps_output = 0 switches the diagnostics off. */
+static int g_traj_nout = 0;
+static const int *g_traj_accum = NULL;
+static tp_particle *g_traj = NULL;
+static int *g_traj_points = NULL;
+
+void tpref_set_trajectory(int nsteps_output, const int *ntraj_accum, tp_particle *ptl_time,
+                          int *points)
+{
+    g_traj_nout = nsteps_output;
+    g_traj_accum = ntraj_accum;
+    g_traj = ptl_time;
+    g_traj_points = points;
+}
+
 int tpref_track_fixed(const tpref_field *f, const tp_track_params *q,
                       int n, tp_particle *ptl, const tp_particle *ptl_init,
                       int *nsteps, double *espect, double *espect_escape,
@@ -723,6 +745,9 @@ int tpref_track_fixed(const tpref_field *f, const tp_track_params *q,
             double dv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
             if (g_carry_stale) memcpy(dv, dcarry, sizeof(dv));
             if (esc_pre && 1 < nd) rm[1]--;
+            const int toff = (g_traj_nout > 0 && i > 0) ? g_traj_accum[i - 1] : 0;
+            const int tcap = g_traj_nout > 0 ? g_traj_accum[i] - toff : 0;
+            int tpts = 1;
             for (int j = q->sstep; j <= q->estep; j++) {
                 dv[0] = 0.0; dv[4] = 0.0; dv[5] = 0.0;  /* drr dpp duu */
                 if (q->flags & TP_FLAG_FIX_STALE_DCOEFF)
@@ -734,6 +759,10 @@ int tpref_track_fixed(const tpref_field *f, const tp_track_params *q,
                     if (nsteps) nsteps[i] += 1;
                     if (esc_after == 0 && j % di == 0)
                         tpref_diff_coeff(f, &ptl_init[i], p, q->dt, dv);
+                    if (g_traj_nout > 0 && j % g_traj_nout == 0) {
+                        if (tpts < tcap) memcpy(&g_traj[toff + tpts], p, sizeof(tp_particle));
+                        tpts++;
+                    }
                 }
                 if (esc_pre == 0 && esc_after == 1 && q->bc_flag == 1 && ntp < nt)
                     se[ntp * nb + tpref_energy_bin(p, nb, q->logemin, q->logde)] += 1.0;
@@ -751,6 +780,7 @@ int tpref_track_fixed(const tpref_field *f, const tp_track_params *q,
                 esc_pre = esc_after;
             }
             memcpy(dcarry, dv, sizeof(dv));
+            if (g_traj_nout > 0 && g_traj_points) g_traj_points[i] = tpts;
         }
     }
     double t2 = omp_get_wtime();

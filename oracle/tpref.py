@@ -75,6 +75,7 @@ def lib():
         L.tpref_sincos.argtypes = [d, P(d), P(d)]
         L.tpref_ellint.argtypes = [d, P(d), P(d)]
         L.tpref_ellint_ref.argtypes = [d, P(d), P(d)]
+        L.tpref_set_trajectory.argtypes = [i, P(i), P(d), P(i)]
         L.tpref_mlp_nparams.restype = i
         L.tpref_bf16_round.argtypes = [d]
         L.tpref_bf16_round.restype = d
@@ -225,6 +226,25 @@ def track_fixed(field, params, ptl, ptl_init=None, nthreads=1):
         raise RuntimeError(f"tpref_track_fixed failed: {rc}")
     return dict(ptl=ptl, nsteps=nsteps, espect=esp, espect_escape=ese,
                 dcoeff=dco, remain=rem, seconds=secs.value)
+
+
+def track_fixed_traj(field, params, ptl, nsteps_output, ntraj_accum, ptl_time,
+                     ptl_init=None, nthreads=1):
+    """track_fixed with trajectory diagnostics; ptl_time (float64, shape
+    (ntraj_accum[-1], 11)) holds the caller's initial points and receives the
+    others; returns the track_fixed result plus 'ptl_time' and 'points'."""
+    ntraj_accum = np.ascontiguousarray(ntraj_accum, dtype=np.int32)
+    ptl_time = np.ascontiguousarray(ptl_time, dtype=np.float64).copy()
+    points = np.zeros(ntraj_accum.size, dtype=np.int32)
+    lib().tpref_set_trajectory(int(nsteps_output), abi.iptr(ntraj_accum), abi.dptr(ptl_time),
+                               abi.iptr(points))
+    try:
+        r = track_fixed(field, params, ptl, ptl_init=ptl_init, nthreads=nthreads)
+    finally:
+        lib().tpref_set_trajectory(0, None, None, None)
+    r["ptl_time"] = ptl_time
+    r["points"] = points
+    return r
 
 
 def bs_track(field, params, ptl, nthreads=1):

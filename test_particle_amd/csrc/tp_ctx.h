@@ -57,6 +57,12 @@ struct tp_ctx {
     int *bs_tune_block = nullptr;
     /* learned surrogate: network, optimiser state and training buffers */
     struct MlpState *mlp = nullptr;
+    /* trajectory diagnostics armed for the next run (traj_nout > 0) */
+    int traj_nout = 0;
+    tp_particle *traj = nullptr;
+    int *traj_accum = nullptr, *traj_pts = nullptr;
+    size_t traj_cap = 0;
+    int traj_ncap = 0;
 };
 
 void tp_bs_release(tp_ctx *c);

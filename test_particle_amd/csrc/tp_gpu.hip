@@ -68,6 +68,13 @@ __global__ void __launch_bounds__(256) k_track(TrackArgs a)
     float fux = (float)p.ux, fuy = (float)p.uy, fuz = (float)p.uz;
     const float dtf = (float)a.dt;
     const float hf = (float)(a.qm * (0.5 * a.dt));
+    /* trajectory diagnostics: this particle's next slot and its slot range */
+    int tpts = 0, toff = 0, tcap = 0;
+    if (live && a.traj) {
+        tpts = a.traj_pts[i];
+        toff = i > 0 ? a.traj_accum[i - 1] : 0;
+        tcap = a.traj_accum[i] - toff;
+    }
     GridCache gc;
     gc_init(gc);
     __syncthreads();
@@ -85,6 +92,19 @@ __global__ void __launch_bounds__(256) k_track(TrackArgs a)
             }
             particle_bc(a, p, xs, ys, zs, esc_after);
             nst++;
+            /* src/tracking.c:440-447: the particle after the step, every
+             * nsteps_output steps; slots past the particle's range are
+             * counted, not written (the reference writes them) */
+            if (a.traj && j % a.traj_nout == 0) {
+                if (tpts < tcap) {
+                    tp_particle r;
+                    r.x = p.x; r.y = p.y; r.z = p.z; r.qm = a.stage[i].qm;
+                    r.ux = p.ux; r.uy = p.uy; r.uz = p.uz; r.t = p.t;
+                    r.xshift = xs; r.yshift = ys; r.zshift = zs;
+                    a.traj[toff + tpts] = r;
+                }
+                tpts++;
+            }
         }
         if (live && esc == 0 && esc_after == 1 && a.bc == 1 && ntp < a.nt_out)
             atomicAdd(&a.espect_esc[ntp * a.nbins +
@@ -113,6 +133,7 @@ __global__ void __launch_bounds__(256) k_track(TrackArgs a)
         a.xs[i] = xs; a.ys[i] = ys; a.zs[i] = zs;
         a.esc[i] = esc;
         a.nsteps[i] = nst;
+        if (a.traj) a.traj_pts[i] = tpts;
     }
 }
 
@@ -453,6 +474,9 @@ void tp_gpu_finalize(tp_ctx *c)
     free(c->evs);
     tp_bs_release(c);
     tp_mlp_release(c);
+    hipFree(c->traj);
+    hipFree(c->traj_accum);
+    hipFree(c->traj_pts);
     free(c->bs_tune_block);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
@@ -777,6 +801,7 @@ int tp_gpu_run_fixed(tp_ctx *c, const tp_track_params *p)
     if (p->precision != TP_FP64 && p->precision != TP_FP32_MIXED) return TP_E_ARG;
     const bool mlp = field_kind(c) == FK_MLP;
     if (mlp && (p->pusher != TP_PUSH_BORIS || p->precision != TP_FP32_MIXED)) return TP_E_ARG;
+    if (mlp && c->traj_nout > 0) return TP_E_ARG;
     /* surrogate field: the exact-field kernel runs the particles that fell back */
     track_fn fn = pick_track(mlp ? FK_MODES : field_kind(c), p->pusher, p->precision);
     if (!fn) return TP_E_ARG;
@@ -864,6 +889,10 @@ int tp_gpu_run_fixed(tp_ctx *c, const tp_track_params *p)
     a.st = S.st; a.esc = S.esc; a.nsteps = S.nsteps;
     a.espect = c->espect; a.espect_esc = c->espect_esc;
     a.partials = c->partials; a.rdec = c->rdec;
+    if (c->traj_nout > 0) {
+        a.traj = c->traj; a.traj_accum = c->traj_accum; a.traj_pts = c->traj_pts;
+        a.traj_nout = c->traj_nout; a.stage = S.stage;
+    }
 
     /* frame and D-row counters at step sstep (src/tracking.c:317, 328) */
     int ntp = (int)floor(p->sstep / (double)p->einterval_t);
@@ -975,6 +1004,63 @@ int tp_gpu_read_diagnostics(tp_ctx *c, double *espect, double *espect_escape, do
     free(h);
     free(r);
     return rc;
+}
+
+__global__ void k_fill_int(int *a, int n, int v)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+int tp_gpu_track_fixed_traj(tp_ctx *c, const tp_track_params *p, int n, tp_particle *ptl,
+                            const tp_particle *ptl_init, int *nsteps, double *espect,
+                            double *espect_escape, double *dcoeff, int *nptl_remain,
+                            int nsteps_output, const int *ntraj_accum, tp_particle *ptl_time,
+                            int *points)
+{
+    if (!c || !p || n < 0 || nsteps_output <= 0 || (n > 0 && (!ntraj_accum || !ptl_time)))
+        return TP_E_ARG;
+    if (field_kind(c) == FK_MLP) return TP_E_ARG;
+    /* slot ranges must be non-decreasing from 0 */
+    for (int i = 0; i < n; i++)
+        if (ntraj_accum[i] < (i > 0 ? ntraj_accum[i - 1] : 0)) return TP_E_ARG;
+    const size_t ntraj = n > 0 ? (size_t)ntraj_accum[n - 1] : 0;
+    HIPCHK(hipSetDevice(c->device));
+    if (ntraj > c->traj_cap || n > c->traj_ncap) {
+        hipFree(c->traj); hipFree(c->traj_accum); hipFree(c->traj_pts);
+        c->traj = nullptr; c->traj_accum = nullptr; c->traj_pts = nullptr;
+        c->traj_cap = 0; c->traj_ncap = 0;
+        const size_t t = ntraj > 0 ? ntraj : 1, k = n > 0 ? (size_t)n : 1;
+        if (hipMalloc(&c->traj, t * sizeof(tp_particle)) != hipSuccess ||
+            hipMalloc(&c->traj_accum, k * sizeof(int)) != hipSuccess ||
+            hipMalloc(&c->traj_pts, k * sizeof(int)) != hipSuccess)
+            return TP_E_NOMEM;
+        c->traj_cap = t;
+        c->traj_ncap = (int)k;
+    }
+    if (n > 0) {
+        HIPCHK(hipMemcpyAsync(c->traj, ptl_time, ntraj * sizeof(tp_particle),
+                              hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->traj_accum, ntraj_accum, n * sizeof(int),
+                              hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(k_fill_int, dim3((n + 255) / 256), dim3(256), 0, c->stream,
+                           c->traj_pts, n, 1);
+        HIPCHK(hipGetLastError());
+    }
+    c->traj_nout = nsteps_output;
+    int rc = tp_gpu_track_fixed(c, p, n, ptl, ptl_init, nsteps, espect, espect_escape, dcoeff,
+                                nptl_remain);
+    c->traj_nout = 0;
+    if (rc) return rc;
+    if (n > 0) {
+        HIPCHK(hipMemcpyAsync(ptl_time, c->traj, ntraj * sizeof(tp_particle),
+                              hipMemcpyDeviceToHost, c->stream));
+        if (points)
+            HIPCHK(hipMemcpyAsync(points, c->traj_pts, n * sizeof(int), hipMemcpyDeviceToHost,
+                                  c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    return TP_OK;
 }
 
 int tp_gpu_track_fixed(tp_ctx *c, const tp_track_params *p, int n, tp_particle *ptl,

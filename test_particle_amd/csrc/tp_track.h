@@ -35,6 +35,14 @@ struct TrackArgs {
     double *partials;
     int *rdec;
     const int *idx;      /* step kernels: particle g of the launch is idx[g] (null: g) */
+    /* trajectory diagnostics (traj != null): records of particle i go to
+     * traj[traj_accum[i-1] + k], k = traj_pts[i], ... every traj_nout steps;
+     * qm of the records from stage[i] (the particles as uploaded) */
+    tp_particle *traj;
+    const int *traj_accum;
+    int *traj_pts;
+    int traj_nout;
+    const tp_particle *stage;
 };
 
 /* particle_bc, src/tracking.c:1172-1221 */

@@ -39,6 +39,8 @@ def lib():
         L.tp_gpu_eval_field.argtypes = [vp, i, P(d), P(d), i]
         L.tp_gpu_track_fixed.argtypes = [vp, P(abi.TpTrackParams), i, P(d), P(d), P(i),
                                          P(d), P(d), P(d), P(i)]
+        L.tp_gpu_track_fixed_traj.argtypes = [vp, P(abi.TpTrackParams), i, P(d), P(d), P(i),
+                                              P(d), P(d), P(d), P(i), i, P(i), P(d), P(i)]
         L.tp_gpu_load_particles.argtypes = [vp, i, P(d), P(d)]
         L.tp_gpu_run_fixed.argtypes = [vp, P(abi.TpTrackParams)]
         L.tp_gpu_store_particles.argtypes = [vp, P(d), P(i)]
@@ -174,6 +176,32 @@ class Tracker:
                                         dptr(dco), iptr(rem)), "track_fixed")
         return dict(ptl=ptl, nsteps=nsteps, espect=esp, espect_escape=ese, dcoeff=dco,
                     remain=rem)
+
+    def track_fixed_traj(self, params, ptl, nsteps_output, ntraj_accum, ptl_time,
+                         ptl_init=None):
+        """track_fixed with the reference's trajectory diagnostics
+        (traj_diagnose = 1); ptl_time holds the initial points on entry."""
+        ptl = abi.as_particles(ptl).copy()
+        ptl_init = ptl.copy() if ptl_init is None else abi.as_particles(ptl_init)
+        n = ptl.shape[0]
+        nd = abi.n_dcoeff_rows(params)
+        acc = np.ascontiguousarray(ntraj_accum, dtype=np.int32)
+        pt = abi.as_particles(ptl_time).copy()
+        if acc.size != n or (n and pt.shape[0] < acc[-1]):
+            raise ValueError("ntraj_accum must have n entries and ptl_time ntraj_accum[-1] rows")
+        nsteps = np.zeros(n, dtype=np.int32)
+        points = np.zeros(n, dtype=np.int32)
+        esp = np.zeros((params.nt_out, params.nbins))
+        ese = np.zeros((params.nt_out, params.nbins))
+        dco = np.zeros((8, nd))
+        rem = np.zeros(nd, dtype=np.int32)
+        _check(lib().tp_gpu_track_fixed_traj(self._ctx, ctypes.byref(params), n, dptr(ptl),
+                                             dptr(ptl_init), iptr(nsteps), dptr(esp), dptr(ese),
+                                             dptr(dco), iptr(rem), int(nsteps_output),
+                                             iptr(acc), dptr(pt), iptr(points)),
+               "track_fixed_traj")
+        return dict(ptl=ptl, nsteps=nsteps, espect=esp, espect_escape=ese, dcoeff=dco,
+                    remain=rem, ptl_time=pt, points=points)
 
     def load(self, ptl, ptl_init=None):
         ptl = abi.as_particles(ptl)

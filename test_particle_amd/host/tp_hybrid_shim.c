@@ -7,7 +7,8 @@
  * (src/tracking.c:659-663, include/tracking.h:78-82); compile the reference's
  * src/tracking.c with -Dparticle_tracking_hybrid=particle_tracking_hybrid_cpu
  * so the CPU loop stays available for what the GPU path does not cover
- * (trajectory diagnostics, adaptive stepping, system 3, two time slices).
+ * (adaptive stepping, system 3, two time slices).  Trajectory runs
+ * (traj_diagnose = 1) go to tp_gpu_track_fixed_traj.
  *
  * The field source is taken from the reference's globals once per call;
  * results go through the reference's own collect_espectrum and
@@ -89,7 +90,7 @@ void particle_tracking_hybrid(int mpi_rank, int nptl, double dt, int nbins,
 {
     int rc = -1;
     if (!tp_shim_ctx && tp_gpu_init(0, &tp_shim_ctx) != TP_OK) tp_shim_ctx = NULL;
-    if (tp_shim_ctx && tracking_method == 0 && traj_diagnose == 0)
+    if (tp_shim_ctx && tracking_method == 0 && (traj_diagnose == 0 || traj_diagnose == 1))
         rc = tp_shim_set_field(tp_shim_ctx);
     if (rc != 0) {
         particle_tracking_hybrid_cpu(mpi_rank, nptl, dt, nbins, nt_out, bc_flag,
@@ -122,6 +123,22 @@ void particle_tracking_hybrid(int mpi_rank, int nptl, double dt, int nbins,
     double *d = calloc((size_t)8 * nd, sizeof(double));
     int *remain = calloc(nd, sizeof(int));
     double t1 = omp_get_wtime();
+    if (traj_diagnose == 1) {
+        /* trajectory run (src/diagnostics.c:555-557): ptl_time filled, no
+         * spectrum or diffusion files (src/tracking.c:173, :688) */
+        rc = tp_gpu_track_fixed_traj(tp_shim_ctx, &p, nptl, (tp_particle *)ptl,
+                                     (const tp_particle *)ptl_init, nsteps_ptl_tracking,
+                                     NULL, NULL, NULL, NULL, nsteps_output, ntraj_accum,
+                                     (tp_particle *)ptl_time, NULL);
+        if (rc != TP_OK) {
+            fprintf(stderr, "tp_gpu_track_fixed_traj: %s\n", tp_gpu_strerror(rc));
+            MPI_Abort(MPI_COMM_WORLD, 1);
+        }
+        double t2 = omp_get_wtime();
+        if (mpi_rank == 0) printf("Elapsed time: %lf s\n", t2 - t1);
+        free(espect); free(espect_esc); free(tot); free(d); free(remain);
+        return;
+    }
     rc = tp_gpu_track_fixed(tp_shim_ctx, &p, nptl, (tp_particle *)ptl,
                             (const tp_particle *)ptl_init, nsteps_ptl_tracking, espect,
                             espect_esc, d, remain);

@@ -92,6 +92,27 @@ def track_case(name, binary, spec, ptl, extra=None, tmp=None):
              **(extra or {}))
 
 
+def traj_case(name, binary, spec, ptl, extra=None, tmp=None):
+    """The fixed-step loop with trajectory diagnostics: every particle has
+    estep/nsteps_output + 1 slots, slot 0 its initial state."""
+    sp = os.path.join(tmp, "spec.txt")
+    write_spec(sp, **spec)
+    ptl.tofile(os.path.join(tmp, "ptl.bin"))
+    od = os.path.join(tmp, "t")
+    os.makedirs(od, exist_ok=True)
+    run(binary, "traj", sp, "ptl.bin", od, cwd=tmp)
+    rd = lambda f, dt=np.float64: np.fromfile(os.path.join(od, f), dtype=dt)
+    nb, nt = spec["nbins"], spec["nt_out"]
+    np.savez(os.path.join(HERE, name + ".npz"), ptl_in=ptl,
+             ptl_out=rd("ptl.bin").reshape(-1, 11),
+             traj=rd("traj.bin").reshape(ptl.shape[0], -1, 11),
+             points=rd("points.bin", np.int32),
+             espect=rd("espect.bin").reshape(nt, nb),
+             spec=np.array(json.dumps({k: float(v) if isinstance(v, np.floating) else v
+                                       for k, v in spec.items()})),
+             **(extra or {}))
+
+
 def main():
     build_ref()
     rng = np.random.default_rng(7)
@@ -193,6 +214,15 @@ def main():
                    dict(gspec, tmax=300 * 0.01 + 1e-9, bc=0, nbins=50, nt_out=3,
                         emin=1e-3, emax=1.0), gptl,
                    extra=dict(egrid=eg, bgrid=bg, lo=glo, hi=ghi, dt=0.01), tmp=tmp)
+        # --- trajectory diagnostics (traj_diagnose = 1) --------------------
+        traj_case("traj_grid", "ref_harness",
+                  dict(gspec, tmax=300 * 0.01 + 1e-9, bc=0, nbins=50, nt_out=3,
+                       emin=1e-3, emax=1.0, nsteps_output=7), gptl,
+                  extra=dict(egrid=eg, bgrid=bg, lo=glo, hi=ghi, dt=0.01), tmp=tmp)
+        traj_case("traj_open", "ref_harness",
+                  dict(system=0, tmax=600 * 0.05 + 1e-9, dt=0.05, bc=1,
+                       nbins=30, nt_out=6, emin=0.01, emax=1.0, nsteps_output=25,
+                       **box(olo, ohi)), optl[:64].copy(), tmp=tmp)
         # --- the reference's own output files (format) ---------------------
         sp = os.path.join(tmp, "spec.txt")
         write_spec(sp, system=2, nx=5, ny=5, nz=5, tmax=300 * dt_c1 + 1e-9,

@@ -108,3 +108,40 @@ def test_dropin_abc(tmp_path):
         a = np.array(lc.split(), dtype=float)
         b = np.array(lg.split(), dtype=float)
         assert np.allclose(a[:7], b[:7], rtol=2e-4) and a[8] == b[8]
+
+
+def test_dropin_trajectory_diagnostics(tmp_path):
+    """traj_diagnose = 1 through particle_tracking_hybrid: the reference
+    program with the GPU shim fills ptl_time as the reference does (within
+    1e-10: the GPU's sin/cos differ from libm by an ulp)."""
+    _need("ref_harness", "ref_harness_gpu")
+    rng = np.random.default_rng(41)
+    nz, ny, nx = 7, 6, 8
+    lo, hi = (0.0, 0.0, 0.0), (7.0, 5.0, 6.0)
+    g = rng.normal(0, 1, (2, nz, ny, nx, 3)).astype(np.float32)
+    g[1, ..., 2] += 2.0
+    g.tofile(os.path.join(tmp_path, "grid.bin"))
+    ptl = init_particles(100, lo, hi, 0.3, seed=42)
+    steps, nout = 400, 9
+    spec = dict(system=4, nx=nx, ny=ny, nz=nz, tmax=steps * 0.02 + 1e-9, dt=0.02, bc=0,
+                nbins=40, nt_out=4, emin=1e-3, emax=1.0, nsteps_output=nout,
+                grid=os.path.join(tmp_path, "grid.bin"),
+                xmin=lo[0], ymin=lo[1], zmin=lo[2], xmax=hi[0], ymax=hi[1], zmax=hi[2])
+    outs = []
+    for b in ("ref_harness", "ref_harness_gpu"):
+        d = os.path.join(str(tmp_path), b)
+        os.makedirs(os.path.join(d, "o"), exist_ok=True)
+        _spec(os.path.join(d, "spec.txt"), **spec)
+        ptl.tofile(os.path.join(d, "p.bin"))
+        print(f"[dropin] {b} trajh", flush=True)
+        subprocess.run([os.path.join(REF, b), "trajh", "spec.txt", "p.bin", "o"], cwd=d,
+                       check=True, timeout=120, stdout=subprocess.DEVNULL,
+                       env=dict(os.environ, OMP_NUM_THREADS="1"))
+        outs.append((np.fromfile(os.path.join(d, "o", "traj.bin")).reshape(100, -1, 11),
+                     np.fromfile(os.path.join(d, "o", "ptl.bin")).reshape(-1, 11)))
+    (tc, pc), (tg, pg) = outs
+    assert tc.shape[1] == steps // nout + 1
+    assert np.all(tc[:, -1, 7] > 0)            # the last slot was written
+    scale = np.maximum(np.abs(tc), 1.0)
+    assert np.max(np.abs(tg - tc) / scale) < 1e-10
+    assert np.max(np.abs(pg - pc) / np.maximum(np.abs(pc), 1.0)) < 1e-10

@@ -289,3 +289,71 @@ def test_track_growing_launch_counts_one_context(gpu):
             fresh.set_field_abc(1.0, 1.0, 1.0)
             ref = fresh.track_fixed(prm, ptl)
         _compare_track(r, ref, f"{steps} steps")
+
+
+def _traj_case(name):
+    import json as _json
+    from conftest import golden
+    g = golden(name)
+    spec = _json.loads(str(g["spec"]))
+    dt = spec["dt"]
+    estep = int(spec["tmax"] / dt)
+    prm = abi.make_params(dt=dt, estep=estep,
+                          lo=(spec["xmin"], spec["ymin"], spec["zmin"]),
+                          hi=(spec["xmax"], spec["ymax"], spec["zmax"]),
+                          bc_flag=spec["bc"], nbins=spec["nbins"], nt_out=spec["nt_out"],
+                          emin=spec["emin"], emax=spec["emax"])
+    ptl = g["ptl_in"]
+    n = ptl.shape[0]
+    nout = int(spec["nsteps_output"])
+    slots = estep // nout + 1
+    accum = (np.arange(1, n + 1) * slots).astype(np.int32)
+    ptl_time = np.zeros((n * slots, 11))
+    ptl_time[::slots] = ptl
+    return g, prm, ptl, nout, accum, ptl_time, slots
+
+
+@pytest.mark.parametrize("name", ["traj_grid", "traj_open"])
+def test_trajectory_diagnostics(gpu, oracle_tp, name):
+    """traj_diagnose = 1: the GPU's trajectory records bitwise those of the
+    oracle (same sin/cos), and within 1e-10 of the reference binary's own
+    records (libm sin/cos)."""
+    print(f"[case] trajectory diagnostics {name}", flush=True)
+    g, prm, ptl, nout, accum, ptl_time, slots = _traj_case(name)
+    if name == "traj_grid":
+        gpu.set_field_grid(g["egrid"], g["bgrid"], g["lo"], g["hi"], abi.TP_GATHER_REF_COMPAT)
+        f = oracle_tp.Field.grid(g["egrid"], g["bgrid"], g["lo"], g["hi"])
+    else:
+        gpu.set_field_test()
+        f = oracle_tp.Field.test()
+    r = gpu.track_fixed_traj(prm, ptl, nout, accum, ptl_time)
+    ref = oracle_tp.track_fixed_traj(f, prm, ptl, nout, accum, ptl_time, nthreads=1)
+    bitwise(r["ptl_time"], ref["ptl_time"], name + " records")
+    bitwise(r["points"], ref["points"], name + " points")
+    bitwise(r["ptl"], ref["ptl"], name + " particles")
+    bitwise(r["espect"], ref["espect"], name + " spectrum")
+    n = ptl.shape[0]
+    tr = r["ptl_time"].reshape(n, slots, 11)
+    gold = g["traj"]
+    scale = np.maximum(np.abs(gold), 1.0)
+    assert np.max(np.abs(tr - gold) / scale) < 1e-10
+    assert np.array_equal(r["points"], g["points"])
+
+
+def test_trajectory_fp32_last_record_is_final_state(gpu):
+    """FP32 Boris, one record at the last step: it is the final particle."""
+    from test_particle_amd.particles import init_particles
+    n, steps = 300, 400
+    print(f"[case] trajectory FP32 {n} x {steps}", flush=True)
+    gpu.set_field_abc(1.0, 1.0, 1.0)
+    ptl = init_particles(n, BOX[0], BOX[1], 0.7, seed=21)
+    prm = _params(steps, 4, abi.TP_PUSH_BORIS, precision=abi.TP_FP32_MIXED)
+    accum = (np.arange(1, n + 1) * 3).astype(np.int32)    # one spare slot each
+    pt = np.zeros((3 * n, 11))
+    pt[::3] = ptl
+    r = gpu.track_fixed_traj(prm, ptl, steps, accum, pt)
+    rec = r["ptl_time"].reshape(n, 3, 11)
+    assert np.array_equal(r["points"], np.full(n, 2))
+    bitwise(rec[:, 1], r["ptl"], "last record")
+    bitwise(rec[:, 0], ptl, "initial slot kept")
+    assert not rec[:, 2].any()

@@ -160,3 +160,50 @@ def test_track_loop_open_boundary(oracle_libm):
 def test_track_loop_pic_grid(oracle_libm):
     _track(oracle_libm, "track_grid",
            lambda g: oracle_libm.Field.grid(g["egrid"], g["bgrid"], g["lo"], g["hi"]))
+
+
+def traj_layout(n, estep, nsteps_output, ptl):
+    """Slots as the golden harness lays them out: estep/nsteps_output + 1
+    per particle, slot 0 the initial state."""
+    slots = estep // nsteps_output + 1
+    accum = (np.arange(1, n + 1) * slots).astype(np.int32)
+    ptl_time = np.zeros((n * slots, 11))
+    ptl_time[::slots] = ptl
+    return slots, accum, ptl_time
+
+
+def _traj(oracle, name, field):
+    g = golden(name)
+    spec = json.loads(str(g["spec"]))
+    dt = spec["dt"]
+    estep = int(spec["tmax"] / dt)
+    prm = abi.make_params(dt=dt, estep=estep,
+                          lo=(spec["xmin"], spec["ymin"], spec["zmin"]),
+                          hi=(spec["xmax"], spec["ymax"], spec["zmax"]),
+                          bc_flag=spec["bc"], nbins=spec["nbins"],
+                          nt_out=spec["nt_out"], emin=spec["emin"],
+                          emax=spec["emax"])
+    ptl = g["ptl_in"]
+    n = ptl.shape[0]
+    nout = int(spec["nsteps_output"])
+    slots, accum, ptl_time = traj_layout(n, estep, nout, ptl)
+    r = oracle.track_fixed_traj(field(g), prm, ptl, nout, accum, ptl_time, nthreads=1)
+    _bitwise(r["ptl"], g["ptl_out"])
+    _bitwise(r["ptl_time"].reshape(n, slots, 11), g["traj"])
+    _bitwise(r["points"], g["points"])
+    _bitwise(r["espect"], g["espect"])
+    return g, r
+
+
+def test_trajectory_diagnostics_pic_grid(oracle_libm):
+    """traj_diagnose = 1 (src/tracking.c:440-447): snapshots every 7 steps
+    of a 300-step PIC-grid run, the reference's records bit for bit."""
+    g, r = _traj(oracle_libm, "traj_grid",
+                 lambda g: oracle_libm.Field.grid(g["egrid"], g["bgrid"], g["lo"], g["hi"]))
+    assert np.all(r["points"] == 300 // 7 + 1)
+
+
+def test_trajectory_diagnostics_open_boundary(oracle_libm):
+    """Open boundaries: escaped particles stop recording."""
+    g, r = _traj(oracle_libm, "traj_open", lambda g: oracle_libm.Field.test())
+    assert r["points"].min() < r["points"].max()
