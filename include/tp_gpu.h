@@ -123,6 +123,19 @@ int tp_gpu_set_field_grid(tp_ctx *ctx, int nx, int ny, int nz,
                           const double lo[3], const double hi[3],
                           const float *efield, const float *bfield,
                           int gather_mode);
+/* Two time slices of the PIC fields (the reference's multi_tframe = 1,
+ * src/interpolation.c:97-100, src/electric_field.c:407-437,
+ * src/magnetic_field.c:593-622): at time t the field is
+ *   dt * F_next + (1 - dt) * F_prev,  dt = t/grid_dt - floor(t/grid_dt),
+ * each slice gathered with the same trilinear weights and summed in double
+ * (the reference gather then rounds the blend to float).  F_next and F_prev
+ * are what the reference loads as efa/maga (slice ct + 1) and efb/magb
+ * (slice ct - 1) (src/magnetic_field.c:440-530); layouts as for
+ * tp_gpu_set_field_grid. */
+int tp_gpu_set_field_grid2(tp_ctx *ctx, int nx, int ny, int nz, const double lo[3],
+                           const double hi[3], const float *efield_next,
+                           const float *bfield_next, const float *efield_prev,
+                           const float *bfield_prev, double grid_dt, int gather_mode);
 /* system 1: nwlcs wire-loop current systems; rows of 14 doubles in the
  * column order of the WLCS configuration file read by read_wlcs
  * (src/wlcs.c:97-179): cur_wr x_wr y_wr z_wr theta phi t0 cur_lp r_lp

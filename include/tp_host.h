@@ -42,6 +42,22 @@ int tp_host_append_espectrum(const char *fname, int nbins, int nt_out,
 int tp_host_write_diff_coeffs(const char *fname, int nd, const double *dcoeff,
                               const int *nptl_remain);
 
+/* PIC field input from the reference's binary files ("<path>ex_<ct>.gda" ...,
+ * nx*ny*nz floats each, x fastest), as read_efields_float_binary and
+ * read_bfields_float_binary load them (src/electric_field.c:257-350,
+ * src/magnetic_field.c:440-530): values scaled by e0 (= v0 B0) and b0 in
+ * double and stored as float, interleaved into the float3 node arrays that
+ * tp_gpu_set_field_grid / tp_gpu_set_field_grid2 take.  path includes its
+ * trailing separator.
+ *   two_slices = 0: slice ct into e_next, b_next;
+ *   two_slices = 1: slice min(ct + 1, nt - 1) into e_next, b_next and slice
+ *                   max(ct - 1, 0) into e_prev, b_prev (the reference's
+ *                   "next" and "previous" slices).
+ * Returns TP_E_ARG for a missing or short file (the reference exits). */
+int tp_host_read_pic_gda(const char *path, int ct, int nt, int two_slices, int nx, int ny,
+                         int nz, double e0, double b0, float *e_next, float *b_next,
+                         float *e_prev, float *b_prev);
+
 #ifdef __cplusplus
 }
 #endif

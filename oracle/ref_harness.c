@@ -25,6 +25,7 @@
  *   traj    SPEC PTL OUTDIR  the same loop with trajectory diagnostics
  *                            (traj_diagnose = 1, src/tracking.c:440-447)
  *   trajh   SPEC PTL OUTDIR  particle_tracking_hybrid with traj_diagnose = 1
+ *   gda     SPEC PATH CT OUT the PIC float field readers (.gda files)
  *   hybrid  SPEC PTL         particle_tracking_hybrid (src/tracking.c:659) with
  *                            the t=0 spectrum of src/diagnostics.c:50; writes
  *                            data/espectrum.dat and data/diff_coeffs.dat
@@ -70,6 +71,8 @@ void set_variabls_interpolation(grids *sgrid, domain *sdomain, int mt);
 void set_variables_bfield(grids *sgrid, domain *sdomain, double B0_field, int mt, int dtype);
 void set_variables_efield(grids *sgrid, domain *sdomain, double E0_field, int mt, int dtype);
 void set_variables_velocity(grids *sgrid, domain *sdomain, double v0_field, int mt, int dtype);
+void read_bfields_binary(char *filepath, int ct, int data_type);
+void read_efields_binary(char *filepath, int ct, int data_type);
 void initialize_bfield(void);
 void initialize_efield(void);
 void initialize_vfield(void);
@@ -100,6 +103,8 @@ void particle_tracking_hybrid(int mpi_rank, int nptl, double dt, int nbins,
 extern param_ff param;                          /* src/force_free.c:33 */
 extern f3 *magc_float;                          /* src/magnetic_field.c:35 */
 extern f3 *efc_float;                           /* src/electric_field.c:35 */
+extern f3 *maga_float, *magb_float;             /* two time slices */
+extern f3 *efa_float, *efb_float;
 extern d3 *vc_double;                           /* src/velocity_field.c:34 */
 extern wlcs *config;                            /* src/wlcs.c:12 */
 extern int nwlcs, iBessel;                      /* src/wlcs.c:13 */
@@ -107,8 +112,9 @@ extern double normI;                            /* src/wlcs.c:14 */
 extern double emin, emax, logemin, logemax, logde; /* src/diagnostics.c:32 */
 
 typedef struct {
-    int system, bc, nbins, nt_out, nx, ny, nz, nwlcs, ibessel, nsteps_output;
-    double lo[3], hi[3], tmax, qm, dt, pmass, emin, emax, ff_b0, ff_lambda;
+    int system, bc, nbins, nt_out, nx, ny, nz, nwlcs, ibessel, nsteps_output, mt, grid_nt;
+    double lo[3], hi[3], tmax, qm, dt, pmass, emin, emax, ff_b0, ff_lambda, grid_dt;
+    double gda_b0, gda_e0;
     double wlcs_omega, wlcs_normi;
     char grid[512], vgrid[512], wlcs[512];
 } spec_t;
@@ -151,8 +157,8 @@ static void read_spec(const char *path, spec_t *s)
 #define D(k) if (!strcmp(key, #k)) { s->k = strtod(val, NULL); continue; }
 #define S(k) if (!strcmp(key, #k)) { strcpy(s->k, val); continue; }
         I(system) I(bc) I(nbins) I(nt_out) I(nx) I(ny) I(nz) I(nwlcs) I(ibessel)
-        I(nsteps_output)
-        D(tmax) D(qm) D(dt) D(pmass) D(emin) D(emax) D(ff_b0) D(ff_lambda)
+        I(nsteps_output) I(mt) I(grid_nt)
+        D(grid_dt) D(gda_b0) D(gda_e0) D(tmax) D(qm) D(dt) D(pmass) D(emin) D(emax) D(ff_b0) D(ff_lambda)
         D(wlcs_omega) D(wlcs_normi)
         S(grid) S(vgrid) S(wlcs)
         if (!strcmp(key, "xmin")) { s->lo[0] = strtod(val, NULL); continue; }
@@ -185,16 +191,17 @@ static void setup(const spec_t *s)
     dom.zmin_ptl = dom.zmin; dom.zmax_ptl = dom.zmax;
     dom.tmin = 0.0; dom.tmax = s->tmax;
     if (s->nx > 1) {
-        g.nx = s->nx; g.ny = s->ny; g.nz = s->nz; g.nt = 1;
+        g.nx = s->nx; g.ny = s->ny; g.nz = s->nz; g.nt = s->grid_nt > 0 ? s->grid_nt : 1;
         g.dx = (dom.xmax - dom.xmin) / (g.nx - 1.0);   /* src/domain.c:548 */
         g.dy = (dom.ymax - dom.ymin) / (g.ny - 1.0);
         g.dz = (dom.zmax - dom.zmin) / (g.nz - 1.0);
-        g.dt = 0.0;
+        g.dt = s->grid_dt;
     }
-    set_variables_tracking(&g, &dom, s->system, 0);
+    const int mt = s->mt;   /* two time slices of the PIC fields */
+    set_variables_tracking(&g, &dom, s->system, mt);
     set_ptl_params_tracking(s->bc, s->qm);
     set_variables_emfields(s->system);
-    set_variabls_interpolation(&g, &dom, 0);
+    set_variabls_interpolation(&g, &dom, mt);
     size_t nn = (size_t)g.nx * g.ny * g.nz, n;
     if (s->system == 2) {
         set_variables_ff(&g, &dom);
@@ -210,15 +217,29 @@ static void setup(const spec_t *s)
         } else {
             memset(vc_double, 0, nn * sizeof(d3));
         }
-    } else if (s->system == 4) {
-        set_variables_bfield(&g, &dom, 1.0, 0, (int)sizeof(float));
-        set_variables_efield(&g, &dom, 1.0, 0, (int)sizeof(float));
+    } else if (s->system == 4 && s->gda_b0 > 0.0) {
+        /* fields come from .gda files (mode "gda") */
+        set_variables_bfield(&g, &dom, s->gda_b0, mt, (int)sizeof(float));
+        set_variables_efield(&g, &dom, s->gda_e0, mt, (int)sizeof(float));
         initialize_bfield();
         initialize_efield();
-        float *v = slurp(s->grid, &n);     /* E (float3 AoS) then B */
-        if (n != 2 * nn * sizeof(f3)) die("grid size");
-        memcpy(efc_float, v, nn * sizeof(f3));
-        memcpy(magc_float, (char *)v + nn * sizeof(f3), nn * sizeof(f3));
+    } else if (s->system == 4) {
+        set_variables_bfield(&g, &dom, 1.0, mt, (int)sizeof(float));
+        set_variables_efield(&g, &dom, 1.0, mt, (int)sizeof(float));
+        initialize_bfield();
+        initialize_efield();
+        float *v = slurp(s->grid, &n);
+        if (!mt) {                          /* E (float3 AoS) then B */
+            if (n != 2 * nn * sizeof(f3)) die("grid size");
+            memcpy(efc_float, v, nn * sizeof(f3));
+            memcpy(magc_float, (char *)v + nn * sizeof(f3), nn * sizeof(f3));
+        } else {                            /* E, B of slice a, then of slice b */
+            if (n != 4 * nn * sizeof(f3)) die("grid size");
+            memcpy(efa_float, v, nn * sizeof(f3));
+            memcpy(maga_float, (char *)v + nn * sizeof(f3), nn * sizeof(f3));
+            memcpy(efb_float, (char *)v + 2 * nn * sizeof(f3), nn * sizeof(f3));
+            memcpy(magb_float, (char *)v + 3 * nn * sizeof(f3), nn * sizeof(f3));
+        }
         free(v);
     } else if (s->system == 1) {
         /* same table read_wlcs (src/wlcs.c:53-222) builds */
@@ -397,6 +418,26 @@ int main(int argc, char **argv)
         spit(path, p, n);
         snprintf(path, sizeof(path), "%s/traj.bin", argv[4]);
         spit(path, traj, sizeof(particles) * (size_t)np * slots);
+    } else if (!strcmp(mode, "gda")) {
+        /* read_efields_binary / read_bfields_binary (src/electric_field.c:
+         * 257-361, src/magnetic_field.c:440-546) from PATH (with its trailing
+         * slash) at time slice CT; writes the loaded float3 arrays */
+        int ct = atoi(argv[4]);
+        read_efields_binary(argv[3], ct, (int)sizeof(float));
+        read_bfields_binary(argv[3], ct, (int)sizeof(float));
+        size_t nn = (size_t)s.nx * s.ny * s.nz;
+        FILE *fo = fopen(argv[5], "wb");
+        if (!fo) die("gda out");
+        if (!s.mt) {
+            fwrite(efc_float, sizeof(f3), nn, fo);
+            fwrite(magc_float, sizeof(f3), nn, fo);
+        } else {
+            fwrite(efa_float, sizeof(f3), nn, fo);
+            fwrite(maga_float, sizeof(f3), nn, fo);
+            fwrite(efb_float, sizeof(f3), nn, fo);
+            fwrite(magb_float, sizeof(f3), nn, fo);
+        }
+        fclose(fo);
     } else if (!strcmp(mode, "hybrid")) {
         particles *p = slurp(argv[3], &n);
         int np = (int)(n / sizeof(particles));

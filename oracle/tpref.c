@@ -423,7 +423,32 @@ static void get_emf(const tpref_field *f, double x, double y, double z,
         /* getemf_pic_test_particle, src/emfields.c:153-166 */
         long c[8];
         double w[8];
-        if (f->gather_mode == TP_GATHER_REF_COMPAT) {
+        if (f->egrid_b) {
+            /* two time slices (multi_tframe = 1): get_float_efield_at_point
+             * / get_float_bfield_at_point (src/electric_field.c:407-437,
+             * src/magnetic_field.c:593-622): each slice summed in double,
+             * blended dt * next + (1 - dt) * previous with dt the fraction of
+             * t / grid_dt (src/interpolation.c:97-100), returned as float
+             * (the textbook mode keeps the double) */
+            double a[6], b[6], v[6];
+            if (f->gather_mode == TP_GATHER_REF_COMPAT)
+                trilinear_ref(f->nx, f->ny, f->nz, f->lo, f->hi, x, y, z, c, w);
+            else
+                trilinear_textbook(f->nx, f->ny, f->nz, f->lo, f->hi, x, y, z, c, w);
+            gather_double(f->egrid, NULL, c, w, a);
+            gather_double(f->bgrid, NULL, c, w, a + 3);
+            gather_double(f->egrid_b, NULL, c, w, b);
+            gather_double(f->bgrid_b, NULL, c, w, b + 3);
+            double r = t / f->grid_dt;
+            double fl = floor(r);
+            int it1 = (fl > -2147483649.0 && fl < 2147483648.0) ? (int)fl : (int)0x80000000;
+            double dtf = r - it1;
+            for (int k = 0; k < 6; k++) v[k] = dtf * a[k] + (1.0 - dtf) * b[k];
+            if (f->gather_mode == TP_GATHER_REF_COMPAT)
+                for (int k = 0; k < 6; k++) v[k] = (double)(float)v[k];
+            o->Ex = v[0]; o->Ey = v[1]; o->Ez = v[2];
+            o->Bx = v[3]; o->By = v[4]; o->Bz = v[5];
+        } else if (f->gather_mode == TP_GATHER_REF_COMPAT) {
             float e[3], b[3];
             trilinear_ref(f->nx, f->ny, f->nz, f->lo, f->hi, x, y, z, c, w);
             gather_float_ref(f->egrid, c, w, e);

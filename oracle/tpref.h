@@ -45,6 +45,11 @@ typedef struct tpref_field {
     int ibessel;
     double omega, norm_i;
     int wlcs_mode;          /* TP_GATHER_REF_COMPAT: the reference's ellint */
+    /* PIC grid, second time slice (multi_tframe = 1): egrid/bgrid hold the
+     * next slice (the reference's efa/maga), egrid_b/bgrid_b the previous
+     * one (efb/magb), grid_dt the time between slices */
+    const float *egrid_b, *bgrid_b;
+    double grid_dt;
 } tpref_field;
 
 /* 1: take sin/cos from libm (as the reference does); 0: tp_sincos, the

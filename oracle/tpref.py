@@ -39,6 +39,9 @@ class TprefField(ctypes.Structure):
         ("omega", ctypes.c_double),
         ("norm_i", ctypes.c_double),
         ("wlcs_mode", ctypes.c_int),
+        ("egrid_b", ctypes.POINTER(ctypes.c_float)),
+        ("bgrid_b", ctypes.POINTER(ctypes.c_float)),
+        ("grid_dt", ctypes.c_double),
     ]
 
 
@@ -134,6 +137,19 @@ class Field:
         f.s.egrid = abi.fptr(e)
         f.s.bgrid = abi.fptr(b)
         f.s.gather_mode = mode
+        return f
+
+    @classmethod
+    def grid2(cls, e_next, b_next, e_prev, b_prev, lo, hi, grid_dt,
+              mode=abi.TP_GATHER_REF_COMPAT):
+        """Two time slices: B = dt B_next + (1 - dt) B_prev, dt the fraction
+        of t / grid_dt (the reference's multi_tframe = 1)."""
+        f = cls.grid(e_next, b_next, lo, hi, mode)
+        eb = f._hold(e_prev, np.float32)
+        bb = f._hold(b_prev, np.float32)
+        f.s.egrid_b = abi.fptr(eb)
+        f.s.bgrid_b = abi.fptr(bb)
+        f.s.grid_dt = float(grid_dt)
         return f
 
     @classmethod

@@ -33,7 +33,7 @@ struct tp_ctx {
     FieldDev f;
     bool have_field = false;
     /* owned field buffers */
-    float4 *grid = nullptr;
+    float4 *grid = nullptr, *grid_b = nullptr;
     double *vgrid = nullptr, *modes = nullptr;
     float *modes32 = nullptr;
     WlcsSys *wl = nullptr;

@@ -26,6 +26,8 @@ enum {
     FK_MODES = 5,     /* multi-mode turbulence */
     FK_WLCS = 6,      /* wire-loop current systems */
     FK_MLP = 7,       /* learned surrogate of the modes, exact modes as fallback */
+    FK_GRID2_REF = 8, /* PIC grid, two time slices, reference gather */
+    FK_GRID2_TB = 9,  /* PIC grid, two time slices, textbook gather */
 };
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -66,6 +68,10 @@ struct FieldDev {
     int nx, ny, nz;
     double lo[3], hi[3], d[3];
     const float4 *grid;
+    /* two time slices: grid = next slice, grid_b = previous slice, gdt the
+     * time between slices */
+    const float4 *grid_b;
+    double gdt;
     /* modes: rows kx ky kz ax ay az phi */
     int nmodes;
     const double *modes;
@@ -506,6 +512,38 @@ __device__ __forceinline__ void get_emf64(const FieldDev &f, double x, double y,
         }
         o.ex = ax; o.ey = ay; o.ez = az;
         o.bx = bx; o.by = by; o.bz = bz;
+    } else if (FK == FK_GRID2_REF || FK == FK_GRID2_TB) {
+        /* two time slices (the reference's multi_tframe = 1,
+         * src/electric_field.c:407-437, src/magnetic_field.c:593-622): each
+         * slice summed over the corners in double, blended
+         * dt * next + (1 - dt) * previous with dt the fraction of t / gdt
+         * (src/interpolation.c:97-100); the reference gather returns float */
+        unsigned c[8];
+        double w[8];
+        if (FK == FK_GRID2_REF) trilinear_ref(f.nx, f.ny, f.nz, f.lo, f.hi, f.d, x, y, z, c, w);
+        else trilinear_tb(f.nx, f.ny, f.nz, f.lo, f.hi, f.d, x, y, z, c, w);
+        double a[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, b[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const float4 p = f.grid[2 * c[k]], q = f.grid[2 * c[k] + 1];
+            const float4 pb = f.grid_b[2 * c[k]], qb = f.grid_b[2 * c[k] + 1];
+            a[0] = a[0] + (double)p.x * w[k]; a[1] = a[1] + (double)p.y * w[k];
+            a[2] = a[2] + (double)p.z * w[k]; a[3] = a[3] + (double)p.w * w[k];
+            a[4] = a[4] + (double)q.x * w[k]; a[5] = a[5] + (double)q.y * w[k];
+            b[0] = b[0] + (double)pb.x * w[k]; b[1] = b[1] + (double)pb.y * w[k];
+            b[2] = b[2] + (double)pb.z * w[k]; b[3] = b[3] + (double)pb.w * w[k];
+            b[4] = b[4] + (double)qb.x * w[k]; b[5] = b[5] + (double)qb.y * w[k];
+        }
+        const double r = t / f.gdt;
+        const double dtf = r - (double)tp_floor_to_int(floor(r));
+        double v[6];
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            v[k] = dtf * a[k] + (1.0 - dtf) * b[k];
+            if (FK == FK_GRID2_REF) v[k] = (double)(float)v[k];
+        }
+        o.ex = v[0]; o.ey = v[1]; o.ez = v[2];
+        o.bx = v[3]; o.by = v[4]; o.bz = v[5];
     } else if (FK == FK_MODES || FK == FK_MLP) {
         /* four independent cosines per iteration (instruction-level
          * parallelism for the FP64 FMA chains), summed in mode order */

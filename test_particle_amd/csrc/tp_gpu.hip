@@ -330,6 +330,8 @@ static track_fn pick_track(int fk, int push, int prec)
     case FK_GRID_TB: return pick_track_fk<FK_GRID_TB>(push, prec);
     case FK_MODES: return pick_track_fk<FK_MODES>(push, prec);
     case FK_WLCS: return pick_track_fk<FK_WLCS>(push, prec);
+    case FK_GRID2_REF: return pick_track_fk<FK_GRID2_REF>(push, prec);
+    case FK_GRID2_TB: return pick_track_fk<FK_GRID2_TB>(push, prec);
     }
     return nullptr;
 }
@@ -352,6 +354,8 @@ static void dispatch_dcoeff(int fk, dim3 g, dim3 b, hipStream_t s, const TrackAr
     case FK_MODES: launch_dcoeff<FK_MODES>(g, b, s, a, tstep); break;
     case FK_WLCS: launch_dcoeff<FK_WLCS>(g, b, s, a, tstep); break;
     case FK_MLP: launch_dcoeff<FK_MODES>(g, b, s, a, tstep); break;
+    case FK_GRID2_REF: launch_dcoeff<FK_GRID2_REF>(g, b, s, a, tstep); break;
+    case FK_GRID2_TB: launch_dcoeff<FK_GRID2_TB>(g, b, s, a, tstep); break;
     }
 }
 
@@ -377,6 +381,8 @@ static void dispatch_load(int fk, dim3 g, dim3 b, hipStream_t s, const FieldDev 
     case FK_MODES: launch_load<FK_MODES>(g, b, s, f, n, cur, ini, S); break;
     case FK_WLCS: launch_load<FK_WLCS>(g, b, s, f, n, cur, ini, S); break;
     case FK_MLP: launch_load<FK_MODES>(g, b, s, f, n, cur, ini, S); break;
+    case FK_GRID2_REF: launch_load<FK_GRID2_REF>(g, b, s, f, n, cur, ini, S); break;
+    case FK_GRID2_TB: launch_load<FK_GRID2_TB>(g, b, s, f, n, cur, ini, S); break;
     }
 }
 
@@ -400,6 +406,8 @@ static void dispatch_eval(int fk, dim3 g, dim3 b, hipStream_t s, const FieldDev 
     case FK_MODES: launch_eval<FK_MODES>(g, b, s, f, n, pts, out, prec); break;
     case FK_WLCS: launch_eval<FK_WLCS>(g, b, s, f, n, pts, out, prec); break;
     case FK_MLP: launch_eval<FK_MODES>(g, b, s, f, n, pts, out, prec); break;
+    case FK_GRID2_REF: launch_eval<FK_GRID2_REF>(g, b, s, f, n, pts, out, prec); break;
+    case FK_GRID2_TB: launch_eval<FK_GRID2_TB>(g, b, s, f, n, pts, out, prec); break;
     }
 }
 
@@ -448,6 +456,7 @@ int tp_gpu_init(int device, tp_ctx **out)
 static void free_field(tp_ctx *c)
 {
     hipFree(c->grid); c->grid = nullptr;
+    hipFree(c->grid_b); c->grid_b = nullptr;
     hipFree(c->vgrid); c->vgrid = nullptr;
     hipFree(c->modes); c->modes = nullptr;
     hipFree(c->modes32); c->modes32 = nullptr;
@@ -595,6 +604,34 @@ int tp_gpu_set_field_grid(tp_ctx *c, int nx, int ny, int nz, const double lo[3],
     c->f.grid = c->grid;
     c->f.kind = mode == TP_GATHER_REF_COMPAT ? FK_GRID_REF : FK_GRID_TB;
     c->have_field = true;
+    return TP_OK;
+}
+
+int tp_gpu_set_field_grid2(tp_ctx *c, int nx, int ny, int nz, const double lo[3],
+                           const double hi[3], const float *ef_next, const float *bf_next,
+                           const float *ef_prev, const float *bf_prev, double grid_dt,
+                           int mode)
+{
+    if (!c || !ef_prev || !bf_prev || !(grid_dt > 0.0) || !isfinite(grid_dt)) return TP_E_ARG;
+    int rc = tp_gpu_set_field_grid(c, nx, ny, nz, lo, hi, ef_next, bf_next, mode);
+    if (rc) return rc;
+    const size_t nn = (size_t)nx * ny * nz;
+    float *h = (float *)malloc(nn * 8 * sizeof(float));
+    if (!h) { free_field(c); return TP_E_NOMEM; }
+    for (size_t k = 0; k < nn; k++) {
+        float *o = h + 8 * k;
+        o[0] = ef_prev[3 * k]; o[1] = ef_prev[3 * k + 1]; o[2] = ef_prev[3 * k + 2];
+        o[3] = bf_prev[3 * k]; o[4] = bf_prev[3 * k + 1]; o[5] = bf_prev[3 * k + 2];
+        o[6] = 0.f; o[7] = 0.f;
+    }
+    hipError_t e = hipMalloc(&c->grid_b, nn * 8 * sizeof(float));
+    if (e != hipSuccess) { free(h); c->grid_b = nullptr; free_field(c); return TP_E_NOMEM; }
+    e = hipMemcpy(c->grid_b, h, nn * 8 * sizeof(float), hipMemcpyHostToDevice);
+    free(h);
+    if (e != hipSuccess) { free_field(c); return TP_E_HIP; }
+    c->f.grid_b = c->grid_b;
+    c->f.gdt = grid_dt;
+    c->f.kind = mode == TP_GATHER_REF_COMPAT ? FK_GRID2_REF : FK_GRID2_TB;
     return TP_OK;
 }
 

@@ -129,3 +129,54 @@ int tp_host_write_diff_coeffs(const char *fname, int nd, const double *d, const 
     fclose(fp);
     return TP_OK;
 }
+
+/* one component file into every third float of out, scaled (float)(v * scale) */
+static int read_component(const char *path, const char *comp, int ct, size_t nn,
+                          double scale, float *buf, float *out, int k)
+{
+    char fname[4096];
+    int m = snprintf(fname, sizeof(fname), "%s%s_%d.gda", path, comp, ct);
+    if (m < 0 || (size_t)m >= sizeof(fname)) return TP_E_ARG;
+    FILE *fp = fopen(fname, "rb");
+    if (!fp) return TP_E_ARG;
+    size_t got = fread(buf, sizeof(float), nn, fp);
+    fclose(fp);
+    if (got != nn) return TP_E_ARG;
+    for (size_t i = 0; i < nn; i++) out[3 * i + k] = (float)((double)buf[i] * scale);
+    return TP_OK;
+}
+
+static int read_slice(const char *path, int ct, size_t nn, double e0, double b0, float *buf,
+                      float *e, float *b)
+{
+    static const char *ec[3] = {"ex", "ey", "ez"}, *bc[3] = {"bx", "by", "bz"};
+    for (int k = 0; k < 3; k++) {
+        int rc = read_component(path, ec[k], ct, nn, e0, buf, e, k);
+        if (rc) return rc;
+        rc = read_component(path, bc[k], ct, nn, b0, buf, b, k);
+        if (rc) return rc;
+    }
+    return TP_OK;
+}
+
+int tp_host_read_pic_gda(const char *path, int ct, int nt, int two_slices, int nx, int ny,
+                         int nz, double e0, double b0, float *e_next, float *b_next,
+                         float *e_prev, float *b_prev)
+{
+    if (!path || nx < 1 || ny < 1 || nz < 1 || ct < 0 || !e_next || !b_next) return TP_E_ARG;
+    if (two_slices && (nt < 1 || ct >= nt || !e_prev || !b_prev)) return TP_E_ARG;
+    const size_t nn = (size_t)nx * ny * nz;
+    float *buf = (float *)malloc(nn * sizeof(float));
+    if (!buf) return TP_E_NOMEM;
+    int rc;
+    if (!two_slices) {
+        rc = read_slice(path, ct, nn, e0, b0, buf, e_next, b_next);
+    } else {
+        /* src/magnetic_field.c:470-524: previous = ct - 1 (ct at 0),
+         * next = ct + 1 (ct at the last slice) */
+        rc = read_slice(path, ct > 0 ? ct - 1 : ct, nn, e0, b0, buf, e_prev, b_prev);
+        if (!rc) rc = read_slice(path, ct < nt - 1 ? ct + 1 : ct, nn, e0, b0, buf, e_next, b_next);
+    }
+    free(buf);
+    return rc;
+}

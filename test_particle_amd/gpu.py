@@ -35,6 +35,9 @@ def lib():
         L.tp_gpu_set_field_grid.argtypes = [vp, i, i, i, P(d), P(d), P(ctypes.c_float),
                                             P(ctypes.c_float), i]
         L.tp_gpu_set_field_wlcs.argtypes = [vp, i, P(d), i, d, d, i]
+        L.tp_gpu_set_field_grid2.argtypes = [vp, i, i, i, P(d), P(d), P(ctypes.c_float),
+                                             P(ctypes.c_float), P(ctypes.c_float),
+                                             P(ctypes.c_float), d, i]
         L.tp_gpu_set_field_modes.argtypes = [vp, i, P(d), d]
         L.tp_gpu_eval_field.argtypes = [vp, i, P(d), P(d), i]
         L.tp_gpu_track_fixed.argtypes = [vp, P(abi.TpTrackParams), i, P(d), P(d), P(i),
@@ -65,6 +68,7 @@ def lib():
         L.tp_host_write_espectrum_t0.argtypes = [cs, cs, i, d, d, P(d), i]
         L.tp_host_append_espectrum.argtypes = [cs, i, i, d, d, P(d)]
         L.tp_host_write_diff_coeffs.argtypes = [cs, i, P(d), P(i)]
+        L.tp_host_read_pic_gda.argtypes = [cs, i, i, i, i, i, i, d, d, P(f), P(f), P(f), P(f)]
         _lib = L
     return _lib
 
@@ -139,6 +143,18 @@ class Tracker:
         nz, ny, nx = e.shape[:3]
         _check(lib().tp_gpu_set_field_grid(self._ctx, nx, ny, nz, _vec3(lo), _vec3(hi),
                                            fptr(e), fptr(b), mode), "set_field_grid")
+
+    def set_field_grid2(self, e_next, b_next, e_prev, b_prev, lo, hi, grid_dt,
+                        mode=abi.TP_GATHER_REF_COMPAT):
+        """Two time slices, blended dt * next + (1 - dt) * previous."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in (e_next, b_next, e_prev,
+                                                                     b_prev)]
+        if any(a.shape != arrs[0].shape for a in arrs):
+            raise ValueError("the four slices must have the same shape")
+        nz, ny, nx = arrs[0].shape[:3]
+        _check(lib().tp_gpu_set_field_grid2(self._ctx, nx, ny, nz, _vec3(lo), _vec3(hi),
+                                            *(fptr(a) for a in arrs), float(grid_dt), mode),
+               "set_field_grid2")
 
     def set_field_wlcs(self, rows, ibessel=0, omega=0.001, norm_i=1.0,
                        mode=abi.TP_GATHER_REF_COMPAT):
@@ -349,3 +365,18 @@ def write_outputs(dirname, ptl_init, result, nbins, emin, emax, bc_flag=0):
     r = np.ascontiguousarray(result["remain"], dtype=np.int32)
     _check(L.tp_host_write_diff_coeffs(os.path.join(dirname, "diff_coeffs.dat").encode(),
                                        d.shape[1], dptr(d), iptr(r)), "write_diff_coeffs")
+
+
+def read_pic_gda(path, ct, nx, ny, nz, e0, b0, nt=None, two_slices=False):
+    """The reference's PIC field files (include/tp_host.h,
+    tp_host_read_pic_gda): returns (e, b) of slice ct, or with two_slices
+    (e_next, b_next, e_prev, b_prev), each float32 (nz, ny, nx, 3)."""
+    shape = (nz, ny, nx, 3)
+    en, bn = np.zeros(shape, np.float32), np.zeros(shape, np.float32)
+    ep = np.zeros(shape, np.float32) if two_slices else None
+    bp = np.zeros(shape, np.float32) if two_slices else None
+    p = path if path.endswith(os.sep) else path + os.sep
+    _check(lib().tp_host_read_pic_gda(p.encode(), int(ct), int(nt or 0), 1 if two_slices else 0,
+                                      nx, ny, nz, float(e0), float(b0), fptr(en), fptr(bn),
+                                      fptr(ep), fptr(bp)), "read_pic_gda")
+    return (en, bn, ep, bp) if two_slices else (en, bn)

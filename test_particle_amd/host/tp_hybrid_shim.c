@@ -7,8 +7,9 @@
  * (src/tracking.c:659-663, include/tracking.h:78-82); compile the reference's
  * src/tracking.c with -Dparticle_tracking_hybrid=particle_tracking_hybrid_cpu
  * so the CPU loop stays available for what the GPU path does not cover
- * (adaptive stepping, system 3, two time slices).  Trajectory runs
- * (traj_diagnose = 1) go to tp_gpu_track_fixed_traj.
+ * (adaptive stepping, system 3, double-precision grids).  Trajectory runs
+ * (traj_diagnose = 1) go to tp_gpu_track_fixed_traj, two PIC time slices
+ * to tp_gpu_set_field_grid2.
  *
  * The field source is taken from the reference's globals once per call;
  * results go through the reference's own collect_espectrum and
@@ -46,8 +47,8 @@ extern struct grids simul_grid;
 extern int system_type, multi_tframe, data_type;
 extern double charge_mass;
 extern struct param_ff param;
-extern struct bfields_float *magc_float;
-extern struct efields_float *efc_float;
+extern struct bfields_float *magc_float, *maga_float, *magb_float;
+extern struct efields_float *efc_float, *efa_float, *efb_float;
 extern struct vfields_double *vc_double;
 extern struct wlcs *config;
 extern int nwlcs, iBessel;
@@ -73,7 +74,16 @@ static int tp_shim_set_field(tp_ctx *c)
         return tp_gpu_set_field_abc(c, ffA, ffB, ffC, param.lambda0, param.B0);
     }
     case 4:
-        if (multi_tframe != 0 || data_type != (int)sizeof(float)) return -1;
+        if (data_type != (int)sizeof(float)) return -1;
+        if (multi_tframe == 1)
+            /* two time slices: efa/maga = next, efb/magb = previous
+             * (src/magnetic_field.c:470-524) */
+            return tp_gpu_set_field_grid2(c, simul_grid.nx, simul_grid.ny, simul_grid.nz, lo,
+                                          hi, (const float *)efa_float,
+                                          (const float *)maga_float,
+                                          (const float *)efb_float,
+                                          (const float *)magb_float, simul_grid.dt,
+                                          TP_GATHER_REF_COMPAT);
         return tp_gpu_set_field_grid(c, simul_grid.nx, simul_grid.ny, simul_grid.nz, lo, hi,
                                      (const float *)efc_float, (const float *)magc_float,
                                      TP_GATHER_REF_COMPAT);

@@ -223,6 +223,45 @@ def main():
                   dict(system=0, tmax=600 * 0.05 + 1e-9, dt=0.05, bc=1,
                        nbins=30, nt_out=6, emin=0.01, emax=1.0, nsteps_output=25,
                        **box(olo, ohi)), optl[:64].copy(), tmp=tmp)
+        # --- PIC grid with two time slices (multi_tframe = 1) --------------
+        rng2 = np.random.default_rng(99)
+        slices = [rng2.normal(0, 1, (nz, ny, nx, 3)).astype(np.float32) for _ in range(4)]
+        slices[1][..., 2] += 1.5                 # B of slice a
+        slices[3][..., 2] += 1.5                 # B of slice b
+        np.concatenate([a.ravel() for a in slices]).tofile(os.path.join(tmp, "grid2.bin"))
+        g2spec = dict(gspec, mt=1, grid_dt=0.37, grid=os.path.join(tmp, "grid2.bin"))
+        g2p = np.column_stack([rng2.uniform(-0.3, 5.3, 400), rng2.uniform(-1.2, 3.2, 400),
+                               rng2.uniform(1.8, 5.2, 400), rng2.uniform(0.0, 3.0, 400)])
+        fields_case("fields_grid2", "ref_harness", g2spec, g2p,
+                    extra=dict(ea=slices[0], ba=slices[1], eb=slices[2], bb=slices[3],
+                               lo=glo, hi=ghi, grid_dt=0.37), tmp=tmp)
+        g2ptl = init_particles(32, glo, ghi, 0.3, seed=17)
+        track_case("track_grid2", "ref_harness",
+                   dict(g2spec, tmax=300 * 0.01 + 1e-9, bc=0, nbins=50, nt_out=3,
+                        emin=1e-3, emax=1.0), g2ptl,
+                   extra=dict(ea=slices[0], ba=slices[1], eb=slices[2], bb=slices[3],
+                              lo=glo, hi=ghi, grid_dt=0.37, dt=0.01), tmp=tmp)
+        # --- PIC field input from .gda files (one and two time slices) -----
+        rng3 = np.random.default_rng(77)
+        gdir = os.path.join(tmp, "gda") + "/"
+        os.makedirs(gdir, exist_ok=True)
+        files = {}
+        for ct in range(4):
+            for comp in ("ex", "ey", "ez", "bx", "by", "bz"):
+                a = rng3.normal(0, 1, nz * ny * nx).astype(np.float32)
+                a.tofile(f"{gdir}{comp}_{ct}.gda")
+                files[f"{comp}_{ct}"] = a
+        out = {}
+        for mt in (0, 1):
+            for ct in (0, 2, 3):
+                sp = os.path.join(tmp, "spec.txt")
+                write_spec(sp, **dict(gspec, mt=mt, grid_nt=4, grid_dt=0.5, gda_b0=2.5,
+                                      gda_e0=0.3))
+                run("ref_harness", "gda", sp, gdir, ct, "gout.bin", cwd=tmp)
+                out[f"mt{mt}_ct{ct}"] = np.fromfile(os.path.join(tmp, "gout.bin"),
+                                                    dtype=np.float32)
+        np.savez(os.path.join(HERE, "gda_slices.npz"), nx=nx, ny=ny, nz=nz, nt=4, b0=2.5,
+                 e0=0.3, **files, **out)
         # --- the reference's own output files (format) ---------------------
         sp = os.path.join(tmp, "spec.txt")
         write_spec(sp, system=2, nx=5, ny=5, nz=5, tmax=300 * dt_c1 + 1e-9,

@@ -145,3 +145,25 @@ def test_dropin_trajectory_diagnostics(tmp_path):
     scale = np.maximum(np.abs(tc), 1.0)
     assert np.max(np.abs(tg - tc) / scale) < 1e-10
     assert np.max(np.abs(pg - pc) / np.maximum(np.abs(pc), 1.0)) < 1e-10
+
+
+def test_dropin_pic_grid_two_time_slices(tmp_path):
+    """System 4 with two time slices (multi_tframe = 1): the reference
+    program with the GPU shim writes the same espectrum.dat and
+    diff_coeffs.dat as the reference."""
+    _need("ref_harness", "ref_harness_gpu")
+    rng = np.random.default_rng(51)
+    nz, ny, nx = 9, 8, 10
+    lo, hi = (0.0, 0.0, 0.0), (9.0, 7.0, 8.0)
+    g = rng.normal(0, 1, (4, nz, ny, nx, 3)).astype(np.float32)   # E, B next; E, B prev
+    g[1, ..., 2] += 2.0
+    g[3, ..., 2] += 2.0
+    g.tofile(os.path.join(tmp_path, "grid2.bin"))
+    ptl = init_particles(400, lo, hi, 0.3, seed=52)
+    spec = dict(system=4, nx=nx, ny=ny, nz=nz, tmax=600 * 0.02 + 1e-9, dt=0.02, bc=0,
+                nbins=60, nt_out=6, emin=1e-3, emax=1.0, mt=1, grid_dt=1.3,
+                grid=os.path.join(tmp_path, "grid2.bin"),
+                xmin=lo[0], ymin=lo[1], zmin=lo[2], xmax=hi[0], ymax=hi[1], zmax=hi[2])
+    cpu, gpu = _run_both(str(tmp_path), "ref_harness", "ref_harness_gpu", spec, ptl)
+    assert set(cpu) == {"diff_coeffs.dat", "espectrum.dat"}
+    assert cpu == gpu

@@ -357,3 +357,39 @@ def test_trajectory_fp32_last_record_is_final_state(gpu):
     bitwise(rec[:, 1], r["ptl"], "last record")
     bitwise(rec[:, 0], ptl, "initial slot kept")
     assert not rec[:, 2].any()
+
+
+@pytest.mark.parametrize("mode", [abi.TP_GATHER_REF_COMPAT, abi.TP_GATHER_TEXTBOOK])
+def test_field_grid_two_time_slices(gpu, oracle_tp, mode):
+    """multi_tframe = 1: the GPU blend of two slices bitwise the oracle's
+    (which is the reference's bit for bit in the reference mode)."""
+    from conftest import golden
+    g = golden("fields_grid2")
+    print(f"[case] two-slice grid fields mode {mode}", flush=True)
+    gpu.set_field_grid2(g["ea"], g["ba"], g["eb"], g["bb"], g["lo"], g["hi"],
+                        float(g["grid_dt"]), mode)
+    out = gpu.eval_field(g["pts"])
+    f = oracle_tp.Field.grid2(g["ea"], g["ba"], g["eb"], g["bb"], g["lo"], g["hi"],
+                              float(g["grid_dt"]), mode)
+    bitwise(out, oracle_tp.eval_field(f, g["pts"]), "two-slice fields")
+    if mode == abi.TP_GATHER_REF_COMPAT:
+        bitwise(out, g["out"], "two-slice fields vs reference")
+
+
+def test_track_grid_two_time_slices(gpu, oracle_tp):
+    """300 Wirz steps through the time-blended grid (the particle time moves
+    through eight slice intervals): bitwise the oracle."""
+    from conftest import golden
+    g = golden("track_grid2")
+    print("[case] two-slice grid tracking", flush=True)
+    gpu.set_field_grid2(g["ea"], g["ba"], g["eb"], g["bb"], g["lo"], g["hi"],
+                        float(g["grid_dt"]))
+    prm = _params(300, 3, abi.TP_PUSH_WIRZ, lo=g["lo"], hi=g["hi"], dt=float(g["dt"]),
+                  nbins=50, emin=1e-3, emax=1.0)
+    r = gpu.track_fixed(prm, g["ptl_in"])
+    f = oracle_tp.Field.grid2(g["ea"], g["ba"], g["eb"], g["bb"], g["lo"], g["hi"],
+                              float(g["grid_dt"]))
+    ref = oracle_tp.track_fixed(f, prm, g["ptl_in"], nthreads=1)
+    _compare_track(r, ref, "two-slice grid")
+    scale = np.maximum(np.abs(g["ptl_out"]), 1.0)
+    assert np.max(np.abs(r["ptl"] - g["ptl_out"]) / scale) < 1e-10

@@ -36,3 +36,25 @@ def test_spectrum_bins_formulas():
     assert eb[0] == 0.5e-2 and ei[0] == 1e-2
     assert np.isclose(eb[3], 10 ** (-2 + logde * 2.5))
     assert np.isclose(ei[3], 1e-2 * (10 ** (logde * 3) - 10 ** (logde * 2)))
+
+
+@pytest.mark.parametrize("two", [False, True])
+@pytest.mark.parametrize("ct", [0, 2, 3])
+def test_read_pic_gda_matches_reference_loader(tmp_path, two, ct):
+    """The .gda field input (tp_host_read_pic_gda) loads exactly what the
+    reference's read_efields/read_bfields_binary load: the same slices
+    (previous ct - 1 and next ct + 1, clamped at the ends), the same
+    double-then-float scaling, the same node order."""
+    from conftest import golden
+    from test_particle_amd.gpu import read_pic_gda
+    g = golden("gda_slices")
+    nx, ny, nz, nt = int(g["nx"]), int(g["ny"]), int(g["nz"]), int(g["nt"])
+    for k in g.files:
+        if k[:2] in ("ex", "ey", "ez", "bx", "by", "bz"):
+            g[k].tofile(os.path.join(str(tmp_path), k + ".gda"))
+    ref = g[f"mt{int(two)}_ct{ct}"].reshape(-1, nz, ny, nx, 3)
+    out = read_pic_gda(str(tmp_path), ct, nx, ny, nz, float(g["e0"]), float(g["b0"]), nt=nt,
+                       two_slices=two)
+    assert len(out) == ref.shape[0]
+    for a, b in zip(out, ref):
+        assert np.array_equal(a, b)

@@ -207,3 +207,27 @@ def test_trajectory_diagnostics_open_boundary(oracle_libm):
     """Open boundaries: escaped particles stop recording."""
     g, r = _traj(oracle_libm, "traj_open", lambda g: oracle_libm.Field.test())
     assert r["points"].min() < r["points"].max()
+
+
+def test_pic_grid_two_time_slices(oracle_libm):
+    """multi_tframe = 1: each slice summed in double, blended by the
+    fraction of t / grid_dt, returned as float -- the reference's values bit
+    for bit at points and times spread over several slice intervals."""
+    g = golden("fields_grid2")
+    f = oracle_libm.Field.grid2(g["ea"], g["ba"], g["eb"], g["bb"], g["lo"], g["hi"],
+                                float(g["grid_dt"]))
+    _bitwise(oracle_libm.eval_field(f, g["pts"]), g["out"])
+    # at t = 0 and t = grid_dt (dt = 0) the next slice has no weight
+    pts = g["pts"][:20].copy()
+    z = np.zeros_like(g["ea"])
+    f0 = oracle_libm.Field.grid2(z, z, g["eb"], g["bb"], g["lo"], g["hi"],
+                                 float(g["grid_dt"]))
+    for t in (0.0, float(g["grid_dt"])):
+        pts[:, 3] = t
+        _bitwise(oracle_libm.eval_field(f, pts), oracle_libm.eval_field(f0, pts))
+
+
+def test_track_loop_pic_grid_two_time_slices(oracle_libm):
+    _track(oracle_libm, "track_grid2",
+           lambda g: oracle_libm.Field.grid2(g["ea"], g["ba"], g["eb"], g["bb"], g["lo"],
+                                             g["hi"], float(g["grid_dt"])))
