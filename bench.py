@@ -19,7 +19,8 @@ invocation, else the plain FP64 push; the plain value is reported beside
 it.  Extra workloads under "forms": the ABC field
 (16,384 particles x 10,000 steps FP64 Boris and Wirz; 4M particles x 5,000
 steps FP32 against FP64) and the batched adaptive Bulirsch-Stoer stepper
-(1,000-mode turbulence, 1M particles).
+(1,000-mode turbulence, 1M particles), the learned-surrogate workload and
+the PIC grid with two time slices.
 
     python bench.py [--gpus 1] [--steps 5000] [--warmup 200]
 """
@@ -75,9 +76,9 @@ def main():
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 form")
-    ap.add_argument("--forms", default="abc,bs,surrogate",
-                    help="extra workloads reported under 'forms' (abc, bs, surrogate; '' "
-                         "for none)")
+    ap.add_argument("--forms", default="abc,bs,surrogate,grid2",
+                    help="extra workloads reported under 'forms' (abc, bs, surrogate, "
+                         "grid2; '' for none)")
     ap.add_argument("--sur-epochs", type=int, default=4, help="surrogate training epochs")
     ap.add_argument("--sur-sigma", type=float, default=4.0,
                     help="spread of the surrogate's Fourier frequencies (lattice units)")
@@ -185,6 +186,8 @@ def main():
         forms.update(forms_bs(args))
     if "surrogate" in extra:
         forms.update(forms_surrogate(args))
+    if "grid2" in extra:
+        forms.update(forms_grid2(args))
 
     out = {
         "metric": METRIC, "value": chosen_value, "unit": "particle-steps/s", "n_gpus": 1,
@@ -421,6 +424,47 @@ def forms_surrogate(args):
         "residual_median": float(np.median(res)), "residual_p90": float(np.quantile(res, 0.9)),
         "residual_p99": float(np.quantile(res, 0.99)),
         "fraction_residual_below_1e-3": float(np.mean(res < 1e-3))}
+    return out
+
+
+def forms_grid2(args):
+    """PIC fields with two time slices (the reference's multi_tframe = 1):
+    the PIC-grid workload with a second, independent 256^3 slice, the field
+    blended in time between them (slice spacing 20 time units, so the 1000
+    timed steps cross several blend intervals); FP64 and FP32 Boris, 1M
+    particles.  Algorithmic bytes: 384 per particle-step (two slices)."""
+    from test_particle_amd import abi
+    from test_particle_amd.fields import gyroperiod, pic_turbulence_grid
+    from test_particle_amd.gpu import Tracker
+    from test_particle_amd.particles import init_particles
+    e1, b1, lo, hi = pic_turbulence_grid(args.grid, seed=args.seed)
+    e0, b0, _, _ = pic_turbulence_grid(args.grid, seed=args.seed + 1)
+    n, K = args.particles, 1000
+    ptl = init_particles(n, lo, hi, 0.7, seed=args.seed)
+    dt = 0.01 * gyroperiod(0.7, 1.0)
+    out = {}
+    with Tracker(0) as tr:
+        tr.set_field_grid2(e1, b1, e0, b0, lo, hi, 20.0)
+        del e1, b1, e0, b0
+        for prec, name in ((abi.TP_FP64, "pic_grid2_1M_fp64"), (abi.TP_FP32_MIXED,
+                                                                "pic_grid2_1M_fp32")):
+            prm = abi.make_params(dt=dt, estep=K, lo=lo, hi=hi, pusher=abi.TP_PUSH_BORIS,
+                                  precision=prec, nbins=102, nt_out=10, emin=1e-4, emax=1.0)
+            tr.load(ptl)
+            tr.run(abi.make_params(dt=dt, estep=100, lo=lo, hi=hi, pusher=abi.TP_PUSH_BORIS,
+                                   precision=prec, nbins=102, nt_out=1, emin=1e-4,
+                                   emax=1.0))
+            tr.load(ptl)
+            t0 = time.perf_counter()
+            tr.run(prm)
+            w = time.perf_counter() - t0
+            kms, launches = tr.last_kernel_time()
+            out[name] = {"value": n * K / w, "dtype": "f64" if prec == abi.TP_FP64 else
+                         "f32 push and gather, f64 positions", "particles": n, "steps": K,
+                         "algorithmic_GBps": 384.0 * n * K / (kms * 1e-3) / 1e9,
+                         "kernel_ms_per_launch": kms / max(launches, 1)}
+            log(f"[bench] {name}: {n * K / w:.4e} particle-steps/s, "
+                f"{out[name]['algorithmic_GBps']:.0f} GB/s algorithmic")
     return out
 
 

@@ -208,31 +208,81 @@ __device__ __forceinline__ void trilinear_tb(int nx, int ny, int nz,
  * bit-identical to the uncached gather. */
 #define TP_GC_INVALID 0xFFFFFFFFu
 
-struct GridCache {
+/* NS time slices: v[48 s + 6 k + component] of corner k in slice s (slice 0
+ * = grid, slice 1 = grid_b) */
+template <int NS>
+struct GridCacheN {
     unsigned k0, k7;
-    float v[48];
+    float v[48 * NS];
+};
+typedef GridCacheN<1> GridCache;
+
+/* the cache a field kind's step kernel carries */
+template <int FK>
+struct CacheFor {
+    typedef GridCacheN<(FK == FK_GRID2_REF || FK == FK_GRID2_TB) ? 2 : 1> type;
 };
 
-__device__ __forceinline__ void gc_init(GridCache &gc) { gc.k0 = TP_GC_INVALID; gc.k7 = 0u; }
+template <int NS>
+__device__ __forceinline__ void gc_init(GridCacheN<NS> &gc) { gc.k0 = TP_GC_INVALID; gc.k7 = 0u; }
 
-__device__ __forceinline__ void gc_fill(const FieldDev &f, GridCache &gc, const unsigned *c,
-                                        bool clamped)
+template <int NS>
+__device__ __forceinline__ void gc_fill(const FieldDev &f, GridCacheN<NS> &gc,
+                                        const unsigned *c, bool clamped)
 {
     if (c[0] != gc.k0 || c[7] != gc.k7) {
 #pragma unroll
-        for (int k = 0; k < 8; k++) {
-            const float4 p = f.grid[2 * (size_t)c[k]];
-            const float2 q = *(const float2 *)(f.grid + 2 * (size_t)c[k] + 1);
-            gc.v[6 * k + 0] = p.x; gc.v[6 * k + 1] = p.y; gc.v[6 * k + 2] = p.z;
-            gc.v[6 * k + 3] = p.w; gc.v[6 * k + 4] = q.x; gc.v[6 * k + 5] = q.y;
+        for (int s = 0; s < NS; s++) {
+            const float4 *g = s == 0 ? f.grid : f.grid_b;
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const float4 p = g[2 * (size_t)c[k]];
+                const float2 q = *(const float2 *)(g + 2 * (size_t)c[k] + 1);
+                float *v = gc.v + 48 * s + 6 * k;
+                v[0] = p.x; v[1] = p.y; v[2] = p.z;
+                v[3] = p.w; v[4] = q.x; v[5] = q.y;
+            }
         }
         gc.k0 = clamped ? TP_GC_INVALID : c[0];
         gc.k7 = c[7];
     }
 }
 
+/* two time slices through the cache: the same sums, blend and rounding as
+ * get_emf64<FK_GRID2_*> */
 template <int FK>
-__device__ __forceinline__ void grid_emf64_cached(const FieldDev &f, GridCache &gc, double x,
+__device__ __forceinline__ void grid2_emf64_cached(const FieldDev &f, GridCacheN<2> &gc,
+                                                   double x, double y, double z, double t,
+                                                   Emf &o)
+{
+    unsigned c[8];
+    double w[8];
+    bool cl = false;
+    if (FK == FK_GRID2_REF) trilinear_ref(f.nx, f.ny, f.nz, f.lo, f.hi, f.d, x, y, z, c, w, &cl);
+    else trilinear_tb(f.nx, f.ny, f.nz, f.lo, f.hi, f.d, x, y, z, c, w);
+    gc_fill(f, gc, c, cl);
+    double a[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, b[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+#pragma unroll
+        for (int m = 0; m < 6; m++) {
+            a[m] = a[m] + (double)gc.v[6 * k + m] * w[k];
+            b[m] = b[m] + (double)gc.v[48 + 6 * k + m] * w[k];
+        }
+    const double r = t / f.gdt;
+    const double dtf = r - (double)tp_floor_to_int(floor(r));
+    double v[6];
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+        v[m] = dtf * a[m] + (1.0 - dtf) * b[m];
+        if (FK == FK_GRID2_REF) v[m] = (double)(float)v[m];
+    }
+    o.ex = v[0]; o.ey = v[1]; o.ez = v[2];
+    o.bx = v[3]; o.by = v[4]; o.bz = v[5];
+}
+
+template <int FK>
+__device__ __forceinline__ void grid_emf64_cached(const FieldDev &f, GridCacheN<1> &gc, double x,
                                                   double y, double z, Emf &o)
 {
     unsigned c[8];
@@ -271,7 +321,7 @@ __device__ __forceinline__ void grid_emf64_cached(const FieldDev &f, GridCache &
 }
 
 template <int FK>
-__device__ __forceinline__ void grid_emf32_cached(const FieldDev &f, GridCache &gc, double x,
+__device__ __forceinline__ void grid_emf32_cached(const FieldDev &f, GridCacheN<1> &gc, double x,
                                                   double y, double z, EmfF &o)
 {
     unsigned c[8];
@@ -670,20 +720,32 @@ __device__ __forceinline__ void get_emf32(const FieldDev &f, double x, double y,
 }
 
 /* field evaluation inside the step loop: grids go through the cell cache */
-template <int FK>
-__device__ __forceinline__ void emf64_step(const FieldDev &f, GridCache &gc, double x,
+template <int FK, class GC>
+__device__ __forceinline__ void emf64_step(const FieldDev &f, GC &gc, double x,
                                            double y, double z, double t, Emf &o)
 {
-    if (FK == FK_GRID_REF || FK == FK_GRID_TB) grid_emf64_cached<FK>(f, gc, x, y, z, o);
-    else get_emf64<FK>(f, x, y, z, t, o);
+    if constexpr (FK == FK_GRID_REF || FK == FK_GRID_TB)
+        grid_emf64_cached<FK>(f, gc, x, y, z, o);
+    else if constexpr (FK == FK_GRID2_REF || FK == FK_GRID2_TB)
+        grid2_emf64_cached<FK>(f, gc, x, y, z, t, o);
+    else
+        get_emf64<FK>(f, x, y, z, t, o);
 }
 
-template <int FK>
-__device__ __forceinline__ void emf32_step(const FieldDev &f, GridCache &gc, double x,
+template <int FK, class GC>
+__device__ __forceinline__ void emf32_step(const FieldDev &f, GC &gc, double x,
                                            double y, double z, double t, EmfF &o)
 {
-    if (FK == FK_GRID_REF || FK == FK_GRID_TB) grid_emf32_cached<FK>(f, gc, x, y, z, o);
-    else get_emf32<FK>(f, x, y, z, t, o);
+    if constexpr (FK == FK_GRID_REF || FK == FK_GRID_TB) {
+        grid_emf32_cached<FK>(f, gc, x, y, z, o);
+    } else if constexpr (FK == FK_GRID2_REF || FK == FK_GRID2_TB) {
+        Emf e;
+        grid2_emf64_cached<FK>(f, gc, x, y, z, t, e);
+        o.bx = (float)e.bx; o.by = (float)e.by; o.bz = (float)e.bz;
+        o.ex = (float)e.ex; o.ey = (float)e.ey; o.ez = (float)e.ez;
+    } else {
+        get_emf32<FK>(f, x, y, z, t, o);
+    }
 }
 
 /* ------------------------------------------------------------------ */
@@ -694,8 +756,8 @@ struct Ptl {
 };
 
 /* tracking_wirz, src/tracking.c:749-892, in the reference's order */
-template <int FK>
-__device__ __forceinline__ void wirz_step64(const FieldDev &f, GridCache &gc, Ptl &p,
+template <int FK, class GC>
+__device__ __forceinline__ void wirz_step64(const FieldDev &f, GC &gc, Ptl &p,
                                             double dt, double qm)
 {
     Emf e;
@@ -767,8 +829,8 @@ __device__ __forceinline__ void wirz_step64(const FieldDev &f, GridCache &gc, Pt
 }
 
 /* relativistic Boris (same operations as the oracle's tpref_boris_step) */
-template <int FK>
-__device__ __forceinline__ void boris_step64(const FieldDev &f, GridCache &gc, Ptl &p,
+template <int FK, class GC>
+__device__ __forceinline__ void boris_step64(const FieldDev &f, GC &gc, Ptl &p,
                                              double dt, double qm)
 {
     Emf e;
@@ -832,8 +894,8 @@ __device__ __forceinline__ void boris_push32(const EmfF &e, Ptl &p, float &ux, f
     p.t += dt;
 }
 
-template <int FK>
-__device__ __forceinline__ void boris_step32(const FieldDev &f, GridCache &gc, Ptl &p,
+template <int FK, class GC>
+__device__ __forceinline__ void boris_step32(const FieldDev &f, GC &gc, Ptl &p,
                                              float &ux, float &uy, float &uz,
                                              double dt, float dtf, float hf)
 {

@@ -36,7 +36,7 @@
  * step (or at estep); k_dcoeff then evaluates the D terms on the state
  * after that step.  The spectrum (any cadence) and the escape spectrum are
  * binned here. */
-template <int PUSH, int FK, int PREC>
+template <int PUSH, int FK, int PREC, bool TRAJ>
 __global__ void __launch_bounds__(256) k_track(TrackArgs a)
 {
     extern __shared__ unsigned hist[];
@@ -68,14 +68,10 @@ __global__ void __launch_bounds__(256) k_track(TrackArgs a)
     float fux = (float)p.ux, fuy = (float)p.uy, fuz = (float)p.uz;
     const float dtf = (float)a.dt;
     const float hf = (float)(a.qm * (0.5 * a.dt));
-    /* trajectory diagnostics: this particle's next slot and its slot range */
-    int tpts = 0, toff = 0, tcap = 0;
-    if (live && a.traj) {
-        tpts = a.traj_pts[i];
-        toff = i > 0 ? a.traj_accum[i - 1] : 0;
-        tcap = a.traj_accum[i] - toff;
-    }
-    GridCache gc;
+    /* trajectory diagnostics: this particle's next slot (its slot range is
+     * read when a point is recorded) */
+    int tpts = (TRAJ && live) ? a.traj_pts[i] : 0;
+    typename CacheFor<FK>::type gc;
     gc_init(gc);
     __syncthreads();
 
@@ -95,8 +91,9 @@ __global__ void __launch_bounds__(256) k_track(TrackArgs a)
             /* src/tracking.c:440-447: the particle after the step, every
              * nsteps_output steps; slots past the particle's range are
              * counted, not written (the reference writes them) */
-            if (a.traj && j % a.traj_nout == 0) {
-                if (tpts < tcap) {
+            if (TRAJ && j % a.traj_nout == 0) {
+                const int toff = i > 0 ? a.traj_accum[i - 1] : 0;
+                if (tpts < a.traj_accum[i] - toff) {
                     tp_particle r;
                     r.x = p.x; r.y = p.y; r.z = p.z; r.qm = a.stage[i].qm;
                     r.ux = p.ux; r.uy = p.uy; r.uz = p.uz; r.t = p.t;
@@ -133,7 +130,7 @@ __global__ void __launch_bounds__(256) k_track(TrackArgs a)
         a.xs[i] = xs; a.ys[i] = ys; a.zs[i] = zs;
         a.esc[i] = esc;
         a.nsteps[i] = nst;
-        if (a.traj) a.traj_pts[i] = tpts;
+        if (TRAJ) a.traj_pts[i] = tpts;
     }
 }
 
@@ -303,35 +300,43 @@ __global__ void __launch_bounds__(256) k_eval(FieldDev f, int n, const double *p
 
 
 
-template <int PUSH, int FK, int PREC>
+template <int PUSH, int FK, int PREC, bool TRAJ>
 static void launch_track(dim3 g, dim3 b, size_t sh, hipStream_t s, const TrackArgs &a)
 {
-    hipLaunchKernelGGL((k_track<PUSH, FK, PREC>), g, b, sh, s, a);
+    hipLaunchKernelGGL((k_track<PUSH, FK, PREC, TRAJ>), g, b, sh, s, a);
 }
 
-template <int FK>
-static track_fn pick_track_fk(int push, int prec)
+/* trajectory recording is a separate instantiation, so the kernels without
+ * it carry none of its registers */
+template <int FK, bool TRAJ>
+static track_fn pick_track_fkt(int push, int prec)
 {
     if (prec == TP_FP64) {
-        if (push == TP_PUSH_WIRZ) return launch_track<TP_PUSH_WIRZ, FK, PREC64>;
-        return launch_track<TP_PUSH_BORIS, FK, PREC64>;
+        if (push == TP_PUSH_WIRZ) return launch_track<TP_PUSH_WIRZ, FK, PREC64, TRAJ>;
+        return launch_track<TP_PUSH_BORIS, FK, PREC64, TRAJ>;
     }
-    if (push == TP_PUSH_BORIS) return launch_track<TP_PUSH_BORIS, FK, PREC32>;
+    if (push == TP_PUSH_BORIS) return launch_track<TP_PUSH_BORIS, FK, PREC32, TRAJ>;
     return nullptr;
 }
 
-static track_fn pick_track(int fk, int push, int prec)
+template <int FK>
+static track_fn pick_track_fk(int push, int prec, bool traj)
+{
+    return traj ? pick_track_fkt<FK, true>(push, prec) : pick_track_fkt<FK, false>(push, prec);
+}
+
+static track_fn pick_track(int fk, int push, int prec, bool traj)
 {
     switch (fk) {
-    case FK_TEST: return pick_track_fk<FK_TEST>(push, prec);
-    case FK_ABC: return pick_track_fk<FK_ABC>(push, prec);
-    case FK_ABCV: return pick_track_fk<FK_ABCV>(push, prec);
-    case FK_GRID_REF: return pick_track_fk<FK_GRID_REF>(push, prec);
-    case FK_GRID_TB: return pick_track_fk<FK_GRID_TB>(push, prec);
-    case FK_MODES: return pick_track_fk<FK_MODES>(push, prec);
-    case FK_WLCS: return pick_track_fk<FK_WLCS>(push, prec);
-    case FK_GRID2_REF: return pick_track_fk<FK_GRID2_REF>(push, prec);
-    case FK_GRID2_TB: return pick_track_fk<FK_GRID2_TB>(push, prec);
+    case FK_TEST: return pick_track_fk<FK_TEST>(push, prec, traj);
+    case FK_ABC: return pick_track_fk<FK_ABC>(push, prec, traj);
+    case FK_ABCV: return pick_track_fk<FK_ABCV>(push, prec, traj);
+    case FK_GRID_REF: return pick_track_fk<FK_GRID_REF>(push, prec, traj);
+    case FK_GRID_TB: return pick_track_fk<FK_GRID_TB>(push, prec, traj);
+    case FK_MODES: return pick_track_fk<FK_MODES>(push, prec, traj);
+    case FK_WLCS: return pick_track_fk<FK_WLCS>(push, prec, traj);
+    case FK_GRID2_REF: return pick_track_fk<FK_GRID2_REF>(push, prec, traj);
+    case FK_GRID2_TB: return pick_track_fk<FK_GRID2_TB>(push, prec, traj);
     }
     return nullptr;
 }
@@ -840,7 +845,8 @@ int tp_gpu_run_fixed(tp_ctx *c, const tp_track_params *p)
     if (mlp && (p->pusher != TP_PUSH_BORIS || p->precision != TP_FP32_MIXED)) return TP_E_ARG;
     if (mlp && c->traj_nout > 0) return TP_E_ARG;
     /* surrogate field: the exact-field kernel runs the particles that fell back */
-    track_fn fn = pick_track(mlp ? FK_MODES : field_kind(c), p->pusher, p->precision);
+    track_fn fn = pick_track(mlp ? FK_MODES : field_kind(c), p->pusher, p->precision,
+                             c->traj_nout > 0);
     if (!fn) return TP_E_ARG;
     HIPCHK(hipSetDevice(c->device));
     const int n = c->s.n;
