@@ -76,9 +76,12 @@ def main():
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 form")
-    ap.add_argument("--forms", default="abc,bs,surrogate,grid2",
+    ap.add_argument("--forms", default="abc,bs,surrogate,grid2,poincare",
                     help="extra workloads reported under 'forms' (abc, bs, surrogate, "
-                         "grid2; '' for none)")
+                         "grid2, poincare; '' for none)")
+    ap.add_argument("--pmap-side", type=int, default=512,
+                    help="Poincare workload: start points per side of the plane")
+    ap.add_argument("--pmap-ncross", type=int, default=8, help="Poincare crossings per line")
     ap.add_argument("--sur-epochs", type=int, default=4, help="surrogate training epochs")
     ap.add_argument("--sur-sigma", type=float, default=4.0,
                     help="spread of the surrogate's Fourier frequencies (lattice units)")
@@ -188,6 +191,8 @@ def main():
         forms.update(forms_surrogate(args))
     if "grid2" in extra:
         forms.update(forms_grid2(args))
+    if "poincare" in extra:
+        forms.update(forms_poincare(args, not args.no_cpu))
 
     out = {
         "metric": METRIC, "value": chosen_value, "unit": "particle-steps/s", "n_gpus": 1,
@@ -465,6 +470,66 @@ def forms_grid2(args):
                          "kernel_ms_per_launch": kms / max(launches, 1)}
             log(f"[bench] {name}: {n * K / w:.4e} particle-steps/s, "
                 f"{out[name]['algorithmic_GBps']:.0f} GB/s algorithmic")
+    return out
+
+
+def forms_poincare(args, with_cpu=True):
+    """Poincare map of the PIC-grid workload's magnetic field (the
+    reference's run_type 2): a plane z = 128.3 of side x side start points,
+    RK4 on B/|B| with step 0.05 cells and the reference's float trilinear
+    gather, the reference's crossing test, `ncross` crossings per line,
+    length cap 2e4.  A step is one RK4 step of one field line (four field
+    evaluations); the value is total steps / wall time of the library call.
+    The CPU oracle runs the first lines of the same plane (OpenMP), and the
+    GPU's crossing points on those lines are compared bit for bit."""
+    from test_particle_amd import abi
+    from test_particle_amd.fields import pic_turbulence_grid
+    from test_particle_amd.gpu import Tracker, pmap_starts
+    _, b, lo, hi = pic_turbulence_grid(args.grid, seed=args.seed)
+    side, nc = args.pmap_side, args.pmap_ncross
+    st = pmap_starts(2, (0.0, 0.0, 128.3), min_h=lo[0] + 0.25, max_h=hi[0] - 0.25,
+                     min_v=lo[1] + 0.25, max_v=hi[1] - 0.25, nh=side, nv=side)
+    prm = abi.make_pmap_params(face=2, ncross=nc, step=0.05, length_max=2e4, lo=lo, hi=hi,
+                               mode=abi.TP_PMAP_REF_COMPAT)
+    out = {}
+    e0 = np.zeros_like(b)
+    with Tracker(0) as tr:
+        tr.set_field_grid(e0, b, lo, hi, abi.TP_GATHER_REF_COMPAT)
+        del e0
+        warm = abi.make_pmap_params(face=2, ncross=1, step=0.05, length_max=50.0, lo=lo, hi=hi)
+        tr.poincare_map(warm, st[:65536])
+        t0 = time.perf_counter()
+        r = tr.poincare_map(prm, st)
+        w = time.perf_counter() - t0
+        kms, launches = tr.last_kernel_time()
+        steps = int(r["nsteps"].sum())
+        name = f"poincare_{side}x{side}_lines_fp64"
+        out[name] = {"value": steps / w, "unit": "field-line RK4 steps/s", "dtype": "f64",
+                     "lines": int(st.shape[0]), "ncross": nc, "rk4_steps": steps,
+                     "kernel_ms": kms, "launches": launches,
+                     "kernel_rate": steps / (kms * 1e-3),
+                     "field_evals_per_s": 4.0 * steps / w,
+                     "lines_reaching_ncross": int((r["ncrossed"] == nc).sum())}
+        log(f"[bench] {name}: {steps / w:.4e} RK4 steps/s ({steps} steps, {w:.2f}s, "
+            f"{launches} launches)")
+        if with_cpu:
+            from oracle import tpref
+            cores = host_cores()
+            f = tpref.Field.grid(np.zeros_like(b), b, lo, hi, abi.TP_GATHER_REF_COMPAT)
+            pilot = tpref.poincare_map(f, prm, st[:4 * cores], nthreads=cores)
+            rate = pilot["nsteps"].sum() / max(pilot["seconds"], 1e-6)
+            per_line = pilot["nsteps"].mean()
+            m = int(np.clip(12.0 * rate / per_line, 4 * cores, 65536))
+            ref = tpref.poincare_map(f, prm, st[:m], nthreads=cores)
+            cpu_rate = ref["nsteps"].sum() / ref["seconds"]
+            same = all(np.array_equal(r[k][:m], ref[k]) for k in ("xpos", "ypos", "zpos"))
+            out[name]["cpu_baseline"] = {
+                "value": cpu_rate, "unit": "field-line RK4 steps/s", "cores": cores,
+                "kind": "port", "sample": f"first {m} lines of the same plane, FP64 oracle "
+                                          f"with OpenMP ({cpu_model()})",
+                "gpu_sample_bitwise_equal": bool(same)}
+            log(f"[bench] {name} CPU {cpu_rate:.3e} RK4 steps/s on {cores} threads "
+                f"({m} lines); GPU sample bitwise={same}")
     return out
 
 

@@ -260,6 +260,45 @@ int tp_gpu_tune_adaptive(tp_ctx *ctx, const tp_bs_params *p, const char *cache_p
 int tp_gpu_eval_field(tp_ctx *ctx, int n, const double *pts, double *out,
                       int precision);
 
+/* --- Poincare map of the magnetic field ------------------------------------
+ * run_type 2 of the reference (src/poincare_map.c:39-258): each field line is
+ * traced from its start point by classical RK4 on the unit vector B/|B| with
+ * the field at t = 0 (next_location, :268-320), every stage point folded once
+ * into the periodic box (field_boundary, :326-352), and the positions where it
+ * crosses the plane face = start[face] are recorded (:166-184).  The field
+ * is the context's field; the reference's systems 3 and 4 are the PIC grid
+ * kinds (tp_gpu_set_field_grid, _grid2 evaluated at t = 0); the analytic
+ * kinds have no counterpart there. */
+#define TP_PMAP_REF_COMPAT 0  /* the crossing test of src/poincare_map.c:168-173:
+                                 the face coordinate before a step is read before
+                                 that step's periodic fold, so a fold of the face
+                                 axis counts as a crossing, and the first step
+                                 from the start counts as one */
+#define TP_PMAP_PERIODIC   1  /* a crossing is a sign change within one step of
+                                 the distance to the plane reduced into
+                                 [-L/2, L/2] (L the box length along face) */
+
+typedef struct tp_pmap_params {
+    int face;               /* normal of the cross section: 0 x, 1 y, 2 z */
+    int ncross;             /* crossings recorded per field line, >= 0 */
+    int mode;               /* TP_PMAP_* */
+    int chunk;              /* RK4 steps per line per kernel launch (0: 16384) */
+    double step;            /* field-line step, > 0 */
+    double length_max;      /* traced length at which a line stops */
+    double lo[3], hi[3];    /* periodic box (simul_domain) */
+} tp_pmap_params;
+
+/* Traces npoints field lines from starts[3*i] (host arrays).  xpos, ypos,
+ * zpos have npoints*(ncross+1) entries each (the reference's datasets,
+ * src/poincare_map.c:196-240): slot i*(ncross+1) is start i, slot
+ * i*(ncross+1)+k its k-th crossing (the position after the step that
+ * crossed); slots not reached keep 2*hi (:55-80).  ncrossed[i] and
+ * nsteps[i] (either may be NULL) receive the crossings found and the RK4
+ * steps taken. */
+int tp_gpu_poincare_map(tp_ctx *ctx, const tp_pmap_params *p, int npoints,
+                        const double *starts, double *xpos, double *ypos, double *zpos,
+                        int *ncrossed, long long *nsteps);
+
 /* --- learned field surrogate ---------------------------------------------
  * No counterpart in the reference (its closest slot is the field dispatch
  * get_emf, src/emfields.c:32-60).  A coordinate network stands in for the

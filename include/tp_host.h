@@ -1,7 +1,8 @@
 /*
  * tp_host.h -- host-side helpers of libtpgpu.so for callers that replace the
- * reference's tracking loop: they turn the raw arrays returned by
- * tp_gpu_track_fixed into the reference's output files, byte for byte.
+ * reference's tracking loop or Poincare map: they turn the raw arrays
+ * returned by tp_gpu_track_fixed into the reference's output files, byte for
+ * byte, read its field files and lay out its Poincare start points.
  * Plain C, no GPU use; each returns 0 or TP_E_ARG (bad arguments or a file
  * that cannot be opened).
  */
@@ -57,6 +58,16 @@ int tp_host_write_diff_coeffs(const char *fname, int nd, const double *dcoeff,
 int tp_host_read_pic_gda(const char *path, int ct, int nt, int two_slices, int nx, int ny,
                          int nz, double e0, double b0, float *e_next, float *b_next,
                          float *e_prev, float *b_prev);
+
+/* Start points of a Poincare map (src/poincare_map.c:83-158).  single = 1:
+ * the one point p0.  single = 0: nh*nv points on the plane through p0
+ * normal to axis face; point ipoint = iv*nh + ih sits at dh*ih + min_h on
+ * the first axis other than face and at dv*iv + min_v on the second, with
+ * dh = (max_h - min_h)/(nh - 1), or max_h - min_h when nh = 1 (dv alike).
+ * starts receives 3 doubles per point.  Returns the number of points, or
+ * TP_E_ARG. */
+int tp_host_pmap_starts(int face, int single, const double p0[3], double min_h, double max_h,
+                        double min_v, double max_v, int nh, int nv, double *starts);
 
 #ifdef __cplusplus
 }

@@ -32,12 +32,22 @@
  *   diff    SPEC PTL0 PTL DT OUT  calc_diff_coeff (src/diagnostics.c:659)
  *   ellint  SPEC KS OUT      ellint (src/wlcs.c:460); KS: n f64 moduli k,
  *                            OUT: n*(E, K)
+ *   pmap    SPEC CFG OUT     Poincare map: the "Poincare map configuration"
+ *                            block of CFG read by config_poincare_map
+ *                            (src/poincare_map.c:357-514), lines traced with
+ *                            next_location (:268-320); OUT: xpos, ypos, zpos,
+ *                            each npoints*(ncross+1) f64
+ *   pmaph   SPEC CFG OUT     poincare_map (src/poincare_map.c:39) as main
+ *                            calls it (src/main.c:44-46), in the build with
+ *                            the GPU drop-in; reads back
+ *                            data/poincare_map.h5, OUT as for pmap
  */
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 #include <math.h>
 #include <mpi.h>
+#include <hdf5.h>
 
 /* Layouts of the reference's structs (include/particle_info.h:6-12,
  * include/domain.h:5-24, include/emfields.h:7-10, include/wlcs.h:24-37,
@@ -98,6 +108,18 @@ void particle_tracking_hybrid(int mpi_rank, int nptl, double dt, int nbins,
         int *ntraj_accum, int tracking_method, int traj_diagnose,
         particles *ptl_init, int *nsteps_ptl_tracking, particles *ptl,
         particles *ptl_time);
+
+/* Poincare map (src/poincare_map.c:18-31): the configuration globals that
+ * config_poincare_map sets, and the RK4 step */
+int config_poincare_map(int mpi_rank, char *config_file_name);
+void next_location(double *fx, double *fy, double *fz, double dstep);
+extern int map_for_single_point, face_cross_section, nhorizontal, nvertical, ncross;
+extern double fx0, fy0, fz0, min_horizontal, max_horizontal, min_vertical, max_vertical;
+extern double step_fieldline, fieldline_length_max;
+/* defined only where the GPU drop-in is linked: the reference's own
+ * poincare_map needs parallel HDF5 (H5Pset_fapl_mpio) for its output */
+int poincare_map(int mpi_rank, int mpi_size, char *config_file_name, domain *sdomain)
+    __attribute__((weak));
 
 /* reference globals this driver sets directly */
 extern param_ff param;                          /* src/force_free.c:33 */
@@ -178,6 +200,8 @@ static void read_spec(const char *path, spec_t *s)
 
 /* Mirrors the setup src/domain.c:314-409 performs per system type, with
  * the field arrays filled from files instead of HDF5/binary readers. */
+static domain g_dom;
+
 static void setup(const spec_t *s)
 {
     domain dom;
@@ -190,6 +214,7 @@ static void setup(const spec_t *s)
     dom.ymin_ptl = dom.ymin; dom.ymax_ptl = dom.ymax;
     dom.zmin_ptl = dom.zmin; dom.zmax_ptl = dom.zmax;
     dom.tmin = 0.0; dom.tmax = s->tmax;
+    g_dom = dom;
     if (s->nx > 1) {
         g.nx = s->nx; g.ny = s->ny; g.nz = s->nz; g.nt = s->grid_nt > 0 ? s->grid_nt : 1;
         g.dx = (dom.xmax - dom.xmin) / (g.nx - 1.0);   /* src/domain.c:548 */
@@ -447,6 +472,83 @@ int main(int argc, char **argv)
         calc_energy_spectrum(0, np, p, s.nbins, s.nt_out, s.pmass, s.bc);
         particle_tracking_hybrid(0, np, s.dt, s.nbins, s.nt_out, s.bc,
             s.nsteps_output, s.pmass, NULL, 0, 0, p0, nsteps, p, NULL);
+    } else if (!strcmp(mode, "pmap")) {
+        /* the tracing loop of src/poincare_map.c:83-185 around the
+         * reference's next_location, one MPI rank */
+        config_poincare_map(0, argv[3]);
+        const int np = map_for_single_point ? 1 : nhorizontal * nvertical;
+        const size_t ns = (size_t)ncross + 1;
+        double *xp = malloc(sizeof(double) * ns * np), *yp = malloc(sizeof(double) * ns * np),
+               *zp = malloc(sizeof(double) * ns * np);
+        for (size_t i = 0; i < ns * np; i++) {
+            xp[i] = g_dom.xmax * 2;
+            yp[i] = g_dom.ymax * 2;
+            zp[i] = g_dom.zmax * 2;
+        }
+        const double dh = nhorizontal > 1 ? (max_horizontal - min_horizontal) / (nhorizontal - 1)
+                                          : max_horizontal - min_horizontal;
+        const double dv = nvertical > 1 ? (max_vertical - min_vertical) / (nvertical - 1)
+                                        : max_vertical - min_vertical;
+        for (int ip = 0; ip < np; ip++) {
+            double pos[3] = {fx0, fy0, fz0};
+            if (!map_for_single_point) {
+                int axis = 0;
+                for (int a = 0; a < 3; a++) {
+                    if (a == face_cross_section) continue;
+                    pos[a] = axis == 0 ? dh * (ip % nhorizontal) + min_horizontal
+                                       : dv * (ip / nhorizontal) + min_vertical;
+                    axis++;
+                }
+            }
+            const double c0 = pos[face_cross_section];
+            double *xs = xp + ns * ip, *ys = yp + ns * ip, *zs = zp + ns * ip;
+            xs[0] = pos[0]; ys[0] = pos[1]; zs[0] = pos[2];
+            double length = 0.0;
+            for (int k = 1; k <= ncross; k++) {
+                double cross = 1.0;
+                while (cross > 0 && length < fieldline_length_max) {
+                    double before = pos[face_cross_section];
+                    next_location(&pos[0], &pos[1], &pos[2], step_fieldline);
+                    cross = (before - c0) * (pos[face_cross_section] - c0);
+                    length += step_fieldline;
+                }
+                if (!(length < fieldline_length_max)) break;
+                xs[k] = pos[0]; ys[k] = pos[1]; zs[k] = pos[2];
+            }
+        }
+        FILE *fo = fopen(argv[4], "wb");
+        if (!fo) die("pmap out");
+        fwrite(xp, sizeof(double), ns * np, fo);
+        fwrite(yp, sizeof(double), ns * np, fo);
+        fwrite(zp, sizeof(double), ns * np, fo);
+        fclose(fo);
+    } else if (!strcmp(mode, "pmaph")) {
+        if (!poincare_map) die("pmaph needs the build with the GPU drop-in");
+        poincare_map(0, 1, argv[3], &g_dom);
+        hid_t fh = H5Fopen("data/poincare_map.h5", H5F_ACC_RDONLY, H5P_DEFAULT);
+        if (fh < 0) die("data/poincare_map.h5");
+        FILE *fo = fopen(argv[4], "wb");
+        if (!fo) die("pmaph out");
+        const char *names[3] = {"/poincare_map/xpos", "/poincare_map/ypos",
+                                "/poincare_map/zpos"};
+        for (int k = 0; k < 3; k++) {
+            hid_t ds = H5Dopen(fh, names[k], H5P_DEFAULT);
+            if (ds < 0) die(names[k]);
+            hid_t sp = H5Dget_space(ds);
+            hsize_t dims[2] = {0, 0};
+            if (H5Sget_simple_extent_ndims(sp) != 2) die("pmaph rank");
+            H5Sget_simple_extent_dims(sp, dims, NULL);
+            size_t cnt = (size_t)dims[0] * dims[1];
+            double *buf = malloc(sizeof(double) * cnt + 1);
+            if (H5Dread(ds, H5T_NATIVE_DOUBLE, H5S_ALL, H5S_ALL, H5P_DEFAULT, buf) < 0)
+                die("pmaph read");
+            fwrite(buf, sizeof(double), cnt, fo);
+            free(buf);
+            H5Sclose(sp);
+            H5Dclose(ds);
+        }
+        fclose(fo);
+        H5Fclose(fh);
     } else {
         die("unknown mode");
     }

@@ -17,6 +17,9 @@
  *   loop       src/tracking.c:90-190, :297-462
  *   diagnostics ptl_energy_fixed src/diagnostics.c:187-205,
  *              calc_diff_coeff :659-705
+ *   poincare   the Poincare map (run_type 2): tracing loop
+ *              src/poincare_map.c:83-185, next_location :268-320,
+ *              field_boundary :326-352
  * plus, for what the reference does not have, a relativistic Boris push
  * and a multi-mode turbulence field.
  *
@@ -27,7 +30,8 @@
  *
  * Pins (tests/): golden vectors written by the reference binary through
  * oracle/ref_harness.c (fields of systems 0/2/4, Wirz trajectories,
- * calc_diff_coeff, spectrum/D sums of the full loop); closed forms (Wirz is
+ * calc_diff_coeff, spectrum/D sums of the full loop, Poincare crossing
+ * points); closed forms (Wirz is
  * exact in uniform B; Boris rotation angle 2 atan(Omega dt / 2 gamma));
  * scipy ellipk/ellipe for the elliptic integrals; brute-force mode sums.
  */
@@ -1121,5 +1125,117 @@ int tpref_bs_track(const tpref_field *f, const tp_bs_params *prm, int n, tp_part
     if (totals) {
         totals[0] = acc; totals[1] = rej; totals[2] = rhs; totals[3] = failed;
     }
+    return TP_OK;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Poincare map (run_type 2, src/poincare_map.c:39-258)                    */
+/* ---------------------------------------------------------------------- */
+/* field_boundary, src/poincare_map.c:326-352: each axis folded once */
+static void pmap_fold(const tp_pmap_params *q, double *v)
+{
+    for (int d = 0; d < 3; d++) {
+        const double len = q->hi[d] - q->lo[d];
+        if (v[d] > q->hi[d]) v[d] -= len;
+    }
+    for (int d = 0; d < 3; d++) {
+        const double len = q->hi[d] - q->lo[d];
+        if (v[d] < q->lo[d]) v[d] += len;
+    }
+}
+
+/* B / |B| at t = 0 (get_magnetic_field, src/emfields.c:76-96, and the
+ * normalisation of src/poincare_map.c:282-285) */
+static void pmap_unit(const tpref_field *f, const double *v, double *u)
+{
+    emf_t e;
+    get_emf(f, v[0], v[1], v[2], 0.0, &e);
+    double ib = 1.0 / sqrt(e.Bx * e.Bx + e.By * e.By + e.Bz * e.Bz);
+    u[0] = e.Bx * ib;
+    u[1] = e.By * ib;
+    u[2] = e.Bz * ib;
+}
+
+/* next_location, src/poincare_map.c:268-320; *folded gets the face
+ * coordinate after the initial fold */
+static void pmap_rk4(const tpref_field *f, const tp_pmap_params *q, double *p, double *folded)
+{
+    double k1[3], k2[3], k3[3], k4[3], t[3];
+    const double ds = q->step;
+    pmap_fold(q, p);
+    *folded = p[q->face];
+    pmap_unit(f, p, k1);
+    for (int d = 0; d < 3; d++) t[d] = p[d] + 0.5 * ds * k1[d];
+    pmap_fold(q, t);
+    pmap_unit(f, t, k2);
+    for (int d = 0; d < 3; d++) t[d] = p[d] + 0.5 * ds * k2[d];
+    pmap_fold(q, t);
+    pmap_unit(f, t, k3);
+    for (int d = 0; d < 3; d++) t[d] = p[d] + ds * k3[d];
+    pmap_fold(q, t);
+    pmap_unit(f, t, k4);
+    for (int d = 0; d < 3; d++) p[d] += (ds / 6) * (k1[d] + 2 * k2[d] + 2 * k3[d] + k4[d]);
+}
+
+/* Whether the step from `before` (face coordinate before the step's fold) /
+ * `folded` (after it) to `after` crossed the plane c0. */
+static int pmap_crossed(const tp_pmap_params *q, double c0, double before, double folded,
+                        double after)
+{
+    if (q->mode == TP_PMAP_REF_COMPAT)
+        return !((before - c0) * (after - c0) > 0);     /* src/poincare_map.c:168-171 */
+    const double L = q->hi[q->face] - q->lo[q->face];
+    double d0 = folded - c0, d1 = after - c0;
+    d0 = d0 - L * rint(d0 / L);
+    d1 = d1 - L * rint(d1 / L);
+    return (d0 * d1 < 0 && fabs(d0 - d1) < 0.5 * L) || (d1 == 0 && d0 != 0);
+}
+
+int tpref_poincare_map(const tpref_field *f, const tp_pmap_params *q, int n,
+                       const double *starts, double *xpos, double *ypos, double *zpos,
+                       int *ncrossed, long long *nsteps, int nthreads, double *seconds)
+{
+    if (n < 0 || q->face < 0 || q->face > 2 || q->ncross < 0 || !(q->step > 0.0) ||
+        !isfinite(q->length_max) || !(q->length_max / q->step < 0x1p50))
+        return TP_E_ARG;
+    if (q->mode != TP_PMAP_REF_COMPAT && q->mode != TP_PMAP_PERIODIC) return TP_E_ARG;
+    const int ns = q->ncross + 1;
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    double t1 = omp_get_wtime();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+    for (int i = 0; i < n; i++) {
+        double *xs = xpos + (size_t)i * ns, *ys = ypos + (size_t)i * ns,
+               *zs = zpos + (size_t)i * ns;
+        /* slots a line does not reach keep 2 * hi (src/poincare_map.c:75-80) */
+        for (int s = 0; s < ns; s++) {
+            xs[s] = q->hi[0] * 2;
+            ys[s] = q->hi[1] * 2;
+            zs[s] = q->hi[2] * 2;
+        }
+        double p[3] = {starts[3 * i], starts[3 * i + 1], starts[3 * i + 2]};
+        xs[0] = p[0]; ys[0] = p[1]; zs[0] = p[2];
+        const double c0 = p[q->face];
+        double len = 0.0;
+        long long st = 0;
+        int k = 1;
+        /* src/poincare_map.c:166-184: trace until the next crossing, record
+         * it, stop at ncross crossings or when the length reaches the cap */
+        while (k <= q->ncross && len < q->length_max) {
+            int hit;
+            do {
+                double before = p[q->face], folded;
+                pmap_rk4(f, q, p, &folded);
+                len += q->step;
+                st++;
+                hit = pmap_crossed(q, c0, before, folded, p[q->face]);
+            } while (!hit && len < q->length_max);
+            if (!(len < q->length_max)) break;
+            xs[k] = p[0]; ys[k] = p[1]; zs[k] = p[2];
+            k++;
+        }
+        if (ncrossed) ncrossed[i] = k - 1;
+        if (nsteps) nsteps[i] = st;
+    }
+    if (seconds) *seconds = omp_get_wtime() - t1;
     return TP_OK;
 }

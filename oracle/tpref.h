@@ -82,6 +82,11 @@ int  tpref_track_fixed(const tpref_field *f, const tp_track_params *prm,
 int  tpref_bs_track(const tpref_field *f, const tp_bs_params *prm, int n,
                     tp_particle *ptl, int *nsteps, long long *totals,
                     int nthreads, double *seconds);
+/* Poincare map of the field at t = 0 (src/poincare_map.c:39-258); arguments
+ * as for tp_gpu_poincare_map.  Returns 0 or TP_E_ARG. */
+int  tpref_poincare_map(const tpref_field *f, const tp_pmap_params *prm, int n,
+                        const double *starts, double *xpos, double *ypos, double *zpos,
+                        int *ncrossed, long long *nsteps, int nthreads, double *seconds);
 /* exported for the tests of the shared routines */
 void tpref_sincos(double x, double *s, double *c);
 void tpref_ellint(double k, double *elle, double *ellk);

@@ -75,6 +75,9 @@ def lib():
         L.tpref_bs_track.argtypes = [P(TprefField), P(abi.TpBsParams), i, P(d), P(i),
                                      P(ctypes.c_longlong), i, P(d)]
         L.tpref_bs_track.restype = i
+        L.tpref_poincare_map.argtypes = [P(TprefField), P(abi.TpPmapParams), i, P(d), P(d),
+                                         P(d), P(d), P(i), P(ctypes.c_longlong), i, P(d)]
+        L.tpref_poincare_map.restype = i
         L.tpref_sincos.argtypes = [d, P(d), P(d)]
         L.tpref_ellint.argtypes = [d, P(d), P(d)]
         L.tpref_ellint_ref.argtypes = [d, P(d), P(d)]
@@ -261,6 +264,29 @@ def track_fixed_traj(field, params, ptl, nsteps_output, ntraj_accum, ptl_time,
     r["ptl_time"] = ptl_time
     r["points"] = points
     return r
+
+
+def poincare_map(field, params, starts, nthreads=1):
+    """tpref_poincare_map: dict(xpos, ypos, zpos, ncrossed, nsteps, seconds)."""
+    st = np.ascontiguousarray(starts, dtype=np.float64).reshape(-1, 3)
+    n = st.shape[0]
+    ns = params.ncross + 1
+    xp, yp, zp = (np.zeros((n, ns)) for _ in range(3))
+    nc = np.zeros(n, np.int32)
+    nst = np.zeros(n, np.int64)
+    sec = ctypes.c_double()
+    P = ctypes.POINTER
+    rc = lib().tpref_poincare_map(ctypes.byref(field.s), ctypes.byref(params), n,
+                                  st.ctypes.data_as(P(ctypes.c_double)),
+                                  xp.ctypes.data_as(P(ctypes.c_double)),
+                                  yp.ctypes.data_as(P(ctypes.c_double)),
+                                  zp.ctypes.data_as(P(ctypes.c_double)),
+                                  nc.ctypes.data_as(P(ctypes.c_int)),
+                                  nst.ctypes.data_as(P(ctypes.c_longlong)), int(nthreads),
+                                  ctypes.byref(sec))
+    if rc != 0:
+        raise ValueError(f"tpref_poincare_map: {rc}")
+    return dict(xpos=xp, ypos=yp, zpos=zp, ncrossed=nc, nsteps=nst, seconds=sec.value)
 
 
 def bs_track(field, params, ptl, nthreads=1):

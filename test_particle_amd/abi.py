@@ -37,6 +37,8 @@ TP_GATHER_REF_COMPAT = 0
 TP_GATHER_TEXTBOOK = 1
 TP_FLAG_FIX_STALE_DCOEFF = 1
 TP_FLAG_CHECK_FINITE = 2
+TP_PMAP_REF_COMPAT = 0
+TP_PMAP_PERIODIC = 1
 
 
 class TpTrackParams(ctypes.Structure):
@@ -119,6 +121,33 @@ class TpMlpTrainStats(ctypes.Structure):
         ("sample_ms", ctypes.c_double),
         ("train_ms", ctypes.c_double),
     ]
+
+
+class TpPmapParams(ctypes.Structure):
+    _fields_ = [
+        ("face", ctypes.c_int),
+        ("ncross", ctypes.c_int),
+        ("mode", ctypes.c_int),
+        ("chunk", ctypes.c_int),
+        ("step", ctypes.c_double),
+        ("length_max", ctypes.c_double),
+        ("lo", ctypes.c_double * 3),
+        ("hi", ctypes.c_double * 3),
+    ]
+
+
+def make_pmap_params(*, face, ncross, step, length_max, lo, hi, mode=TP_PMAP_REF_COMPAT,
+                     chunk=0):
+    """tp_pmap_params of a Poincare map (the reference's "Poincare map
+    configuration" block: face, step, maximum length, crossing count; lo/hi
+    the simulation domain)."""
+    p = TpPmapParams()
+    p.face, p.ncross, p.mode, p.chunk = int(face), int(ncross), int(mode), int(chunk)
+    p.step, p.length_max = float(step), float(length_max)
+    for d in range(3):
+        p.lo[d] = float(lo[d])
+        p.hi[d] = float(hi[d])
+    return p
 
 
 def make_mlp_train_params(*, nsamples, period, batch=16384, epochs=4, lr=2e-3, lr_min=2e-5,

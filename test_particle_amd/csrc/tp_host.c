@@ -180,3 +180,27 @@ int tp_host_read_pic_gda(const char *path, int ct, int nt, int two_slices, int n
     free(buf);
     return rc;
 }
+
+int tp_host_pmap_starts(int face, int single, const double p0[3], double min_h, double max_h,
+                        double min_v, double max_v, int nh, int nv, double *starts)
+{
+    if (face < 0 || face > 2 || !p0 || !starts) return TP_E_ARG;
+    if (single) {
+        starts[0] = p0[0]; starts[1] = p0[1]; starts[2] = p0[2];
+        return 1;
+    }
+    if (nh < 1 || nv < 1 || (long long)nh * nv > 0x7fffffffLL) return TP_E_ARG;
+    const double dh = nh > 1 ? (max_h - min_h) / (nh - 1) : max_h - min_h;
+    const double dv = nv > 1 ? (max_v - min_v) / (nv - 1) : max_v - min_v;
+    /* the two in-plane axes in increasing order */
+    const int ah = face == 0 ? 1 : 0;
+    const int av = face == 2 ? 1 : 2;
+    const int np = nh * nv;
+    for (int ip = 0; ip < np; ip++) {
+        double *s = starts + 3 * (size_t)ip;
+        s[0] = p0[0]; s[1] = p0[1]; s[2] = p0[2];
+        s[ah] = dh * (ip % nh) + min_h;
+        s[av] = dv * (ip / nh) + min_v;
+    }
+    return np;
+}

@@ -62,12 +62,15 @@ def lib():
                                              P(abi.TpMlpTrainStats)]
         L.tp_gpu_set_field_mlp.argtypes = [vp, d, i]
         L.tp_gpu_mlp_fallback_count.argtypes = [vp, P(ctypes.c_longlong)]
+        L.tp_gpu_poincare_map.argtypes = [vp, P(abi.TpPmapParams), i, P(d), P(d), P(d), P(d),
+                                          P(i), P(ctypes.c_longlong)]
         cs = ctypes.c_char_p
         L.tp_host_spectrum_bins.argtypes = [i, d, d, P(d), P(d)]
         L.tp_host_spectrum_t0.argtypes = [i, P(d), i, d, d, P(d)]
         L.tp_host_write_espectrum_t0.argtypes = [cs, cs, i, d, d, P(d), i]
         L.tp_host_append_espectrum.argtypes = [cs, i, i, d, d, P(d)]
         L.tp_host_write_diff_coeffs.argtypes = [cs, i, P(d), P(i)]
+        L.tp_host_pmap_starts.argtypes = [i, i, P(d), d, d, d, d, i, i, P(d)]
         L.tp_host_read_pic_gda.argtypes = [cs, i, i, i, i, i, i, d, d, P(f), P(f), P(f), P(f)]
         _lib = L
     return _lib
@@ -318,6 +321,20 @@ class Tracker:
                "mlp_fallback_count")
         return v.value
 
+    def poincare_map(self, params, starts):
+        """tp_gpu_poincare_map: returns dict(xpos, ypos, zpos) of shape
+        (npoints, ncross + 1), ncrossed (npoints,) and nsteps (npoints,)."""
+        st = np.ascontiguousarray(starts, dtype=np.float64).reshape(-1, 3)
+        n = st.shape[0]
+        ns = params.ncross + 1
+        xp, yp, zp = (np.zeros((n, ns)) for _ in range(3))
+        nc = np.zeros(n, np.int32)
+        nst = np.zeros(n, np.int64)
+        _check(lib().tp_gpu_poincare_map(
+            self._ctx, ctypes.byref(params), n, dptr(st), dptr(xp), dptr(yp), dptr(zp),
+            iptr(nc), nst.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong))), "poincare_map")
+        return dict(xpos=xp, ypos=yp, zpos=zp, ncrossed=nc, nsteps=nst)
+
     def last_kernel_time(self):
         ms = ctypes.c_double()
         k = ctypes.c_int()
@@ -380,3 +397,19 @@ def read_pic_gda(path, ct, nx, ny, nz, e0, b0, nt=None, two_slices=False):
                                       nx, ny, nz, float(e0), float(b0), fptr(en), fptr(bn),
                                       fptr(ep), fptr(bp)), "read_pic_gda")
     return (en, bn, ep, bp) if two_slices else (en, bn)
+
+
+def pmap_starts(face, p0, single=False, min_h=0.0, max_h=0.0, min_v=0.0, max_v=0.0, nh=1,
+                nv=1):
+    """Start points of a Poincare map (include/tp_host.h, tp_host_pmap_starts):
+    (npoints, 3) float64, point iv*nh + ih at (min_h + ih dh, min_v + iv dv)
+    on the two axes other than face."""
+    n = 1 if single else int(nh) * int(nv)
+    out = np.zeros((n, 3))
+    p = np.ascontiguousarray(p0, dtype=np.float64)
+    rc = lib().tp_host_pmap_starts(int(face), 1 if single else 0, dptr(p), float(min_h),
+                                   float(max_h), float(min_v), float(max_v), int(nh), int(nv),
+                                   dptr(out))
+    if rc < 0:
+        _check(rc, "pmap_starts")
+    return out

@@ -3,6 +3,7 @@
     python -m test_particle_amd.profile_run grid  [--steps 1000]   # PIC-grid tracking
     python -m test_particle_amd.profile_run abc   [--steps 1000]   # ABC field tracking
     python -m test_particle_amd.profile_run copy  [--mib 1024]     # plain copy kernel
+    python -m test_particle_amd.profile_run pmap  [--side 512]     # Poincare map, PIC grid
 
 ``copy`` launches one device-to-device copy kernel (torch's) of a known size;
 its FETCH_SIZE / WRITE_SIZE readings calibrate those counters against bytes.
@@ -16,7 +17,7 @@ import numpy as np
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["grid", "abc", "copy", "modes", "surrogate"])
+    ap.add_argument("what", choices=["grid", "abc", "copy", "modes", "surrogate", "pmap"])
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--particles", type=int, default=1 << 20)
     ap.add_argument("--pusher", default="boris", choices=["boris", "wirz"])
@@ -30,6 +31,9 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-3, help="surrogate residual tolerance")
     ap.add_argument("--epochs", type=int, default=0, help="surrogate training epochs")
     ap.add_argument("--exact", action="store_true", help="surrogate workload, exact field")
+    ap.add_argument("--side", type=int, default=512, help="Poincare start points per side")
+    ap.add_argument("--ncross", type=int, default=2, help="Poincare crossings per line")
+    ap.add_argument("--chunk", type=int, default=0, help="Poincare RK4 steps per launch")
     a = ap.parse_args()
     if a.what == "copy":
         import torch
@@ -95,6 +99,24 @@ def main():
         print(f"surrogate({'exact' if a.exact else 'net'}, tol {a.tol}): {a.particles} x "
               f"{a.steps} steps, {a.particles * a.steps / wall:.4e} particle-steps/s, "
               f"kernels {ms:.2f} ms / {k} launches, fallbacks {nfb}", flush=True)
+        tr.close()
+        return
+    if a.what == "pmap":
+        # the Poincare workload of bench.py: 256^3 turbulent grid, plane z = 128.3
+        from .gpu import pmap_starts
+        _, b, lo, hi = pic_turbulence_grid(256)
+        tr.set_field_grid(np.zeros_like(b), b, lo, hi, abi.TP_GATHER_REF_COMPAT)
+        st = pmap_starts(2, (0.0, 0.0, 128.3), min_h=lo[0] + 0.25, max_h=hi[0] - 0.25,
+                         min_v=lo[1] + 0.25, max_v=hi[1] - 0.25, nh=a.side, nv=a.side)
+        prm = abi.make_pmap_params(face=2, ncross=a.ncross, step=0.05, length_max=2e4, lo=lo,
+                                   hi=hi, chunk=a.chunk)
+        t0 = time.perf_counter()
+        r = tr.poincare_map(prm, st)
+        wall = time.perf_counter() - t0
+        ms, k = tr.last_kernel_time()
+        steps = int(r["nsteps"].sum())
+        print(f"pmap: {st.shape[0]} lines x {a.ncross} crossings, {steps} RK4 steps, "
+              f"{steps / wall:.4e} steps/s, kernels {ms:.2f} ms / {k} launches", flush=True)
         tr.close()
         return
     pusher = abi.TP_PUSH_BORIS if a.pusher == "boris" else abi.TP_PUSH_WIRZ

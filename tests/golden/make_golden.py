@@ -20,6 +20,7 @@ ROOT = os.path.abspath(os.path.join(HERE, "..", ".."))
 sys.path.insert(0, ROOT)
 
 from test_particle_amd.particles import init_particles  # noqa: E402
+from test_particle_amd.poincare import write_config  # noqa: E402
 
 REF = os.path.join(ROOT, "oracle", "_ref")
 TWO_PI = 2.0 * np.pi
@@ -111,6 +112,25 @@ def traj_case(name, binary, spec, ptl, extra=None, tmp=None):
              spec=np.array(json.dumps({k: float(v) if isinstance(v, np.floating) else v
                                        for k, v in spec.items()})),
              **(extra or {}))
+
+
+PMAP_KEYS = ("face", "step", "length_max", "ncross", "single", "x0", "y0", "z0", "min_h",
+             "max_h", "min_v", "max_v", "nh", "nv")
+
+
+def pmap_case(name, spec, cfg, extra, tmp):
+    """Poincare map of the reference's next_location and config parser
+    (ref_harness mode pmap)."""
+    sp = os.path.join(tmp, "spec.txt")
+    write_spec(sp, **spec)
+    cp = os.path.join(tmp, "pmap.cfg")
+    write_config(cp, cfg)
+    run("ref_harness", "pmap", sp, cp, "pm.bin", cwd=tmp)
+    np_ = 1 if cfg["single"] else cfg["nh"] * cfg["nv"]
+    out = np.fromfile(os.path.join(tmp, "pm.bin")).reshape(3, np_, cfg["ncross"] + 1)
+    np.savez(os.path.join(HERE, name + ".npz"), xpos=out[0], ypos=out[1], zpos=out[2],
+             cfg=np.array(json.dumps({k: cfg[k] for k in PMAP_KEYS})),
+             **extra)
 
 
 def main():
@@ -262,6 +282,40 @@ def main():
                                                     dtype=np.float32)
         np.savez(os.path.join(HERE, "gda_slices.npz"), nx=nx, ny=ny, nz=nz, nt=4, b0=2.5,
                  e0=0.3, **files, **out)
+        # --- Poincare map (run_type 2) --------------------------------------
+        rng4 = np.random.default_rng(31)
+        plo, phi = (0.0, -1.0, 2.0), (6.0, 4.0, 5.0)
+        pnz, pny, pnx = 8, 10, 12
+        # a guide field along y with perturbations: lines cross y planes
+        pb = (0.35 * rng4.normal(0, 1, (pnz, pny, pnx, 3))).astype(np.float32)
+        pb[..., 1] += 1.0
+        pe = np.zeros_like(pb)
+        np.concatenate([pe.ravel(), pb.ravel()]).tofile(os.path.join(tmp, "pgrid.bin"))
+        pspec = dict(system=4, nx=pnx, ny=pny, nz=pnz, tmax=1.0, dt=0.01,
+                     grid=os.path.join(tmp, "pgrid.bin"), **box(plo, phi))
+        pext = dict(bgrid=pb, lo=plo, hi=phi)
+        pmap_case("pmap_grid", pspec,
+                  dict(face=1, step=0.05, length_max=27.0, ncross=12, single=0, x0=0.0,
+                       y0=1.3, z0=0.0, min_h=0.2, max_h=5.7, min_v=2.1, max_v=4.9, nh=5, nv=4),
+                  pext, tmp)
+        # a single point on the box face z = zmin, crossing z planes
+        pmap_case("pmap_single", pspec,
+                  dict(face=2, step=0.04, length_max=80.0, ncross=25, single=1, x0=3.1,
+                       y0=0.4, z0=2.0, min_h=0.0, max_h=0.0, min_v=0.0, max_v=0.0, nh=1,
+                       nv=1), pext, tmp)
+        # one row of x planes (nv = 1), field of the two-slice setup at t = 0
+        pb2 = [(0.3 * rng4.normal(0, 1, (pnz, pny, pnx, 3))).astype(np.float32)
+               for _ in range(2)]
+        pb2[0][..., 2] += 1.0                    # slice a (next): guide along z
+        pb2[1][..., 0] += 1.0                    # slice b (previous): guide along x
+        np.concatenate([pe.ravel(), pb2[0].ravel(), pe.ravel(),
+                        pb2[1].ravel()]).tofile(os.path.join(tmp, "pgrid2.bin"))
+        pmap_case("pmap_grid2", dict(pspec, mt=1, grid_dt=0.5,
+                                     grid=os.path.join(tmp, "pgrid2.bin")),
+                  dict(face=0, step=0.05, length_max=24.0, ncross=10, single=0, x0=2.2,
+                       y0=0.0, z0=0.0, min_h=-0.8, max_h=3.6, min_v=3.3, max_v=3.3, nh=7,
+                       nv=1),
+                  dict(b_next=pb2[0], b_prev=pb2[1], lo=plo, hi=phi, grid_dt=0.5), tmp)
         # --- the reference's own output files (format) ---------------------
         sp = os.path.join(tmp, "spec.txt")
         write_spec(sp, system=2, nx=5, ny=5, nz=5, tmax=300 * dt_c1 + 1e-9,

@@ -167,3 +167,69 @@ def test_dropin_pic_grid_two_time_slices(tmp_path):
     cpu, gpu = _run_both(str(tmp_path), "ref_harness", "ref_harness_gpu", spec, ptl)
     assert set(cpu) == {"diff_coeffs.dat", "espectrum.dat"}
     assert cpu == gpu
+
+
+def _pmap_run(tmp, binary, mode, spec, cfg):
+    from test_particle_amd.poincare import write_config
+    d = os.path.join(tmp, f"{binary}_{mode}")
+    os.makedirs(os.path.join(d, "data"), exist_ok=True)
+    _spec(os.path.join(d, "spec.txt"), **spec)
+    write_config(os.path.join(d, "init.dat"), cfg)
+    print(f"[dropin] {binary} {mode}", flush=True)
+    subprocess.run([os.path.join(REF, binary), mode, "spec.txt", "init.dat", "out.bin"], cwd=d,
+                   check=True, timeout=120, stdout=subprocess.DEVNULL)
+    np_ = 1 if cfg["single"] else cfg["nh"] * cfg["nv"]
+    return np.fromfile(os.path.join(d, "out.bin")).reshape(3, np_, cfg["ncross"] + 1)
+
+
+def _pmap_spec(tmp, name, e, b, lo, hi, **kw):
+    path = os.path.join(tmp, f"{name}.bin")
+    np.concatenate([a.ravel() for a in (*e, *b)]).astype(np.float32).tofile(path)
+    nz, ny, nx = b[0].shape[:3]
+    return dict(system=4, nx=nx, ny=ny, nz=nz, tmax=1.0, dt=0.01, grid=path,
+                xmin=float(lo[0]), ymin=float(lo[1]), zmin=float(lo[2]), xmax=float(hi[0]),
+                ymax=float(hi[1]), zmax=float(hi[2]), **kw)
+
+
+@pytest.mark.parametrize("name", ["pmap_grid", "pmap_single", "pmap_grid2"])
+def test_dropin_poincare_map_golden(tmp_path, name):
+    """run_type 2 through the reference's poincare_map entry replaced by
+    test_particle_amd/host/tp_pmap_shim.c: the reference's config parser
+    reads the block, the lines are traced on the GPU and
+    data/poincare_map.h5 holds the reference's crossing points bit for bit
+    (read back from the file)."""
+    import json
+    from conftest import golden
+    _need("ref_harness_gpu")
+    g = golden(name)
+    cfg = json.loads(str(g["cfg"]))
+    if name == "pmap_grid2":
+        z = np.zeros_like(g["b_next"])
+        spec = _pmap_spec(str(tmp_path), name, (z, z), (g["b_next"], g["b_prev"]), g["lo"],
+                          g["hi"], mt=1, grid_dt=float(g["grid_dt"]))
+        # the file holds E, B of slice a then E, B of slice b
+        np.concatenate([z.ravel(), g["b_next"].ravel(), z.ravel(),
+                        g["b_prev"].ravel()]).tofile(spec["grid"])
+    else:
+        z = np.zeros_like(g["bgrid"])
+        spec = _pmap_spec(str(tmp_path), name, (z,), (g["bgrid"],), g["lo"], g["hi"])
+    out = _pmap_run(str(tmp_path), "ref_harness_gpu", "pmaph", spec, cfg)
+    for k, key in enumerate(("xpos", "ypos", "zpos")):
+        assert np.array_equal(out[k], g[key]), key
+
+
+def test_dropin_poincare_map_turbulent_grid(tmp_path):
+    """A 33^3 turbulent PIC grid, a 12 x 10 plane of start points: the
+    reference (next_location driven by the harness) and the program with
+    the GPU drop-in write the same map."""
+    from test_particle_amd.fields import pic_turbulence_grid
+    _need("ref_harness", "ref_harness_gpu")
+    e, b, lo, hi = pic_turbulence_grid(n=32, seed=8, db_rms=0.8)
+    spec = _pmap_spec(str(tmp_path), "turb", (np.zeros_like(b),), (b,), lo, hi)
+    cfg = dict(face=2, step=0.05, length_max=150.0, ncross=8, single=0, x0=0.0, y0=0.0,
+               z0=9.3, min_h=0.5, max_h=31.5, min_v=1.0, max_v=30.0, nh=12, nv=10)
+    cpu = _pmap_run(str(tmp_path), "ref_harness", "pmap", spec, cfg)
+    gpu = _pmap_run(str(tmp_path), "ref_harness_gpu", "pmaph", spec, cfg)
+    assert np.array_equal(cpu, gpu)
+    filled = (cpu[0] != 2 * hi[0]).sum(axis=1) - 1
+    assert filled.max() == 8 and filled.min() < 8

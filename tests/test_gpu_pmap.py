@@ -1,0 +1,210 @@
+"""Poincare maps on the GPU (tp_gpu_poincare_map, csrc/tp_fieldline.hip).
+
+* the reference's own crossing points (golden vectors of
+  src/poincare_map.c's RK4 step and config parser on PIC grids) bit for
+  bit, through any chunking of the steps into launches;
+* the CPU oracle bit for bit for the analytic fields (ABC, multi-mode,
+  with the shared sin/cos) and the textbook PIC gather, in both crossing
+  modes, on lines that end at different times (the active-list compaction
+  between launches);
+* a 65,536-line map of a turbulent PIC grid: every periodic-mode record lies
+  within one step of its plane, the run is deterministic, and a sample of
+  lines equals the oracle bit for bit;
+* argument and state errors.
+"""
+import json
+
+import numpy as np
+import pytest
+
+from conftest import golden
+from test_particle_amd import abi
+from test_particle_amd.gpu import pmap_starts
+
+pytestmark = pytest.mark.gpu
+
+TWO_PI = 2.0 * np.pi
+
+
+def case(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    from test_particle_amd.gpu import Tracker
+    t = Tracker(0)
+    yield t
+    t.close()
+
+
+def bitwise(a, b, what=""):
+    a, b = np.asarray(a), np.asarray(b)
+    assert a.shape == b.shape, what
+    bad = ~((a == b) | (np.isnan(a) & np.isnan(b)))
+    if bad.any():
+        i = tuple(np.argwhere(bad)[0])
+        raise AssertionError(f"{what}: {bad.sum()} differ, first {i}: {a[i]!r} vs {b[i]!r}")
+
+
+def same_map(r, ref, what):
+    for k in ("xpos", "ypos", "zpos"):
+        bitwise(r[k], ref[k], f"{what} {k}")
+    assert np.array_equal(r["ncrossed"], ref["ncrossed"]), what
+    if "nsteps" in ref:
+        assert np.array_equal(r["nsteps"], ref["nsteps"]), what
+
+
+def _golden_case(gpu, name):
+    g = golden(name)
+    c = json.loads(str(g["cfg"]))
+    if name == "pmap_grid2":
+        z = np.zeros_like(g["b_next"])
+        gpu.set_field_grid2(z, g["b_next"], z, g["b_prev"], g["lo"], g["hi"],
+                            float(g["grid_dt"]))
+    else:
+        gpu.set_field_grid(np.zeros_like(g["bgrid"]), g["bgrid"], g["lo"], g["hi"])
+    st = pmap_starts(c["face"], (c["x0"], c["y0"], c["z0"]), single=bool(c["single"]),
+                     min_h=c["min_h"], max_h=c["max_h"], min_v=c["min_v"], max_v=c["max_v"],
+                     nh=c["nh"], nv=c["nv"])
+    return g, c, st
+
+
+@pytest.mark.parametrize("name", ["pmap_grid", "pmap_single", "pmap_grid2"])
+@pytest.mark.parametrize("chunk", [0, 37])
+def test_reference_golden_maps(gpu, name, chunk):
+    case(f"poincare golden {name}, chunk {chunk}")
+    g, c, st = _golden_case(gpu, name)
+    prm = abi.make_pmap_params(face=c["face"], ncross=c["ncross"], step=c["step"],
+                               length_max=c["length_max"], lo=g["lo"], hi=g["hi"], chunk=chunk)
+    r = gpu.poincare_map(prm, st)
+    ref = dict(xpos=g["xpos"], ypos=g["ypos"], zpos=g["zpos"],
+               ncrossed=(g["xpos"] != 2 * g["hi"][0]).sum(axis=1) - 1)
+    same_map(r, ref, name)
+    if chunk:
+        assert gpu.last_kernel_time()[1] > 1
+
+
+def _starts(n, lo, hi, face, seed):
+    rng = np.random.default_rng(seed)
+    st = np.column_stack([rng.uniform(lo[d], hi[d], n) for d in range(3)])
+    st[:, face] = rng.uniform(lo[face], hi[face])          # one plane
+    return st
+
+
+FIELDS = ["abc", "modes", "grid_tb", "grid_ref"]
+
+
+def _set(gpu, oracle, kind, seed=0):
+    if kind == "abc":
+        gpu.set_field_abc(1.0, 1.0, 1.0)
+        return oracle.Field.abc(1.0, 1.0, 1.0), (0.0,) * 3, (TWO_PI,) * 3
+    if kind == "modes":
+        from test_particle_amd.fields import mode_table
+        rows = mode_table(48, kmin=0.5, kmax=8.0, rms=0.8, seed=5)
+        gpu.set_field_modes(rows, 1.0)
+        L = TWO_PI / 0.5
+        return oracle.Field.modes(rows, 1.0), (0.0,) * 3, (L,) * 3
+    rng = np.random.default_rng(40 + seed)
+    lo, hi = (0.0, -1.0, 2.0), (6.0, 4.0, 5.0)
+    b = (0.6 * rng.normal(0, 1, (8, 10, 12, 3))).astype(np.float32)
+    b[..., 2] += 0.5
+    e = np.zeros_like(b)
+    mode = abi.TP_GATHER_TEXTBOOK if kind == "grid_tb" else abi.TP_GATHER_REF_COMPAT
+    gpu.set_field_grid(e, b, lo, hi, mode)
+    return oracle.Field.grid(e, b, lo, hi, mode), lo, hi
+
+
+@pytest.mark.parametrize("kind", FIELDS)
+@pytest.mark.parametrize("mode", [abi.TP_PMAP_REF_COMPAT, abi.TP_PMAP_PERIODIC])
+def test_matches_oracle(gpu, oracle_tp, kind, mode):
+    f, lo, hi = _set(gpu, oracle_tp, kind)
+    face = 2 if kind.startswith("grid") else 1
+    n = 300
+    ncross, lmax = {"abc": (6, 25.0), "modes": (6, 60.0)}.get(kind, (16, 60.0))
+    st = _starts(n, lo, hi, face, seed=7)
+    step = 0.03
+    case(f"poincare {kind} mode {mode}: {n} lines, ncross {ncross}, length {lmax}")
+    prm = abi.make_pmap_params(face=face, ncross=ncross, step=step, length_max=lmax, lo=lo,
+                               hi=hi, mode=mode, chunk=200)
+    r = gpu.poincare_map(prm, st)
+    ref = oracle_tp.poincare_map(f, prm, st, nthreads=8)
+    same_map(r, ref, f"{kind} mode {mode}")
+    # lines end at different times: some reach ncross, some the length cap
+    nc = r["ncrossed"]
+    assert (nc == ncross).any() and (nc < ncross).any(), np.bincount(nc)
+
+
+def test_chunking_does_not_change_results(gpu, oracle_tp):
+    _set(gpu, oracle_tp, "abc")
+    st = _starts(1000, (0.0,) * 3, (TWO_PI,) * 3, 0, seed=9)
+    out = []
+    for chunk in (1, 13, 0):
+        case(f"poincare abc chunk {chunk}")
+        prm = abi.make_pmap_params(face=0, ncross=6, step=0.05, length_max=15.0,
+                                   lo=(0.0,) * 3, hi=(TWO_PI,) * 3,
+                                   mode=abi.TP_PMAP_PERIODIC, chunk=chunk)
+        out.append(gpu.poincare_map(prm, st))
+        ms, launches = gpu.last_kernel_time()
+        if chunk == 1:
+            assert launches == out[-1]["nsteps"].max()
+    same_map(out[0], out[1], "chunk 1 vs 13")
+    same_map(out[0], out[2], "chunk 1 vs default")
+
+
+def test_turbulent_grid_map_at_scale(gpu, oracle_tp):
+    from test_particle_amd.fields import pic_turbulence_grid
+    e, b, lo, hi = pic_turbulence_grid(n=64, seed=3, db_rms=0.6)
+    gpu.set_field_grid(np.zeros_like(b), b, lo, hi)
+    st = pmap_starts(2, (0.0, 0.0, 17.25), min_h=0.25, max_h=63.75, min_v=0.25, max_v=63.75,
+                     nh=256, nv=256)
+    step, ncross = 0.1, 6
+    prm = abi.make_pmap_params(face=2, ncross=ncross, step=step, length_max=600.0, lo=lo, hi=hi,
+                               mode=abi.TP_PMAP_PERIODIC)
+    case(f"poincare turbulent 64^3 grid: {st.shape[0]} lines, ncross {ncross}")
+    r = gpu.poincare_map(prm, st)
+    r2 = gpu.poincare_map(prm, st)
+    same_map(r, r2, "rerun")
+    nc = r["ncrossed"]
+    print(f"crossings per line: {np.bincount(nc, minlength=ncross + 1).tolist()}")
+    # a line that wanders far from the guide direction can miss its plane
+    assert (nc >= 1).mean() > 0.99 and nc.max() == ncross
+    L = hi[2] - lo[2]
+    for i in range(0, st.shape[0], 997):
+        z = r["zpos"][i, 1:nc[i] + 1]
+        off = (z - st[i, 2] + 0.5 * L) % L - 0.5 * L
+        assert np.abs(off).max() <= step, i
+        assert (r["zpos"][i, nc[i] + 1:] == 2 * hi[2]).all()
+    # every 1365th line plus lines of each crossing count (the rare ones
+    # included)
+    sample = set(range(0, st.shape[0], st.shape[0] // 48))
+    for k in range(ncross + 1):
+        sample.update(np.flatnonzero(nc == k)[:4].tolist())
+    sample = np.array(sorted(sample))
+    case(f"oracle on {sample.size} sample lines")
+    ref = oracle_tp.poincare_map(oracle_tp.Field.grid(np.zeros_like(b), b, lo, hi), prm,
+                                 st[sample], nthreads=8)
+    sub = {k: r[k][sample] for k in ("xpos", "ypos", "zpos", "ncrossed", "nsteps")}
+    same_map(sub, ref, "sample")
+
+
+def test_errors(gpu):
+    from test_particle_amd.gpu import TPError, Tracker
+    st = np.array([[0.1, 0.2, 0.3]])
+    gpu.set_field_abc()
+    for kw in (dict(face=-1), dict(step=-0.1), dict(length_max=np.nan), dict(mode=5),
+               dict(chunk=-2), dict(hi=(0.0, 1.0, 1.0))):
+        args = dict(face=0, ncross=2, step=0.1, length_max=5.0, lo=(0.0,) * 3, hi=(1.0,) * 3)
+        args.update(kw)
+        with pytest.raises(TPError) as ei:
+            gpu.poincare_map(abi.make_pmap_params(**args), st)
+        assert ei.value.code == abi.TP_E_ARG, kw
+    with Tracker(0) as fresh:
+        with pytest.raises(TPError) as ei:
+            fresh.poincare_map(abi.make_pmap_params(face=0, ncross=2, step=0.1, length_max=5.0,
+                                                    lo=(0.0,) * 3, hi=(1.0,) * 3), st)
+        assert ei.value.code == abi.TP_E_STATE
+    # no lines: nothing to do
+    r = gpu.poincare_map(abi.make_pmap_params(face=0, ncross=2, step=0.1, length_max=5.0,
+                                              lo=(0.0,) * 3, hi=(1.0,) * 3), np.zeros((0, 3)))
+    assert r["xpos"].shape == (0, 3)
