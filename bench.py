@@ -19,8 +19,9 @@ invocation, else the plain FP64 push; the plain value is reported beside
 it.  Extra workloads under "forms": the ABC field
 (16,384 particles x 10,000 steps FP64 Boris and Wirz; 4M particles x 5,000
 steps FP32 against FP64) and the batched adaptive Bulirsch-Stoer stepper
-(1,000-mode turbulence, 1M particles), the learned-surrogate workload and
-the PIC grid with two time slices.
+(1,000-mode turbulence, 1M particles), the learned-surrogate workload, the
+PIC grid with two time slices and the Poincare map of the PIC grid's B
+(262,144 field lines).
 
     python bench.py [--gpus 1] [--steps 5000] [--warmup 200]
 """
@@ -480,8 +481,9 @@ def forms_poincare(args, with_cpu=True):
     gather, the reference's crossing test, `ncross` crossings per line,
     length cap 2e4.  A step is one RK4 step of one field line (four field
     evaluations); the value is total steps / wall time of the library call.
-    The CPU oracle runs the first lines of the same plane (OpenMP), and the
-    GPU's crossing points on those lines are compared bit for bit."""
+    The CPU baseline is the reference's own tracing on the first rows of the
+    same plane (the oracle where the reference harness is not built), and
+    the GPU's crossing points on those lines are compared bit for bit."""
     from test_particle_amd import abi
     from test_particle_amd.fields import pic_turbulence_grid
     from test_particle_amd.gpu import Tracker, pmap_starts
@@ -513,24 +515,85 @@ def forms_poincare(args, with_cpu=True):
         log(f"[bench] {name}: {steps / w:.4e} RK4 steps/s ({steps} steps, {w:.2f}s, "
             f"{launches} launches)")
         if with_cpu:
-            from oracle import tpref
-            cores = host_cores()
-            f = tpref.Field.grid(np.zeros_like(b), b, lo, hi, abi.TP_GATHER_REF_COMPAT)
-            pilot = tpref.poincare_map(f, prm, st[:4 * cores], nthreads=cores)
-            rate = pilot["nsteps"].sum() / max(pilot["seconds"], 1e-6)
-            per_line = pilot["nsteps"].mean()
-            m = int(np.clip(12.0 * rate / per_line, 4 * cores, 65536))
-            ref = tpref.poincare_map(f, prm, st[:m], nthreads=cores)
-            cpu_rate = ref["nsteps"].sum() / ref["seconds"]
-            same = all(np.array_equal(r[k][:m], ref[k]) for k in ("xpos", "ypos", "zpos"))
-            out[name]["cpu_baseline"] = {
-                "value": cpu_rate, "unit": "field-line RK4 steps/s", "cores": cores,
-                "kind": "port", "sample": f"first {m} lines of the same plane, FP64 oracle "
-                                          f"with OpenMP ({cpu_model()})",
-                "gpu_sample_bitwise_equal": bool(same)}
-            log(f"[bench] {name} CPU {cpu_rate:.3e} RK4 steps/s on {cores} threads "
-                f"({m} lines); GPU sample bitwise={same}")
+            base = pmap_reference_baseline(b, lo, hi, prm, side, r)
+            if base is None:
+                base = pmap_port_baseline(b, lo, hi, prm, st, r)
+            out[name]["cpu_baseline"] = base
+            log(f"[bench] {name} CPU ({base['kind']}) {base['value']:.3e} RK4 steps/s on "
+                f"{base['cores']} threads; GPU sample bitwise={base['gpu_sample_bitwise_equal']}")
     return out
+
+
+def pmap_reference_baseline(b, lo, hi, prm, side, r):
+    """The reference's own Poincare tracing -- its config_poincare_map and
+    next_location compiled from its sources into oracle/_ref/ref_harness
+    (mode pmap: the lines spread over OpenMP threads as in
+    src/poincare_map.c:122) -- on the first rows of the same plane of start
+    points, sized from a one-row pilot to about 12 s; its crossing points are
+    compared with the GPU's bit for bit.  None when the harness is absent."""
+    import subprocess
+    import tempfile
+    from test_particle_amd.poincare import write_config
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    if not os.path.exists(exe):
+        return None
+    cores = host_cores()
+    dv = (hi[1] - 0.25 - (lo[1] + 0.25)) / (side - 1)
+    with tempfile.TemporaryDirectory() as tmp:
+        grid = os.path.join(tmp, "grid.bin")
+        with open(grid, "wb") as f:
+            np.zeros(b.size, np.float32).tofile(f)          # E is not used
+            np.ascontiguousarray(b, np.float32).tofile(f)
+        nz, ny, nx = b.shape[:3]
+        with open(os.path.join(tmp, "spec.txt"), "w") as f:
+            f.write(f"system 4\nnx {nx}\nny {ny}\nnz {nz}\ntmax 1.0\ndt 0.01\ngrid {grid}\n")
+            for k, a in zip(("x", "y", "z"), range(3)):
+                f.write(f"{k}min {float(lo[a])!r}\n{k}max {float(hi[a])!r}\n")
+
+        def run(rows):
+            cfg = dict(face=2, step=prm.step, length_max=prm.length_max, ncross=prm.ncross,
+                       single=0, x0=0.0, y0=0.0, z0=128.3, min_h=lo[0] + 0.25,
+                       max_h=hi[0] - 0.25, min_v=lo[1] + 0.25,
+                       max_v=lo[1] + 0.25 + dv * (rows - 1), nh=side, nv=rows)
+            write_config(os.path.join(tmp, "init.dat"), cfg)
+            p = subprocess.run([exe, "pmap", "spec.txt", "init.dat", "pm.bin"], cwd=tmp,
+                               check=True, capture_output=True, text=True, timeout=300,
+                               env=dict(os.environ, OMP_NUM_THREADS=str(cores)))
+            kv = dict(line.split() for line in p.stdout.splitlines()
+                      if line.startswith("pmap_"))
+            out = np.fromfile(os.path.join(tmp, "pm.bin")).reshape(3, side * rows, -1)
+            return float(kv["pmap_seconds"]), int(kv["pmap_steps"]), out
+
+        sec, steps, _ = run(1)
+        rows = int(np.clip(12.0 / max(sec, 1e-3), 1, side))
+        sec, steps, out = run(rows)
+        m = side * rows
+        same = all(np.array_equal(out[k], r[key][:m])
+                   for k, key in enumerate(("xpos", "ypos", "zpos")))
+    return {"value": steps / sec, "unit": "field-line RK4 steps/s", "cores": cores,
+            "kind": "reference",
+            "sample": f"first {m} lines ({rows} rows) of the same plane: the reference's "
+                      f"next_location and config parser (oracle/_ref/ref_harness pmap, "
+                      f"OpenMP over lines) on {cpu_model()}",
+            "gpu_sample_bitwise_equal": bool(same)}
+
+
+def pmap_port_baseline(b, lo, hi, prm, st, r):
+    """The CPU oracle (OpenMP) on the first lines of the plane, about 12 s."""
+    from oracle import tpref
+    from test_particle_amd import abi
+    cores = host_cores()
+    f = tpref.Field.grid(np.zeros_like(b), b, lo, hi, abi.TP_GATHER_REF_COMPAT)
+    pilot = tpref.poincare_map(f, prm, st[:4 * cores], nthreads=cores)
+    rate = pilot["nsteps"].sum() / max(pilot["seconds"], 1e-6)
+    m = int(np.clip(12.0 * rate / pilot["nsteps"].mean(), 4 * cores, 65536))
+    ref = tpref.poincare_map(f, prm, st[:m], nthreads=cores)
+    same = all(np.array_equal(r[k][:m], ref[k]) for k in ("xpos", "ypos", "zpos"))
+    return {"value": ref["nsteps"].sum() / ref["seconds"], "unit": "field-line RK4 steps/s",
+            "cores": cores, "kind": "port",
+            "sample": f"first {m} lines of the same plane, FP64 oracle with OpenMP "
+                      f"({cpu_model()})",
+            "gpu_sample_bitwise_equal": bool(same)}
 
 
 def cpu_baseline(e, b, lo, hi, ptl, params, tr):

@@ -48,6 +48,7 @@
 #include <math.h>
 #include <mpi.h>
 #include <hdf5.h>
+#include <omp.h>
 
 /* Layouts of the reference's structs (include/particle_info.h:6-12,
  * include/domain.h:5-24, include/emfields.h:7-10, include/wlcs.h:24-37,
@@ -474,7 +475,8 @@ int main(int argc, char **argv)
             s.nsteps_output, s.pmass, NULL, 0, 0, p0, nsteps, p, NULL);
     } else if (!strcmp(mode, "pmap")) {
         /* the tracing loop of src/poincare_map.c:83-185 around the
-         * reference's next_location, one MPI rank */
+         * reference's next_location, one MPI rank, lines over OpenMP threads
+         * as :122 does; prints the loop's wall time and RK4 step count */
         config_poincare_map(0, argv[3]);
         const int np = map_for_single_point ? 1 : nhorizontal * nvertical;
         const size_t ns = (size_t)ncross + 1;
@@ -489,6 +491,9 @@ int main(int argc, char **argv)
                                           : max_horizontal - min_horizontal;
         const double dv = nvertical > 1 ? (max_vertical - min_vertical) / (nvertical - 1)
                                         : max_vertical - min_vertical;
+        long long steps = 0;
+        const double t_start = omp_get_wtime();
+#pragma omp parallel for reduction(+ : steps)
         for (int ip = 0; ip < np; ip++) {
             double pos[3] = {fx0, fy0, fz0};
             if (!map_for_single_point) {
@@ -511,11 +516,14 @@ int main(int argc, char **argv)
                     next_location(&pos[0], &pos[1], &pos[2], step_fieldline);
                     cross = (before - c0) * (pos[face_cross_section] - c0);
                     length += step_fieldline;
+                    steps++;
                 }
                 if (!(length < fieldline_length_max)) break;
                 xs[k] = pos[0]; ys[k] = pos[1]; zs[k] = pos[2];
             }
         }
+        printf("pmap_seconds %.6f\npmap_steps %lld\npmap_threads %d\n",
+               omp_get_wtime() - t_start, steps, omp_get_max_threads());
         FILE *fo = fopen(argv[4], "wb");
         if (!fo) die("pmap out");
         fwrite(xp, sizeof(double), ns * np, fo);
