@@ -135,6 +135,69 @@ def test_matches_oracle(gpu, oracle_tp, kind, mode):
     assert (nc == ncross).any() and (nc < ncross).any(), np.bincount(nc)
 
 
+def _set_more(gpu, oracle, kind, ibessel=1):
+    rng = np.random.default_rng(61)
+    lo, hi = (0.0, -1.0, 2.0), (6.0, 4.0, 5.0)
+    if kind == "test":
+        gpu.set_field_test()
+        return oracle.Field.test(), lo, hi
+    if kind == "wlcs":
+        g = golden(f"fields_wlcs_b{ibessel}")
+        kw = dict(ibessel=int(g["ibessel"]), omega=float(g["omega"]), norm_i=float(g["norm_i"]))
+        gpu.set_field_wlcs(g["rows"], **kw)
+        return oracle.Field.wlcs(g["rows"], **kw), (-1.0,) * 3, (2.0,) * 3
+    b = [(0.6 * rng.normal(0, 1, (8, 10, 12, 3))).astype(np.float32) for _ in range(2)]
+    b[0][..., 0] += 0.5                      # next slice: unused at t = 0
+    b[1][..., 2] += 0.5                      # previous slice
+    e = np.zeros_like(b[0])
+    mode = abi.TP_GATHER_TEXTBOOK if kind == "grid2_tb" else abi.TP_GATHER_REF_COMPAT
+    gpu.set_field_grid2(e, b[0], e, b[1], lo, hi, 0.7, mode)
+    return oracle.Field.grid2(e, b[0], e, b[1], lo, hi, 0.7, mode), lo, hi
+
+
+@pytest.mark.parametrize("kind", ["grid2_ref", "grid2_tb", "test"])
+@pytest.mark.parametrize("mode", [abi.TP_PMAP_REF_COMPAT, abi.TP_PMAP_PERIODIC])
+def test_matches_oracle_more_fields(gpu, oracle_tp, kind, mode):
+    """Two time slices at t = 0 (only the previous slice enters) and the
+    uniform test field (straight lines along z), bit for bit."""
+    f, lo, hi = _set_more(gpu, oracle_tp, kind)
+    st = _starts(300, lo, hi, 2, seed=13)
+    prm = abi.make_pmap_params(face=2, ncross=12, step=0.03, length_max=40.0, lo=lo, hi=hi,
+                               mode=mode, chunk=300)
+    case(f"poincare {kind} mode {mode}: 300 lines")
+    r = gpu.poincare_map(prm, st)
+    ref = oracle_tp.poincare_map(f, prm, st, nthreads=8)
+    same_map(r, ref, f"{kind} mode {mode}")
+    if kind == "test":
+        # straight lines along z: x and y never change, one crossing per
+        # period (periodic test) or per fold and per period (reference's)
+        assert (r["xpos"][:, 1:][r["xpos"][:, 1:] != 2 * hi[0]] == np.repeat(
+            st[:, :1], 12, axis=1)[r["xpos"][:, 1:] != 2 * hi[0]]).all()
+        per = int(np.floor(40.0 / (hi[2] - lo[2])))
+        expect = min(12, per if mode == abi.TP_PMAP_PERIODIC else 2 * per + 1)
+        assert np.all(r["ncrossed"] == expect), np.bincount(r["ncrossed"])
+
+
+@pytest.mark.parametrize("ibessel,tol", [(0, 1e-9), (1, 1e-6)])
+def test_wlcs_lines_follow_the_oracle(gpu, oracle_tp, ibessel, tol):
+    """The WLCS field differs from the oracle's by ~1e-12 with static wires
+    and ~2e-6 with the float Bessel wires (the device math library's acos,
+    j0f and j1f), so crossing points agree to a tolerance rather than bit for
+    bit."""
+    f, lo, hi = _set_more(gpu, oracle_tp, "wlcs", ibessel)
+    st = _starts(200, lo, hi, 0, seed=17)
+    prm = abi.make_pmap_params(face=0, ncross=6, step=0.01, length_max=12.0, lo=lo, hi=hi,
+                               mode=abi.TP_PMAP_PERIODIC)
+    case(f"poincare wlcs ibessel {ibessel}: 200 lines")
+    r = gpu.poincare_map(prm, st)
+    ref = oracle_tp.poincare_map(f, prm, st, nthreads=8)
+    same = r["ncrossed"] == ref["ncrossed"]
+    assert same.mean() > 0.98
+    for k in ("xpos", "ypos", "zpos"):
+        d = np.abs(r[k][same] - ref[k][same])
+        assert d.max() < tol, (k, d.max())
+
+
 def test_chunking_does_not_change_results(gpu, oracle_tp):
     _set(gpu, oracle_tp, "abc")
     st = _starts(1000, (0.0,) * 3, (TWO_PI,) * 3, 0, seed=9)
