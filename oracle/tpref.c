@@ -1196,7 +1196,7 @@ int tpref_poincare_map(const tpref_field *f, const tp_pmap_params *q, int n,
                        int *ncrossed, long long *nsteps, int nthreads, double *seconds)
 {
     if (n < 0 || q->face < 0 || q->face > 2 || q->ncross < 0 || !(q->step > 0.0) ||
-        !isfinite(q->length_max) || !(q->length_max / q->step < 0x1p50))
+        !isfinite(q->step) || !isfinite(q->length_max) || !(q->length_max / q->step < 0x1p50))
         return TP_E_ARG;
     if (q->mode != TP_PMAP_REF_COMPAT && q->mode != TP_PMAP_PERIODIC) return TP_E_ARG;
     const int ns = q->ncross + 1;

@@ -396,7 +396,8 @@ extern "C" int tp_gpu_poincare_map(tp_ctx *c, const tp_pmap_params *p, int n,
     /* the traced length grows by step per RK4 step: a finite length_max
      * bounds the steps of every line, and length_max / step < 2^50 keeps
      * len + step > len */
-    if (!(p->step > 0.0) || !isfinite(p->length_max) || !(p->length_max / p->step < 0x1p50))
+    if (!(p->step > 0.0) || !isfinite(p->step) || !isfinite(p->length_max) ||
+        !(p->length_max / p->step < 0x1p50))
         return TP_E_ARG;
     for (int d = 0; d < 3; d++)
         if (!(p->hi[d] > p->lo[d]) || !isfinite(p->hi[d] - p->lo[d])) return TP_E_ARG;

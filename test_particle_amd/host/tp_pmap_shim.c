@@ -102,7 +102,14 @@ int poincare_map(int mpi_rank, int mpi_size, char *config_file_name, domain *sdo
     /* this rank's points: all of them for a single point (rank 0 traces it),
      * otherwise the reference's block split with the remainder on the low
      * ranks */
-    const int total = map_for_single_point ? 1 : nhorizontal * nvertical;
+    const long long total_ll = map_for_single_point ? 1 : (long long)nhorizontal * nvertical;
+    if (total_ll < 1 || total_ll * (ncross + 1LL) > 0x7fffffffLL) {
+        /* the MPI gather counts are ints */
+        fprintf(stderr, "poincare_map: %lld points x %d slots is out of range\n", total_ll,
+                ncross + 1);
+        MPI_Abort(MPI_COMM_WORLD, 1);
+    }
+    const int total = (int)total_ll;
     int npoints, nshift;
     if (map_for_single_point) {
         npoints = mpi_rank == 0 ? 1 : 0;

@@ -173,7 +173,8 @@ def test_zero_crossings_and_bad_arguments(oracle_libm):
     r = oracle_libm.poincare_map(f, prm, st)
     assert r["ncrossed"][0] == 0 and r["nsteps"][0] == 0
     assert np.array_equal(r["ypos"][0], [1.37])
-    for kw in (dict(face=3), dict(step=0.0), dict(length_max=np.inf), dict(ncross=-1),
+    for kw in (dict(face=3), dict(step=0.0), dict(step=np.inf), dict(length_max=np.inf),
+               dict(ncross=-1),
                dict(step=1e-20, length_max=1.0), dict(mode=2)):
         args = dict(face=1, ncross=3, step=0.05, length_max=10.0, lo=LO, hi=HI)
         args.update(kw)
