@@ -514,6 +514,30 @@ def forms_poincare(args, with_cpu=True):
                      "lines_reaching_ncross": int((r["ncrossed"] == nc).sum())}
         log(f"[bench] {name}: {steps / w:.4e} RK4 steps/s ({steps} steps, {w:.2f}s, "
             f"{launches} launches)")
+        # FP32 form: the same lines with the field, unit vectors and RK4 sum
+        # in FP32 (positions, length and crossing test in FP64)
+        prm32 = abi.make_pmap_params(face=2, ncross=nc, step=0.05, length_max=2e4, lo=lo,
+                                     hi=hi, mode=abi.TP_PMAP_REF_COMPAT,
+                                     precision=abi.TP_FP32_MIXED)
+        t0 = time.perf_counter()
+        r32 = tr.poincare_map(prm32, st)
+        w32 = time.perf_counter() - t0
+        k32, l32 = tr.last_kernel_time()
+        s32 = int(r32["nsteps"].sum())
+        both = (r["ncrossed"] >= 2) & (r32["ncrossed"] >= 2)
+        dist = np.sqrt(sum((r32[k][both, 2] - r[k][both, 2]) ** 2
+                           for k in ("xpos", "ypos", "zpos")))
+        name32 = f"poincare_{side}x{side}_lines_fp32"
+        out[name32] = {"value": s32 / w32, "unit": "field-line RK4 steps/s",
+                       "dtype": "f32 field, unit vector and RK4 sum; f64 positions",
+                       "rk4_steps": s32, "kernel_ms": k32, "launches": l32,
+                       "vs_fp64_form": (s32 / w32) / (steps / w),
+                       "same_crossing_count": float(np.mean(r32["ncrossed"] == r["ncrossed"])),
+                       "second_record_dist_median": float(np.median(dist)),
+                       "second_record_dist_p95": float(np.quantile(dist, 0.95))}
+        log(f"[bench] {name32}: {s32 / w32:.4e} RK4 steps/s ({w32:.2f}s); same crossing "
+            f"count {out[name32]['same_crossing_count']:.4f}, 2nd record distance median "
+            f"{np.median(dist):.2e} p95 {np.quantile(dist, 0.95):.2e} cells")
         if with_cpu:
             base = pmap_reference_baseline(b, lo, hi, prm, side, r)
             if base is None:

@@ -283,6 +283,10 @@ typedef struct tp_pmap_params {
     int ncross;             /* crossings recorded per field line, >= 0 */
     int mode;               /* TP_PMAP_* */
     int chunk;              /* RK4 steps per line per kernel launch (0: 16384) */
+    int precision;          /* TP_FP64: the reference's arithmetic, bitwise;
+                               TP_FP32_MIXED: field, unit vector and RK4 sum in
+                               FP32, positions, traced length and crossing test
+                               in FP64 */
     double step;            /* field-line step, > 0 */
     double length_max;      /* traced length at which a line stops */
     double lo[3], hi[3];    /* periodic box (simul_domain) */

@@ -129,6 +129,7 @@ class TpPmapParams(ctypes.Structure):
         ("ncross", ctypes.c_int),
         ("mode", ctypes.c_int),
         ("chunk", ctypes.c_int),
+        ("precision", ctypes.c_int),
         ("step", ctypes.c_double),
         ("length_max", ctypes.c_double),
         ("lo", ctypes.c_double * 3),
@@ -137,12 +138,13 @@ class TpPmapParams(ctypes.Structure):
 
 
 def make_pmap_params(*, face, ncross, step, length_max, lo, hi, mode=TP_PMAP_REF_COMPAT,
-                     chunk=0):
+                     chunk=0, precision=TP_FP64):
     """tp_pmap_params of a Poincare map (the reference's "Poincare map
     configuration" block: face, step, maximum length, crossing count; lo/hi
     the simulation domain)."""
     p = TpPmapParams()
     p.face, p.ncross, p.mode, p.chunk = int(face), int(ncross), int(mode), int(chunk)
+    p.precision = int(precision)
     p.step, p.length_max = float(step), float(length_max)
     for d in range(3):
         p.lo[d] = float(lo[d])

@@ -78,15 +78,17 @@ __device__ __forceinline__ void bc_fill(const float4 *g, BCache &bc, const unsig
 /* grid_indices / get_trilinar_parameters along one axis
  * (src/interpolation.c:36-105): lower and upper corner and the fraction; a
  * point beyond either end of the box uses the wrap cell n-1 -> 0 */
-__device__ __forceinline__ void pm_axis(double x, double lo, double hi, double d, int n,
-                                        int &i1, int &i2, double &fr)
+template <bool FAST>
+__device__ __forceinline__ void pm_axis(double x, double lo, double hi, double d, double inv_d,
+                                        int n, int &i1, int &i2, double &fr)
 {
-    const double q = (x - lo) / d;
+    /* FAST (the FP32 form only): multiplication by 1/d instead of division */
+    const double q = FAST ? (x - lo) * inv_d : (x - lo) / d;
     i1 = tp_floor_to_int(floor(q));
     i2 = i1 + 1;
     fr = q - i1;
-    if (x > hi) { i1 = n - 1; i2 = 0; fr = x / d - (n - 1); }
-    if (x < lo) { i1 = n - 1; i2 = 0; fr = x / d + 1.0; }
+    if (x > hi) { i1 = n - 1; i2 = 0; fr = (FAST ? x * inv_d : x / d) - (n - 1); }
+    if (x < lo) { i1 = n - 1; i2 = 0; fr = (FAST ? x * inv_d : x / d) + 1.0; }
 }
 
 /* The reference's trilinear parameters (corner order and x/z-exchanged
@@ -94,14 +96,16 @@ __device__ __forceinline__ void pm_axis(double x, double lo, double hi, double d
  * with the cell's B values in bc.  Flat node indices are formed only when
  * the cell changes: in 32 bits when every corner index lies inside the
  * grid, otherwise in 64 bits with the reference's clamp into the array. */
+template <bool FAST>
 __device__ __forceinline__ void pm_ref_cell(const FieldDev &f, const float4 *g, BCache &bc,
-                                            double x, double y, double z, double *w)
+                                            const double *inv_d, double x, double y, double z,
+                                            double *w)
 {
     int i1[3], i2[3];
     double fx, fy, fz;
-    pm_axis(x, f.lo[0], f.hi[0], f.d[0], f.nx, i1[0], i2[0], fx);
-    pm_axis(y, f.lo[1], f.hi[1], f.d[1], f.ny, i1[1], i2[1], fy);
-    pm_axis(z, f.lo[2], f.hi[2], f.d[2], f.nz, i1[2], i2[2], fz);
+    pm_axis<FAST>(x, f.lo[0], f.hi[0], f.d[0], inv_d[0], f.nx, i1[0], i2[0], fx);
+    pm_axis<FAST>(y, f.lo[1], f.hi[1], f.d[1], inv_d[1], f.ny, i1[1], i2[1], fy);
+    pm_axis<FAST>(z, f.lo[2], f.hi[2], f.d[2], inv_d[2], f.nz, i1[2], i2[2], fz);
     const bool same = bc.valid && i1[0] == bc.key[0] && i2[0] == bc.key[1] &&
                       i1[1] == bc.key[2] && i2[1] == bc.key[3] && i1[2] == bc.key[4] &&
                       i2[2] == bc.key[5];
@@ -159,7 +163,7 @@ __device__ __forceinline__ void pm_bfield(const FieldDev &f, BCache &bc, double 
 {
     if (FK == FK_GRID_REF || FK == FK_GRID2_REF) {
         double w[8];
-        pm_ref_cell(f, FK == FK_GRID_REF ? f.grid : f.grid_b, bc, x, y, z, w);
+        pm_ref_cell<false>(f, FK == FK_GRID_REF ? f.grid : f.grid_b, bc, nullptr, x, y, z, w);
         if (FK == FK_GRID_REF) {
             /* get_float_bfield_at_point, one slice (src/magnetic_field.c:
              * 603-608): float accumulators rounded after every corner */
@@ -215,6 +219,43 @@ __device__ __forceinline__ void pm_unit(const FieldDev &f, BCache &bc, double x,
     ux = bx * ib; uy = by * ib; uz = bz * ib;
 }
 
+/* The FP32 form: B in FP32 (float weights and FMA accumulation on the grid,
+ * the tracker's FP32 field evaluators otherwise) and the unit vector by the
+ * hardware reciprocal square root. */
+template <int FK>
+__device__ __forceinline__ void pm_unit32(const FieldDev &f, BCache &bc, const double *inv_d,
+                                          double x, double y, double z, float &ux, float &uy,
+                                          float &uz)
+{
+    float bx, by, bz;
+    if (FK == FK_GRID_REF || FK == FK_GRID2_REF || FK == FK_GRID_TB || FK == FK_GRID2_TB) {
+        double w[8];
+        const float4 *g = (FK == FK_GRID_REF || FK == FK_GRID_TB) ? f.grid : f.grid_b;
+        if (FK == FK_GRID_REF || FK == FK_GRID2_REF) {
+            pm_ref_cell<true>(f, g, bc, inv_d, x, y, z, w);
+        } else {
+            unsigned c[8];
+            trilinear_tb(f.nx, f.ny, f.nz, f.lo, f.hi, f.d, x, y, z, c, w);
+            bc_fill(g, bc, c, false);
+        }
+        float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const float wk = (float)w[k];
+            a0 = fmaf(bc.v[3 * k + 0], wk, a0);
+            a1 = fmaf(bc.v[3 * k + 1], wk, a1);
+            a2 = fmaf(bc.v[3 * k + 2], wk, a2);
+        }
+        bx = a0; by = a1; bz = a2;
+    } else {
+        EmfF e;
+        get_emf32<FK>(f, x, y, z, 0.0, e);
+        bx = e.bx; by = e.by; bz = e.bz;
+    }
+    const float ib = rsqrtf(fmaf(bz, bz, fmaf(by, by, bx * bx)));
+    ux = bx * ib; uy = by * ib; uz = bz * ib;
+}
+
 /* field_boundary (src/poincare_map.c:326-352): one fold per axis */
 __device__ __forceinline__ double pm_fold(double v, double lo, double hi, double len)
 {
@@ -228,6 +269,7 @@ struct PmapArgs {
     int face, ncross, mode, chunk, stride;
     double step, lmax;
     double lo[3], hi[3], L[3];
+    double inv_d[3];             /* 1 / grid spacing (FP32 form) */
     double *px, *py, *pz, *len;  /* line state */
     const double *c0;            /* plane coordinate of each line */
     int *k;                      /* next slot while running; -(next slot) once done */
@@ -247,7 +289,7 @@ __device__ __forceinline__ double pm_offset(double d, double L)
 }
 
 /* Lane gi traces line act[gi] for up to a.chunk RK4 steps. */
-template <int FK>
+template <int FK, int PREC>
 __global__ void __launch_bounds__(256) k_pmap(PmapArgs a, int nact, const int *act,
                                               int *act_next, unsigned *cnt)
 {
@@ -271,32 +313,58 @@ __global__ void __launch_bounds__(256) k_pmap(PmapArgs a, int nact, const int *a
             y = pm_fold(y, a.lo[1], a.hi[1], a.L[1]);
             z = pm_fold(z, a.lo[2], a.hi[2], a.L[2]);
             const double pre_folded = pm_pick(a.face, x, y, z);
-            /* the four stages in one loop body (one inlined field gather):
-             * stage s evaluates at x + c_s b_{s-1}, c = (0, h, h, step), and
-             * the weighted sum ((b1 + 2 b2) + 2 b3) + b4 is built in the
-             * reference's order */
-            double ux, uy, uz, sx, sy, sz;
+            if (PREC == PREC64) {
+                /* the four stages in one loop body (one inlined field
+                 * gather): stage s evaluates at x + c_s b_{s-1}, c = (0, h,
+                 * h, step), and the weighted sum ((b1 + 2 b2) + 2 b3) + b4
+                 * is built in the reference's order */
+                double ux, uy, uz, sx, sy, sz;
 #pragma unroll 1
-            for (int s = 0; s < 4; s++) {
-                double tx = x, ty = y, tz = z;
-                if (s > 0) {
-                    const double cs = s == 3 ? a.step : h;
-                    tx = pm_fold(x + cs * ux, a.lo[0], a.hi[0], a.L[0]);
-                    ty = pm_fold(y + cs * uy, a.lo[1], a.hi[1], a.L[1]);
-                    tz = pm_fold(z + cs * uz, a.lo[2], a.hi[2], a.L[2]);
+                for (int s = 0; s < 4; s++) {
+                    double tx = x, ty = y, tz = z;
+                    if (s > 0) {
+                        const double cs = s == 3 ? a.step : h;
+                        tx = pm_fold(x + cs * ux, a.lo[0], a.hi[0], a.L[0]);
+                        ty = pm_fold(y + cs * uy, a.lo[1], a.hi[1], a.L[1]);
+                        tz = pm_fold(z + cs * uz, a.lo[2], a.hi[2], a.L[2]);
+                    }
+                    pm_unit<FK>(a.f, bc, tx, ty, tz, ux, uy, uz);
+                    if (s == 0) {
+                        sx = ux; sy = uy; sz = uz;
+                    } else if (s < 3) {
+                        sx = sx + 2 * ux; sy = sy + 2 * uy; sz = sz + 2 * uz;
+                    } else {
+                        sx = sx + ux; sy = sy + uy; sz = sz + uz;
+                    }
                 }
-                pm_unit<FK>(a.f, bc, tx, ty, tz, ux, uy, uz);
-                if (s == 0) {
-                    sx = ux; sy = uy; sz = uz;
-                } else if (s < 3) {
-                    sx = sx + 2 * ux; sy = sy + 2 * uy; sz = sz + 2 * uz;
-                } else {
-                    sx = sx + ux; sy = sy + uy; sz = sz + uz;
+                x = x + s6 * sx;
+                y = y + s6 * sy;
+                z = z + s6 * sz;
+            } else {
+                /* FP32 form: unit vectors and their weighted sum in FP32,
+                 * stage points and positions in FP64 */
+                float ux, uy, uz, sx, sy, sz;
+#pragma unroll 1
+                for (int s = 0; s < 4; s++) {
+                    double tx = x, ty = y, tz = z;
+                    if (s > 0) {
+                        const double cs = s == 3 ? a.step : h;
+                        tx = pm_fold(x + cs * (double)ux, a.lo[0], a.hi[0], a.L[0]);
+                        ty = pm_fold(y + cs * (double)uy, a.lo[1], a.hi[1], a.L[1]);
+                        tz = pm_fold(z + cs * (double)uz, a.lo[2], a.hi[2], a.L[2]);
+                    }
+                    pm_unit32<FK>(a.f, bc, a.inv_d, tx, ty, tz, ux, uy, uz);
+                    if (s == 0) {
+                        sx = ux; sy = uy; sz = uz;
+                    } else {
+                        const float wgt = s < 3 ? 2.f : 1.f;
+                        sx = fmaf(wgt, ux, sx); sy = fmaf(wgt, uy, sy); sz = fmaf(wgt, uz, sz);
+                    }
                 }
+                x = x + s6 * (double)sx;
+                y = y + s6 * (double)sy;
+                z = z + s6 * (double)sz;
             }
-            x = x + s6 * sx;
-            y = y + s6 * sy;
-            z = z + s6 * sz;
             len = len + a.step;
             ns++;
             const double post = pm_pick(a.face, x, y, z);
@@ -366,19 +434,25 @@ __global__ void k_pmap_zero(unsigned *cnt) { cnt[threadIdx.x] = 0u; }
 
 typedef void (*pmap_fn)(PmapArgs, int, const int *, int *, unsigned *);
 
-pmap_fn pick_pmap(int fk)
+template <int FK>
+pmap_fn pick_prec(int prec)
+{
+    return prec == TP_FP64 ? k_pmap<FK, PREC64> : k_pmap<FK, PREC32>;
+}
+
+pmap_fn pick_pmap(int fk, int prec)
 {
     switch (fk) {
-    case FK_TEST: return k_pmap<FK_TEST>;
+    case FK_TEST: return pick_prec<FK_TEST>(prec);
     case FK_ABC:
-    case FK_ABCV: return k_pmap<FK_ABC>;        /* B does not depend on the velocity grid */
-    case FK_GRID_REF: return k_pmap<FK_GRID_REF>;
-    case FK_GRID_TB: return k_pmap<FK_GRID_TB>;
+    case FK_ABCV: return pick_prec<FK_ABC>(prec);        /* B does not depend on the velocity grid */
+    case FK_GRID_REF: return pick_prec<FK_GRID_REF>(prec);
+    case FK_GRID_TB: return pick_prec<FK_GRID_TB>(prec);
     case FK_MODES:
-    case FK_MLP: return k_pmap<FK_MODES>;       /* the exact mode sum */
-    case FK_WLCS: return k_pmap<FK_WLCS>;
-    case FK_GRID2_REF: return k_pmap<FK_GRID2_REF>;
-    case FK_GRID2_TB: return k_pmap<FK_GRID2_TB>;
+    case FK_MLP: return pick_prec<FK_MODES>(prec);       /* the exact mode sum */
+    case FK_WLCS: return pick_prec<FK_WLCS>(prec);
+    case FK_GRID2_REF: return pick_prec<FK_GRID2_REF>(prec);
+    case FK_GRID2_TB: return pick_prec<FK_GRID2_TB>(prec);
     default: return nullptr;
     }
 }
@@ -393,6 +467,7 @@ extern "C" int tp_gpu_poincare_map(tp_ctx *c, const tp_pmap_params *p, int n,
     if (p->face < 0 || p->face > 2 || p->ncross < 0 || p->chunk < 0 || p->chunk > PM_CHUNK_MAX)
         return TP_E_ARG;
     if (p->mode != TP_PMAP_REF_COMPAT && p->mode != TP_PMAP_PERIODIC) return TP_E_ARG;
+    if (p->precision != TP_FP64 && p->precision != TP_FP32_MIXED) return TP_E_ARG;
     /* the traced length grows by step per RK4 step: a finite length_max
      * bounds the steps of every line, and length_max / step < 2^50 keeps
      * len + step > len */
@@ -403,7 +478,7 @@ extern "C" int tp_gpu_poincare_map(tp_ctx *c, const tp_pmap_params *p, int n,
         if (!(p->hi[d] > p->lo[d]) || !isfinite(p->hi[d] - p->lo[d])) return TP_E_ARG;
     if (n > 0 && (!starts || !xpos || !ypos || !zpos)) return TP_E_ARG;
     if (!c->have_field) return TP_E_STATE;
-    const pmap_fn fn = pick_pmap(field_kind(c));
+    const pmap_fn fn = pick_pmap(field_kind(c), p->precision);
     if (!fn) return TP_E_STATE;
     c->last_ms = 0.0;
     c->last_launches = 0;
@@ -433,6 +508,7 @@ extern "C" int tp_gpu_poincare_map(tp_ctx *c, const tp_pmap_params *p, int n,
         a.lo[d] = p->lo[d];
         a.hi[d] = p->hi[d];
         a.L[d] = p->hi[d] - p->lo[d];
+        a.inv_d[d] = c->f.d[d] != 0.0 ? 1.0 / c->f.d[d] : 0.0;
     }
     a.px = dv;
     a.py = dv + n;

@@ -198,6 +198,53 @@ def test_wlcs_lines_follow_the_oracle(gpu, oracle_tp, ibessel, tol):
         assert d.max() < tol, (k, d.max())
 
 
+def _fp32_vs_fp64(gpu, prm, st):
+    prm.precision = abi.TP_FP64
+    r64 = gpu.poincare_map(prm, st)
+    prm.precision = abi.TP_FP32_MIXED
+    r32 = gpu.poincare_map(prm, st)
+    prm.precision = abi.TP_FP64
+    return r64, r32
+
+
+def test_fp32_form_straight_lines(gpu):
+    """Uniform field on a grid (straight lines): the FP32 form records the
+    same crossings as the FP64 form, positions within 1e-5."""
+    lo, hi = (0.0, 0.0, 0.0), (4.0, 3.0, 5.0)
+    b = np.broadcast_to(np.array([0.3, 0.8, 0.5], np.float32), (5, 5, 5, 3)).copy()
+    gpu.set_field_grid(np.zeros_like(b), b, lo, hi)
+    st = _starts(500, lo, hi, 1, seed=19)
+    prm = abi.make_pmap_params(face=1, ncross=10, step=0.05, length_max=60.0, lo=lo, hi=hi,
+                               mode=abi.TP_PMAP_PERIODIC)
+    case("poincare FP32 form, uniform grid field")
+    r64, r32 = _fp32_vs_fp64(gpu, prm, st)
+    assert np.array_equal(r32["ncrossed"], r64["ncrossed"])
+    assert np.array_equal(r32["nsteps"], r64["nsteps"])
+    for k in ("xpos", "ypos", "zpos"):
+        assert np.abs(r32[k] - r64[k]).max() < 1e-5, k
+
+
+@pytest.mark.parametrize("kind", ["grid_ref", "grid_tb", "abc", "modes"])
+def test_fp32_form_first_crossings(gpu, oracle_tp, kind):
+    """Turbulent and chaotic fields: the FP32 form's first crossing lies
+    within 1e-3 of the FP64 one for 95 % of the lines (the FP64 form is the
+    oracle's bit for bit)."""
+    f, lo, hi = _set(gpu, oracle_tp, kind)
+    face = 2 if kind.startswith("grid") else 1
+    st = _starts(2000, lo, hi, face, seed=23)
+    prm = abi.make_pmap_params(face=face, ncross=3, step=0.02, length_max=60.0, lo=lo, hi=hi,
+                               mode=abi.TP_PMAP_PERIODIC)
+    case(f"poincare FP32 form vs FP64, {kind}")
+    r64, r32 = _fp32_vs_fp64(gpu, prm, st)
+    both = (r64["ncrossed"] >= 1) & (r32["ncrossed"] >= 1)
+    assert both.mean() > 0.5
+    d = np.sqrt(sum((r32[k][both, 1] - r64[k][both, 1]) ** 2 for k in ("xpos", "ypos", "zpos")))
+    print(f"first-crossing distance FP32 vs FP64: median {np.median(d):.2e}, "
+          f"p95 {np.quantile(d, 0.95):.2e}; same count {np.mean(r32['ncrossed'] == r64['ncrossed']):.3f}")
+    assert np.quantile(d, 0.95) < 1e-3
+    assert np.mean(r32["ncrossed"] == r64["ncrossed"]) > 0.9
+
+
 def test_chunking_does_not_change_results(gpu, oracle_tp):
     _set(gpu, oracle_tp, "abc")
     st = _starts(1000, (0.0,) * 3, (TWO_PI,) * 3, 0, seed=9)
@@ -256,7 +303,7 @@ def test_errors(gpu):
     st = np.array([[0.1, 0.2, 0.3]])
     gpu.set_field_abc()
     for kw in (dict(face=-1), dict(step=-0.1), dict(length_max=np.nan), dict(mode=5),
-               dict(chunk=-2), dict(hi=(0.0, 1.0, 1.0))):
+               dict(chunk=-2), dict(hi=(0.0, 1.0, 1.0)), dict(precision=3)):
         args = dict(face=0, ncross=2, step=0.1, length_max=5.0, lo=(0.0,) * 3, hi=(1.0,) * 3)
         args.update(kw)
         with pytest.raises(TPError) as ei:
