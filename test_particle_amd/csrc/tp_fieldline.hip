@@ -256,6 +256,55 @@ __device__ __forceinline__ void pm_unit32(const FieldDev &f, BCache &bc, const d
     ux = bx * ib; uy = by * ib; uz = bz * ib;
 }
 
+/* FP32 form on a grid: a stage point is the step's base cell (from the
+ * folded FP64 position, once per step) plus an FP32 offset q in cell units.
+ * Points past the last node fold by n - 1 cells (the grid is endpoint
+ * inclusive over the periodic box, so that is one box length), the cell
+ * index is kept inside the grid, and the weights follow the field's gather
+ * convention (the reference's exchanges the x and z fractions). */
+template <int FK>
+__device__ __forceinline__ void pm_grid32(const FieldDev &f, BCache &bc, const int *base,
+                                          const float *q, float &bx, float &by, float &bz)
+{
+    constexpr bool REF = FK == FK_GRID_REF || FK == FK_GRID2_REF;
+    const float4 *g = (FK == FK_GRID_REF || FK == FK_GRID_TB) ? f.grid : f.grid_b;
+    const int n[3] = {f.nx, f.ny, f.nz};
+    int i[3];
+    float fr[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const float fl = floorf(q[a]);
+        int k = base[a] + (int)fl;
+        fr[a] = q[a] - fl;
+        if (k >= n[a] - 1) k -= n[a] - 1;
+        if (k < 0) k += n[a] - 1;
+        i[a] = min(max(k, 0), n[a] - 2);
+    }
+    if (!(bc.valid && i[0] == bc.key[0] && i[1] == bc.key[1] && i[2] == bc.key[2])) {
+        const unsigned sx = (unsigned)f.nx, sxy = (unsigned)f.nx * (unsigned)f.ny;
+        const unsigned b0 = (unsigned)i[0] + sx * (unsigned)i[1] + sxy * (unsigned)i[2];
+        const unsigned c[8] = {b0, b0 + sxy, b0 + sx, b0 + 1u, b0 + 1u + sx, b0 + 1u + sxy,
+                               b0 + sx + sxy, b0 + 1u + sx + sxy};
+        bc_load(g, bc, c);
+        bc.key[0] = i[0]; bc.key[1] = i[1]; bc.key[2] = i[2];
+        bc.valid = true;
+    }
+    /* corners in the order (x1y1z1, x1y1z2, x1y2z1, x2y1z1, x2y2z1, x2y1z2,
+     * x1y2z2, x2y2z2) */
+    const float fx = REF ? fr[2] : fr[0], fy = fr[1], fz = REF ? fr[0] : fr[2];
+    const float gx = 1.f - fx, gy = 1.f - fy, gz = 1.f - fz;
+    const float w[8] = {gx * gy * gz, gx * gy * fz, gx * fy * gz, fx * gy * gz,
+                        fx * fy * gz, fx * gy * fz, gx * fy * fz, fx * fy * fz};
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        a0 = fmaf(bc.v[3 * k + 0], w[k], a0);
+        a1 = fmaf(bc.v[3 * k + 1], w[k], a1);
+        a2 = fmaf(bc.v[3 * k + 2], w[k], a2);
+    }
+    bx = a0; by = a1; bz = a2;
+}
+
 /* field_boundary (src/poincare_map.c:326-352): one fold per axis */
 __device__ __forceinline__ double pm_fold(double v, double lo, double hi, double len)
 {
@@ -270,6 +319,7 @@ struct PmapArgs {
     double step, lmax;
     double lo[3], hi[3], L[3];
     double inv_d[3];             /* 1 / grid spacing (FP32 form) */
+    float hd32[3], sd32[3];      /* step / 2 / dx and step / dx (FP32 form) */
     double *px, *py, *pz, *len;  /* line state */
     const double *c0;            /* plane coordinate of each line */
     int *k;                      /* next slot while running; -(next slot) once done */
@@ -340,9 +390,48 @@ __global__ void __launch_bounds__(256) k_pmap(PmapArgs a, int nact, const int *a
                 x = x + s6 * sx;
                 y = y + s6 * sy;
                 z = z + s6 * sz;
+            } else if (FK == FK_GRID_REF || FK == FK_GRID_TB || FK == FK_GRID2_REF ||
+                       FK == FK_GRID2_TB) {
+                /* FP32 form on a grid: base cell and remainder from the
+                 * folded FP64 position once per step, stage offsets,
+                 * field, unit vectors and their weighted sum in FP32 */
+                int base[3];
+                float r0[3];
+                const double p3[3] = {x, y, z};
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    const double qd = (p3[d] - a.lo[d]) * a.inv_d[d];
+                    const double fl = floor(qd);
+                    base[d] = tp_floor_to_int(fl);
+                    r0[d] = (float)(qd - fl);
+                }
+                float ux = 0.f, uy = 0.f, uz = 0.f, sx, sy, sz;
+#pragma unroll 1
+                for (int s = 0; s < 4; s++) {
+                    float q[3] = {r0[0], r0[1], r0[2]};
+                    if (s > 0) {
+                        const float *cs = s == 3 ? a.sd32 : a.hd32;   /* c_s / dx per axis */
+                        q[0] = fmaf(cs[0], ux, r0[0]);
+                        q[1] = fmaf(cs[1], uy, r0[1]);
+                        q[2] = fmaf(cs[2], uz, r0[2]);
+                    }
+                    float bx, by, bz;
+                    pm_grid32<FK>(a.f, bc, base, q, bx, by, bz);
+                    const float ib = rsqrtf(fmaf(bz, bz, fmaf(by, by, bx * bx)));
+                    ux = bx * ib; uy = by * ib; uz = bz * ib;
+                    if (s == 0) {
+                        sx = ux; sy = uy; sz = uz;
+                    } else {
+                        const float wgt = s < 3 ? 2.f : 1.f;
+                        sx = fmaf(wgt, ux, sx); sy = fmaf(wgt, uy, sy); sz = fmaf(wgt, uz, sz);
+                    }
+                }
+                x = x + s6 * (double)sx;
+                y = y + s6 * (double)sy;
+                z = z + s6 * (double)sz;
             } else {
-                /* FP32 form: unit vectors and their weighted sum in FP32,
-                 * stage points and positions in FP64 */
+                /* FP32 form, analytic fields: unit vectors and their
+                 * weighted sum in FP32, stage points and positions in FP64 */
                 float ux, uy, uz, sx, sy, sz;
 #pragma unroll 1
                 for (int s = 0; s < 4; s++) {
@@ -509,6 +598,8 @@ extern "C" int tp_gpu_poincare_map(tp_ctx *c, const tp_pmap_params *p, int n,
         a.hi[d] = p->hi[d];
         a.L[d] = p->hi[d] - p->lo[d];
         a.inv_d[d] = c->f.d[d] != 0.0 ? 1.0 / c->f.d[d] : 0.0;
+        a.hd32[d] = (float)(0.5 * p->step * a.inv_d[d]);
+        a.sd32[d] = (float)(p->step * a.inv_d[d]);
     }
     a.px = dv;
     a.py = dv + n;
