@@ -263,6 +263,13 @@ def forms_abc(args):
             out[name] = {"value": n1 * k1 / w, "dtype": "f64", "particles": n1, "steps": k1,
                          "dxx_last": float(d["dcoeff"][1, -1] / max(d["remain"][-1], 1))}
             log(f"[bench] {name}: {n1 * k1 / w:.4e} particle-steps/s")
+            if push == abi.TP_PUSH_WIRZ and not args.no_cpu:
+                base = abc_reference_baseline(p1, dt, k1, d)
+                if base is not None:
+                    out[name]["cpu_baseline"] = base
+                    log(f"[bench] {name} CPU (reference) {base['value']:.3e} particle-steps/s "
+                        f"on {base['cores']} threads; D_xx last row GPU vs reference "
+                        f"{base['dxx_last_rel_diff_vs_gpu']:.2e}")
         n3, k3 = 1 << 22, 5000
         p3 = init_particles(n3, lo, hi, vthe, seed=args.seed)
         res = {}
@@ -546,6 +553,47 @@ def forms_poincare(args, with_cpu=True):
             log(f"[bench] {name} CPU ({base['kind']}) {base['value']:.3e} RK4 steps/s on "
                 f"{base['cores']} threads; GPU sample bitwise={base['gpu_sample_bitwise_equal']}")
     return out
+
+
+def abc_reference_baseline(ptl, dt, steps, d_gpu):
+    """The reference's own tracking loop on the whole 16,384-particle x
+    10,000-step Wirz workload: particle_tracking_hybrid compiled from its
+    sources with A = B = C = 1 (oracle/_ref/ref_harness_abc1, mode hybrid,
+    OpenMP over the box's threads), timed around the call (which includes
+    the reference's own writes of its two small result files).  Its
+    trajectories differ from the GPU's only through libm's sin/cos against
+    the project's (ulps, amplified by the chaotic ABC flow), so D_xx is
+    compared statistically.  None when the harness is absent."""
+    import subprocess
+    import tempfile
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness_abc1")
+    if not os.path.exists(exe):
+        return None
+    cores = host_cores()
+    two_pi = 2.0 * np.pi
+    with tempfile.TemporaryDirectory() as tmp:
+        os.makedirs(os.path.join(tmp, "data"))
+        with open(os.path.join(tmp, "spec.txt"), "w") as f:
+            f.write(f"system 2\nnx 5\nny 5\nnz 5\ntmax {float(steps * dt + 1e-9)!r}\n"
+                    f"dt {float(dt)!r}\n"
+                    f"bc 0\nnbins 102\nnt_out 100\nemin 0.001\nemax 10.0\n")
+            for k in ("x", "y", "z"):
+                f.write(f"{k}min 0.0\n{k}max {two_pi!r}\n")
+        np.ascontiguousarray(ptl, np.float64).tofile(os.path.join(tmp, "p.bin"))
+        p = subprocess.run([exe, "hybrid", "spec.txt", "p.bin"], cwd=tmp, check=True,
+                           capture_output=True, text=True, timeout=600,
+                           env=dict(os.environ, OMP_NUM_THREADS=str(cores)))
+        kv = dict(line.split() for line in p.stdout.splitlines() if line.startswith("hybrid_"))
+        sec = float(kv["hybrid_seconds"])
+        diff = np.loadtxt(os.path.join(tmp, "data", "diff_coeffs.dat"), skiprows=1)
+    dxx_ref = diff[-1, 1] / diff[-1, 8]
+    dxx_gpu = d_gpu["dcoeff"][1, -1] / d_gpu["remain"][-1]
+    return {"value": ptl.shape[0] * steps / sec, "unit": "particle-steps/s", "cores": cores,
+            "kind": "reference",
+            "sample": f"the whole workload ({ptl.shape[0]} particles x {steps} Wirz steps) "
+                      f"through the reference's particle_tracking_hybrid "
+                      f"(oracle/_ref/ref_harness_abc1, OpenMP) on {cpu_model()}",
+            "dxx_last_rel_diff_vs_gpu": float(abs(dxx_gpu / dxx_ref - 1.0))}
 
 
 def pmap_reference_baseline(b, lo, hi, prm, side, r):

@@ -28,7 +28,8 @@
  *   gda     SPEC PATH CT OUT the PIC float field readers (.gda files)
  *   hybrid  SPEC PTL         particle_tracking_hybrid (src/tracking.c:659) with
  *                            the t=0 spectrum of src/diagnostics.c:50; writes
- *                            data/espectrum.dat and data/diff_coeffs.dat
+ *                            data/espectrum.dat and data/diff_coeffs.dat and
+ *                            prints the call's wall time
  *   diff    SPEC PTL0 PTL DT OUT  calc_diff_coeff (src/diagnostics.c:659)
  *   ellint  SPEC KS OUT      ellint (src/wlcs.c:460); KS: n f64 moduli k,
  *                            OUT: n*(E, K)
@@ -471,8 +472,11 @@ int main(int argc, char **argv)
         memcpy(p0, p, n);
         int *nsteps = malloc(sizeof(int) * (np + 1));
         calc_energy_spectrum(0, np, p, s.nbins, s.nt_out, s.pmass, s.bc);
+        const double t_start = omp_get_wtime();
         particle_tracking_hybrid(0, np, s.dt, s.nbins, s.nt_out, s.bc,
             s.nsteps_output, s.pmass, NULL, 0, 0, p0, nsteps, p, NULL);
+        printf("hybrid_seconds %.6f\nhybrid_threads %d\n", omp_get_wtime() - t_start,
+               omp_get_max_threads());
     } else if (!strcmp(mode, "pmap")) {
         /* the tracing loop of src/poincare_map.c:83-185 around the
          * reference's next_location, one MPI rank, lines over OpenMP threads
