@@ -329,14 +329,37 @@ def forms_bs(args):
         t0 = time.perf_counter()
         st = tr.run_adaptive(prm)
         w = time.perf_counter() - t0
+        fin = tr.store()[0] if not args.no_cpu else None
     rate = st["accepted"] / w
     log(f"[bench] BS: {rate:.4e} accepted steps/s, {st['rhs'] / w:.4e} RHS/s, "
         f"rejected {st['rejected']}, failed {st['failed']}")
-    return {"bs_modes1000_1M": {
+    out = {"bs_modes1000_1M": {
         "value": rate, "unit": "accepted BS steps/s", "dtype": "f64", "particles": n,
         "horizon_gyroperiods": args.bs_tgyro, "rtol": 1e-8, "atol": 1e-10,
         "rhs_per_s": st["rhs"] / w, "accepted": st["accepted"], "rejected": st["rejected"],
         "failed": st["failed"], "kernel_ms": st["kernel_ms"], "wall_s": w}}
+    if fin is not None:
+        # the oracle's sequential BS (OpenMP over particles) on the first
+        # particles of the same set, about 10 s; the GPU's end states of
+        # those particles are compared bit for bit
+        from oracle import tpref
+        cores = host_cores()
+        tpref.set_math(False)
+        f = tpref.Field.modes(rows, 1.0)
+        pilot = tpref.bs_track(f, prm, ptl[:8 * cores], nthreads=cores)
+        per = max(pilot["seconds"], 1e-6) / (8 * cores)          # thread-seconds / particle
+        m = int(np.clip(10.0 * cores / (per * cores), 8 * cores, n))
+        ref = tpref.bs_track(f, prm, ptl[:m], nthreads=cores)
+        same = bool(np.array_equal(ref["ptl"], fin[:m]))
+        out["bs_modes1000_1M"]["cpu_baseline"] = {
+            "value": ref["accepted"] / ref["seconds"], "unit": "accepted BS steps/s",
+            "cores": cores, "kind": "port",
+            "sample": f"first {m} particles of the same set, same horizon and tolerances, "
+                      f"the oracle's BS with OpenMP ({cpu_model()})",
+            "gpu_sample_bitwise_equal": same}
+        log(f"[bench] BS CPU {ref['accepted'] / ref['seconds']:.3e} accepted steps/s on "
+            f"{cores} threads ({m} particles); GPU sample bitwise={same}")
+    return out
 
 
 def forms_surrogate(args):
