@@ -481,7 +481,6 @@ def forms_grid2(args):
     out = {}
     with Tracker(0) as tr:
         tr.set_field_grid2(e1, b1, e0, b0, lo, hi, 20.0)
-        del e1, b1, e0, b0
         for prec, name in ((abi.TP_FP64, "pic_grid2_1M_fp64"), (abi.TP_FP32_MIXED,
                                                                 "pic_grid2_1M_fp32")):
             prm = abi.make_params(dt=dt, estep=K, lo=lo, hi=hi, pusher=abi.TP_PUSH_BORIS,
@@ -501,6 +500,27 @@ def forms_grid2(args):
                          "kernel_ms_per_launch": kms / max(launches, 1)}
             log(f"[bench] {name}: {n * K / w:.4e} particle-steps/s, "
                 f"{out[name]['algorithmic_GBps']:.0f} GB/s algorithmic")
+        if not args.no_cpu:
+            # the oracle (FP64, OpenMP) on the first 65,536 particles for 200
+            # steps; the GPU runs the same sample and must match bit for bit
+            from oracle import tpref
+            cores = host_cores()
+            f = tpref.Field.grid2(e1, b1, e0, b0, lo, hi, 20.0)
+            ns, ks = 65536, 200
+            prm = abi.make_params(dt=dt, estep=ks, lo=lo, hi=hi, pusher=abi.TP_PUSH_BORIS,
+                                  nbins=102, nt_out=2, emin=1e-4, emax=1.0)
+            ref = tpref.track_fixed(f, prm, ptl[:ns], nthreads=cores)
+            g = tr.track_fixed(prm, ptl[:ns])
+            same = bool(np.array_equal(g["ptl"], ref["ptl"]))
+            out["pic_grid2_1M_fp64"]["cpu_baseline"] = {
+                "value": ns * ks / ref["seconds"], "unit": "particle-steps/s", "cores": cores,
+                "kind": "port",
+                "sample": f"first {ns} particles, {ks} steps, same two slices, FP64 oracle "
+                          f"with OpenMP ({cpu_model()})",
+                "gpu_sample_bitwise_equal": same}
+            log(f"[bench] pic_grid2 CPU {ns * ks / ref['seconds']:.3e} particle-steps/s on "
+                f"{cores} threads; GPU sample bitwise={same}")
+    del e1, b1, e0, b0
     return out
 
 
