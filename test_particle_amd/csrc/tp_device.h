@@ -67,6 +67,7 @@ struct FieldDev {
     /* PIC grid, nodes of 8 floats: ex ey ez bx | by bz 0 0 */
     int nx, ny, nz;
     double lo[3], hi[3], d[3];
+    double inv_d[3];                /* 1 / d (the FP32 forms' cell index) */
     const float4 *grid;
     /* two time slices: grid = next slice, grid_b = previous slice, gdt the
      * time between slices */
@@ -212,7 +213,9 @@ __device__ __forceinline__ void trilinear_tb(int nx, int ny, int nz,
  * = grid, slice 1 = grid_b) */
 template <int NS>
 struct GridCacheN {
-    unsigned k0, k7;
+    unsigned k0, k7;   /* textbook gather: flat indices of corners 0 and 7 */
+    int key[6];        /* reference gather: the six corner indices */
+    bool valid;
     float v[48 * NS];
 };
 typedef GridCacheN<1> GridCache;
@@ -224,7 +227,12 @@ struct CacheFor {
 };
 
 template <int NS>
-__device__ __forceinline__ void gc_init(GridCacheN<NS> &gc) { gc.k0 = TP_GC_INVALID; gc.k7 = 0u; }
+__device__ __forceinline__ void gc_init(GridCacheN<NS> &gc)
+{
+    gc.k0 = TP_GC_INVALID;
+    gc.k7 = 0u;
+    gc.valid = false;
+}
 
 template <int NS>
 __device__ __forceinline__ void gc_fill(const FieldDev &f, GridCacheN<NS> &gc,
@@ -248,6 +256,111 @@ __device__ __forceinline__ void gc_fill(const FieldDev &f, GridCacheN<NS> &gc,
     }
 }
 
+template <int NS>
+__device__ __forceinline__ void gc_load(const FieldDev &f, GridCacheN<NS> &gc, const unsigned *c)
+{
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        const float4 *g = s == 0 ? f.grid : f.grid_b;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const float4 p = g[2 * (size_t)c[k]];
+            const float2 q = *(const float2 *)(g + 2 * (size_t)c[k] + 1);
+            float *v = gc.v + 48 * s + 6 * k;
+            v[0] = p.x; v[1] = p.y; v[2] = p.z;
+            v[3] = p.w; v[4] = q.x; v[5] = q.y;
+        }
+    }
+}
+
+/* grid_indices / get_trilinar_parameters along one axis
+ * (src/interpolation.c:36-105): lower and upper corner and the fraction; a
+ * point beyond either end of the box uses the wrap cell n-1 -> 0.  FAST
+ * (the FP32 forms only) multiplies by 1/d instead of dividing by d. */
+template <bool FAST>
+__device__ __forceinline__ void ref_axis(double x, double lo, double hi, double d, double inv_d,
+                                         int n, int &i1, int &i2, double &fr)
+{
+    const double q = FAST ? (x - lo) * inv_d : (x - lo) / d;
+    i1 = tp_floor_to_int(floor(q));
+    i2 = i1 + 1;
+    fr = q - i1;
+    if (x > hi) { i1 = n - 1; i2 = 0; fr = (FAST ? x * inv_d : x / d) - (n - 1); }
+    if (x < lo) { i1 = n - 1; i2 = 0; fr = (FAST ? x * inv_d : x / d) + 1.0; }
+}
+
+/* The reference gather's parameters (the values of trilinear_ref: corner
+ * order and x/z-exchanged weights of src/interpolation.c:141-181) with the
+ * cell's nodes cached in gc.  The cell is identified by its six corner
+ * indices, and flat node indices are formed only when it changes: in 32
+ * bits when every corner index lies inside the grid, otherwise in 64 bits
+ * with the clamp into the array (a clamped cell is never reused). */
+template <bool FAST, int NS>
+__device__ __forceinline__ void ref_cell_cached(const FieldDev &f, GridCacheN<NS> &gc,
+                                                double x, double y, double z, double &fx,
+                                                double &fy, double &fz)
+{
+    int i1[3], i2[3];
+    ref_axis<FAST>(x, f.lo[0], f.hi[0], f.d[0], f.inv_d[0], f.nx, i1[0], i2[0], fx);
+    ref_axis<FAST>(y, f.lo[1], f.hi[1], f.d[1], f.inv_d[1], f.ny, i1[1], i2[1], fy);
+    ref_axis<FAST>(z, f.lo[2], f.hi[2], f.d[2], f.inv_d[2], f.nz, i1[2], i2[2], fz);
+    const bool same = gc.valid && i1[0] == gc.key[0] && i2[0] == gc.key[1] &&
+                      i1[1] == gc.key[2] && i2[1] == gc.key[3] && i1[2] == gc.key[4] &&
+                      i2[2] == gc.key[5];
+    if (!same) {
+        unsigned c[8];
+        bool cl = false;
+        const bool inside = (unsigned)i1[0] < (unsigned)f.nx && (unsigned)i2[0] < (unsigned)f.nx &&
+                            (unsigned)i1[1] < (unsigned)f.ny && (unsigned)i2[1] < (unsigned)f.ny &&
+                            (unsigned)i1[2] < (unsigned)f.nz && (unsigned)i2[2] < (unsigned)f.nz;
+        if (inside) {
+            const unsigned sx = (unsigned)f.nx, sxy = (unsigned)f.nx * (unsigned)f.ny;
+            const unsigned X1 = i1[0], X2 = i2[0];
+            const unsigned Y1 = sx * i1[1], Y2 = sx * i2[1];
+            const unsigned Z1 = sxy * i1[2], Z2 = sxy * i2[2];
+            c[0] = X1 + Y1 + Z1; c[1] = X1 + Y1 + Z2; c[2] = X1 + Y2 + Z1; c[3] = X2 + Y1 + Z1;
+            c[4] = X2 + Y2 + Z1; c[5] = X2 + Y1 + Z2; c[6] = X1 + Y2 + Z2; c[7] = X2 + Y2 + Z2;
+        } else {
+            const long total = (long)f.nx * f.ny * f.nz;
+            const long sx = f.nx, sxy = (long)f.nx * f.ny;
+            long a[8];
+            a[0] = i1[0] + sx * i1[1] + sxy * i1[2];
+            a[1] = i1[0] + sx * i1[1] + sxy * i2[2];
+            a[2] = i1[0] + sx * i2[1] + sxy * i1[2];
+            a[3] = i2[0] + sx * i1[1] + sxy * i1[2];
+            a[4] = i2[0] + sx * i2[1] + sxy * i1[2];
+            a[5] = i2[0] + sx * i1[1] + sxy * i2[2];
+            a[6] = i1[0] + sx * i2[1] + sxy * i2[2];
+            a[7] = i2[0] + sx * i2[1] + sxy * i2[2];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                cl |= (a[k] < 0) | (a[k] >= total);
+                c[k] = (unsigned)(a[k] < 0 ? 0 : (a[k] >= total ? total - 1 : a[k]));
+            }
+        }
+        gc_load(f, gc, c);
+        gc.key[0] = i1[0]; gc.key[1] = i2[0]; gc.key[2] = i1[1];
+        gc.key[3] = i2[1]; gc.key[4] = i1[2]; gc.key[5] = i2[2];
+        gc.valid = !cl;
+    }
+}
+
+/* the reference's corner weights (x and z fractions exchanged,
+ * src/interpolation.c:173-180), in T */
+template <typename T>
+__device__ __forceinline__ void ref_weights(T fx, T fy, T fz, T *w)
+{
+    const T gx = T(1) - fx, gy = T(1) - fy, gz = T(1) - fz;
+    w[0] = gx * gy * gz;
+    w[1] = fx * gy * gz;
+    w[2] = gx * fy * gz;
+    w[3] = gx * gy * fz;
+    w[4] = gx * fy * fz;
+    w[5] = fx * gy * fz;
+    w[6] = fx * fy * gz;
+    w[7] = fx * fy * fz;
+}
+
 /* two time slices through the cache: the same sums, blend and rounding as
  * get_emf64<FK_GRID2_*> */
 template <int FK>
@@ -255,12 +368,16 @@ __device__ __forceinline__ void grid2_emf64_cached(const FieldDev &f, GridCacheN
                                                    double x, double y, double z, double t,
                                                    Emf &o)
 {
-    unsigned c[8];
     double w[8];
-    bool cl = false;
-    if (FK == FK_GRID2_REF) trilinear_ref(f.nx, f.ny, f.nz, f.lo, f.hi, f.d, x, y, z, c, w, &cl);
-    else trilinear_tb(f.nx, f.ny, f.nz, f.lo, f.hi, f.d, x, y, z, c, w);
-    gc_fill(f, gc, c, cl);
+    if (FK == FK_GRID2_REF) {
+        double fx, fy, fz;
+        ref_cell_cached<false, 2>(f, gc, x, y, z, fx, fy, fz);
+        ref_weights<double>(fx, fy, fz, w);
+    } else {
+        unsigned c[8];
+        trilinear_tb(f.nx, f.ny, f.nz, f.lo, f.hi, f.d, x, y, z, c, w);
+        gc_fill(f, gc, c, false);
+    }
     double a[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, b[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < 8; k++)
@@ -285,12 +402,11 @@ template <int FK>
 __device__ __forceinline__ void grid_emf64_cached(const FieldDev &f, GridCacheN<1> &gc, double x,
                                                   double y, double z, Emf &o)
 {
-    unsigned c[8];
     double w[8];
-    bool cl = false;
     if (FK == FK_GRID_REF) {
-        trilinear_ref(f.nx, f.ny, f.nz, f.lo, f.hi, f.d, x, y, z, c, w, &cl);
-        gc_fill(f, gc, c, cl);
+        double fx, fy, fz;
+        ref_cell_cached<false, 1>(f, gc, x, y, z, fx, fy, fz);
+        ref_weights<double>(fx, fy, fz, w);
         /* float accumulators rounded after every corner (src/magnetic_field.c:603-608) */
         float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f, a4 = 0.f, a5 = 0.f;
 #pragma unroll
@@ -304,6 +420,7 @@ __device__ __forceinline__ void grid_emf64_cached(const FieldDev &f, GridCacheN<
         }
         o.ex = a0; o.ey = a1; o.ez = a2; o.bx = a3; o.by = a4; o.bz = a5;
     } else {
+        unsigned c[8];
         trilinear_tb(f.nx, f.ny, f.nz, f.lo, f.hi, f.d, x, y, z, c, w);
         gc_fill(f, gc, c, false);
         double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0, a5 = 0.0;
@@ -324,16 +441,24 @@ template <int FK>
 __device__ __forceinline__ void grid_emf32_cached(const FieldDev &f, GridCacheN<1> &gc, double x,
                                                   double y, double z, EmfF &o)
 {
-    unsigned c[8];
-    double w[8];
-    bool cl = false;
-    if (FK == FK_GRID_REF) trilinear_ref(f.nx, f.ny, f.nz, f.lo, f.hi, f.d, x, y, z, c, w, &cl);
-    else trilinear_tb(f.nx, f.ny, f.nz, f.lo, f.hi, f.d, x, y, z, c, w);
-    gc_fill(f, gc, c, cl);
+    float w[8];
+    if (FK == FK_GRID_REF) {
+        /* FP32 form: cell index by 1/dx, weights in float */
+        double fx, fy, fz;
+        ref_cell_cached<true, 1>(f, gc, x, y, z, fx, fy, fz);
+        ref_weights<float>((float)fx, (float)fy, (float)fz, w);
+    } else {
+        unsigned c[8];
+        double wd[8];
+        trilinear_tb(f.nx, f.ny, f.nz, f.lo, f.hi, f.d, x, y, z, c, wd);
+        gc_fill(f, gc, c, false);
+#pragma unroll
+        for (int k = 0; k < 8; k++) w[k] = (float)wd[k];
+    }
     float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f, a4 = 0.f, a5 = 0.f;
 #pragma unroll
     for (int k = 0; k < 8; k++) {
-        float wk = (float)w[k];
+        const float wk = w[k];
         a0 = fmaf(gc.v[6 * k + 0], wk, a0); a1 = fmaf(gc.v[6 * k + 1], wk, a1);
         a2 = fmaf(gc.v[6 * k + 2], wk, a2); a3 = fmaf(gc.v[6 * k + 3], wk, a3);
         a4 = fmaf(gc.v[6 * k + 4], wk, a4); a5 = fmaf(gc.v[6 * k + 5], wk, a5);
@@ -875,7 +1000,7 @@ __device__ __forceinline__ void boris_push32(const EmfF &e, Ptl &p, float &ux, f
     float fq = hf * __frsqrt_rn(gm2);
     float tx = fq * e.bx, ty = fq * e.by, tz = fq * e.bz;
     float t2 = fmaf(tx, tx, fmaf(ty, ty, tz * tz));
-    float sf = 2.0f / (1.0f + t2);
+    float sf = 2.0f * __builtin_amdgcn_rcpf(1.0f + t2);   /* hardware reciprocal, ~1 ulp */
     float sx = sf * tx, sy = sf * ty, sz = sf * tz;
     float upx = umx + (umy * tz - umz * ty);
     float upy = umy + (umz * tx - umx * tz);

@@ -606,6 +606,7 @@ int tp_gpu_set_field_grid(tp_ctx *c, int nx, int ny, int nz, const double lo[3],
     if (e != hipSuccess) return TP_E_HIP;
     c->f.nx = nx; c->f.ny = ny; c->f.nz = nz;
     set_box(c->f.lo, c->f.hi, c->f.d, lo, hi, nx, ny, nz);
+    for (int a = 0; a < 3; a++) c->f.inv_d[a] = 1.0 / c->f.d[a];
     c->f.grid = c->grid;
     c->f.kind = mode == TP_GATHER_REF_COMPAT ? FK_GRID_REF : FK_GRID_TB;
     c->have_field = true;
