@@ -201,12 +201,13 @@ __device__ __forceinline__ void trilinear_tb(int nx, int ny, int nz,
 /* A particle moves ~0.06 cells per step, so it gathers from the same eight
  * corners for ~10 steps in a row.  The eight corners' six components (48
  * floats) stay in VGPRs and are reloaded only when the corner set changes.
- * The corner set is identified by the flat indices of corners 0 and 7:
- * c7 - c0 = dx + nx dy + nx ny dz with each d either 1 or 1 - n (a wrap
- * cell), and these eight sums are distinct, so (c0, c7) fixes all eight
- * corners.  Cells whose indices were clamped are never reused.  The
- * gathered values are the same floats either way, so results are
- * bit-identical to the uncached gather. */
+ * The reference gather identifies the set by its six per-axis corner
+ * indices (the flat node indices are formed only on a reload); the textbook
+ * gather by the flat indices of corners 0 and 7: c7 - c0 = dx + nx dy +
+ * nx ny dz with each d either 1 or 1 - n (a wrap cell), and these eight
+ * sums are distinct, so (c0, c7) fixes all eight corners.  Cells whose
+ * indices were clamped are never reused.  The gathered values are the same
+ * floats either way, so results are bit-identical to the uncached gather. */
 #define TP_GC_INVALID 0xFFFFFFFFu
 
 /* NS time slices: v[48 s + 6 k + component] of corner k in slice s (slice 0
