@@ -23,6 +23,8 @@
 #ifndef TP_GPU_H
 #define TP_GPU_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -136,6 +138,50 @@ int tp_gpu_set_field_grid2(tp_ctx *ctx, int nx, int ny, int nz, const double lo[
                            const double hi[3], const float *efield_next,
                            const float *bfield_next, const float *efield_prev,
                            const float *bfield_prev, double grid_dt, int gather_mode);
+
+/* --- PIC field input pipeline ------------------------------------------
+ * The reference's loaders read one component at a time as a plane of floats
+ * and scatter it, scaled, into the float3 node arrays on the host
+ * (read_efields_float_binary, src/electric_field.c:257-300, for .gda files,
+ * x fastest; read_pic_efields_float_h5, src/electric_field.c:447-507, and
+ * read_pic_bfields_float_h5, src/magnetic_field.c:632-692, for HDF5, x
+ * slowest and transposed).  Here the raw planes go to the GPU in slabs along
+ * their slowest axis through two pinned staging buffers (the host fills one
+ * while the other is copied and laid out), and a GPU kernel scales, transposes
+ * and interleaves them into the nodes: node values are (float)(v * e0) and
+ * (float)(v * b0) in double, as in the reference, so the field is the same as
+ * tp_gpu_set_field_grid / _grid2 of the reference's arrays. */
+#define TP_PLANES_X_FASTEST 0   /* plane index (k*ny + j)*nx + i: .gda files */
+#define TP_PLANES_X_SLOWEST 1   /* plane index (i*ny + j)*nz + k: PIC HDF5 */
+
+/* Source of the planes: writes component comp (0..5 = ex ey ez bx by bz) of
+ * slice (0 = the next or only slice, 1 = the previous slice) for slowest-axis
+ * indices [s0, s1) to dst, (s1 - s0) * (the product of the two other
+ * dimensions) floats in the plane's order.  Returns 0, or nonzero to stop the
+ * load (which then returns TP_E_ARG and leaves no field set). */
+typedef int (*tp_plane_reader)(void *user, int slice, int comp, long long s0, long long s1,
+                               float *dst);
+
+/* Loads one slice (two_slices = 0, as tp_gpu_set_field_grid) or two
+ * (two_slices = 1, as tp_gpu_set_field_grid2 with grid_dt) from read.
+ * stage_bytes is the size of each pinned staging buffer (0: 64 MiB; at least
+ * one slab of the six components is staged at a time). */
+int tp_gpu_load_field_grid(tp_ctx *ctx, int nx, int ny, int nz, const double lo[3],
+                           const double hi[3], int order, int two_slices, double e0, double b0,
+                           double grid_dt, int gather_mode, tp_plane_reader read, void *user,
+                           size_t stage_bytes);
+/* The same from host arrays: planes_next (and planes_prev, or NULL for one
+ * slice) hold the six components' planes one after another, 6*nx*ny*nz
+ * floats in the given order, unscaled. */
+int tp_gpu_set_field_grid_planes(tp_ctx *ctx, int nx, int ny, int nz, const double lo[3],
+                                 const double hi[3], int order, double e0, double b0,
+                                 const float *planes_next, const float *planes_prev,
+                                 double grid_dt, int gather_mode);
+/* Copies the nodes of a grid field back to the host: slice 0 (the next or
+ * only slice) or 1 (the previous slice) as 8 floats per node, x fastest:
+ * ex ey ez bx by bz 0 0 (the layout the gathers read).  nx, ny, nz must be
+ * the grid's (TP_E_ARG otherwise); TP_E_STATE without a grid field. */
+int tp_gpu_get_field_nodes(tp_ctx *ctx, int slice, int nx, int ny, int nz, float *nodes);
 /* system 1: nwlcs wire-loop current systems; rows of 14 doubles in the
  * column order of the WLCS configuration file read by read_wlcs
  * (src/wlcs.c:97-179): cur_wr x_wr y_wr z_wr theta phi t0 cur_lp r_lp

@@ -3,8 +3,8 @@
  * reference's tracking loop or Poincare map: they turn the raw arrays
  * returned by tp_gpu_track_fixed into the reference's output files, byte for
  * byte, read its field files and lay out its Poincare start points.
- * Plain C, no GPU use; each returns 0 or TP_E_ARG (bad arguments or a file
- * that cannot be opened).
+ * Plain C; only tp_host_load_pic_gda uses the GPU (through tp_gpu.h).  Each
+ * returns 0 or TP_E_ARG (bad arguments or a file that cannot be opened).
  */
 #ifndef TP_HOST_H
 #define TP_HOST_H
@@ -58,6 +58,16 @@ int tp_host_write_diff_coeffs(const char *fname, int nd, const double *dcoeff,
 int tp_host_read_pic_gda(const char *path, int ct, int nt, int two_slices, int nx, int ny,
                          int nz, double e0, double b0, float *e_next, float *b_next,
                          float *e_prev, float *b_prev);
+
+/* The same files straight onto the GPU: each component file is read in
+ * z slabs into pinned staging memory and laid out on the GPU
+ * (tp_gpu_load_field_grid, TP_PLANES_X_FASTEST), giving the field
+ * tp_gpu_set_field_grid (two_slices = 0) or tp_gpu_set_field_grid2
+ * (two_slices = 1, grid_dt) would give for the arrays tp_host_read_pic_gda
+ * returns.  TP_E_ARG for a missing or short file. */
+int tp_host_load_pic_gda(tp_ctx *ctx, const char *path, int ct, int nt, int two_slices, int nx,
+                         int ny, int nz, const double lo[3], const double hi[3], double e0,
+                         double b0, double grid_dt, int gather_mode);
 
 /* Start points of a Poincare map (src/poincare_map.c:83-158).  single = 1:
  * the one point p0.  single = 0: nh*nv points on the plane through p0

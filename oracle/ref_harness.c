@@ -26,6 +26,10 @@
  *                            (traj_diagnose = 1, src/tracking.c:440-447)
  *   trajh   SPEC PTL OUTDIR  particle_tracking_hybrid with traj_diagnose = 1
  *   gda     SPEC PATH CT OUT the PIC float field readers (.gda files)
+ *   h5      SPEC FILE GROUP OUT  the PIC HDF5 readers read_pic_efields_h5 and
+ *                            read_pic_bfields_h5 (src/electric_field.c:447-507,
+ *                            src/magnetic_field.c:632-692); OUT: the loaded
+ *                            efc_float then magc_float
  *   hybrid  SPEC PTL         particle_tracking_hybrid (src/tracking.c:659) with
  *                            the t=0 spectrum of src/diagnostics.c:50; writes
  *                            data/espectrum.dat and data/diff_coeffs.dat and
@@ -85,6 +89,8 @@ void set_variables_efield(grids *sgrid, domain *sdomain, double E0_field, int mt
 void set_variables_velocity(grids *sgrid, domain *sdomain, double v0_field, int mt, int dtype);
 void read_bfields_binary(char *filepath, int ct, int data_type);
 void read_efields_binary(char *filepath, int ct, int data_type);
+void read_pic_efields_h5(char *fname, char *gname, int data_type);
+void read_pic_bfields_h5(char *fname, char *gname, int data_type);
 void initialize_bfield(void);
 void initialize_efield(void);
 void initialize_vfield(void);
@@ -245,7 +251,7 @@ static void setup(const spec_t *s)
             memset(vc_double, 0, nn * sizeof(d3));
         }
     } else if (s->system == 4 && s->gda_b0 > 0.0) {
-        /* fields come from .gda files (mode "gda") */
+        /* fields come from .gda files (mode "gda") or HDF5 (mode "h5") */
         set_variables_bfield(&g, &dom, s->gda_b0, mt, (int)sizeof(float));
         set_variables_efield(&g, &dom, s->gda_e0, mt, (int)sizeof(float));
         initialize_bfield();
@@ -464,6 +470,17 @@ int main(int argc, char **argv)
             fwrite(efb_float, sizeof(f3), nn, fo);
             fwrite(magb_float, sizeof(f3), nn, fo);
         }
+        fclose(fo);
+    } else if (!strcmp(mode, "h5")) {
+        /* one time slice (the readers implement multi_tframe = 0 only) */
+        if (s.mt) die("h5: one time slice only");
+        read_pic_efields_h5(argv[3], argv[4], (int)sizeof(float));
+        read_pic_bfields_h5(argv[3], argv[4], (int)sizeof(float));
+        size_t nn = (size_t)s.nx * s.ny * s.nz;
+        FILE *fo = fopen(argv[5], "wb");
+        if (!fo) die("h5 out");
+        fwrite(efc_float, sizeof(f3), nn, fo);
+        fwrite(magc_float, sizeof(f3), nn, fo);
         fclose(fo);
     } else if (!strcmp(mode, "hybrid")) {
         particles *p = slurp(argv[3], &n);

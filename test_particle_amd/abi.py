@@ -13,6 +13,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 # TPGPU_LIB selects another build of the library (A/B timing of variants)
 LIB_PATH = os.environ.get("TPGPU_LIB") or os.path.join(HERE, "libtpgpu.so")
+# the HDF5 PIC field input (include/tp_h5.h), linked to libtpgpu.so beside it
+H5_LIB_PATH = os.path.join(HERE, "libtph5.so")
 
 PARTICLE_FIELDS = ("x", "y", "z", "qm", "ux", "uy", "uz", "t",
                    "xshift", "yshift", "zshift")
@@ -25,6 +27,10 @@ TP_E_HIP = -3
 TP_E_NONFINITE = -4
 TP_E_STATE = -5
 TP_E_NODEV = -6
+
+# plane orders of the PIC field input (tp_gpu_load_field_grid)
+TP_PLANES_X_FASTEST = 0
+TP_PLANES_X_SLOWEST = 1
 
 TP_PUSH_WIRZ = 0
 TP_PUSH_BORIS = 1

@@ -10,6 +10,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libtpgpu.so")
+OUT_H5 = os.path.join(HERE, "libtph5.so")
+# serial HDF5 (headers and library) for libtph5.so
+H5_ROOT = os.environ.get("HDF5_ROOT", "/opt/conda")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -ffp-contract=off: FP64 kernels must perform the same IEEE operations as
@@ -48,7 +51,19 @@ def build(verbose=False):
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", OUT + ".tmp", *objs],
          verbose)
     os.replace(OUT + ".tmp", OUT)
+    build_h5(verbose)
     return OUT
+
+
+def build_h5(verbose=False):
+    """libtph5.so: the HDF5 PIC field input (include/tp_h5.h), linked to the
+    serial HDF5 library and to libtpgpu.so beside it."""
+    _run([CC, *C_FLAGS, "-I", os.path.join(H5_ROOT, "include"), "-shared", "-o",
+          OUT_H5 + ".tmp", os.path.join(CSRC, "tp_host_h5.c"), "-L", HERE, "-ltpgpu",
+          "-L", os.path.join(H5_ROOT, "lib"), "-lhdf5",
+          "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + os.path.join(H5_ROOT, "lib")], verbose)
+    os.replace(OUT_H5 + ".tmp", OUT_H5)
+    return OUT_H5
 
 
 if __name__ == "__main__":

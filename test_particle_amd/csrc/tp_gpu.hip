@@ -641,6 +641,253 @@ int tp_gpu_set_field_grid2(tp_ctx *c, int nx, int ny, int nz, const double lo[3]
     return TP_OK;
 }
 
+/* ------------------------------------------------------------------ */
+/* PIC field input: raw component planes -> interleaved nodes          */
+/* ------------------------------------------------------------------ */
+
+/* the reference's scaling: data[i] * E0 in double, stored as float
+ * (src/electric_field.c:276, :481) */
+__device__ __forceinline__ float plane_scale(float v, double s) { return (float)((double)v * s); }
+
+__device__ __forceinline__ void store_node(float4 *grid, size_t node, const float v[6],
+                                           double e0, double b0)
+{
+    grid[2 * node] = make_float4(plane_scale(v[0], e0), plane_scale(v[1], e0),
+                                 plane_scale(v[2], e0), plane_scale(v[3], b0));
+    grid[2 * node + 1] = make_float4(plane_scale(v[4], b0), plane_scale(v[5], b0), 0.f, 0.f);
+}
+
+/* x-fastest planes: a slab of cnt nodes starting at node0; plane c of the
+ * slab at src + c*cnt.  Reads and writes are both contiguous in the node
+ * index, so one lane per node. */
+__global__ void __launch_bounds__(256) k_planes_xf(const float *__restrict__ src, size_t cnt,
+                                                   size_t node0, float4 *__restrict__ grid,
+                                                   double e0, double b0)
+{
+    for (size_t n = (size_t)blockIdx.x * 256 + threadIdx.x; n < cnt;
+         n += (size_t)gridDim.x * 256) {
+        float v[6];
+#pragma unroll
+        for (int c = 0; c < 6; c++) v[c] = src[c * cnt + n];
+        store_node(grid, node0 + n, v, e0, b0);
+    }
+}
+
+/* x-slowest planes: a slab of ci x-indices starting at i0, plane c of the
+ * slab at src + c*ci*ny*nz with index (il*ny + j)*nz + k.  A block moves a
+ * 32 (k) x 32 (il) tile of one j through LDS: the loads run along k, the node
+ * stores along x (32 consecutive nodes = 1 KiB per row). */
+__global__ void __launch_bounds__(256) k_planes_xs(const float *__restrict__ src, int ci, int i0,
+                                                   int nx, int ny, int nz,
+                                                   float4 *__restrict__ grid, double e0,
+                                                   double b0)
+{
+    __shared__ float t[6][32][33];
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const int k0 = blockIdx.x * 32, il0 = blockIdx.y * 32;
+    const size_t cnt = (size_t)ci * ny * nz;
+    for (int j = blockIdx.z; j < ny; j += gridDim.z) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int il = il0 + ty + 8 * r, k = k0 + tx;
+            if (il < ci && k < nz) {
+                const size_t at = ((size_t)il * ny + j) * nz + k;
+#pragma unroll
+                for (int c = 0; c < 6; c++) t[c][ty + 8 * r][tx] = src[c * cnt + at];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int k = k0 + ty + 8 * r, il = il0 + tx;
+            if (il < ci && k < nz) {
+                float v[6];
+#pragma unroll
+                for (int c = 0; c < 6; c++) v[c] = t[c][tx][ty + 8 * r];
+                store_node(grid, ((size_t)k * ny + j) * nx + i0 + il, v, e0, b0);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+namespace {
+struct Staging {
+    float *pin[2] = {nullptr, nullptr};   /* pinned host slabs */
+    float *dev[2] = {nullptr, nullptr};   /* their device copies */
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    bool used[2] = {false, false};
+    size_t floats = 0;                    /* capacity of each, in floats */
+};
+
+void staging_free(Staging &s)
+{
+    for (int b = 0; b < 2; b++) {
+        if (s.pin[b]) hipHostFree(s.pin[b]);
+        hipFree(s.dev[b]);
+        if (s.ev[b]) hipEventDestroy(s.ev[b]);
+    }
+}
+
+int staging_alloc(Staging &s, size_t floats)
+{
+    s.floats = floats;
+    for (int b = 0; b < 2; b++) {
+        if (hipHostMalloc((void **)&s.pin[b], floats * sizeof(float), hipHostMallocDefault) !=
+            hipSuccess) {
+            s.pin[b] = nullptr;
+            return TP_E_NOMEM;
+        }
+        if (hipMalloc((void **)&s.dev[b], floats * sizeof(float)) != hipSuccess) {
+            s.dev[b] = nullptr;
+            return TP_E_NOMEM;
+        }
+        if (hipEventCreateWithFlags(&s.ev[b], hipEventDisableTiming) != hipSuccess) {
+            s.ev[b] = nullptr;
+            return TP_E_HIP;
+        }
+    }
+    return TP_OK;
+}
+
+/* One slice through the two staging slabs: while the GPU copies and lays
+ * out slab b, the host fills slab b ^ 1 (after the event of its previous use).
+ * The caller synchronizes the stream before the slabs are freed. */
+int load_slice(tp_ctx *c, Staging &s, float4 *dst, int nx, int ny, int nz, int order,
+               double e0, double b0, tp_plane_reader rd, void *user, int slice)
+{
+    const size_t inner = order == TP_PLANES_X_FASTEST ? (size_t)nx * ny : (size_t)ny * nz;
+    const long long outer = order == TP_PLANES_X_FASTEST ? nz : nx;
+    const long long rows = (long long)(s.floats / (6 * inner));
+    int b = 0;
+    for (long long s0 = 0; s0 < outer; s0 += rows, b ^= 1) {
+        const long long s1 = s0 + rows < outer ? s0 + rows : outer;
+        const size_t cnt = (size_t)(s1 - s0) * inner;
+        if (s.used[b]) HIPCHK(hipEventSynchronize(s.ev[b]));
+        for (int comp = 0; comp < 6; comp++)
+            if (rd(user, slice, comp, s0, s1, s.pin[b] + comp * cnt)) return TP_E_ARG;
+        HIPCHK(hipMemcpyAsync(s.dev[b], s.pin[b], 6 * cnt * sizeof(float),
+                              hipMemcpyHostToDevice, c->stream));
+        if (order == TP_PLANES_X_FASTEST) {
+            const size_t want = (cnt + 255) / 256;
+            const unsigned nb = (unsigned)(want < 65536 ? want : 65536);
+            hipLaunchKernelGGL(k_planes_xf, dim3(nb), dim3(256), 0, c->stream, s.dev[b], cnt,
+                               (size_t)s0 * inner, dst, e0, b0);
+        } else {
+            const int ci = (int)(s1 - s0);
+            dim3 g((nz + 31) / 32, (ci + 31) / 32, ny < 65535 ? ny : 65535);
+            hipLaunchKernelGGL(k_planes_xs, g, dim3(256), 0, c->stream, s.dev[b], ci, (int)s0,
+                               nx, ny, nz, dst, e0, b0);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(s.ev[b], c->stream));
+        s.used[b] = true;
+    }
+    return TP_OK;
+}
+
+struct HostPlanes {
+    const float *p[2];
+    size_t nn, inner;
+};
+
+int host_planes_read(void *user, int slice, int comp, long long s0, long long s1, float *dst)
+{
+    const HostPlanes *h = (const HostPlanes *)user;
+    memcpy(dst, h->p[slice] + comp * h->nn + (size_t)s0 * h->inner,
+           (size_t)(s1 - s0) * h->inner * sizeof(float));
+    return 0;
+}
+}  // namespace
+
+int tp_gpu_load_field_grid(tp_ctx *c, int nx, int ny, int nz, const double lo[3],
+                           const double hi[3], int order, int two_slices, double e0, double b0,
+                           double grid_dt, int mode, tp_plane_reader rd, void *user,
+                           size_t stage_bytes)
+{
+    if (!c || nx < 2 || ny < 2 || nz < 2 || !lo || !hi || !rd) return TP_E_ARG;
+    if (order != TP_PLANES_X_FASTEST && order != TP_PLANES_X_SLOWEST) return TP_E_ARG;
+    if (two_slices != 0 && two_slices != 1) return TP_E_ARG;
+    if (two_slices && (!(grid_dt > 0.0) || !isfinite(grid_dt))) return TP_E_ARG;
+    if (mode != TP_GATHER_REF_COMPAT && mode != TP_GATHER_TEXTBOOK) return TP_E_ARG;
+    const size_t nn = (size_t)nx * ny * nz;
+    if (nn >= 0xFFFFFFFFull) return TP_E_ARG;    /* 32-bit corner indices */
+    HIPCHK(hipSetDevice(c->device));
+    free_field(c);
+    memset(&c->f, 0, sizeof(c->f));
+    /* whole slabs of the six components, at least one, at most the grid */
+    const size_t inner = order == TP_PLANES_X_FASTEST ? (size_t)nx * ny : (size_t)ny * nz;
+    const size_t outer = order == TP_PLANES_X_FASTEST ? (size_t)nz : (size_t)nx;
+    size_t rows = (stage_bytes ? stage_bytes : ((size_t)64 << 20)) / (6 * inner * sizeof(float));
+    if (rows < 1) rows = 1;
+    if (rows > outer) rows = outer;
+    if (order == TP_PLANES_X_SLOWEST && rows > (size_t)65535 * 32) rows = (size_t)65535 * 32;
+    if (hipMalloc(&c->grid, nn * 8 * sizeof(float)) != hipSuccess) {
+        c->grid = nullptr;
+        return TP_E_NOMEM;
+    }
+    if (two_slices && hipMalloc(&c->grid_b, nn * 8 * sizeof(float)) != hipSuccess) {
+        c->grid_b = nullptr;
+        free_field(c);
+        return TP_E_NOMEM;
+    }
+    Staging s;
+    int rc = staging_alloc(s, rows * 6 * inner);
+    if (rc == TP_OK)
+        rc = load_slice(c, s, c->grid, nx, ny, nz, order, e0, b0, rd, user, 0);
+    if (rc == TP_OK && two_slices)
+        rc = load_slice(c, s, c->grid_b, nx, ny, nz, order, e0, b0, rd, user, 1);
+    /* every copy out of the slabs ends before they are freed */
+    const hipError_t se = hipStreamSynchronize(c->stream);
+    staging_free(s);
+    if (rc == TP_OK && se != hipSuccess) rc = TP_E_HIP;
+    if (rc != TP_OK) {
+        free_field(c);
+        memset(&c->f, 0, sizeof(c->f));
+        return rc;
+    }
+    c->f.nx = nx; c->f.ny = ny; c->f.nz = nz;
+    set_box(c->f.lo, c->f.hi, c->f.d, lo, hi, nx, ny, nz);
+    for (int a = 0; a < 3; a++) c->f.inv_d[a] = 1.0 / c->f.d[a];
+    c->f.grid = c->grid;
+    if (two_slices) {
+        c->f.grid_b = c->grid_b;
+        c->f.gdt = grid_dt;
+        c->f.kind = mode == TP_GATHER_REF_COMPAT ? FK_GRID2_REF : FK_GRID2_TB;
+    } else {
+        c->f.kind = mode == TP_GATHER_REF_COMPAT ? FK_GRID_REF : FK_GRID_TB;
+    }
+    c->have_field = true;
+    return TP_OK;
+}
+
+int tp_gpu_set_field_grid_planes(tp_ctx *c, int nx, int ny, int nz, const double lo[3],
+                                 const double hi[3], int order, double e0, double b0,
+                                 const float *planes_next, const float *planes_prev,
+                                 double grid_dt, int mode)
+{
+    if (!planes_next || nx < 2 || ny < 2 || nz < 2) return TP_E_ARG;
+    HostPlanes h;
+    h.p[0] = planes_next;
+    h.p[1] = planes_prev;
+    h.nn = (size_t)nx * ny * nz;
+    h.inner = order == TP_PLANES_X_FASTEST ? (size_t)nx * ny : (size_t)ny * nz;
+    return tp_gpu_load_field_grid(c, nx, ny, nz, lo, hi, order, planes_prev ? 1 : 0, e0, b0,
+                                  grid_dt, mode, host_planes_read, &h, 0);
+}
+
+int tp_gpu_get_field_nodes(tp_ctx *c, int slice, int nx, int ny, int nz, float *nodes)
+{
+    if (!c || !nodes || (slice != 0 && slice != 1)) return TP_E_ARG;
+    const float4 *g = slice ? c->grid_b : c->grid;
+    if (!c->have_field || !g) return TP_E_STATE;
+    if (nx != c->f.nx || ny != c->f.ny || nz != c->f.nz) return TP_E_ARG;
+    HIPCHK(hipSetDevice(c->device));
+    const size_t nn = (size_t)c->f.nx * c->f.ny * c->f.nz;
+    HIPCHK(hipMemcpy(nodes, g, nn * 8 * sizeof(float), hipMemcpyDeviceToHost));
+    return TP_OK;
+}
+
 int tp_gpu_set_field_wlcs(tp_ctx *c, int nwlcs, const double *rows, int ibessel,
                           double omega, double norm_i, int mode)
 {

@@ -6,9 +6,12 @@
  * files from the raw arrays the GPU library returns, with the same formulas
  * for the bin centres and widths and the same printf formats.
  */
+#define _FILE_OFFSET_BITS 64
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
+#include <sys/types.h>
 
 #include "../../include/tp_host.h"
 
@@ -178,6 +181,56 @@ int tp_host_read_pic_gda(const char *path, int ct, int nt, int two_slices, int n
         if (!rc) rc = read_slice(path, ct < nt - 1 ? ct + 1 : ct, nn, e0, b0, buf, e_next, b_next);
     }
     free(buf);
+    return rc;
+}
+
+/* the open component files of the slices being loaded, read in slabs */
+typedef struct {
+    FILE *fp[2][6];
+    size_t inner;
+} gda_files;
+
+static int gda_read(void *user, int slice, int comp, long long s0, long long s1, float *dst)
+{
+    gda_files *g = (gda_files *)user;
+    FILE *fp = g->fp[slice][comp];
+    const size_t cnt = (size_t)(s1 - s0) * g->inner;
+    if (fseeko(fp, (off_t)s0 * (off_t)g->inner * (off_t)sizeof(float), SEEK_SET) != 0) return 1;
+    return fread(dst, sizeof(float), cnt, fp) == cnt ? 0 : 1;
+}
+
+int tp_host_load_pic_gda(tp_ctx *ctx, const char *path, int ct, int nt, int two_slices, int nx,
+                         int ny, int nz, const double lo[3], const double hi[3], double e0,
+                         double b0, double grid_dt, int gather_mode)
+{
+    static const char *names[6] = {"ex", "ey", "ez", "bx", "by", "bz"};
+    if (!ctx || !path || nx < 1 || ny < 1 || nz < 1 || ct < 0) return TP_E_ARG;
+    if (two_slices != 0 && two_slices != 1) return TP_E_ARG;
+    if (two_slices && (nt < 1 || ct >= nt)) return TP_E_ARG;
+    /* slice 0 = next (ct + 1, ct at the last slice), slice 1 = previous
+     * (ct - 1, ct at 0), src/magnetic_field.c:470-524; one slice: ct */
+    const int cts[2] = {two_slices ? (ct < nt - 1 ? ct + 1 : ct) : ct, ct > 0 ? ct - 1 : ct};
+    gda_files g;
+    memset(&g, 0, sizeof(g));
+    g.inner = (size_t)nx * ny;
+    const long long need = (long long)nx * ny * nz * (long long)sizeof(float);
+    int rc = TP_OK;
+    for (int sl = 0; sl <= two_slices && rc == TP_OK; sl++)
+        for (int k = 0; k < 6 && rc == TP_OK; k++) {
+            char fname[4096];
+            int m = snprintf(fname, sizeof(fname), "%s%s_%d.gda", path, names[k], cts[sl]);
+            if (m < 0 || (size_t)m >= sizeof(fname)) { rc = TP_E_ARG; break; }
+            g.fp[sl][k] = fopen(fname, "rb");
+            if (!g.fp[sl][k] || fseeko(g.fp[sl][k], 0, SEEK_END) != 0 ||
+                ftello(g.fp[sl][k]) < need)
+                rc = TP_E_ARG;
+        }
+    if (rc == TP_OK)
+        rc = tp_gpu_load_field_grid(ctx, nx, ny, nz, lo, hi, TP_PLANES_X_FASTEST, two_slices, e0,
+                                    b0, grid_dt, gather_mode, gda_read, &g, 0);
+    for (int sl = 0; sl < 2; sl++)
+        for (int k = 0; k < 6; k++)
+            if (g.fp[sl][k]) fclose(g.fp[sl][k]);
     return rc;
 }
 

@@ -13,6 +13,11 @@ from .abi import dptr, fptr, iptr
 
 _lib = None
 
+# tp_plane_reader: (user, slice, comp, s0, s1, dst) -> 0 or nonzero to stop
+PLANE_READER = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_longlong, ctypes.c_longlong,
+                                ctypes.POINTER(ctypes.c_float))
+
 
 def lib():
     global _lib
@@ -39,6 +44,11 @@ def lib():
                                              P(ctypes.c_float), P(ctypes.c_float),
                                              P(ctypes.c_float), d, i]
         L.tp_gpu_set_field_modes.argtypes = [vp, i, P(d), d]
+        L.tp_gpu_load_field_grid.argtypes = [vp, i, i, i, P(d), P(d), i, i, d, d, d, i,
+                                             PLANE_READER, vp, ctypes.c_size_t]
+        L.tp_gpu_set_field_grid_planes.argtypes = [vp, i, i, i, P(d), P(d), i, d, d,
+                                                   P(ctypes.c_float), P(ctypes.c_float), d, i]
+        L.tp_gpu_get_field_nodes.argtypes = [vp, i, i, i, i, P(ctypes.c_float)]
         L.tp_gpu_eval_field.argtypes = [vp, i, P(d), P(d), i]
         L.tp_gpu_track_fixed.argtypes = [vp, P(abi.TpTrackParams), i, P(d), P(d), P(i),
                                          P(d), P(d), P(d), P(i)]
@@ -72,6 +82,7 @@ def lib():
         L.tp_host_write_diff_coeffs.argtypes = [cs, i, P(d), P(i)]
         L.tp_host_pmap_starts.argtypes = [i, i, P(d), d, d, d, d, i, i, P(d)]
         L.tp_host_read_pic_gda.argtypes = [cs, i, i, i, i, i, i, d, d, P(f), P(f), P(f), P(f)]
+        L.tp_host_load_pic_gda.argtypes = [vp, cs, i, i, i, i, i, i, P(d), P(d), d, d, d, i]
         _lib = L
     return _lib
 
@@ -158,6 +169,77 @@ class Tracker:
         _check(lib().tp_gpu_set_field_grid2(self._ctx, nx, ny, nz, _vec3(lo), _vec3(hi),
                                             *(fptr(a) for a in arrs), float(grid_dt), mode),
                "set_field_grid2")
+
+    def set_field_grid_planes(self, planes_next, lo, hi, order=abi.TP_PLANES_X_FASTEST,
+                              e0=1.0, b0=1.0, planes_prev=None, grid_dt=0.0,
+                              mode=abi.TP_GATHER_REF_COMPAT):
+        """PIC field from raw component planes (tp_gpu_set_field_grid_planes):
+        planes_* of shape (6, nz, ny, nx) for TP_PLANES_X_FASTEST or
+        (6, nx, ny, nz) for TP_PLANES_X_SLOWEST, components ex ey ez bx by bz,
+        unscaled; values become (float)(v * e0) and (float)(v * b0)."""
+        pn = np.ascontiguousarray(planes_next, dtype=np.float32)
+        if pn.ndim != 4 or pn.shape[0] != 6:
+            raise ValueError("planes must have shape (6, n0, n1, n2)")
+        pp = None if planes_prev is None else np.ascontiguousarray(planes_prev,
+                                                                   dtype=np.float32)
+        if pp is not None and pp.shape != pn.shape:
+            raise ValueError("the two slices must have the same shape")
+        a, bb, c = pn.shape[1:]
+        nz, ny, nx = (a, bb, c) if order == abi.TP_PLANES_X_FASTEST else (c, bb, a)
+        _check(lib().tp_gpu_set_field_grid_planes(self._ctx, nx, ny, nz, _vec3(lo), _vec3(hi),
+                                                  int(order), float(e0), float(b0), fptr(pn),
+                                                  None if pp is None else fptr(pp),
+                                                  float(grid_dt), mode), "set_field_grid_planes")
+
+    def load_field_grid(self, reader, shape, lo, hi, order=abi.TP_PLANES_X_FASTEST,
+                        two_slices=False, e0=1.0, b0=1.0, grid_dt=0.0,
+                        mode=abi.TP_GATHER_REF_COMPAT, stage_bytes=0):
+        """PIC field through a plane reader (tp_gpu_load_field_grid): reader
+        (slice, comp, s0, s1) returns the float32 values of that slab in the
+        plane's order, or None to stop the load; shape = (nx, ny, nz)."""
+        nx, ny, nz = (int(v) for v in shape)
+
+        want = (ny * nx if order == abi.TP_PLANES_X_FASTEST else ny * nz)
+
+        def cb(user, slice_, comp, s0, s1, dst):
+            try:
+                v = reader(slice_, comp, s0, s1)
+                if v is None:
+                    return 1
+                v = np.ascontiguousarray(v, dtype=np.float32).ravel()
+                if v.size != (s1 - s0) * want:
+                    return 1
+                ctypes.memmove(dst, v.ctypes.data, v.nbytes)
+                return 0
+            except Exception:   # an exception must not cross the C frames
+                return 1
+
+        fn = PLANE_READER(cb)
+        _check(lib().tp_gpu_load_field_grid(self._ctx, nx, ny, nz, _vec3(lo), _vec3(hi),
+                                            int(order), 1 if two_slices else 0, float(e0),
+                                            float(b0), float(grid_dt), mode, fn, None,
+                                            int(stage_bytes)), "load_field_grid")
+
+    def load_pic_gda(self, path, ct, shape, lo, hi, e0, b0, nt=0, two_slices=False,
+                     grid_dt=0.0, mode=abi.TP_GATHER_REF_COMPAT):
+        """The reference's .gda field files straight onto the GPU
+        (tp_host_load_pic_gda); shape = (nx, ny, nz)."""
+        nx, ny, nz = (int(v) for v in shape)
+        p = path if path.endswith(os.sep) else path + os.sep
+        _check(lib().tp_host_load_pic_gda(self._ctx, p.encode(), int(ct), int(nt),
+                                          1 if two_slices else 0, nx, ny, nz, _vec3(lo),
+                                          _vec3(hi), float(e0), float(b0), float(grid_dt),
+                                          mode), "load_pic_gda")
+
+    def field_nodes(self, shape, slice_=0):
+        """The grid field's nodes as the gathers read them
+        (tp_gpu_get_field_nodes); shape = (nx, ny, nz) of the grid.  Returns
+        float32 (nz, ny, nx, 8): ex ey ez bx by bz 0 0."""
+        nx, ny, nz = (int(v) for v in shape)
+        out = np.zeros((nz, ny, nx, 8), np.float32)
+        _check(lib().tp_gpu_get_field_nodes(self._ctx, int(slice_), nx, ny, nz, fptr(out)),
+               "field_nodes")
+        return out
 
     def set_field_wlcs(self, rows, ibessel=0, omega=0.001, norm_i=1.0,
                        mode=abi.TP_GATHER_REF_COMPAT):

@@ -4,6 +4,7 @@
     python -m test_particle_amd.profile_run abc   [--steps 1000]   # ABC field tracking
     python -m test_particle_amd.profile_run copy  [--mib 1024]     # plain copy kernel
     python -m test_particle_amd.profile_run pmap  [--side 512]     # Poincare map, PIC grid
+    python -m test_particle_amd.profile_run fieldload [--side 384] [--h5 DIR]  # PIC field input
 
 ``copy`` launches one device-to-device copy kernel (torch's) of a known size;
 its FETCH_SIZE / WRITE_SIZE readings calibrate those counters against bytes.
@@ -15,9 +16,58 @@ import time
 import numpy as np
 
 
+def field_load(tr, n, h5dir):
+    """A side^3 PIC grid from raw planes: the GPU pipeline (pinned slabs,
+    transpose and interleave on the GPU) against the host path of the
+    reference's loaders (scale and transpose on the host, then
+    tp_gpu_set_field_grid)."""
+    from . import abi
+    rng = np.random.default_rng(1)
+    lo, hi = (0.0, 0.0, 0.0), (float(n), float(n), float(n))
+    planes = rng.standard_normal((6, n, n, n), dtype=np.float32)   # x slowest
+    gb = planes.nbytes / 1e9
+    print(f"fieldload: {n}^3 nodes, {gb:.2f} GB of raw planes", flush=True)
+    for order, name in ((abi.TP_PLANES_X_SLOWEST, "x-slowest (HDF5)"),
+                        (abi.TP_PLANES_X_FASTEST, "x-fastest (.gda)")):
+        for rep in range(2):
+            t0 = time.perf_counter()
+            tr.set_field_grid_planes(planes, lo, hi, order=order, e0=0.3, b0=1.7)
+            w = time.perf_counter() - t0
+            print(f"  planes {name} #{rep}: {w * 1e3:.1f} ms, {gb / w:.2f} GB/s", flush=True)
+    t0 = time.perf_counter()
+    p = planes.astype(np.float64)
+    e = np.ascontiguousarray((p[:3] * 0.3).astype(np.float32).transpose(3, 2, 1, 0))
+    b = np.ascontiguousarray((p[3:] * 1.7).astype(np.float32).transpose(3, 2, 1, 0))
+    del p
+    t1 = time.perf_counter()
+    tr.set_field_grid(e, b, lo, hi)
+    t2 = time.perf_counter()
+    print(f"  host path: scale+transpose (numpy) {(t1 - t0) * 1e3:.1f} ms, "
+          f"tp_gpu_set_field_grid {(t2 - t1) * 1e3:.1f} ms, {gb / (t2 - t0):.2f} GB/s",
+          flush=True)
+    if h5dir:
+        from . import h5
+        fname = f"{h5dir}/fields_0.h5"
+        h5.write_pic_planes(fname, "Timestep_0", planes)
+        for rep in range(2):
+            t0 = time.perf_counter()
+            h5.load_pic(tr, fname, "Timestep_0", (n, n, n), lo, hi, 0.3, 1.7)
+            w = time.perf_counter() - t0
+            print(f"  h5 load_pic #{rep}: {w * 1e3:.1f} ms, {gb / w:.2f} GB/s", flush=True)
+        t0 = time.perf_counter()
+        e2, b2 = h5.read_pic(fname, "Timestep_0", (n, n, n), 0.3, 1.7)
+        t1 = time.perf_counter()
+        tr.set_field_grid(e2, b2, lo, hi)
+        t2 = time.perf_counter()
+        print(f"  h5 host path: tp_h5_read_pic {(t1 - t0) * 1e3:.1f} ms, "
+              f"tp_gpu_set_field_grid {(t2 - t1) * 1e3:.1f} ms", flush=True)
+        assert np.array_equal(e2, e) and np.array_equal(b2, b)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("what", choices=["grid", "abc", "copy", "modes", "surrogate", "pmap"])
+    ap.add_argument("what", choices=["grid", "abc", "copy", "modes", "surrogate", "pmap",
+                                       "fieldload"])
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--particles", type=int, default=1 << 20)
     ap.add_argument("--pusher", default="boris", choices=["boris", "wirz"])
@@ -34,6 +84,7 @@ def main():
     ap.add_argument("--side", type=int, default=512, help="Poincare start points per side")
     ap.add_argument("--ncross", type=int, default=2, help="Poincare crossings per line")
     ap.add_argument("--chunk", type=int, default=0, help="Poincare RK4 steps per launch")
+    ap.add_argument("--h5", default="", help="fieldload: also through an HDF5 file in DIR")
     a = ap.parse_args()
     if a.what == "copy":
         import torch
@@ -117,6 +168,10 @@ def main():
         steps = int(r["nsteps"].sum())
         print(f"pmap: {st.shape[0]} lines x {a.ncross} crossings, {steps} RK4 steps, "
               f"{steps / wall:.4e} steps/s, kernels {ms:.2f} ms / {k} launches", flush=True)
+        tr.close()
+        return
+    if a.what == "fieldload":
+        field_load(tr, a.side, a.h5)
         tr.close()
         return
     pusher = abi.TP_PUSH_BORIS if a.pusher == "boris" else abi.TP_PUSH_WIRZ

@@ -6,6 +6,7 @@ runs it on seeded inputs.  Every fixture is an .npz of inputs and the
 reference's outputs; nothing of the reference's source is stored.
 
     python tests/golden/make_golden.py      # in the build container only
+    python tests/golden/make_golden.py h5   # only the PIC HDF5 input fixture
 """
 import json
 import os
@@ -133,8 +134,39 @@ def pmap_case(name, spec, cfg, extra, tmp):
              **extra)
 
 
+def h5_case(tmp):
+    """PIC HDF5 field input: a file of the reference's layout (datasets ex ey
+    ez cbx cby cbz, x slowest, larger than the block read) written with
+    libtph5.so, loaded by the reference's read_pic_efields_h5 and
+    read_pic_bfields_h5."""
+    import ctypes
+    from test_particle_amd import build
+    build.build_h5()
+    L = ctypes.CDLL(build.OUT_H5)
+    rng = np.random.default_rng(91)
+    nx, ny, nz = 7, 5, 6
+    dims = np.array([9, 6, 8], dtype=np.int64)        # (x, y, z) extents in the file
+    planes = rng.normal(0, 1, (6, *dims)).astype(np.float32)
+    fname = os.path.join(tmp, "fields_3.h5")
+    rc = L.tp_h5_write_pic_planes(fname.encode(), b"Timestep_3",
+                                  dims.ctypes.data_as(ctypes.c_void_p),
+                                  planes.ctypes.data_as(ctypes.c_void_p))
+    assert rc == 0
+    sp = os.path.join(tmp, "spec.txt")
+    write_spec(sp, system=4, nx=nx, ny=ny, nz=nz, tmax=1.0, dt=0.01, gda_b0=2.5, gda_e0=0.3,
+               **box((0.0, 0.0, 0.0), (1.0, 1.0, 1.0)))
+    run("ref_harness", "h5", sp, fname, "Timestep_3", "hout.bin", cwd=tmp)
+    out = np.fromfile(os.path.join(tmp, "hout.bin"), dtype=np.float32).reshape(2, nz, ny, nx, 3)
+    np.savez(os.path.join(HERE, "pic_h5.npz"), nx=nx, ny=ny, nz=nz, e0=0.3, b0=2.5,
+             planes=planes, gname="Timestep_3", e=out[0], b=out[1])
+
+
 def main():
     build_ref()
+    if sys.argv[1:] == ["h5"]:
+        with tempfile.TemporaryDirectory() as tmp:
+            h5_case(tmp)
+        return
     rng = np.random.default_rng(7)
     lo2pi, hi2pi = (0.0, 0.0, 0.0), (TWO_PI, TWO_PI, TWO_PI)
     with tempfile.TemporaryDirectory() as tmp:
@@ -282,6 +314,7 @@ def main():
                                                     dtype=np.float32)
         np.savez(os.path.join(HERE, "gda_slices.npz"), nx=nx, ny=ny, nz=nz, nt=4, b0=2.5,
                  e0=0.3, **files, **out)
+        h5_case(tmp)
         # --- Poincare map (run_type 2) --------------------------------------
         rng4 = np.random.default_rng(31)
         plo, phi = (0.0, -1.0, 2.0), (6.0, 4.0, 5.0)

@@ -1,5 +1,5 @@
-"""CPU-side checks of the boundary: libtpgpu.so loads and exports every
-function include/tp_gpu.h declares; the Python mirror's structures match the
+"""CPU-side checks of the boundary: libtpgpu.so and libtph5.so load and
+export every function their headers in include/ declare; the Python mirror's structures match the
 C layout."""
 import ctypes
 import os
@@ -14,9 +14,14 @@ HEADERS = [os.path.join(ROOT, "include", f) for f in os.listdir(os.path.join(ROO
            if f.endswith(".h")]
 
 
-def declared_functions():
+# include/tp_h5.h is the HDF5 input library's; every other header is libtpgpu.so's
+H5_HEADER = os.path.join(ROOT, "include", "tp_h5.h")
+H5_LIB = os.path.join(os.path.dirname(abi.LIB_PATH), "libtph5.so")
+
+
+def declared_functions(headers=None):
     names = set()
-    for h in HEADERS:
+    for h in headers or [h for h in HEADERS if h != H5_HEADER]:
         src = open(h).read()
         src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
         for m in re.finditer(r"\b(?:int|void|const char\s*\*)\s*\**\s*(tp_\w+)\s*\(", src):
@@ -35,6 +40,15 @@ def test_header_declares_entry_points():
 def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(abi.LIB_PATH)
     missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, f"not exported: {missing}"
+
+
+@pytest.mark.skipif(not os.path.exists(H5_LIB), reason="libtph5.so not built")
+def test_h5_library_exports_every_declared_symbol():
+    names = declared_functions([H5_HEADER])
+    assert "tp_h5_load_pic" in names and "tp_h5_read_pic" in names
+    lib = ctypes.CDLL(H5_LIB)
+    missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, f"not exported: {missing}"
 
 
