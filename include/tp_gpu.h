@@ -164,8 +164,10 @@ typedef int (*tp_plane_reader)(void *user, int slice, int comp, long long s0, lo
 
 /* Loads one slice (two_slices = 0, as tp_gpu_set_field_grid) or two
  * (two_slices = 1, as tp_gpu_set_field_grid2 with grid_dt) from read.
- * stage_bytes is the size of each pinned staging buffer (0: 64 MiB; at least
- * one slab of the six components is staged at a time). */
+ * stage_bytes is the size of each pinned staging buffer (0: 32 MiB; at least
+ * one slab of the six components is staged at a time).  The context keeps
+ * its two staging buffers (pinned host memory and device memory of that
+ * size each) for later loads, until tp_gpu_finalize. */
 int tp_gpu_load_field_grid(tp_ctx *ctx, int nx, int ny, int nz, const double lo[3],
                            const double hi[3], int order, int two_slices, double e0, double b0,
                            double grid_dt, int gather_mode, tp_plane_reader read, void *user,

@@ -63,6 +63,12 @@ struct tp_ctx {
     int *traj_accum = nullptr, *traj_pts = nullptr;
     size_t traj_cap = 0;
     int traj_ncap = 0;
+    /* grid input: two pinned host slabs and their device copies, kept for
+     * the next load (freed by tp_gpu_finalize) */
+    float *stage_pin[2] = {nullptr, nullptr};
+    float *stage_dev[2] = {nullptr, nullptr};
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
+    size_t stage_floats = 0;
 };
 
 void tp_bs_release(tp_ctx *c);

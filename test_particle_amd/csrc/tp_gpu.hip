@@ -470,6 +470,10 @@ static void free_field(tp_ctx *c)
     c->s.derived = false;
 }
 
+namespace {
+void staging_release(tp_ctx *c);
+}
+
 void tp_gpu_finalize(tp_ctx *c)
 {
     if (!c) return;
@@ -492,6 +496,7 @@ void tp_gpu_finalize(tp_ctx *c)
     hipFree(c->traj_accum);
     hipFree(c->traj_pts);
     free(c->bs_tune_block);
+    staging_release(c);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -580,69 +585,10 @@ int tp_gpu_set_velocity_grid(tp_ctx *c, int nx, int ny, int nz, const double lo[
     return TP_OK;
 }
 
-int tp_gpu_set_field_grid(tp_ctx *c, int nx, int ny, int nz, const double lo[3],
-                          const double hi[3], const float *ef, const float *bf, int mode)
-{
-    if (!c || nx < 2 || ny < 2 || nz < 2 || !lo || !hi || !ef || !bf) return TP_E_ARG;
-    if (mode != TP_GATHER_REF_COMPAT && mode != TP_GATHER_TEXTBOOK) return TP_E_ARG;
-    size_t nn = (size_t)nx * ny * nz;
-    if (nn >= 0xFFFFFFFFull) return TP_E_ARG;    /* 32-bit corner indices */
-    HIPCHK(hipSetDevice(c->device));
-    free_field(c);
-    memset(&c->f, 0, sizeof(c->f));
-    /* interleave: node k -> float4 {ex,ey,ez,bx}, float4 {by,bz,0,0} */
-    float *h = (float *)malloc(nn * 8 * sizeof(float));
-    if (!h) return TP_E_NOMEM;
-    for (size_t k = 0; k < nn; k++) {
-        float *o = h + 8 * k;
-        o[0] = ef[3 * k]; o[1] = ef[3 * k + 1]; o[2] = ef[3 * k + 2];
-        o[3] = bf[3 * k]; o[4] = bf[3 * k + 1]; o[5] = bf[3 * k + 2];
-        o[6] = 0.f; o[7] = 0.f;
-    }
-    hipError_t e = hipMalloc(&c->grid, nn * 8 * sizeof(float));
-    if (e != hipSuccess) { free(h); c->grid = nullptr; return TP_E_NOMEM; }
-    e = hipMemcpy(c->grid, h, nn * 8 * sizeof(float), hipMemcpyHostToDevice);
-    free(h);
-    if (e != hipSuccess) return TP_E_HIP;
-    c->f.nx = nx; c->f.ny = ny; c->f.nz = nz;
-    set_box(c->f.lo, c->f.hi, c->f.d, lo, hi, nx, ny, nz);
-    for (int a = 0; a < 3; a++) c->f.inv_d[a] = 1.0 / c->f.d[a];
-    c->f.grid = c->grid;
-    c->f.kind = mode == TP_GATHER_REF_COMPAT ? FK_GRID_REF : FK_GRID_TB;
-    c->have_field = true;
-    return TP_OK;
-}
-
-int tp_gpu_set_field_grid2(tp_ctx *c, int nx, int ny, int nz, const double lo[3],
-                           const double hi[3], const float *ef_next, const float *bf_next,
-                           const float *ef_prev, const float *bf_prev, double grid_dt,
-                           int mode)
-{
-    if (!c || !ef_prev || !bf_prev || !(grid_dt > 0.0) || !isfinite(grid_dt)) return TP_E_ARG;
-    int rc = tp_gpu_set_field_grid(c, nx, ny, nz, lo, hi, ef_next, bf_next, mode);
-    if (rc) return rc;
-    const size_t nn = (size_t)nx * ny * nz;
-    float *h = (float *)malloc(nn * 8 * sizeof(float));
-    if (!h) { free_field(c); return TP_E_NOMEM; }
-    for (size_t k = 0; k < nn; k++) {
-        float *o = h + 8 * k;
-        o[0] = ef_prev[3 * k]; o[1] = ef_prev[3 * k + 1]; o[2] = ef_prev[3 * k + 2];
-        o[3] = bf_prev[3 * k]; o[4] = bf_prev[3 * k + 1]; o[5] = bf_prev[3 * k + 2];
-        o[6] = 0.f; o[7] = 0.f;
-    }
-    hipError_t e = hipMalloc(&c->grid_b, nn * 8 * sizeof(float));
-    if (e != hipSuccess) { free(h); c->grid_b = nullptr; free_field(c); return TP_E_NOMEM; }
-    e = hipMemcpy(c->grid_b, h, nn * 8 * sizeof(float), hipMemcpyHostToDevice);
-    free(h);
-    if (e != hipSuccess) { free_field(c); return TP_E_HIP; }
-    c->f.grid_b = c->grid_b;
-    c->f.gdt = grid_dt;
-    c->f.kind = mode == TP_GATHER_REF_COMPAT ? FK_GRID2_REF : FK_GRID2_TB;
-    return TP_OK;
-}
-
 /* ------------------------------------------------------------------ */
-/* PIC field input: raw component planes -> interleaved nodes          */
+/* PIC grid input: raw component planes or the reference's float3 node */
+/* arrays -> 32-byte nodes {ex ey ez bx | by bz 0 0}, x fastest, built */
+/* on the GPU from slabs staged through pinned host memory             */
 /* ------------------------------------------------------------------ */
 
 /* the reference's scaling: data[i] * E0 in double, stored as float
@@ -711,66 +657,108 @@ __global__ void __launch_bounds__(256) k_planes_xs(const float *__restrict__ src
     }
 }
 
+/* float3 E and B node arrays (x fastest, already scaled): a slab of cnt
+ * nodes, E at src, B at src + 3*cnt. */
+__global__ void __launch_bounds__(256) k_nodes_aos(const float *__restrict__ src, size_t cnt,
+                                                  size_t node0, float4 *__restrict__ grid)
+{
+    for (size_t n = (size_t)blockIdx.x * 256 + threadIdx.x; n < cnt;
+         n += (size_t)gridDim.x * 256) {
+        const float *e = src + 3 * n, *b = src + 3 * cnt + 3 * n;
+        grid[2 * (node0 + n)] = make_float4(e[0], e[1], e[2], b[0]);
+        grid[2 * (node0 + n) + 1] = make_float4(b[1], b[2], 0.f, 0.f);
+    }
+}
+
+/* source layouts of the pipeline: the two public plane orders, and the
+ * reference's float3 node arrays (tp_gpu_set_field_grid / _grid2), read
+ * as component 0 (E, 3 floats per node) and 1 (B) */
+enum { LAYOUT_XF = TP_PLANES_X_FASTEST, LAYOUT_XS = TP_PLANES_X_SLOWEST, LAYOUT_AOS = 2 };
+
 namespace {
+/* the context's two staging slabs for one load */
 struct Staging {
     float *pin[2] = {nullptr, nullptr};   /* pinned host slabs */
     float *dev[2] = {nullptr, nullptr};   /* their device copies */
     hipEvent_t ev[2] = {nullptr, nullptr};
     bool used[2] = {false, false};
     size_t floats = 0;                    /* capacity of each, in floats */
+    size_t rows = 0;                      /* slowest-axis rows per slab */
 };
 
-void staging_free(Staging &s)
+void staging_release(tp_ctx *c)
 {
     for (int b = 0; b < 2; b++) {
-        if (s.pin[b]) hipHostFree(s.pin[b]);
-        hipFree(s.dev[b]);
-        if (s.ev[b]) hipEventDestroy(s.ev[b]);
+        if (c->stage_pin[b]) hipHostFree(c->stage_pin[b]);
+        hipFree(c->stage_dev[b]);
+        if (c->stage_ev[b]) hipEventDestroy(c->stage_ev[b]);
+        c->stage_pin[b] = c->stage_dev[b] = nullptr;
+        c->stage_ev[b] = nullptr;
     }
+    c->stage_floats = 0;
 }
 
-int staging_alloc(Staging &s, size_t floats)
+/* slabs of at least `floats` floats: the context's, grown when too small */
+int staging_get(tp_ctx *c, Staging &s, size_t floats)
 {
-    s.floats = floats;
-    for (int b = 0; b < 2; b++) {
-        if (hipHostMalloc((void **)&s.pin[b], floats * sizeof(float), hipHostMallocDefault) !=
-            hipSuccess) {
-            s.pin[b] = nullptr;
-            return TP_E_NOMEM;
+    if (c->stage_floats < floats) {
+        staging_release(c);
+        for (int b = 0; b < 2; b++) {
+            if (hipHostMalloc((void **)&c->stage_pin[b], floats * sizeof(float),
+                              hipHostMallocDefault) != hipSuccess) {
+                c->stage_pin[b] = nullptr;
+                staging_release(c);
+                return TP_E_NOMEM;
+            }
+            if (hipMalloc((void **)&c->stage_dev[b], floats * sizeof(float)) != hipSuccess) {
+                c->stage_dev[b] = nullptr;
+                staging_release(c);
+                return TP_E_NOMEM;
+            }
+            if (hipEventCreateWithFlags(&c->stage_ev[b], hipEventDisableTiming) != hipSuccess) {
+                c->stage_ev[b] = nullptr;
+                staging_release(c);
+                return TP_E_HIP;
+            }
         }
-        if (hipMalloc((void **)&s.dev[b], floats * sizeof(float)) != hipSuccess) {
-            s.dev[b] = nullptr;
-            return TP_E_NOMEM;
-        }
-        if (hipEventCreateWithFlags(&s.ev[b], hipEventDisableTiming) != hipSuccess) {
-            s.ev[b] = nullptr;
-            return TP_E_HIP;
-        }
+        c->stage_floats = floats;
     }
+    for (int b = 0; b < 2; b++) {
+        s.pin[b] = c->stage_pin[b];
+        s.dev[b] = c->stage_dev[b];
+        s.ev[b] = c->stage_ev[b];
+    }
+    s.floats = c->stage_floats;
     return TP_OK;
 }
 
 /* One slice through the two staging slabs: while the GPU copies and lays
  * out slab b, the host fills slab b ^ 1 (after the event of its previous use).
  * The caller synchronizes the stream before the slabs are freed. */
-int load_slice(tp_ctx *c, Staging &s, float4 *dst, int nx, int ny, int nz, int order,
+int load_slice(tp_ctx *c, Staging &s, float4 *dst, int nx, int ny, int nz, int layout,
                double e0, double b0, tp_plane_reader rd, void *user, int slice)
 {
-    const size_t inner = order == TP_PLANES_X_FASTEST ? (size_t)nx * ny : (size_t)ny * nz;
-    const long long outer = order == TP_PLANES_X_FASTEST ? nz : nx;
-    const long long rows = (long long)(s.floats / (6 * inner));
+    const size_t inner = layout == LAYOUT_XS ? (size_t)ny * nz : (size_t)nx * ny;
+    const long long outer = layout == LAYOUT_XS ? nx : nz;
+    const long long rows = (long long)s.rows;
+    /* six planes of cnt floats, or two float3 arrays of 3*cnt */
+    const int ncomp = layout == LAYOUT_AOS ? 2 : 6;
     int b = 0;
     for (long long s0 = 0; s0 < outer; s0 += rows, b ^= 1) {
         const long long s1 = s0 + rows < outer ? s0 + rows : outer;
         const size_t cnt = (size_t)(s1 - s0) * inner;
+        const size_t per = cnt * (6 / ncomp);
         if (s.used[b]) HIPCHK(hipEventSynchronize(s.ev[b]));
-        for (int comp = 0; comp < 6; comp++)
-            if (rd(user, slice, comp, s0, s1, s.pin[b] + comp * cnt)) return TP_E_ARG;
+        for (int comp = 0; comp < ncomp; comp++)
+            if (rd(user, slice, comp, s0, s1, s.pin[b] + comp * per)) return TP_E_ARG;
         HIPCHK(hipMemcpyAsync(s.dev[b], s.pin[b], 6 * cnt * sizeof(float),
                               hipMemcpyHostToDevice, c->stream));
-        if (order == TP_PLANES_X_FASTEST) {
-            const size_t want = (cnt + 255) / 256;
-            const unsigned nb = (unsigned)(want < 65536 ? want : 65536);
+        const size_t want = (cnt + 255) / 256;
+        const unsigned nb = (unsigned)(want < 65536 ? want : 65536);
+        if (layout == LAYOUT_AOS) {
+            hipLaunchKernelGGL(k_nodes_aos, dim3(nb), dim3(256), 0, c->stream, s.dev[b], cnt,
+                               (size_t)s0 * inner, dst);
+        } else if (layout == LAYOUT_XF) {
             hipLaunchKernelGGL(k_planes_xf, dim3(nb), dim3(256), 0, c->stream, s.dev[b], cnt,
                                (size_t)s0 * inner, dst, e0, b0);
         } else {
@@ -798,6 +786,24 @@ int host_planes_read(void *user, int slice, int comp, long long s0, long long s1
            (size_t)(s1 - s0) * h->inner * sizeof(float));
     return 0;
 }
+
+/* the float3 E and B arrays of one or two slices, read in z slabs */
+struct HostNodes {
+    const float *a[2][2];   /* [slice][E, B] */
+    size_t inner;
+};
+
+int host_nodes_read(void *user, int slice, int comp, long long s0, long long s1, float *dst)
+{
+    const HostNodes *h = (const HostNodes *)user;
+    memcpy(dst, h->a[slice][comp] + 3 * (size_t)s0 * h->inner,
+           3 * (size_t)(s1 - s0) * h->inner * sizeof(float));
+    return 0;
+}
+
+int load_grid(tp_ctx *c, int nx, int ny, int nz, const double lo[3], const double hi[3],
+              int layout, int two_slices, double e0, double b0, double grid_dt, int mode,
+              tp_plane_reader rd, void *user, size_t stage_bytes);
 }  // namespace
 
 int tp_gpu_load_field_grid(tp_ctx *c, int nx, int ny, int nz, const double lo[3],
@@ -805,8 +811,46 @@ int tp_gpu_load_field_grid(tp_ctx *c, int nx, int ny, int nz, const double lo[3]
                            double grid_dt, int mode, tp_plane_reader rd, void *user,
                            size_t stage_bytes)
 {
-    if (!c || nx < 2 || ny < 2 || nz < 2 || !lo || !hi || !rd) return TP_E_ARG;
     if (order != TP_PLANES_X_FASTEST && order != TP_PLANES_X_SLOWEST) return TP_E_ARG;
+    return load_grid(c, nx, ny, nz, lo, hi, order, two_slices, e0, b0, grid_dt, mode, rd, user,
+                     stage_bytes);
+}
+
+int tp_gpu_set_field_grid(tp_ctx *c, int nx, int ny, int nz, const double lo[3],
+                          const double hi[3], const float *ef, const float *bf, int mode)
+{
+    if (!ef || !bf) return TP_E_ARG;
+    HostNodes h;
+    h.a[0][0] = ef;
+    h.a[0][1] = bf;
+    h.a[1][0] = h.a[1][1] = nullptr;
+    h.inner = (size_t)nx * ny;
+    return load_grid(c, nx, ny, nz, lo, hi, LAYOUT_AOS, 0, 1.0, 1.0, 0.0, mode, host_nodes_read,
+                     &h, 0);
+}
+
+int tp_gpu_set_field_grid2(tp_ctx *c, int nx, int ny, int nz, const double lo[3],
+                           const double hi[3], const float *ef_next, const float *bf_next,
+                           const float *ef_prev, const float *bf_prev, double grid_dt,
+                           int mode)
+{
+    if (!ef_next || !bf_next || !ef_prev || !bf_prev) return TP_E_ARG;
+    HostNodes h;
+    h.a[0][0] = ef_next;
+    h.a[0][1] = bf_next;
+    h.a[1][0] = ef_prev;
+    h.a[1][1] = bf_prev;
+    h.inner = (size_t)nx * ny;
+    return load_grid(c, nx, ny, nz, lo, hi, LAYOUT_AOS, 1, 1.0, 1.0, grid_dt, mode,
+                     host_nodes_read, &h, 0);
+}
+
+namespace {
+int load_grid(tp_ctx *c, int nx, int ny, int nz, const double lo[3], const double hi[3],
+              int layout, int two_slices, double e0, double b0, double grid_dt, int mode,
+              tp_plane_reader rd, void *user, size_t stage_bytes)
+{
+    if (!c || nx < 2 || ny < 2 || nz < 2 || !lo || !hi || !rd) return TP_E_ARG;
     if (two_slices != 0 && two_slices != 1) return TP_E_ARG;
     if (two_slices && (!(grid_dt > 0.0) || !isfinite(grid_dt))) return TP_E_ARG;
     if (mode != TP_GATHER_REF_COMPAT && mode != TP_GATHER_TEXTBOOK) return TP_E_ARG;
@@ -816,12 +860,12 @@ int tp_gpu_load_field_grid(tp_ctx *c, int nx, int ny, int nz, const double lo[3]
     free_field(c);
     memset(&c->f, 0, sizeof(c->f));
     /* whole slabs of the six components, at least one, at most the grid */
-    const size_t inner = order == TP_PLANES_X_FASTEST ? (size_t)nx * ny : (size_t)ny * nz;
-    const size_t outer = order == TP_PLANES_X_FASTEST ? (size_t)nz : (size_t)nx;
-    size_t rows = (stage_bytes ? stage_bytes : ((size_t)64 << 20)) / (6 * inner * sizeof(float));
+    const size_t inner = layout == LAYOUT_XS ? (size_t)ny * nz : (size_t)nx * ny;
+    const size_t outer = layout == LAYOUT_XS ? (size_t)nx : (size_t)nz;
+    size_t rows = (stage_bytes ? stage_bytes : ((size_t)32 << 20)) / (6 * inner * sizeof(float));
     if (rows < 1) rows = 1;
     if (rows > outer) rows = outer;
-    if (order == TP_PLANES_X_SLOWEST && rows > (size_t)65535 * 32) rows = (size_t)65535 * 32;
+    if (layout == LAYOUT_XS && rows > (size_t)65535 * 32) rows = (size_t)65535 * 32;
     if (hipMalloc(&c->grid, nn * 8 * sizeof(float)) != hipSuccess) {
         c->grid = nullptr;
         return TP_E_NOMEM;
@@ -832,14 +876,15 @@ int tp_gpu_load_field_grid(tp_ctx *c, int nx, int ny, int nz, const double lo[3]
         return TP_E_NOMEM;
     }
     Staging s;
-    int rc = staging_alloc(s, rows * 6 * inner);
+    int rc = staging_get(c, s, rows * 6 * inner);
+    s.rows = rows;
     if (rc == TP_OK)
-        rc = load_slice(c, s, c->grid, nx, ny, nz, order, e0, b0, rd, user, 0);
+        rc = load_slice(c, s, c->grid, nx, ny, nz, layout, e0, b0, rd, user, 0);
     if (rc == TP_OK && two_slices)
-        rc = load_slice(c, s, c->grid_b, nx, ny, nz, order, e0, b0, rd, user, 1);
-    /* every copy out of the slabs ends before they are freed */
+        rc = load_slice(c, s, c->grid_b, nx, ny, nz, layout, e0, b0, rd, user, 1);
+    /* every copy out of the slabs ends before the call returns (the next
+     * load refills them) */
     const hipError_t se = hipStreamSynchronize(c->stream);
-    staging_free(s);
     if (rc == TP_OK && se != hipSuccess) rc = TP_E_HIP;
     if (rc != TP_OK) {
         free_field(c);
@@ -860,6 +905,8 @@ int tp_gpu_load_field_grid(tp_ctx *c, int nx, int ny, int nz, const double lo[3]
     c->have_field = true;
     return TP_OK;
 }
+
+}  // namespace
 
 int tp_gpu_set_field_grid_planes(tp_ctx *c, int nx, int ny, int nz, const double lo[3],
                                  const double hi[3], int order, double e0, double b0,

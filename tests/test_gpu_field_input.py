@@ -243,7 +243,7 @@ def test_h5_file_equals_reference_loaders(gpu, tmp_path):
 
 @pytest.mark.parametrize("order", [abi.TP_PLANES_X_FASTEST, abi.TP_PLANES_X_SLOWEST])
 def test_large_grid(gpu, order):
-    """256 x 192 x 160 nodes (several 64 MiB slabs): the nodes equal the
+    """256 x 192 x 160 nodes (several 32 MiB slabs): the nodes equal the
     closed form."""
     nx, ny, nz = 256, 192, 160
     case(f"large grid order={order}")
