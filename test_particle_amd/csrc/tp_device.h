@@ -279,13 +279,20 @@ __device__ __forceinline__ void gc_load(const FieldDev &f, GridCacheN<NS> &gc, c
  * point beyond either end of the box uses the wrap cell n-1 -> 0.  FAST
  * (the FP32 forms only) multiplies by 1/d instead of dividing by d. */
 template <bool FAST>
-__device__ __forceinline__ void ref_axis(double x, double lo, double hi, double d, double inv_d,
-                                         int n, int &i1, int &i2, double &fr)
+__device__ __forceinline__ void ref_axis_in(double x, double lo, double d, double inv_d, int &i1,
+                                            int &i2, double &fr)
 {
     const double q = FAST ? (x - lo) * inv_d : (x - lo) / d;
     i1 = tp_floor_to_int(floor(q));
     i2 = i1 + 1;
     fr = q - i1;
+}
+
+/* the wrap cell, for a point outside [lo, hi] */
+template <bool FAST>
+__device__ __forceinline__ void ref_axis_out(double x, double lo, double hi, double d,
+                                             double inv_d, int n, int &i1, int &i2, double &fr)
+{
     if (x > hi) { i1 = n - 1; i2 = 0; fr = (FAST ? x * inv_d : x / d) - (n - 1); }
     if (x < lo) { i1 = n - 1; i2 = 0; fr = (FAST ? x * inv_d : x / d) + 1.0; }
 }
@@ -302,9 +309,17 @@ __device__ __forceinline__ void ref_cell_cached(const FieldDev &f, GridCacheN<NS
                                                 double &fy, double &fz)
 {
     int i1[3], i2[3];
-    ref_axis<FAST>(x, f.lo[0], f.hi[0], f.d[0], f.inv_d[0], f.nx, i1[0], i2[0], fx);
-    ref_axis<FAST>(y, f.lo[1], f.hi[1], f.d[1], f.inv_d[1], f.ny, i1[1], i2[1], fy);
-    ref_axis<FAST>(z, f.lo[2], f.hi[2], f.d[2], f.inv_d[2], f.nz, i1[2], i2[2], fz);
+    ref_axis_in<FAST>(x, f.lo[0], f.d[0], f.inv_d[0], i1[0], i2[0], fx);
+    ref_axis_in<FAST>(y, f.lo[1], f.d[1], f.inv_d[1], i1[1], i2[1], fy);
+    ref_axis_in<FAST>(z, f.lo[2], f.d[2], f.inv_d[2], i1[2], i2[2], fz);
+    /* points outside the box take the wrap cell; the branch is skipped by
+     * a wave none of whose points is outside (the usual case) */
+    if (x > f.hi[0] || x < f.lo[0] || y > f.hi[1] || y < f.lo[1] || z > f.hi[2] ||
+        z < f.lo[2]) {
+        ref_axis_out<FAST>(x, f.lo[0], f.hi[0], f.d[0], f.inv_d[0], f.nx, i1[0], i2[0], fx);
+        ref_axis_out<FAST>(y, f.lo[1], f.hi[1], f.d[1], f.inv_d[1], f.ny, i1[1], i2[1], fy);
+        ref_axis_out<FAST>(z, f.lo[2], f.hi[2], f.d[2], f.inv_d[2], f.nz, i1[2], i2[2], fz);
+    }
     const bool same = gc.valid && i1[0] == gc.key[0] && i2[0] == gc.key[1] &&
                       i1[1] == gc.key[2] && i2[1] == gc.key[3] && i1[2] == gc.key[4] &&
                       i2[2] == gc.key[5];

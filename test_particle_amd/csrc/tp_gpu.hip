@@ -36,6 +36,21 @@
  * step (or at estep); k_dcoeff then evaluates the D terms on the state
  * after that step.  The spectrum (any cadence) and the escape spectrum are
  * binned here. */
+/* The kernel's argument block read again from the kernarg segment (k_track's
+ * only argument, so at offset 0).  Values needed only after the step loop
+ * or in its rare branches (output pointers, spectrum settings) are loaded
+ * where they are used through this pointer, which the empty asm makes
+ * opaque, instead of occupying scalar registers through the loop (which
+ * spills them to VGPR lanes and reads them back with a VALU instruction
+ * each, every step). */
+typedef __attribute__((address_space(4))) const TrackArgs KArgs;
+__device__ __forceinline__ KArgs *kargs_again()
+{
+    KArgs *k = (KArgs *)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(k));
+    return k;
+}
+
 template <int PUSH, int FK, int PREC, bool TRAJ>
 __global__ void __launch_bounds__(256) k_track(TrackArgs a)
 {
@@ -103,18 +118,24 @@ __global__ void __launch_bounds__(256) k_track(TrackArgs a)
                 tpts++;
             }
         }
-        if (live && esc == 0 && esc_after == 1 && a.bc == 1 && ntp < a.nt_out)
-            atomicAdd(&a.espect_esc[ntp * a.nbins +
-                      energy_bin(p.ux, p.uy, p.uz, a.nbins, a.logemin, a.logde)], 1u);
+        if (live && esc == 0 && esc_after == 1 && a.bc == 1) {
+            KArgs *k = kargs_again();
+            if (ntp < k->nt_out)
+                atomicAdd(&k->espect_esc[ntp * k->nbins +
+                          energy_bin(p.ux, p.uy, p.uz, k->nbins, k->logemin, k->logde)], 1u);
+        }
         if (j % a.einterval == 0) {
-            if (live && esc_after == 0 && ntp < a.nt_out)
-                atomicAdd(&hist[energy_bin(p.ux, p.uy, p.uz, a.nbins, a.logemin, a.logde)], 1u);
+            KArgs *k = kargs_again();
+            const int nt_out = k->nt_out, nbins = k->nbins;
+            if (live && esc_after == 0 && ntp < nt_out)
+                atomicAdd(&hist[energy_bin(p.ux, p.uy, p.uz, nbins, k->logemin, k->logde)], 1u);
             __syncthreads();
-            if (ntp < a.nt_out) {
-                for (int b = tid; b < a.nbins; b += blockDim.x) {
+            if (ntp < nt_out) {
+                unsigned *espect = k->espect;
+                for (int b = tid; b < nbins; b += blockDim.x) {
                     unsigned h = hist[b];
                     if (h) {
-                        atomicAdd(&a.espect[ntp * a.nbins + b], h);
+                        atomicAdd(&espect[ntp * nbins + b], h);
                         hist[b] = 0u;
                     }
                 }
@@ -125,12 +146,13 @@ __global__ void __launch_bounds__(256) k_track(TrackArgs a)
         esc = esc_after;
     }
     if (live) {
-        a.x[i] = p.x; a.y[i] = p.y; a.z[i] = p.z;
-        a.ux[i] = p.ux; a.uy[i] = p.uy; a.uz[i] = p.uz; a.t[i] = p.t;
-        a.xs[i] = xs; a.ys[i] = ys; a.zs[i] = zs;
-        a.esc[i] = esc;
-        a.nsteps[i] = nst;
-        if (TRAJ) a.traj_pts[i] = tpts;
+        KArgs *k = kargs_again();
+        k->x[i] = p.x; k->y[i] = p.y; k->z[i] = p.z;
+        k->ux[i] = p.ux; k->uy[i] = p.uy; k->uz[i] = p.uz; k->t[i] = p.t;
+        k->xs[i] = xs; k->ys[i] = ys; k->zs[i] = zs;
+        k->esc[i] = esc;
+        k->nsteps[i] = nst;
+        if (TRAJ) k->traj_pts[i] = tpts;
     }
 }
 

@@ -50,13 +50,17 @@ static __device__ __forceinline__ void particle_bc(const TrackArgs &a, Ptl &p, d
                                             double &ys, double &zs, int &esc)
 {
     if (a.bc == 0) {
-        double xd = a.hi[0] - a.lo[0], yd = a.hi[1] - a.lo[1], zd = a.hi[2] - a.lo[2];
-        if (p.x > a.hi[0]) { p.x -= xd; xs += xd; }
-        if (p.y > a.hi[1]) { p.y -= yd; ys += yd; }
-        if (p.z > a.hi[2]) { p.z -= zd; zs += zd; }
-        if (p.x < a.lo[0]) { p.x += xd; xs -= xd; }
-        if (p.y < a.lo[1]) { p.y += yd; ys -= yd; }
-        if (p.z < a.lo[2]) { p.z += zd; zs -= zd; }
+        /* a wave none of whose particles left the box skips the wraps */
+        if (p.x > a.hi[0] || p.y > a.hi[1] || p.z > a.hi[2] || p.x < a.lo[0] ||
+            p.y < a.lo[1] || p.z < a.lo[2]) {
+            double xd = a.hi[0] - a.lo[0], yd = a.hi[1] - a.lo[1], zd = a.hi[2] - a.lo[2];
+            if (p.x > a.hi[0]) { p.x -= xd; xs += xd; }
+            if (p.y > a.hi[1]) { p.y -= yd; ys += yd; }
+            if (p.z > a.hi[2]) { p.z -= zd; zs += zd; }
+            if (p.x < a.lo[0]) { p.x += xd; xs -= xd; }
+            if (p.y < a.lo[1]) { p.y += yd; ys -= yd; }
+            if (p.z < a.lo[2]) { p.z += zd; zs -= zd; }
+        }
     } else if (a.bc == 1) {
         if (p.x > a.hi[0] || p.y > a.hi[1] || p.z > a.hi[2] ||
             p.x < a.lo[0] || p.y < a.lo[1] || p.z < a.lo[2])
