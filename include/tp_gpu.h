@@ -139,6 +139,19 @@ int tp_gpu_set_field_grid2(tp_ctx *ctx, int nx, int ny, int nz, const double lo[
                            const float *bfield_next, const float *efield_prev,
                            const float *bfield_prev, double grid_dt, int gather_mode);
 
+/* Cell-major copy of a PIC grid.  Besides the nodes, the library can keep
+ * every cell's eight corners (the reference gather's corners, six floats
+ * each: 192 bytes per cell, 6x the nodes) so that a particle entering a cell
+ * reads one contiguous record; the gathered values and all results are the
+ * same either way.  TP_CELLS_AUTO (the default) builds the copy when it
+ * takes at most half of the free device memory; TP_CELLS_OFF never;
+ * TP_CELLS_ON always (TP_E_NOMEM if it does not fit).  The mode applies to
+ * the grids set after the call. */
+#define TP_CELLS_AUTO 0
+#define TP_CELLS_OFF  1
+#define TP_CELLS_ON   2
+int tp_gpu_set_grid_cells(tp_ctx *ctx, int mode);
+
 /* --- PIC field input pipeline ------------------------------------------
  * The reference's loaders read one component at a time as a plane of floats
  * and scatter it, scaled, into the float3 node arrays on the host

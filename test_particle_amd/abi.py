@@ -28,6 +28,11 @@ TP_E_NONFINITE = -4
 TP_E_STATE = -5
 TP_E_NODEV = -6
 
+# cell-major copy of PIC grids (tp_gpu_set_grid_cells)
+TP_CELLS_AUTO = 0
+TP_CELLS_OFF = 1
+TP_CELLS_ON = 2
+
 # plane orders of the PIC field input (tp_gpu_load_field_grid)
 TP_PLANES_X_FASTEST = 0
 TP_PLANES_X_SLOWEST = 1

@@ -34,6 +34,8 @@ struct tp_ctx {
     bool have_field = false;
     /* owned field buffers */
     float4 *grid = nullptr, *grid_b = nullptr;
+    float4 *cells = nullptr, *cells_b = nullptr;   /* cell-major copies */
+    int cells_mode = TP_CELLS_AUTO;
     double *vgrid = nullptr, *modes = nullptr;
     float *modes32 = nullptr;
     WlcsSys *wl = nullptr;

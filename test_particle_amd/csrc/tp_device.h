@@ -73,6 +73,12 @@ struct FieldDev {
      * time between slices */
     const float4 *grid_b;
     double gdt;
+    /* cell-major copies of grid and grid_b (null when not built): cell
+     * (i, j, k) = the reference gather's eight corners (i1 = i, i2 = i + 1,
+     * or 0 after the last node) in its corner order, six floats each, 12
+     * float4 per cell, cells indexed like the nodes.  A cell change reads one
+     * contiguous 192-byte record instead of eight nodes. */
+    const float4 *cells, *cells_b;
     /* modes: rows kx ky kz ax ay az phi */
     int nmodes;
     const double *modes;
@@ -329,6 +335,26 @@ __device__ __forceinline__ void ref_cell_cached(const FieldDev &f, GridCacheN<NS
         const bool inside = (unsigned)i1[0] < (unsigned)f.nx && (unsigned)i2[0] < (unsigned)f.nx &&
                             (unsigned)i1[1] < (unsigned)f.ny && (unsigned)i2[1] < (unsigned)f.ny &&
                             (unsigned)i1[2] < (unsigned)f.nz && (unsigned)i2[2] < (unsigned)f.nz;
+        if (inside && f.cells) {
+            /* inside the grid the lower corner fixes the cell (i2 = i1 + 1,
+             * or 0 for the wrap cell): one record */
+            const unsigned cid = (unsigned)i1[0] +
+                                 (unsigned)f.nx * ((unsigned)i1[1] + (unsigned)f.ny * (unsigned)i1[2]);
+#pragma unroll
+            for (int sl = 0; sl < NS; sl++) {
+                const float4 *r = (sl == 0 ? f.cells : f.cells_b) + 12 * (size_t)cid;
+#pragma unroll
+                for (int q = 0; q < 12; q++) {
+                    const float4 v = r[q];
+                    float *o = gc.v + 48 * sl + 4 * q;
+                    o[0] = v.x; o[1] = v.y; o[2] = v.z; o[3] = v.w;
+                }
+            }
+            gc.key[0] = i1[0]; gc.key[1] = i2[0]; gc.key[2] = i1[1];
+            gc.key[3] = i2[1]; gc.key[4] = i1[2]; gc.key[5] = i2[2];
+            gc.valid = true;
+            return;
+        }
         if (inside) {
             const unsigned sx = (unsigned)f.nx, sxy = (unsigned)f.nx * (unsigned)f.ny;
             const unsigned X1 = i1[0], X2 = i2[0];

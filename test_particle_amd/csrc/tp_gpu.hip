@@ -484,6 +484,8 @@ static void free_field(tp_ctx *c)
 {
     hipFree(c->grid); c->grid = nullptr;
     hipFree(c->grid_b); c->grid_b = nullptr;
+    hipFree(c->cells); c->cells = nullptr;
+    hipFree(c->cells_b); c->cells_b = nullptr;
     hipFree(c->vgrid); c->vgrid = nullptr;
     hipFree(c->modes); c->modes = nullptr;
     hipFree(c->modes32); c->modes32 = nullptr;
@@ -696,6 +698,78 @@ __global__ void __launch_bounds__(256) k_nodes_aos(const float *__restrict__ src
  * reference's float3 node arrays (tp_gpu_set_field_grid / _grid2), read
  * as component 0 (E, 3 floats per node) and 1 (B) */
 enum { LAYOUT_XF = TP_PLANES_X_FASTEST, LAYOUT_XS = TP_PLANES_X_SLOWEST, LAYOUT_AOS = 2 };
+
+/* cell (i, j, k) of the cell-major copy: the eight corners of the reference
+ * gather's cell with lower corner (i, j, k), in its corner order (0: x1 y1 z1,
+ * 1: x1 y1 z2, 2: x1 y2 z1, 3: x2 y1 z1, 4: x2 y2 z1, 5: x2 y1 z2,
+ * 6: x1 y2 z2, 7: x2 y2 z2), upper corner i + 1 or 0 after the last node;
+ * six floats per corner, written as 12 float4. */
+__global__ void __launch_bounds__(256) k_build_cells(const float4 *__restrict__ grid, int nx,
+                                                     int ny, int nz, float4 *__restrict__ cells)
+{
+    const size_t nn = (size_t)nx * ny * nz;
+    for (size_t cid = (size_t)blockIdx.x * 256 + threadIdx.x; cid < nn;
+         cid += (size_t)gridDim.x * 256) {
+        const unsigned i = (unsigned)(cid % (unsigned)nx);
+        const unsigned j = (unsigned)((cid / (unsigned)nx) % (unsigned)ny);
+        const unsigned k = (unsigned)(cid / ((size_t)nx * ny));
+        const unsigned X1 = i, X2 = i + 1 < (unsigned)nx ? i + 1 : 0u;
+        const unsigned sx = (unsigned)nx, sxy = (unsigned)nx * (unsigned)ny;
+        const unsigned Y1 = sx * j, Y2 = sx * (j + 1 < (unsigned)ny ? j + 1 : 0u);
+        const unsigned Z1 = sxy * k, Z2 = sxy * (k + 1 < (unsigned)nz ? k + 1 : 0u);
+        const unsigned c[8] = {X1 + Y1 + Z1, X1 + Y1 + Z2, X1 + Y2 + Z1, X2 + Y1 + Z1,
+                               X2 + Y2 + Z1, X2 + Y1 + Z2, X1 + Y2 + Z2, X2 + Y2 + Z2};
+        float v[48];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const float4 a = grid[2 * (size_t)c[q]];
+            const float4 b = grid[2 * (size_t)c[q] + 1];
+            v[6 * q + 0] = a.x; v[6 * q + 1] = a.y; v[6 * q + 2] = a.z;
+            v[6 * q + 3] = a.w; v[6 * q + 4] = b.x; v[6 * q + 5] = b.y;
+        }
+        float4 *o = cells + 12 * cid;
+#pragma unroll
+        for (int q = 0; q < 12; q++)
+            o[q] = make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+    }
+}
+
+/* the cell-major copies of the context's grid(s), by its cells mode */
+static int build_cells(tp_ctx *c)
+{
+    if (c->cells_mode == TP_CELLS_OFF) return TP_OK;
+    const size_t nn = (size_t)c->f.nx * c->f.ny * c->f.nz;
+    const int ns = c->grid_b ? 2 : 1;
+    const size_t bytes = nn * 12 * sizeof(float4);
+    if (c->cells_mode == TP_CELLS_AUTO) {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess || ns * bytes > fr / 2) return TP_OK;
+    }
+    for (int s = 0; s < ns; s++) {
+        float4 **dst = s == 0 ? &c->cells : &c->cells_b;
+        if (hipMalloc((void **)dst, bytes) != hipSuccess) {
+            *dst = nullptr;
+            hipFree(c->cells); c->cells = nullptr;
+            return c->cells_mode == TP_CELLS_ON ? TP_E_NOMEM : TP_OK;
+        }
+        const size_t want = (nn + 255) / 256;
+        const unsigned nb = (unsigned)(want < 65536 ? want : 65536);
+        hipLaunchKernelGGL(k_build_cells, dim3(nb), dim3(256), 0, c->stream,
+                           s == 0 ? c->grid : c->grid_b, c->f.nx, c->f.ny, c->f.nz, *dst);
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->f.cells = c->cells;
+    c->f.cells_b = c->cells_b;
+    return TP_OK;
+}
+
+int tp_gpu_set_grid_cells(tp_ctx *c, int mode)
+{
+    if (!c || mode < TP_CELLS_AUTO || mode > TP_CELLS_ON) return TP_E_ARG;
+    c->cells_mode = mode;
+    return TP_OK;
+}
 
 namespace {
 /* the context's two staging slabs for one load */
@@ -923,6 +997,16 @@ int load_grid(tp_ctx *c, int nx, int ny, int nz, const double lo[3], const doubl
         c->f.kind = mode == TP_GATHER_REF_COMPAT ? FK_GRID2_REF : FK_GRID2_TB;
     } else {
         c->f.kind = mode == TP_GATHER_REF_COMPAT ? FK_GRID_REF : FK_GRID_TB;
+    }
+    /* the reference gather reads cell records (the textbook gather's corner
+     * order differs; it keeps reading nodes) */
+    if (mode == TP_GATHER_REF_COMPAT) {
+        rc = build_cells(c);
+        if (rc != TP_OK) {
+            free_field(c);
+            memset(&c->f, 0, sizeof(c->f));
+            return rc;
+        }
     }
     c->have_field = true;
     return TP_OK;

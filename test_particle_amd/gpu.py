@@ -49,6 +49,7 @@ def lib():
         L.tp_gpu_set_field_grid_planes.argtypes = [vp, i, i, i, P(d), P(d), i, d, d,
                                                    P(ctypes.c_float), P(ctypes.c_float), d, i]
         L.tp_gpu_get_field_nodes.argtypes = [vp, i, i, i, i, P(ctypes.c_float)]
+        L.tp_gpu_set_grid_cells.argtypes = [vp, i]
         L.tp_gpu_eval_field.argtypes = [vp, i, P(d), P(d), i]
         L.tp_gpu_track_fixed.argtypes = [vp, P(abi.TpTrackParams), i, P(d), P(d), P(i),
                                          P(d), P(d), P(d), P(i)]
@@ -169,6 +170,11 @@ class Tracker:
         _check(lib().tp_gpu_set_field_grid2(self._ctx, nx, ny, nz, _vec3(lo), _vec3(hi),
                                             *(fptr(a) for a in arrs), float(grid_dt), mode),
                "set_field_grid2")
+
+    def set_grid_cells(self, mode):
+        """Cell-major copy of the PIC grids set after this call
+        (tp_gpu_set_grid_cells): abi.TP_CELLS_AUTO, _OFF or _ON."""
+        _check(lib().tp_gpu_set_grid_cells(self._ctx, int(mode)), "set_grid_cells")
 
     def set_field_grid_planes(self, planes_next, lo, hi, order=abi.TP_PLANES_X_FASTEST,
                               e0=1.0, b0=1.0, planes_prev=None, grid_dt=0.0,
