@@ -204,6 +204,9 @@ __global__ void __launch_bounds__(256) k_dcoeff(TrackArgs a, int tstep)
             dpp = (ax * ax + ay * ay + az * az) / t;
             double g = sqrt(1.0 + ux * ux + uy * uy + uz * uz);
             q = g - a.g0[i]; dee = q * q / t;
+            /* the uncached node gather: one evaluation per particle, so the
+             * nodes (which the last-level cache holds more of than the
+             * 6x larger cell records) serve it best */
             Emf e;
             get_emf64<FK>(a.f, x, y, z, tt, e);
             double ab = sqrt(e.bx * e.bx + e.by * e.by + e.bz * e.bz);

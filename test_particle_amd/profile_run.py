@@ -85,6 +85,9 @@ def main():
     ap.add_argument("--ncross", type=int, default=2, help="Poincare crossings per line")
     ap.add_argument("--chunk", type=int, default=0, help="Poincare RK4 steps per launch")
     ap.add_argument("--h5", default="", help="fieldload: also through an HDF5 file in DIR")
+    ap.add_argument("--two", action="store_true", help="grid: two time slices (dt 20)")
+    ap.add_argument("--cells", default="auto", choices=["auto", "off", "on"],
+                    help="grid: cell-major copy of the grid")
     a = ap.parse_args()
     if a.what == "copy":
         import torch
@@ -177,8 +180,14 @@ def main():
     pusher = abi.TP_PUSH_BORIS if a.pusher == "boris" else abi.TP_PUSH_WIRZ
     prec = abi.TP_FP32_MIXED if a.fp32 else abi.TP_FP64
     if a.what == "grid":
+        tr.set_grid_cells({"auto": abi.TP_CELLS_AUTO, "off": abi.TP_CELLS_OFF,
+                           "on": abi.TP_CELLS_ON}[a.cells])
         e, b, lo, hi = pic_turbulence_grid(256)
-        tr.set_field_grid(e, b, lo, hi, abi.TP_GATHER_REF_COMPAT)
+        if a.two:
+            e0, b0, _, _ = pic_turbulence_grid(256, seed=20261203)
+            tr.set_field_grid2(e, b, e0, b0, lo, hi, 20.0)
+        else:
+            tr.set_field_grid(e, b, lo, hi, abi.TP_GATHER_REF_COMPAT)
         dt = 0.01 * gyroperiod(0.7, 1.0)
     else:
         lo, hi = (0.0, 0.0, 0.0), (2 * np.pi,) * 3
