@@ -51,6 +51,85 @@ __device__ __forceinline__ KArgs *kargs_again()
     return k;
 }
 
+/* ------------------------------------------------------------------ */
+/* diffusion terms at a D step (calc_diff_coeff, src/diagnostics.c:659-705,
+ * summed as src/tracking.c:347-369 does)                              */
+/* ------------------------------------------------------------------ */
+/* Particles inside the box get fresh terms; escaped ones contribute
+ * drr = dpp = duu = 0 and their last (stale) dxx dyy dzz daa dee, as the
+ * reference's loop-private variables do (src/tracking.c:334-336).  The
+ * initial-particle terms gamma_init, mu_init, theta_init were evaluated by
+ * k_load with the same operations.  Each block writes one row of 8 partial
+ * sums (fixed-order wave and LDS reduction); escaped particles decrement
+ * the remaining count of the next row.  The step kernel evaluates them at
+ * the end of a launch that ends at a D step (particle g of the launch is
+ * particle g, so the blocks and their sums are those of k_dcoeff, which
+ * runs instead after launches over an index list). */
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+/* the eight terms of particle i in state (x .. zs) with field e at it (read
+ * only when escaped == 0); updates its stale terms */
+__device__ __forceinline__ void dterms_particle(KArgs *k, int i, int escaped, double x,
+                                                double y, double z, double ux, double uy,
+                                                double uz, double tt, double xs, double ys,
+                                                double zs, const Emf &e, double v[8])
+{
+    const int n = k->n;
+    double *st = k->st;
+    double dxx = st[i], dyy = st[n + i], dzz = st[2 * n + i];
+    double daa = st[3 * n + i], dee = st[4 * n + i];
+    double drr = 0.0, dpp = 0.0, duu = 0.0;
+    if (k->fix_stale) dxx = dyy = dzz = daa = dee = 0.0;
+    if (escaped == 0) {
+        double t = tt * 2 + k->dt;
+        double q;
+        q = k->x0[i] - (x + xs); dxx = q * q / t;
+        q = k->y0[i] - (y + ys); dyy = q * q / t;
+        q = k->z0[i] - (z + zs); dzz = q * q / t;
+        drr = dxx + dyy + dzz;
+        double ax = k->ux0[i] - ux, ay = k->uy0[i] - uy, az = k->uz0[i] - uz;
+        dpp = (ax * ax + ay * ay + az * az) / t;
+        double g = sqrt(1.0 + ux * ux + uy * uy + uz * uz);
+        q = g - k->g0[i]; dee = q * q / t;
+        double ab = sqrt(e.bx * e.bx + e.by * e.by + e.bz * e.bz);
+        double au = sqrt(ux * ux + uy * uy + uz * uz);
+        double mu = (e.bx * ux + e.by * uy + e.bz * uz) / (ab * au);
+        q = mu - k->mu0[i]; duu = q * q / t;
+        double th = acos(mu) * 180 / TP_PI;
+        q = th - k->th0[i]; daa = q * q / t;
+    }
+    st[i] = dxx; st[n + i] = dyy; st[2 * n + i] = dzz;
+    st[3 * n + i] = daa; st[4 * n + i] = dee;
+    v[0] = drr; v[1] = dxx; v[2] = dyy; v[3] = dzz;
+    v[4] = dpp; v[5] = duu; v[6] = daa; v[7] = dee;
+}
+
+/* the block's row of partial sums and its escape count */
+__device__ __forceinline__ void dterms_reduce(KArgs *k, const double v[8], bool live,
+                                              int escaped, int tid, int tstep,
+                                              double (*wsum)[8])
+{
+    const int lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        double r = wave_sum(v[q]);
+        if (lane == 0) wsum[wave][q] = r;
+    }
+    unsigned long long m = __ballot(live && escaped);
+    if (lane == 0 && m && tstep + 1 < k->nd) atomicAdd(&k->rdec[tstep + 1], __popcll(m));
+    __syncthreads();
+    if (tid < 8) {
+        double r = 0.0;
+        for (int w = 0; w < nwaves; w++) r += wsum[w][tid];
+        k->partials[(size_t)blockIdx.x * 8 + tid] = r;
+    }
+}
+
 template <int PUSH, int FK, int PREC, bool TRAJ>
 __global__ void __launch_bounds__(256) k_track(TrackArgs a)
 {
@@ -145,6 +224,21 @@ __global__ void __launch_bounds__(256) k_track(TrackArgs a)
         }
         esc = esc_after;
     }
+    if (a.dstep >= 0) {
+        /* the D terms of the state after step j1 (a D step) */
+        __shared__ double wsum[4][8];
+        KArgs *k = kargs_again();
+        double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (live) {
+            Emf e = {};
+            if (esc == 0) {
+                if constexpr (PREC == PREC64) emf64_step<FK>(a.f, gc, p.x, p.y, p.z, p.t, e);
+                else get_emf64<FK>(a.f, p.x, p.y, p.z, p.t, e);
+            }
+            dterms_particle(k, i, esc, p.x, p.y, p.z, p.ux, p.uy, p.uz, p.t, xs, ys, zs, e, v);
+        }
+        dterms_reduce(k, v, live, live ? esc : 0, tid, a.dstep, wsum);
+    }
     if (live) {
         KArgs *k = kargs_again();
         k->x[i] = p.x; k->y[i] = p.y; k->z[i] = p.z;
@@ -156,84 +250,30 @@ __global__ void __launch_bounds__(256) k_track(TrackArgs a)
     }
 }
 
-/* ------------------------------------------------------------------ */
-/* diffusion terms at a D step (calc_diff_coeff, src/diagnostics.c:659-705,
- * summed as src/tracking.c:347-369 does)                              */
-/* ------------------------------------------------------------------ */
-/* Particles inside the box get fresh terms; escaped ones contribute
- * drr = dpp = duu = 0 and their last (stale) dxx dyy dzz daa dee, as the
- * reference's loop-private variables do (src/tracking.c:334-336).  The
- * initial-particle terms gamma_init, mu_init, theta_init were evaluated by
- * k_load with the same operations.  Each block writes one row of 8 partial
- * sums (fixed-order wave and LDS reduction); escaped particles decrement
- * the remaining count of the next row. */
-__device__ __forceinline__ double wave_sum(double v)
-{
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
 template <int FK>
 __global__ void __launch_bounds__(256) k_dcoeff(TrackArgs a, int tstep)
 {
     __shared__ double wsum[4][8];
     const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
     const int i = blockIdx.x * blockDim.x + tid;
     const bool live = i < a.n;
     double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int escaped = 0;
+    KArgs *k = kargs_again();
     if (live) {
-        const int n = a.n;
-        double dxx = a.st[i], dyy = a.st[n + i], dzz = a.st[2 * n + i];
-        double daa = a.st[3 * n + i], dee = a.st[4 * n + i];
-        double drr = 0.0, dpp = 0.0, duu = 0.0;
-        if (a.fix_stale) dxx = dyy = dzz = daa = dee = 0.0;
         escaped = a.esc[i];
+        Emf e = {};
+        double x = 0.0, y = 0.0, z = 0.0, ux = 0.0, uy = 0.0, uz = 0.0, tt = 0.0;
+        double xs = 0.0, ys = 0.0, zs = 0.0;
         if (escaped == 0) {
-            double x = a.x[i], y = a.y[i], z = a.z[i];
-            double ux = a.ux[i], uy = a.uy[i], uz = a.uz[i], tt = a.t[i];
-            double t = tt * 2 + a.dt;
-            double q;
-            q = a.x0[i] - (x + a.xs[i]); dxx = q * q / t;
-            q = a.y0[i] - (y + a.ys[i]); dyy = q * q / t;
-            q = a.z0[i] - (z + a.zs[i]); dzz = q * q / t;
-            drr = dxx + dyy + dzz;
-            double ax = a.ux0[i] - ux, ay = a.uy0[i] - uy, az = a.uz0[i] - uz;
-            dpp = (ax * ax + ay * ay + az * az) / t;
-            double g = sqrt(1.0 + ux * ux + uy * uy + uz * uz);
-            q = g - a.g0[i]; dee = q * q / t;
-            /* the uncached node gather: one evaluation per particle, so the
-             * nodes (which the last-level cache holds more of than the
-             * 6x larger cell records) serve it best */
-            Emf e;
+            x = a.x[i]; y = a.y[i]; z = a.z[i];
+            ux = a.ux[i]; uy = a.uy[i]; uz = a.uz[i]; tt = a.t[i];
+            xs = a.xs[i]; ys = a.ys[i]; zs = a.zs[i];
             get_emf64<FK>(a.f, x, y, z, tt, e);
-            double ab = sqrt(e.bx * e.bx + e.by * e.by + e.bz * e.bz);
-            double au = sqrt(ux * ux + uy * uy + uz * uz);
-            double mu = (e.bx * ux + e.by * uy + e.bz * uz) / (ab * au);
-            q = mu - a.mu0[i]; duu = q * q / t;
-            double th = acos(mu) * 180 / TP_PI;
-            q = th - a.th0[i]; daa = q * q / t;
         }
-        a.st[i] = dxx; a.st[n + i] = dyy; a.st[2 * n + i] = dzz;
-        a.st[3 * n + i] = daa; a.st[4 * n + i] = dee;
-        v[0] = drr; v[1] = dxx; v[2] = dyy; v[3] = dzz;
-        v[4] = dpp; v[5] = duu; v[6] = daa; v[7] = dee;
+        dterms_particle(k, i, escaped, x, y, z, ux, uy, uz, tt, xs, ys, zs, e, v);
     }
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-        double r = wave_sum(v[k]);
-        if (lane == 0) wsum[wave][k] = r;
-    }
-    unsigned long long m = __ballot(live && escaped);
-    if (lane == 0 && m && tstep + 1 < a.nd) atomicAdd(&a.rdec[tstep + 1], __popcll(m));
-    __syncthreads();
-    if (tid < 8) {
-        double r = 0.0;
-        for (int w = 0; w < nwaves; w++) r += wsum[w][tid];
-        a.partials[(size_t)blockIdx.x * 8 + tid] = r;
-    }
+    dterms_reduce(k, v, live, escaped, tid, tstep, wsum);
 }
 
 /* Sum the per-block partials of one D row in a fixed order: one block per
@@ -1351,6 +1391,9 @@ int tp_gpu_run_fixed(tp_ctx *c, const tp_track_params *p)
         a.j0 = (int)j0; a.j1 = (int)j1;
         a.first = (j0 == p->sstep) ? 1 : 0;
         a.ntp0 = ntp;
+        /* the D terms in the step kernel, unless it runs over index lists */
+        const bool dstep = j1 % di == 0;
+        a.dstep = (dstep && !mlp) ? tstep : -1;
         HIPCHK(hipEventRecord(c->evs[2 * launches], c->stream));
         if (mlp) {
             /* residual checks before the first launch and at launches that
@@ -1367,9 +1410,11 @@ int tp_gpu_run_fixed(tp_ctx *c, const tp_track_params *p)
         launches++;
         for (long j = j0; j <= j1; j++)
             if (j % p->einterval_t == 0) ntp++;
-        if (j1 % di == 0) {
-            dispatch_dcoeff(field_kind(c), dim3(nblocks), dim3(bs), c->stream, a, tstep);
-            HIPCHK(hipGetLastError());
+        if (dstep) {
+            if (a.dstep < 0) {
+                dispatch_dcoeff(field_kind(c), dim3(nblocks), dim3(bs), c->stream, a, tstep);
+                HIPCHK(hipGetLastError());
+            }
             hipLaunchKernelGGL(k_reduce_row, dim3(8), dim3(256), 0, c->stream, c->partials,
                                nblocks, tstep, nd, c->dco);
             HIPCHK(hipGetLastError());
