@@ -508,6 +508,41 @@ __device__ __forceinline__ void grid_emf32_cached(const FieldDev &f, GridCacheN<
     o.ex = a0; o.ey = a1; o.ez = a2; o.bx = a3; o.by = a4; o.bz = a5;
 }
 
+/* two time slices in the FP32 form: each slice summed over the corners in
+ * float, blended dt * next + (1 - dt) * previous in float (the FP64 form
+ * sums and blends in double, as the reference does) */
+template <int FK>
+__device__ __forceinline__ void grid2_emf32_cached(const FieldDev &f, GridCacheN<2> &gc, double x,
+                                                   double y, double z, double t, EmfF &o)
+{
+    float w[8];
+    if (FK == FK_GRID2_REF) {
+        double fx, fy, fz;
+        ref_cell_cached<true>(f, gc, x, y, z, fx, fy, fz);
+        ref_weights<float>((float)fx, (float)fy, (float)fz, w);
+    } else {
+        unsigned c[8];
+        double wd[8];
+        trilinear_tb(f.nx, f.ny, f.nz, f.lo, f.hi, f.d, x, y, z, c, wd);
+        gc_fill(f, gc, c, false);
+#pragma unroll
+        for (int k = 0; k < 8; k++) w[k] = (float)wd[k];
+    }
+    float a[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, b[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+#pragma unroll
+        for (int m = 0; m < 6; m++) {
+            a[m] = fmaf(gc.v[6 * k + m], w[k], a[m]);
+            b[m] = fmaf(gc.v[48 + 6 * k + m], w[k], b[m]);
+        }
+    const double r = t / f.gdt;
+    const float dtf = (float)(r - floor(r));
+    const float dtg = 1.0f - dtf;
+    o.ex = dtf * a[0] + dtg * b[0]; o.ey = dtf * a[1] + dtg * b[1]; o.ez = dtf * a[2] + dtg * b[2];
+    o.bx = dtf * a[3] + dtg * b[3]; o.by = dtf * a[4] + dtg * b[4]; o.bz = dtf * a[5] + dtg * b[5];
+}
+
 /* ------------------------------------------------------------------ */
 /* WLCS (src/wlcs.c:281-449, 814-903)                                  */
 /* ------------------------------------------------------------------ */
@@ -873,6 +908,31 @@ __device__ __forceinline__ void get_emf32(const FieldDev &f, double x, double y,
         }
         o.ex = ax; o.ey = ay; o.ez = az;
         o.bx = bx; o.by = by; o.bz = bz;
+    } else if (FK == FK_GRID2_TB || FK == FK_GRID2_REF) {
+        /* both slices in float, blended in float */
+        unsigned c[8];
+        double w[8];
+        if (FK == FK_GRID2_REF)
+            trilinear_ref(f.nx, f.ny, f.nz, f.lo, f.hi, f.d, x, y, z, c, w);
+        else
+            trilinear_tb(f.nx, f.ny, f.nz, f.lo, f.hi, f.d, x, y, z, c, w);
+        float a[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, b[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const float wk = (float)w[k];
+            const float4 p = f.grid[2 * c[k]], q = f.grid[2 * c[k] + 1];
+            const float4 pb = f.grid_b[2 * c[k]], qb = f.grid_b[2 * c[k] + 1];
+            a[0] = fmaf(p.x, wk, a[0]); a[1] = fmaf(p.y, wk, a[1]); a[2] = fmaf(p.z, wk, a[2]);
+            a[3] = fmaf(p.w, wk, a[3]); a[4] = fmaf(q.x, wk, a[4]); a[5] = fmaf(q.y, wk, a[5]);
+            b[0] = fmaf(pb.x, wk, b[0]); b[1] = fmaf(pb.y, wk, b[1]); b[2] = fmaf(pb.z, wk, b[2]);
+            b[3] = fmaf(pb.w, wk, b[3]); b[4] = fmaf(qb.x, wk, b[4]); b[5] = fmaf(qb.y, wk, b[5]);
+        }
+        const double r = t / f.gdt;
+        const float dtf = (float)(r - floor(r));
+        const float dtg = 1.0f - dtf;
+        o.ex = dtf * a[0] + dtg * b[0]; o.ey = dtf * a[1] + dtg * b[1];
+        o.ez = dtf * a[2] + dtg * b[2]; o.bx = dtf * a[3] + dtg * b[3];
+        o.by = dtf * a[4] + dtg * b[4]; o.bz = dtf * a[5] + dtg * b[5];
     } else if (FK == FK_MODES || FK == FK_MLP) {
         float bx, by, bz;
         modes_sum32(f.modes32, f.nmodes, (float)x, (float)y, (float)z, bx, by, bz);
@@ -906,10 +966,7 @@ __device__ __forceinline__ void emf32_step(const FieldDev &f, GC &gc, double x,
     if constexpr (FK == FK_GRID_REF || FK == FK_GRID_TB) {
         grid_emf32_cached<FK>(f, gc, x, y, z, o);
     } else if constexpr (FK == FK_GRID2_REF || FK == FK_GRID2_TB) {
-        Emf e;
-        grid2_emf64_cached<FK>(f, gc, x, y, z, t, e);
-        o.bx = (float)e.bx; o.by = (float)e.by; o.bz = (float)e.bz;
-        o.ex = (float)e.ex; o.ey = (float)e.ey; o.ez = (float)e.ez;
+        grid2_emf32_cached<FK>(f, gc, x, y, z, t, o);
     } else {
         get_emf32<FK>(f, x, y, z, t, o);
     }

@@ -131,7 +131,7 @@ __device__ __forceinline__ void dterms_reduce(KArgs *k, const double v[8], bool 
 }
 
 template <int PUSH, int FK, int PREC, bool TRAJ>
-__global__ void __launch_bounds__(256) k_track(TrackArgs a)
+__device__ __forceinline__ void track_body(const TrackArgs &a)
 {
     extern __shared__ unsigned hist[];
     const int tid = threadIdx.x;
@@ -248,6 +248,21 @@ __global__ void __launch_bounds__(256) k_track(TrackArgs a)
         k->nsteps[i] = nst;
         if (TRAJ) k->traj_pts[i] = tpts;
     }
+}
+
+template <int PUSH, int FK, int PREC, bool TRAJ>
+__global__ void __launch_bounds__(256) k_track(TrackArgs a)
+{
+    track_body<PUSH, FK, PREC, TRAJ>(a);
+}
+
+/* the same compiled for 3 waves per SIMD (168 VGPRs), which the FP32
+ * two-slice gather fits without spilling (the compiler's own choice is 2) */
+template <int PUSH, int FK, int PREC, bool TRAJ>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+k_track_w3(TrackArgs a)
+{
+    track_body<PUSH, FK, PREC, TRAJ>(a);
 }
 
 template <int FK>
@@ -368,7 +383,10 @@ __global__ void __launch_bounds__(256) k_eval(FieldDev f, int n, const double *p
 template <int PUSH, int FK, int PREC, bool TRAJ>
 static void launch_track(dim3 g, dim3 b, size_t sh, hipStream_t s, const TrackArgs &a)
 {
-    hipLaunchKernelGGL((k_track<PUSH, FK, PREC, TRAJ>), g, b, sh, s, a);
+    if constexpr (PREC == PREC32 && (FK == FK_GRID2_REF || FK == FK_GRID2_TB))
+        hipLaunchKernelGGL((k_track_w3<PUSH, FK, PREC, TRAJ>), g, b, sh, s, a);
+    else
+        hipLaunchKernelGGL((k_track<PUSH, FK, PREC, TRAJ>), g, b, sh, s, a);
 }
 
 /* trajectory recording is a separate instantiation, so the kernels without

@@ -376,6 +376,47 @@ def test_field_grid_two_time_slices(gpu, oracle_tp, mode):
         bitwise(out, g["out"], "two-slice fields vs reference")
 
 
+@pytest.mark.parametrize("mode", [abi.TP_GATHER_REF_COMPAT, abi.TP_GATHER_TEXTBOOK])
+def test_field_grid_two_time_slices_fp32(gpu, mode):
+    """The FP32 form of the two-slice gather (float sums and blend) within
+    float rounding of the FP64 form: 2e-6 of the field's largest component
+    (eight float products, a float blend)."""
+    from conftest import golden
+    g = golden("fields_grid2")
+    print(f"[case] two-slice grid fields FP32 mode {mode}", flush=True)
+    gpu.set_field_grid2(g["ea"], g["ba"], g["eb"], g["bb"], g["lo"], g["hi"],
+                        float(g["grid_dt"]), mode)
+    f64 = gpu.eval_field(g["pts"])
+    f32 = gpu.eval_field(g["pts"], precision=abi.TP_FP32_MIXED)
+    scale = np.abs(f64).max()
+    assert np.abs(f32 - f64).max() < 2e-6 * scale
+    assert not np.array_equal(f32, f64)      # the float path did run
+
+
+@pytest.mark.parametrize("cells", [abi.TP_CELLS_OFF, abi.TP_CELLS_ON])
+def test_track_grid_two_time_slices_fp32(gpu, cells):
+    """The FP32 Boris form through the two-slice grid stays within float
+    accuracy of the FP64 form over 200 steps (orbits in a smooth field, no
+    chaos over this horizon): positions within 1e-4 of the box, momenta
+    within 1e-4 relative."""
+    from conftest import golden
+    g = golden("track_grid2")
+    print(f"[case] two-slice grid tracking FP32 cells={cells}", flush=True)
+    gpu.set_grid_cells(cells)
+    gpu.set_field_grid2(g["ea"], g["ba"], g["eb"], g["bb"], g["lo"], g["hi"],
+                        float(g["grid_dt"]))
+    gpu.set_grid_cells(abi.TP_CELLS_AUTO)
+    kw = dict(lo=g["lo"], hi=g["hi"], dt=float(g["dt"]), nbins=50, emin=1e-3, emax=1.0)
+    r64 = gpu.track_fixed(_params(200, 2, abi.TP_PUSH_BORIS, **kw), g["ptl_in"])
+    r32 = gpu.track_fixed(_params(200, 2, abi.TP_PUSH_BORIS, precision=abi.TP_FP32_MIXED,
+                                  **kw), g["ptl_in"])
+    box = np.max(np.asarray(g["hi"]) - np.asarray(g["lo"]))
+    dx = np.abs(r32["ptl"][:, 0:3] - r64["ptl"][:, 0:3]).max()
+    u64 = r64["ptl"][:, 4:7]
+    du = np.abs(r32["ptl"][:, 4:7] - u64).max() / np.abs(u64).max()
+    assert dx < 1e-4 * box and du < 1e-4, (dx, du)
+
+
 def test_track_grid_two_time_slices(gpu, oracle_tp):
     """300 Wirz steps through the time-blended grid (the particle time moves
     through eight slice intervals): bitwise the oracle."""
