@@ -403,6 +403,49 @@ __device__ __forceinline__ void ref_weights(T fx, T fy, T fz, T *w)
     w[7] = fx * fy * fz;
 }
 
+/* the reference gather's sums over the cached corners with FP64 weights w:
+ * one slice with float accumulators rounded after every corner
+ * (src/magnetic_field.c:603-608) */
+__device__ __forceinline__ void ref_sum64(const GridCacheN<1> &gc, const double *w, Emf &o)
+{
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f, a4 = 0.f, a5 = 0.f;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        a0 = (float)((double)a0 + (double)gc.v[6 * k + 0] * w[k]);
+        a1 = (float)((double)a1 + (double)gc.v[6 * k + 1] * w[k]);
+        a2 = (float)((double)a2 + (double)gc.v[6 * k + 2] * w[k]);
+        a3 = (float)((double)a3 + (double)gc.v[6 * k + 3] * w[k]);
+        a4 = (float)((double)a4 + (double)gc.v[6 * k + 4] * w[k]);
+        a5 = (float)((double)a5 + (double)gc.v[6 * k + 5] * w[k]);
+    }
+    o.ex = a0; o.ey = a1; o.ez = a2; o.bx = a3; o.by = a4; o.bz = a5;
+}
+
+/* two slices: sums in double, blended dt * next + (1 - dt) * previous,
+ * rounded to float for the reference gather (src/magnetic_field.c:609-621) */
+__device__ __forceinline__ void ref_sum64(const GridCacheN<2> &gc, const double *w, double gdt,
+                                          double t, bool round_float, Emf &o)
+{
+    double a[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, b[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+#pragma unroll
+        for (int m = 0; m < 6; m++) {
+            a[m] = a[m] + (double)gc.v[6 * k + m] * w[k];
+            b[m] = b[m] + (double)gc.v[48 + 6 * k + m] * w[k];
+        }
+    const double r = t / gdt;
+    const double dtf = r - (double)tp_floor_to_int(floor(r));
+    double v[6];
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+        v[m] = dtf * a[m] + (1.0 - dtf) * b[m];
+        if (round_float) v[m] = (double)(float)v[m];
+    }
+    o.ex = v[0]; o.ey = v[1]; o.ez = v[2];
+    o.bx = v[3]; o.by = v[4]; o.bz = v[5];
+}
+
 /* two time slices through the cache: the same sums, blend and rounding as
  * get_emf64<FK_GRID2_*> */
 template <int FK>
@@ -420,24 +463,7 @@ __device__ __forceinline__ void grid2_emf64_cached(const FieldDev &f, GridCacheN
         trilinear_tb(f.nx, f.ny, f.nz, f.lo, f.hi, f.d, x, y, z, c, w);
         gc_fill(f, gc, c, false);
     }
-    double a[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, b[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int k = 0; k < 8; k++)
-#pragma unroll
-        for (int m = 0; m < 6; m++) {
-            a[m] = a[m] + (double)gc.v[6 * k + m] * w[k];
-            b[m] = b[m] + (double)gc.v[48 + 6 * k + m] * w[k];
-        }
-    const double r = t / f.gdt;
-    const double dtf = r - (double)tp_floor_to_int(floor(r));
-    double v[6];
-#pragma unroll
-    for (int m = 0; m < 6; m++) {
-        v[m] = dtf * a[m] + (1.0 - dtf) * b[m];
-        if (FK == FK_GRID2_REF) v[m] = (double)(float)v[m];
-    }
-    o.ex = v[0]; o.ey = v[1]; o.ez = v[2];
-    o.bx = v[3]; o.by = v[4]; o.bz = v[5];
+    ref_sum64(gc, w, f.gdt, t, FK == FK_GRID2_REF, o);
 }
 
 template <int FK>
@@ -449,18 +475,7 @@ __device__ __forceinline__ void grid_emf64_cached(const FieldDev &f, GridCacheN<
         double fx, fy, fz;
         ref_cell_cached<false, 1>(f, gc, x, y, z, fx, fy, fz);
         ref_weights<double>(fx, fy, fz, w);
-        /* float accumulators rounded after every corner (src/magnetic_field.c:603-608) */
-        float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f, a4 = 0.f, a5 = 0.f;
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-            a0 = (float)((double)a0 + (double)gc.v[6 * k + 0] * w[k]);
-            a1 = (float)((double)a1 + (double)gc.v[6 * k + 1] * w[k]);
-            a2 = (float)((double)a2 + (double)gc.v[6 * k + 2] * w[k]);
-            a3 = (float)((double)a3 + (double)gc.v[6 * k + 3] * w[k]);
-            a4 = (float)((double)a4 + (double)gc.v[6 * k + 4] * w[k]);
-            a5 = (float)((double)a5 + (double)gc.v[6 * k + 5] * w[k]);
-        }
-        o.ex = a0; o.ey = a1; o.ez = a2; o.bx = a3; o.by = a4; o.bz = a5;
+        ref_sum64(gc, w, o);
     } else {
         unsigned c[8];
         trilinear_tb(f.nx, f.ny, f.nz, f.lo, f.hi, f.d, x, y, z, c, w);
@@ -478,6 +493,7 @@ __device__ __forceinline__ void grid_emf64_cached(const FieldDev &f, GridCacheN<
         o.ex = a0; o.ey = a1; o.ez = a2; o.bx = a3; o.by = a4; o.bz = a5;
     }
 }
+
 
 template <int FK>
 __device__ __forceinline__ void grid_emf32_cached(const FieldDev &f, GridCacheN<1> &gc, double x,
@@ -870,6 +886,39 @@ __device__ __forceinline__ void modes_sum32(const float *tab32, int nmodes, floa
 /* ------------------------------------------------------------------ */
 /* get_emf, FP32 (precision form)                                      */
 /* ------------------------------------------------------------------ */
+/* The FP64 reference gather at one point (bit for bit get_emf64<FK>) for a
+ * kernel whose cache was filled by the FP32 form's cell selection: the cell
+ * is selected again with the FP64 form's divisions, and when the cache holds
+ * that cell its floats are summed; otherwise the nodes are read. */
+template <int FK, int NS>
+__device__ __forceinline__ void ref_emf64_via_cache(const FieldDev &f, const GridCacheN<NS> &gc,
+                                                    double x, double y, double z, double t,
+                                                    Emf &o)
+{
+    int i1[3], i2[3];
+    double fx, fy, fz;
+    ref_axis_in<false>(x, f.lo[0], f.d[0], f.inv_d[0], i1[0], i2[0], fx);
+    ref_axis_in<false>(y, f.lo[1], f.d[1], f.inv_d[1], i1[1], i2[1], fy);
+    ref_axis_in<false>(z, f.lo[2], f.d[2], f.inv_d[2], i1[2], i2[2], fz);
+    if (x > f.hi[0] || x < f.lo[0] || y > f.hi[1] || y < f.lo[1] || z > f.hi[2] ||
+        z < f.lo[2]) {
+        ref_axis_out<false>(x, f.lo[0], f.hi[0], f.d[0], f.inv_d[0], f.nx, i1[0], i2[0], fx);
+        ref_axis_out<false>(y, f.lo[1], f.hi[1], f.d[1], f.inv_d[1], f.ny, i1[1], i2[1], fy);
+        ref_axis_out<false>(z, f.lo[2], f.hi[2], f.d[2], f.inv_d[2], f.nz, i1[2], i2[2], fz);
+    }
+    const bool same = gc.valid && i1[0] == gc.key[0] && i2[0] == gc.key[1] &&
+                      i1[1] == gc.key[2] && i2[1] == gc.key[3] && i1[2] == gc.key[4] &&
+                      i2[2] == gc.key[5];
+    if (same) {
+        double w[8];
+        ref_weights<double>(fx, fy, fz, w);
+        if constexpr (NS == 1) ref_sum64(gc, w, o);
+        else ref_sum64(gc, w, f.gdt, t, true, o);
+    } else {
+        get_emf64<FK>(f, x, y, z, t, o);
+    }
+}
+
 template <int FK>
 __device__ __forceinline__ void get_emf32(const FieldDev &f, double x, double y,
                                           double z, double t, EmfF &o)

@@ -232,8 +232,12 @@ __device__ __forceinline__ void track_body(const TrackArgs &a)
         if (live) {
             Emf e = {};
             if (esc == 0) {
-                if constexpr (PREC == PREC64) emf64_step<FK>(a.f, gc, p.x, p.y, p.z, p.t, e);
-                else get_emf64<FK>(a.f, p.x, p.y, p.z, p.t, e);
+                if constexpr (PREC == PREC64)
+                    emf64_step<FK>(a.f, gc, p.x, p.y, p.z, p.t, e);
+                else if constexpr (FK == FK_GRID_REF || FK == FK_GRID2_REF)
+                    ref_emf64_via_cache<FK>(a.f, gc, p.x, p.y, p.z, p.t, e);
+                else
+                    get_emf64<FK>(a.f, p.x, p.y, p.z, p.t, e);
             }
             dterms_particle(k, i, esc, p.x, p.y, p.z, p.ux, p.uy, p.uz, p.t, xs, ys, zs, e, v);
         }

@@ -109,3 +109,43 @@ def test_cells_mode_errors(gpu):
     for bad in (-1, 3):
         with pytest.raises(TPError):
             gpu.set_grid_cells(bad)
+
+
+def _mu(b, u):
+    return (b * u).sum(1) / (np.sqrt((b * b).sum(1)) * np.sqrt((u * u).sum(1)))
+
+
+@pytest.mark.parametrize("two", [False, True])
+@pytest.mark.parametrize("cells", [abi.TP_CELLS_OFF, abi.TP_CELLS_ON])
+def test_fp32_form_d_terms_use_the_fp64_field(gpu, two, cells):
+    """The FP32 form's D terms take the field at the particle from the FP64
+    reference gather (served from the FP32 cell cache when that holds the
+    same cell): D_mumu of the last row equals the sum of (mu - mu0)^2 / t
+    over the returned particles with mu from tp_gpu_eval_field (FP64) at
+    their final state, to summation order."""
+    case(f"FP32 D terms two={two} cells={cells}")
+    g = golden("track_grid2" if two else "track_grid")
+    lo, hi = np.asarray(g["lo"], float), np.asarray(g["hi"], float)
+    ptl = init_particles(1000, lo, hi, 0.5, seed=73)
+    gpu.set_grid_cells(cells)
+    if two:
+        gpu.set_field_grid2(g["ea"], g["ba"], g["eb"], g["bb"], lo, hi, float(g["grid_dt"]))
+    else:
+        gpu.set_field_grid(g["egrid"], g["bgrid"], lo, hi)
+    gpu.set_grid_cells(abi.TP_CELLS_AUTO)
+    dt = 0.02
+    prm = abi.make_params(dt=dt, estep=200, lo=lo, hi=hi, pusher=abi.TP_PUSH_BORIS,
+                          precision=abi.TP_FP32_MIXED, nbins=50, nt_out=2, emin=1e-3,
+                          emax=1.0)
+    r = gpu.track_fixed(prm, ptl)
+    p = r["ptl"]
+    assert r["remain"][2] == len(ptl)
+    t = p[:, 7] * 2 + dt
+    f0 = gpu.eval_field(np.column_stack([ptl[:, 0:3], ptl[:, 7]]))
+    f1 = gpu.eval_field(np.column_stack([p[:, 0:3], p[:, 7]]))
+    mu0 = _mu(f0[:, 0:3], ptl[:, 4:7])
+    mu1 = _mu(f1[:, 0:3], p[:, 4:7])
+    duu = ((mu1 - mu0) ** 2 / t).sum()
+    assert abs(r["dcoeff"][5, 2] / duu - 1) < 1e-12
+    dxx = ((ptl[:, 0] - (p[:, 0] + p[:, 8])) ** 2 / t).sum()
+    assert abs(r["dcoeff"][1, 2] / dxx - 1) < 1e-12
