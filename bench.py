@@ -91,8 +91,9 @@ def main():
                     help="adaptive-stepper horizon in gyroperiods")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "particles shard without communication, see DESIGN.md", file=sys.stderr)
+        print(f"bench.py: measures one GPU only (asked for --gpus {args.gpus}); the "
+              "particles would shard across GPUs without communication during the run",
+              file=sys.stderr)
         sys.exit(2)
     if int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: one process per run; WORLD_SIZE > 1 is not supported", file=sys.stderr)
@@ -370,7 +371,7 @@ def forms_surrogate(args):
     dt = 0.01 T_g, periodic in L: the exact FP32 mode sum against the
     network with the exact-field fallback at tolerance 1e-3 (checked every
     100 steps) and, as the plain form, the FP64 push with the exact FP64 mode
-    sum (timed over 500 steps).  D_xx and the spectrum are judged against
+    sum (timed over 500 steps).  D_xx and the spectrum are compared with
     the exact-field FP32 run."""
     from test_particle_amd import abi, surrogate
     from test_particle_amd.fields import gyroperiod
@@ -412,7 +413,7 @@ def forms_surrogate(args):
         t0 = time.perf_counter()
         tr.run(params(k64, abi.TP_FP64))
         v64 = n * k64 / (time.perf_counter() - t0)
-        # exact FP32 (the judge of the surrogate)
+        # exact FP32 (what the surrogate is compared with)
         tr.load(ptl)
         pe = params(K, abi.TP_FP32_MIXED)
         t0 = time.perf_counter()
