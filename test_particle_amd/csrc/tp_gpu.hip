@@ -598,7 +598,8 @@ int tp_gpu_device_info(tp_ctx *c, char *name, int name_len, int *num_cu)
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, c->device));
     if (name && name_len > 0) {
-        snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+        snprintf(name, name_len, "%s%s(%s)", prop.name, prop.name[0] ? " " : "",
+                 prop.gcnArchName);
     }
     if (num_cu) *num_cu = prop.multiProcessorCount;
     return TP_OK;
