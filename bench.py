@@ -77,8 +77,8 @@ def main():
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 form")
-    ap.add_argument("--forms", default="abc,bs,surrogate,grid2,poincare",
-                    help="extra workloads reported under 'forms' (abc, bs, surrogate, "
+    ap.add_argument("--forms", default="abc,bs,surrogate,gridwirz,grid2,poincare",
+                    help="extra workloads reported under 'forms' (abc, bs, surrogate, gridwirz, "
                          "grid2, poincare; '' for none)")
     ap.add_argument("--pmap-side", type=int, default=512,
                     help="Poincare workload: start points per side of the plane")
@@ -191,6 +191,8 @@ def main():
         forms.update(forms_bs(args))
     if "surrogate" in extra:
         forms.update(forms_surrogate(args))
+    if "gridwirz" in extra:
+        forms.update(forms_gridwirz(args, not args.no_cpu))
     if "grid2" in extra:
         forms.update(forms_grid2(args))
     if "poincare" in extra:
@@ -462,6 +464,105 @@ def forms_surrogate(args):
         "residual_p99": float(np.quantile(res, 0.99)),
         "fraction_residual_below_1e-3": float(np.mean(res < 1e-3))}
     return out
+
+
+def forms_gridwirz(args, with_cpu):
+    """The headline grid with the reference's own pusher: FP64 Wirz through
+    the reference gather, 1M particles x 1000 steps.  Its CPU baseline is the
+    reference's particle_tracking_hybrid compiled from its sources
+    (oracle/_ref/ref_harness, mode hybrid, OpenMP over the box's threads) on
+    the first 65,536 particles, the step count sized from a pilot to about
+    10 s; the GPU runs the same sample, and the D_xx of their last rows is
+    compared statistically (the Wirz rotation's sin/cos: libm in the
+    reference, the project's routine on the GPU, differ by ulps, which the
+    turbulent field amplifies)."""
+    from test_particle_amd import abi
+    from test_particle_amd.fields import gyroperiod, pic_turbulence_grid
+    from test_particle_amd.gpu import Tracker
+    from test_particle_amd.particles import init_particles
+    e, b, lo, hi = pic_turbulence_grid(args.grid, seed=args.seed)
+    n, K = args.particles, 1000
+    ptl = init_particles(n, lo, hi, 0.7, seed=args.seed)
+    dt = 0.01 * gyroperiod(0.7, 1.0)
+
+    def params(steps, nt_out):
+        return abi.make_params(dt=dt, estep=steps, lo=lo, hi=hi, pusher=abi.TP_PUSH_WIRZ,
+                               nbins=102, nt_out=nt_out, emin=1e-4, emax=1.0)
+
+    out = {}
+    with Tracker(0) as tr:
+        tr.set_field_grid(e, b, lo, hi)
+        tr.load(ptl)
+        tr.run(params(100, 1))
+        tr.load(ptl)
+        t0 = time.perf_counter()
+        tr.run(params(K, 10))
+        w = time.perf_counter() - t0
+        kms, launches = tr.last_kernel_time()
+        name = "pic_grid_1M_fp64_wirz"
+        out[name] = {"value": n * K / w, "dtype": "f64", "particles": n, "steps": K,
+                     "pusher": "wirz (the reference's tracking_wirz)",
+                     "kernel_ms_per_launch": kms / max(launches, 1)}
+        log(f"[bench] {name}: {n * K / w:.4e} particle-steps/s")
+        if with_cpu:
+            ref = grid_reference_baseline(e, b, lo, hi, ptl, dt, tr, params)
+            if ref is not None:
+                out[name]["cpu_baseline"] = ref
+                log(f"[bench] {name} CPU (reference) {ref['value']:.3e} particle-steps/s on "
+                    f"{ref['cores']} threads; D_xx last row GPU vs reference "
+                    f"{ref['dxx_last_rel_diff_vs_gpu']:.2e}")
+    del e, b
+    return out
+
+
+def grid_reference_baseline(e, b, lo, hi, ptl, dt, tr, params):
+    """The reference's particle_tracking_hybrid on the PIC grid (system 4,
+    fixed-step Wirz) over the first particles; see forms_gridwirz.  None when
+    the harness is absent."""
+    import subprocess
+    import tempfile
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    if not os.path.exists(exe):
+        return None
+    cores = host_cores()
+    nz, ny, nx = e.shape[:3]
+    with tempfile.TemporaryDirectory() as tmp:
+        os.makedirs(os.path.join(tmp, "data"))
+        grid = os.path.join(tmp, "grid.bin")
+        with open(grid, "wb") as f:
+            np.ascontiguousarray(e, np.float32).tofile(f)
+            np.ascontiguousarray(b, np.float32).tofile(f)
+
+        def run(np_, steps):
+            with open(os.path.join(tmp, "spec.txt"), "w") as f:
+                f.write(f"system 4\nnx {nx}\nny {ny}\nnz {nz}\ngrid {grid}\n"
+                        f"tmax {float(steps * dt + 1e-9)!r}\ndt {float(dt)!r}\n"
+                        f"bc 0\nnbins 102\nnt_out {steps // 100}\nemin 0.0001\nemax 1.0\n")
+                for k, a in enumerate(("x", "y", "z")):
+                    f.write(f"{a}min {float(lo[k])!r}\n{a}max {float(hi[k])!r}\n")
+            np.ascontiguousarray(ptl[:np_], np.float64).tofile(os.path.join(tmp, "p.bin"))
+            p = subprocess.run([exe, "hybrid", "spec.txt", "p.bin"], cwd=tmp, check=True,
+                               capture_output=True, text=True, timeout=600,
+                               env=dict(os.environ, OMP_NUM_THREADS=str(cores)))
+            kv = dict(line.split() for line in p.stdout.splitlines()
+                      if line.startswith("hybrid_"))
+            return float(kv["hybrid_seconds"])
+
+        ns = 65536
+        pilot = run(8192, 100)
+        rate = 8192 * 100 / max(pilot, 1e-6)
+        ks = int(min(max(100, round(10.0 * rate / ns / 100) * 100), 5000))
+        sec = run(ns, ks)
+        diff = np.loadtxt(os.path.join(tmp, "data", "diff_coeffs.dat"), skiprows=1)
+    g = tr.track_fixed(params(ks, ks // 100), ptl[:ns])
+    dxx_ref = diff[-1, 1] / diff[-1, 8]
+    dxx_gpu = g["dcoeff"][1, -1] / g["remain"][-1]
+    return {"value": ns * ks / sec, "unit": "particle-steps/s", "cores": cores,
+            "kind": "reference",
+            "sample": f"first {ns} of the same particles, {ks} Wirz steps, same grid, through "
+                      f"the reference's particle_tracking_hybrid (oracle/_ref/ref_harness, "
+                      f"OpenMP) on {cpu_model()}",
+            "dxx_last_rel_diff_vs_gpu": float(abs(dxx_gpu / dxx_ref - 1.0))}
 
 
 def forms_grid2(args):
