@@ -260,8 +260,8 @@ __global__ void __launch_bounds__(256) k_track(TrackArgs a)
     track_body<PUSH, FK, PREC, TRAJ>(a);
 }
 
-/* the same compiled for 3 waves per SIMD (168 VGPRs), which the FP32
- * two-slice gather fits without spilling (the compiler's own choice is 2) */
+/* the same compiled for 3 waves per SIMD (168 VGPRs; the compiler's own
+ * choice is 2 for the kernels launched this way) */
 template <int PUSH, int FK, int PREC, bool TRAJ>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
 k_track_w3(TrackArgs a)
@@ -387,7 +387,11 @@ __global__ void __launch_bounds__(256) k_eval(FieldDev f, int n, const double *p
 template <int PUSH, int FK, int PREC, bool TRAJ>
 static void launch_track(dim3 g, dim3 b, size_t sh, hipStream_t s, const TrackArgs &a)
 {
-    if constexpr (PREC == PREC32 && (FK == FK_GRID2_REF || FK == FK_GRID2_TB))
+    /* 3 waves per SIMD: the FP32 two-slice gather (no spills on its hot
+     * path) and the Wirz push through the reference gather (a few spills;
+     * measured 4 % faster than the compiler's 2 waves) */
+    if constexpr ((PREC == PREC32 && (FK == FK_GRID2_REF || FK == FK_GRID2_TB)) ||
+                  (PUSH == TP_PUSH_WIRZ && FK == FK_GRID_REF))
         hipLaunchKernelGGL((k_track_w3<PUSH, FK, PREC, TRAJ>), g, b, sh, s, a);
     else
         hipLaunchKernelGGL((k_track<PUSH, FK, PREC, TRAJ>), g, b, sh, s, a);
