@@ -33,11 +33,11 @@
 /* step kernel: steps j0..j1 of every particle                         */
 /* ------------------------------------------------------------------ */
 /* One launch covers a run of steps that ends at a diffusion-coefficient
- * step (or at estep); k_dcoeff then evaluates the D terms on the state
- * after that step.  The spectrum (any cadence) and the escape spectrum are
- * binned here. */
-/* The kernel's argument block read again from the kernarg segment (k_track's
- * only argument, so at offset 0).  Values needed only after the step loop
+ * step (or at estep); the kernel evaluates the D terms on the state after
+ * that step (k_dcoeff does for launches over index lists).  The spectrum
+ * (any cadence) and the escape spectrum are binned here. */
+/* The kernel's argument block read again from the kernarg segment (the
+ * first argument of k_track, k_track_w3 and k_dcoeff, so at offset 0).  Values needed only after the step loop
  * or in its rare branches (output pointers, spectrum settings) are loaded
  * where they are used through this pointer, which the empty asm makes
  * opaque, instead of occupying scalar registers through the loop (which
