@@ -77,9 +77,9 @@ def main():
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 form")
-    ap.add_argument("--forms", default="abc,bs,surrogate,gridwirz,grid2,poincare",
+    ap.add_argument("--forms", default="abc,bs,surrogate,gridwirz,grid2,poincare,fieldload",
                     help="extra workloads reported under 'forms' (abc, bs, surrogate, gridwirz, "
-                         "grid2, poincare; '' for none)")
+                         "grid2, poincare, fieldload; '' for none)")
     ap.add_argument("--pmap-side", type=int, default=512,
                     help="Poincare workload: start points per side of the plane")
     ap.add_argument("--pmap-ncross", type=int, default=8, help="Poincare crossings per line")
@@ -191,6 +191,8 @@ def main():
         forms.update(forms_bs(args))
     if "surrogate" in extra:
         forms.update(forms_surrogate(args))
+    if "fieldload" in extra:
+        forms.update(forms_fieldload(args))
     if "gridwirz" in extra:
         forms.update(forms_gridwirz(args, not args.no_cpu))
     if "grid2" in extra:
@@ -563,6 +565,46 @@ def grid_reference_baseline(e, b, lo, hi, ptl, dt, tr, params):
                       f"the reference's particle_tracking_hybrid (oracle/_ref/ref_harness, "
                       f"OpenMP) on {cpu_model()}",
             "dxx_last_rel_diff_vs_gpu": float(abs(dxx_gpu / dxx_ref - 1.0))}
+
+
+def forms_fieldload(args):
+    """The PIC field input pipeline (tp_gpu_load_field_grid): raw component
+    planes of a 384^3 grid (1.36 GB, random values) to the nodes, through
+    pinned slabs and the GPU layout kernels, in the HDF5 (x-slowest) and .gda
+    (x-fastest) orders, and the reference's float3 arrays through
+    tp_gpu_set_field_grid; wall time of the second call of each (the first
+    allocates the context's staging slabs), in GB/s of input."""
+    from test_particle_amd import abi
+    from test_particle_amd.gpu import Tracker
+    n = 384
+    rng = np.random.default_rng(args.seed)
+    planes = rng.standard_normal((6, n, n, n), dtype=np.float32)
+    lo, hi = (0.0, 0.0, 0.0), (float(n),) * 3
+    gb = planes.nbytes / 1e9
+    out = {}
+    with Tracker(0) as tr:
+        tr.set_grid_cells(abi.TP_CELLS_OFF)      # the input path alone
+        for order, name in ((abi.TP_PLANES_X_SLOWEST, "fieldload_384_planes_xslowest"),
+                            (abi.TP_PLANES_X_FASTEST, "fieldload_384_planes_xfastest")):
+            for rep in range(2):
+                t0 = time.perf_counter()
+                tr.set_field_grid_planes(planes, lo, hi, order=order, e0=0.3, b0=1.7)
+                w = time.perf_counter() - t0
+            out[name] = {"value": gb / w, "unit": "GB/s of raw planes", "ms": w * 1e3,
+                         "nodes": n ** 3}
+            log(f"[bench] {name}: {gb / w:.2f} GB/s ({w * 1e3:.1f} ms)")
+        e = np.ascontiguousarray(planes[:3].transpose(3, 2, 1, 0))
+        b = np.ascontiguousarray(planes[3:].transpose(3, 2, 1, 0))
+        del planes
+        for rep in range(2):
+            t0 = time.perf_counter()
+            tr.set_field_grid(e, b, lo, hi)
+            w = time.perf_counter() - t0
+        out["fieldload_384_float3_arrays"] = {"value": gb / w, "unit": "GB/s of float3 arrays",
+                                              "ms": w * 1e3, "nodes": n ** 3}
+        log(f"[bench] fieldload_384_float3_arrays: {gb / w:.2f} GB/s ({w * 1e3:.1f} ms)")
+        tr.set_grid_cells(abi.TP_CELLS_AUTO)
+    return out
 
 
 def forms_grid2(args):
