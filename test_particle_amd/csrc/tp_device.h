@@ -927,11 +927,15 @@ __device__ __forceinline__ void get_emf32(const FieldDev &f, double x, double y,
         o.bx = 0.f; o.by = 0.f; o.bz = 1.f;
         o.ex = 0.f; o.ey = 0.f; o.ez = 0.f;
     } else if (FK == FK_ABC) {
-        float lam = (float)f.lam;
-        float sx, cx, sy, cy, sz, cz;
-        sincosf(lam * (float)x, &sx, &cx);
-        sincosf(lam * (float)y, &sy, &cy);
-        sincosf(lam * (float)z, &sz, &cz);
+        /* the hardware sine and cosine of the phase in revolutions (fraction
+         * taken first), as the FP32 mode sum evaluates its cosines */
+        const float lr = (float)(f.lam * (0.5 / TP_PI));
+        const float px = __builtin_amdgcn_fractf(lr * (float)x);
+        const float py = __builtin_amdgcn_fractf(lr * (float)y);
+        const float pz = __builtin_amdgcn_fractf(lr * (float)z);
+        const float sx = __builtin_amdgcn_sinf(px), cx = __builtin_amdgcn_cosf(px);
+        const float sy = __builtin_amdgcn_sinf(py), cy = __builtin_amdgcn_cosf(py);
+        const float sz = __builtin_amdgcn_sinf(pz), cz = __builtin_amdgcn_cosf(pz);
         float A = (float)f.A, B = (float)f.B, C = (float)f.C, B0 = (float)f.B0;
         o.bx = (A * sz + C * cy) * B0;
         o.by = (B * sx + A * cz) * B0;

@@ -273,6 +273,20 @@ def test_fp32_mixed_precision_form(gpu):
     assert np.abs(g32 / g0 - 1).max() < 1e-4
 
 
+def test_fp32_abc_field_accuracy(gpu):
+    """The FP32 ABC field (hardware sine and cosine of the phase in
+    revolutions) within 4e-6 of the field's scale of the FP64 field, inside
+    the box and up to three box lengths outside it."""
+    rng = np.random.default_rng(27)
+    pts = np.column_stack([rng.uniform(-3 * TWO_PI, 4 * TWO_PI, (4000, 3)), np.zeros(4000)])
+    gpu.set_field_abc(1.0, np.sqrt(2.0 / 3.0), np.sqrt(1.0 / 3.0), 1.3, 2.0)
+    f64 = gpu.eval_field(pts)
+    f32 = gpu.eval_field(pts, precision=abi.TP_FP32_MIXED)
+    err = np.abs(f32 - f64).max() / np.abs(f64).max()
+    print(f"[case] FP32 ABC field: max error {err:.2e} of the field scale", flush=True)
+    assert err < 4e-6
+
+
 def test_track_growing_launch_counts_one_context(gpu):
     """One context, runs whose launch count grows (2 events per launch are
     reallocated as needed): every run equals the same run on a fresh
