@@ -29,7 +29,8 @@
  *   h5      SPEC FILE GROUP OUT  the PIC HDF5 readers read_pic_efields_h5 and
  *                            read_pic_bfields_h5 (src/electric_field.c:447-507,
  *                            src/magnetic_field.c:632-692); OUT: the loaded
- *                            efc_float then magc_float
+ *                            efc_float then magc_float ("-": none); prints
+ *                            the readers' wall time
  *   hybrid  SPEC PTL         particle_tracking_hybrid (src/tracking.c:659) with
  *                            the t=0 spectrum of src/diagnostics.c:50; writes
  *                            data/espectrum.dat and data/diff_coeffs.dat and
@@ -472,16 +473,21 @@ int main(int argc, char **argv)
         }
         fclose(fo);
     } else if (!strcmp(mode, "h5")) {
-        /* one time slice (the readers implement multi_tframe = 0 only) */
+        /* one time slice (the readers implement multi_tframe = 0 only); prints
+         * the readers' wall time; OUT "-" writes nothing */
         if (s.mt) die("h5: one time slice only");
+        const double t_start = omp_get_wtime();
         read_pic_efields_h5(argv[3], argv[4], (int)sizeof(float));
         read_pic_bfields_h5(argv[3], argv[4], (int)sizeof(float));
-        size_t nn = (size_t)s.nx * s.ny * s.nz;
-        FILE *fo = fopen(argv[5], "wb");
-        if (!fo) die("h5 out");
-        fwrite(efc_float, sizeof(f3), nn, fo);
-        fwrite(magc_float, sizeof(f3), nn, fo);
-        fclose(fo);
+        printf("h5_seconds %.6f\n", omp_get_wtime() - t_start);
+        if (strcmp(argv[5], "-") != 0) {
+            size_t nn = (size_t)s.nx * s.ny * s.nz;
+            FILE *fo = fopen(argv[5], "wb");
+            if (!fo) die("h5 out");
+            fwrite(efc_float, sizeof(f3), nn, fo);
+            fwrite(magc_float, sizeof(f3), nn, fo);
+            fclose(fo);
+        }
     } else if (!strcmp(mode, "hybrid")) {
         particles *p = slurp(argv[3], &n);
         int np = (int)(n / sizeof(particles));
