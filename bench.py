@@ -179,6 +179,13 @@ def main():
                                                            else "FP32"),
                 "kernel_ms_per_launch": kms / max(launches, 1),
                 "steps_per_launch": K / max(launches, 1)}
+    # what this GPU reaches on a plain 16-byte-per-lane HBM copy (2 GiB each
+    # way), beside the published peak
+    from test_particle_amd.gpu import measure_peak
+    copy_gbs, _ = measure_peak(abi.TP_PEAK_HBM_COPY, 2 << 30, 3)
+    roofline["peak_measured_copy"] = copy_gbs
+    roofline["frac_of_measured_copy"] = achieved / copy_gbs if achieved else None
+    log(f"[bench] HBM copy microbenchmark {copy_gbs:.0f} GB/s")
 
     cpu = None
     if not args.no_cpu:

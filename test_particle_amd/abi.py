@@ -37,6 +37,13 @@ TP_CELLS_ON = 2
 TP_PLANES_X_FASTEST = 0
 TP_PLANES_X_SLOWEST = 1
 
+# microbenchmarks of the device's achievable peaks (tp_peak_measure)
+TP_PEAK_HBM_COPY = 0
+TP_PEAK_HBM_READ = 1
+TP_PEAK_FP64_FMA = 2
+TP_PEAK_FP32_FMA = 3
+TP_PEAK_BF16_MFMA = 4
+
 TP_PUSH_WIRZ = 0
 TP_PUSH_BORIS = 1
 TP_FP64 = 0

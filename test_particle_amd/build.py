@@ -21,7 +21,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
          "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang",
          "-packed-fp32-ops", "-I", os.path.join(ROOT, "include"), "-I", CSRC]
-SOURCES = ["tp_gpu.hip", "tp_bs.hip", "tp_mlp.hip", "tp_fieldline.hip"]
+SOURCES = ["tp_gpu.hip", "tp_bs.hip", "tp_mlp.hip", "tp_fieldline.hip", "tp_peak.hip"]
 # host-side C (the reference's language), compiled by gcc and linked in
 C_SOURCES = ["tp_host.c"]
 CC = os.environ.get("CC", "gcc")

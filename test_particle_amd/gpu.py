@@ -84,6 +84,7 @@ def lib():
         L.tp_host_pmap_starts.argtypes = [i, i, P(d), d, d, d, d, i, i, P(d)]
         L.tp_host_read_pic_gda.argtypes = [cs, i, i, i, i, i, i, d, d, P(f), P(f), P(f), P(f)]
         L.tp_host_load_pic_gda.argtypes = [vp, cs, i, i, i, i, i, i, P(d), P(d), d, d, d, i]
+        L.tp_peak_measure.argtypes = [i, i, ctypes.c_size_t, i, P(d), P(d)]
         _lib = L
     return _lib
 
@@ -97,6 +98,16 @@ class TPError(RuntimeError):
 def _check(rc, what):
     if rc != 0:
         raise TPError(rc, what)
+
+
+def measure_peak(what, nbytes=0, reps=5, device=0):
+    """tp_peak_measure: (rate, ms) of the fastest of `reps` launches of a
+    microbenchmark kernel -- GB/s for abi.TP_PEAK_HBM_*, TFLOP/s for the
+    arithmetic ones."""
+    v, ms = ctypes.c_double(), ctypes.c_double()
+    _check(lib().tp_peak_measure(device, what, nbytes, reps, ctypes.byref(v),
+                                 ctypes.byref(ms)), "tp_peak_measure")
+    return v.value, ms.value
 
 
 def _vec3(v):
