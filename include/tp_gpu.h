@@ -115,12 +115,13 @@ int tp_gpu_set_field_abc(tp_ctx *ctx, double A, double B, double C,
 /* nodes (nx, ny, nz) over [lo, hi] endpoint-inclusive (dx = (hi-lo)/(n-1),
  * src/domain.c:548-550), x fastest, AoS double3 velocity
  * (get_double_velocity_at_point, src/velocity_field.c:555-584).
- * v == NULL removes it. */
+ * v == NULL removes it; nx*ny*nz must be below 2^32 - 1 (TP_E_ARG). */
 int tp_gpu_set_velocity_grid(tp_ctx *ctx, int nx, int ny, int nz,
                              const double lo[3], const double hi[3],
                              const double *v);
 /* system 4: PIC grids, AoS float3 E and B on the same nodes, x fastest
- * (getemf_pic_test_particle, src/emfields.c:153-166). */
+ * (getemf_pic_test_particle, src/emfields.c:153-166); nx*ny*nz below
+ * 2^32 - 1 (TP_E_ARG), as for every grid loader below. */
 int tp_gpu_set_field_grid(tp_ctx *ctx, int nx, int ny, int nz,
                           const double lo[3], const double hi[3],
                           const float *efield, const float *bfield,

@@ -669,6 +669,7 @@ int tp_gpu_set_velocity_grid(tp_ctx *c, int nx, int ny, int nz, const double lo[
     c->s.derived = false;
     if (!v) return TP_OK;
     if (nx < 2 || ny < 2 || nz < 2 || !lo || !hi) return TP_E_ARG;
+    if ((size_t)nx * ny * nz >= 0xFFFFFFFFull) return TP_E_ARG;   /* 32-bit corner indices */
     size_t bytes = (size_t)nx * ny * nz * 3 * sizeof(double);
     if (hipMalloc(&c->vgrid, bytes) != hipSuccess) { c->vgrid = nullptr; return TP_E_NOMEM; }
     HIPCHK(hipMemcpy(c->vgrid, v, bytes, hipMemcpyHostToDevice));
