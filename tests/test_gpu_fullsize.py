@@ -14,7 +14,13 @@ track a million particles here in seconds:
 * a second identical run gives the same bits (FP64 and FP32 forms);
 * the FP32 form's D_xx is within 1 % of the FP64 form's.
 
-200 steps (two D rows, two spectrum frames)."""
+200 steps (two D rows, two spectrum frames).
+
+The ABC workload is small enough to check whole: 16,384 particles, 10,000
+steps of FP64 Boris and FP64 Wirz, bit for bit against the oracle (a few
+seconds of CPU time on 16 threads)."""
+import os
+
 import numpy as np
 import pytest
 
@@ -110,3 +116,28 @@ def test_fp32_form_deterministic_and_close(full_fp64, tracker, workload):
     err = np.max(np.abs(dxx32 / dxx64 - 1.0))
     print(f"[fullsize] FP32 D_xx rel err {err:.2e}", flush=True)
     assert err < 0.01
+
+
+@pytest.mark.parametrize("pusher", [abi.TP_PUSH_BORIS, abi.TP_PUSH_WIRZ])
+def test_abc_16k_10000_steps_bitwise(pusher, oracle_tp):
+    """ABC A = B = C = 1 over [0, 2 pi]^3, |v| = 0.7c isotropic, dt = 0.01 T_g
+    with B_ref = sqrt(3), 10,000 steps, spectrum every 100 steps (102 bins,
+    1e-3 to 10), D every 100 steps: trajectories, step counts, spectra and
+    remaining counts bitwise, D sums to 1e-12."""
+    from test_particle_amd.gpu import Tracker
+    two_pi = 2.0 * np.pi
+    lo, hi = (0.0, 0.0, 0.0), (two_pi,) * 3
+    dt = 0.01 * two_pi * (1.0 / np.sqrt(1.0 - 0.49)) / np.sqrt(3.0)
+    ptl = init_particles(16384, lo, hi, 0.7, seed=SEED)
+    prm = abi.make_params(dt=dt, estep=10000, lo=lo, hi=hi, pusher=pusher, nbins=102,
+                          nt_out=100, emin=1e-3, emax=10.0)
+    print(f"[fullsize] ABC 16384 x 10000 pusher={pusher}", flush=True)
+    with Tracker(0) as t:
+        t.set_field_abc(1.0, 1.0, 1.0)
+        r = t.track_fixed(prm, ptl)
+    ref = oracle_tp.track_fixed(oracle_tp.Field.abc(1.0, 1.0, 1.0), prm, ptl,
+                                nthreads=min(16, os.cpu_count() or 1))
+    for k in ("ptl", "nsteps", "espect", "espect_escape", "remain"):
+        _same(r[k], ref[k], f"pusher {pusher}: {k}")
+    np.testing.assert_allclose(r["dcoeff"], ref["dcoeff"], rtol=1e-12, atol=1e-15)
+    assert r["espect"].sum() == 100 * 16384
