@@ -18,7 +18,19 @@ track a million particles here in seconds:
 
 The ABC workload is small enough to check whole: 16,384 particles, 10,000
 steps of FP64 Boris and FP64 Wirz, bit for bit against the oracle (a few
-seconds of CPU time on 16 threads)."""
+seconds of CPU time on 16 threads).
+
+The ABC precision workload at its full particle count (4,194,304 particles,
+500 steps): 512 sampled particles of the FP64 run bit for bit the oracle;
+the FP32 form's D_xx within 1 % of FP64 and every particle's energy drift
+below 1e-4.
+
+The adaptive workload at its full particle count (1000 modes, 1,048,576
+particles with |v| log-uniform in [0.1, 0.9]c, rtol 1e-8) over 0.2 of each
+particle's gyroperiod: 256 sampled particles integrated by the oracle alone
+match their rows bit for bit (state and accepted steps), and a permuted
+array -- which changes which lanes of the work queue take which particles --
+gives the permuted rows and the same step totals."""
 import os
 
 import numpy as np
@@ -141,3 +153,60 @@ def test_abc_16k_10000_steps_bitwise(pusher, oracle_tp):
         _same(r[k], ref[k], f"pusher {pusher}: {k}")
     np.testing.assert_allclose(r["dcoeff"], ref["dcoeff"], rtol=1e-12, atol=1e-15)
     assert r["espect"].sum() == 100 * 16384
+
+
+def test_bs_1000_modes_1m_sample_and_permutation(oracle_tp):
+    from test_particle_amd.fields import turbulence_modes
+    from test_particle_amd.gpu import Tracker
+    rows = turbulence_modes(1000, seed=SEED)
+    ptl = init_particles(N, (0, 0, 0), (100, 100, 100), 0.1, kind="loguniform", vmax=0.9,
+                         seed=SEED)
+    prm = abi.make_bs_params(rtol=1e-8, atol=1e-10, t_end_gyro=0.2, b_ref=1.0)
+    perm = np.random.default_rng(13).permutation(N)
+    print(f"[fullsize] BS 1000 modes, {N} particles, 0.2 T_g", flush=True)
+    with Tracker(0) as t:
+        t.set_field_modes(rows, 1.0)
+        r = t.track_adaptive(prm, ptl)
+        rp = t.track_adaptive(prm, ptl[perm])
+    assert r["failed"] == 0 and r["accepted"] >= N
+    _same(rp["ptl"], r["ptl"][perm], "permuted particles")
+    _same(rp["nsteps"], r["nsteps"][perm], "permuted accepted steps")
+    for k in ("accepted", "rejected", "rhs", "failed"):
+        assert rp[k] == r[k], k
+    idx = np.sort(np.random.default_rng(17).choice(N, 256, replace=False))
+    ref = oracle_tp.bs_track(oracle_tp.Field.modes(rows, 1.0), prm, ptl[idx],
+                             nthreads=min(16, os.cpu_count() or 1))
+    _same(r["ptl"][idx], ref["ptl"], "sampled particles vs oracle")
+    _same(r["nsteps"][idx], ref["nsteps"], "sampled accepted steps vs oracle")
+
+
+def test_abc_4m_fp64_sample_and_fp32_form(oracle_tp):
+    from test_particle_amd.gpu import Tracker
+    two_pi = 2.0 * np.pi
+    lo, hi = (0.0, 0.0, 0.0), (two_pi,) * 3
+    dt = 0.01 * two_pi * (1.0 / np.sqrt(1.0 - 0.49)) / np.sqrt(3.0)
+    n = 1 << 22
+    ptl = init_particles(n, lo, hi, 0.7, seed=SEED)
+
+    def prm(precision):
+        return abi.make_params(dt=dt, estep=500, lo=lo, hi=hi, pusher=abi.TP_PUSH_BORIS,
+                               precision=precision, nbins=102, nt_out=5, emin=1e-3,
+                               emax=10.0)
+    print(f"[fullsize] ABC {n} x 500 FP64 and FP32", flush=True)
+    with Tracker(0) as t:
+        t.set_field_abc(1.0, 1.0, 1.0)
+        r64 = t.track_fixed(prm(abi.TP_FP64), ptl)
+        r32 = t.track_fixed(prm(abi.TP_FP32_MIXED), ptl)
+    idx = np.sort(np.random.default_rng(19).choice(n, 512, replace=False))
+    ref = oracle_tp.track_fixed(oracle_tp.Field.abc(1.0, 1.0, 1.0), prm(abi.TP_FP64),
+                                ptl[idx], nthreads=min(16, os.cpu_count() or 1))
+    _same(r64["ptl"][idx], ref["ptl"], "sampled FP64 particles vs oracle")
+    dxx32 = r32["dcoeff"][1, 1:] / np.maximum(r32["remain"][1:], 1)
+    dxx64 = r64["dcoeff"][1, 1:] / np.maximum(r64["remain"][1:], 1)
+    err = np.max(np.abs(dxx32 / dxx64 - 1.0))
+
+    def gamma(p):
+        return np.sqrt(1.0 + p[:, 4] ** 2 + p[:, 5] ** 2 + p[:, 6] ** 2)
+    drift = np.max(np.abs(gamma(r32["ptl"]) / gamma(ptl) - 1.0))
+    print(f"[fullsize] FP32 D_xx rel err {err:.2e}, energy drift {drift:.2e}", flush=True)
+    assert err < 0.01 and drift < 1e-4
