@@ -77,9 +77,9 @@ def main():
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 form")
-    ap.add_argument("--forms", default="abc,bs,surrogate,gridwirz,grid2,poincare,fieldload",
+    ap.add_argument("--forms", default="abc,bs,surrogate,gridwirz,grid2,openbc,poincare,fieldload",
                     help="extra workloads reported under 'forms' (abc, bs, surrogate, gridwirz, "
-                         "grid2, poincare, fieldload; '' for none)")
+                         "grid2, openbc, poincare, fieldload; '' for none)")
     ap.add_argument("--pmap-side", type=int, default=512,
                     help="Poincare workload: start points per side of the plane")
     ap.add_argument("--pmap-ncross", type=int, default=8, help="Poincare crossings per line")
@@ -205,6 +205,8 @@ def main():
         forms.update(forms_gridwirz(args, not args.no_cpu))
     if "grid2" in extra:
         forms.update(forms_grid2(args))
+    if "openbc" in extra:
+        forms.update(forms_openbc(args))
     if "poincare" in extra:
         forms.update(forms_poincare(args, not args.no_cpu))
 
@@ -670,6 +672,57 @@ def fieldload_h5(tr, n, seed, with_cpu):
                           f"{sec * 1e3:.0f} ms"}
             log(f"[bench] {name} CPU (reference loaders) {gb / sec:.2f} GB/s "
                 f"({sec * 1e3:.0f} ms)")
+    return out
+
+
+def forms_openbc(args):
+    """Open boundaries (the reference's bc_flag = 1) on the PIC-grid
+    workload: the 256^3 grid, 1M particles at |v| = 0.7c, Boris, 5000
+    steps, particles that leave the box stop and are counted in the escape
+    spectrum.  value = particle-steps actually taken (the sum of the step
+    counts) per second of wall time, beside the fraction remaining at the
+    end and the rate of the same run with periodic boundaries."""
+    from test_particle_amd import abi
+    from test_particle_amd.fields import gyroperiod, pic_turbulence_grid
+    from test_particle_amd.gpu import Tracker
+    from test_particle_amd.particles import init_particles
+    e, b, lo, hi = pic_turbulence_grid(args.grid, seed=args.seed)
+    n, K = args.particles, 5000
+    ptl = init_particles(n, lo, hi, 0.7, seed=args.seed)
+    dt = 0.01 * gyroperiod(0.7, 1.0)
+    out = {}
+    with Tracker(0) as tr:
+        tr.set_field_grid(e, b, lo, hi, abi.TP_GATHER_REF_COMPAT)
+        for prec, name in ((abi.TP_FP64, "pic_grid_open_1M_fp64"),
+                           (abi.TP_FP32_MIXED, "pic_grid_open_1M_fp32")):
+            def prm(bc, steps=K):
+                return abi.make_params(dt=dt, estep=steps, lo=lo, hi=hi,
+                                       pusher=abi.TP_PUSH_BORIS, bc_flag=bc, precision=prec,
+                                       nbins=102, nt_out=50, emin=1e-4, emax=1.0)
+            tr.load(ptl)
+            tr.run(prm(abi.TP_BC_OPEN, 100))
+            tr.load(ptl)
+            t0 = time.perf_counter()
+            tr.run(prm(abi.TP_BC_OPEN))
+            w = time.perf_counter() - t0
+            _, nsteps = tr.store()
+            d = tr.diagnostics(prm(abi.TP_BC_OPEN))
+            taken = int(nsteps.astype(np.int64).sum())
+            remain = float(d["remain"][-1]) / n
+            escaped = int(d["espect_escape"].sum())
+            tr.load(ptl)
+            t1 = time.perf_counter()
+            tr.run(prm(abi.TP_BC_PERIODIC))
+            wp = time.perf_counter() - t1
+            out[name] = {"value": taken / w, "unit": "particle-steps/s (steps taken)",
+                         "dtype": "f64" if prec == abi.TP_FP64 else
+                         "f32 push and gather, f64 positions",
+                         "particles": n, "steps": K, "remaining_fraction": remain,
+                         "escape_spectrum_counts": escaped,
+                         "periodic_same_run_value": n * K / wp}
+            log(f"[bench] {name}: {taken / w:.4e} particle-steps/s taken, remaining "
+                f"{remain:.3f}, periodic {n * K / wp:.4e}")
+    del e, b
     return out
 
 

@@ -46,6 +46,9 @@ struct tp_ctx {
     double *dco = nullptr, *partials = nullptr;
     int *rdec = nullptr;
     size_t partials_cap = 0, diag_cap = 0, dco_cap = 0;
+    /* open boundaries: indices of the particles still inside, and their count */
+    int *live_idx = nullptr;
+    size_t live_cap = 0;
     int last_n = 0;
     double last_ms = 0.0;
     int last_launches = 0;

@@ -79,7 +79,7 @@ __device__ __forceinline__ void dterms_particle(KArgs *k, int i, int escaped, do
                                                 double uz, double tt, double xs, double ys,
                                                 double zs, const Emf &e, double v[8])
 {
-    const int n = k->n;
+    const int n = k->ntot;
     double *st = k->st;
     double dxx = st[i], dyy = st[n + i], dzz = st[2 * n + i];
     double daa = st[3 * n + i], dee = st[4 * n + i];
@@ -126,7 +126,7 @@ __device__ __forceinline__ void dterms_reduce(KArgs *k, const double v[8], bool 
     if (tid < 8) {
         double r = 0.0;
         for (int w = 0; w < nwaves; w++) r += wsum[w][tid];
-        k->partials[(size_t)blockIdx.x * 8 + tid] = r;
+        k->partials[(size_t)(blockIdx.x + k->pboff) * 8 + tid] = r;
     }
 }
 
@@ -293,6 +293,80 @@ __global__ void __launch_bounds__(256) k_dcoeff(TrackArgs a, int tstep)
         dterms_particle(k, i, escaped, x, y, z, ux, uy, uz, tt, xs, ys, zs, e, v);
     }
     dterms_reduce(k, v, live, escaped, tid, tstep, wsum);
+}
+
+/* The D terms of escaped particles listed in idx[0 .. n): their stale
+ * terms (and the escape count), as the step kernel forms them for its
+ * escaped lanes; block rows from pboff on. */
+__global__ void __launch_bounds__(256) k_dstale(TrackArgs a, int tstep)
+{
+    __shared__ double wsum[4][8];
+    const int tid = threadIdx.x;
+    const int g = blockIdx.x * blockDim.x + tid;
+    const bool live = g < a.n;
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    KArgs *k = kargs_again();
+    if (live) {
+        const Emf e = {};
+        dterms_particle(k, a.idx[g], 1, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, e, v);
+    }
+    dterms_reduce(k, v, live, live ? 1 : 0, tid, tstep, wsum);
+}
+
+/* Open boundaries: stable split of the particles into those still inside
+ * the box (esc == 0) and those that left, each list in index order, in
+ * three passes over blocks of 256: counts, one-block scan, scatter. */
+__global__ void __launch_bounds__(256) k_live_count(const int *esc, int n, int *cnt)
+{
+    __shared__ int w[4];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const unsigned long long m = __ballot(i < n && esc[i] == 0);
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) cnt[blockIdx.x] = w[0] + w[1] + w[2] + w[3];
+}
+
+__global__ void __launch_bounds__(1024) k_live_scan(const int *cnt, int nb, int *off,
+                                                    int *total)
+{
+    __shared__ int s[1024];
+    const int t = threadIdx.x;
+    const int per = (nb + 1023) / 1024;
+    const int b0 = t * per, b1 = min(nb, b0 + per);
+    int sum = 0;
+    for (int b = b0; b < b1; b++) sum += cnt[b];
+    s[t] = sum;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {          /* inclusive scan of the chunk sums */
+        const int v = t >= o ? s[t - o] : 0;
+        __syncthreads();
+        s[t] += v;
+        __syncthreads();
+    }
+    int run = s[t] - sum;
+    for (int b = b0; b < b1; b++) {
+        off[b] = run;
+        run += cnt[b];
+    }
+    if (t == 1023) *total = s[1023];
+}
+
+__global__ void __launch_bounds__(256) k_live_scatter(const int *esc, int n, const int *off,
+                                                      int *inside, int *gone)
+{
+    __shared__ int w[4];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const bool valid = i < n;
+    const bool in = valid && esc[i] == 0;
+    const unsigned long long m = __ballot(in);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) w[wave] = __popcll(m);
+    __syncthreads();
+    int rl = __popcll(m & ((1ull << lane) - 1ull));   /* inside, before i in the block */
+    for (int q = 0; q < wave; q++) rl += w[q];
+    const int base = off[blockIdx.x];                /* inside, in earlier blocks */
+    if (in) inside[base + rl] = i;
+    else if (valid) gone[(blockIdx.x * 256 - base) + ((int)threadIdx.x - rl)] = i;
 }
 
 /* Sum the per-block partials of one D row in a fixed order: one block per
@@ -581,6 +655,7 @@ void tp_gpu_finalize(tp_ctx *c)
     hipFree(c->dco);
     hipFree(c->partials);
     hipFree(c->rdec);
+    hipFree(c->live_idx);
     for (int k = 0; k < c->nev; k++) hipEventDestroy(c->evs[k]);
     free(c->evs);
     tp_bs_release(c);
@@ -1356,7 +1431,9 @@ int tp_gpu_run_fixed(tp_ctx *c, const tp_track_params *p)
 
     const int bs = block_size_for(c, n);
     const int nblocks = n > 0 ? (n + bs - 1) / bs : 0;
-    size_t pneed = (size_t)(nblocks > 0 ? nblocks : 1) * 8;
+    /* partial-sum rows: the step kernel's blocks, or with open boundaries a
+     * launch over the particles inside plus k_dstale's blocks of 256 */
+    size_t pneed = ((size_t)(nblocks > 0 ? nblocks : 1) + (n + 255) / 256 + 2) * 8;
     if (pneed > c->partials_cap) {
         hipFree(c->partials);
         c->partials = nullptr; c->partials_cap = 0;
@@ -1365,6 +1442,25 @@ int tp_gpu_run_fixed(tp_ctx *c, const tp_track_params *p)
     }
     size_t shmem = sizeof(unsigned) * nb;
     if (shmem > 64 * 1024) return TP_E_ARG;
+    /* open boundaries: after each launch the particles are split (stably)
+     * into those still inside and those that left, and once some have left
+     * the next launch runs over the first list, so the lanes of escaped
+     * particles do not idle through the remaining steps.  Its D terms are
+     * formed in the step kernel for the particles inside and by k_dstale
+     * (stale terms, escape count) for the others; per-particle results and
+     * spectra are unchanged, the D sums are summed in that block order.
+     * Buffer: inside[n], gone[n], block counts and offsets, total. */
+    const bool compact = p->bc_flag == 1 && !mlp && n > 0;
+    const int nb256 = (n + 255) / 256;
+    const size_t live_need = 2 * (size_t)n + 2 * (size_t)nb256 + 1;
+    if (compact && live_need > c->live_cap) {
+        hipFree(c->live_idx);
+        c->live_idx = nullptr; c->live_cap = 0;
+        if (hipMalloc(&c->live_idx, live_need * sizeof(int)) != hipSuccess) return TP_E_NOMEM;
+        c->live_cap = live_need;
+    }
+    int *l_in = c->live_idx, *l_gone = l_in + n, *l_cnt = l_gone + n;
+    int *l_off = l_cnt + nb256, *l_tot = l_off + nb256;
     /* launches: each ends at the next D step (a multiple of di) or estep */
     long nlaunch = 0;
     for (long j0 = p->sstep; j0 <= p->estep; nlaunch++) {
@@ -1389,7 +1485,7 @@ int tp_gpu_run_fixed(tp_ctx *c, const tp_track_params *p)
     TrackArgs a;
     memset(&a, 0, sizeof(a));
     a.f = c->f;
-    a.n = n; a.bc = p->bc_flag; a.nbins = nb; a.nt_out = nt; a.einterval = p->einterval_t;
+    a.n = n; a.ntot = n; a.pboff = 0; a.bc = p->bc_flag; a.nbins = nb; a.nt_out = nt; a.einterval = p->einterval_t;
     a.di = di; a.fix_stale = (p->flags & TP_FLAG_FIX_STALE_DCOEFF) ? 1 : 0;
     a.nd = nd;
     a.dt = p->dt; a.qm = p->qm;
@@ -1413,6 +1509,9 @@ int tp_gpu_run_fixed(tp_ctx *c, const tp_track_params *p)
     int ntp = (int)floor(p->sstep / (double)p->einterval_t);
     int tstep = 1;
     int launches = 0;
+    bool use_list = false;      /* the launch runs over l_in[0 .. nlive) */
+    int nlive = n;
+    int row_blocks = nblocks;   /* partial-sum rows of the launch's D row */
     for (long j0 = p->sstep; j0 <= p->estep && nblocks > 0;) {
         long j1 = (j0 % di == 0) ? j0 : (j0 / di + 1) * (long)di;
         if (j1 > p->estep) j1 = p->estep;
@@ -1422,8 +1521,28 @@ int tp_gpu_run_fixed(tp_ctx *c, const tp_track_params *p)
         /* the D terms in the step kernel, unless it runs over index lists */
         const bool dstep = j1 % di == 0;
         a.dstep = (dstep && !mlp) ? tstep : -1;
+        row_blocks = nblocks;
         HIPCHK(hipEventRecord(c->evs[2 * launches], c->stream));
-        if (mlp) {
+        if (use_list) {
+            const int nbl = (nlive + bs - 1) / bs, nesc = n - nlive;
+            if (nlive > 0) {
+                TrackArgs b = a;
+                b.idx = l_in;
+                b.n = nlive;
+                fn(dim3(nbl), dim3(bs), shmem, c->stream, b);
+                HIPCHK(hipGetLastError());
+            }
+            row_blocks = nbl;
+            if (dstep && nesc > 0) {
+                TrackArgs b = a;
+                b.idx = l_gone;
+                b.n = nesc;
+                b.pboff = nbl;
+                hipLaunchKernelGGL(k_dstale, dim3((nesc + 255) / 256), dim3(256), 0,
+                                   c->stream, b, tstep);
+                row_blocks = nbl + (nesc + 255) / 256;
+            }
+        } else if (mlp) {
             /* residual checks before the first launch and at launches that
              * start after a multiple of the check interval */
             const int ci = c->f.mlp.check;
@@ -1444,9 +1563,24 @@ int tp_gpu_run_fixed(tp_ctx *c, const tp_track_params *p)
                 HIPCHK(hipGetLastError());
             }
             hipLaunchKernelGGL(k_reduce_row, dim3(8), dim3(256), 0, c->stream, c->partials,
-                               nblocks, tstep, nd, c->dco);
+                               row_blocks, tstep, nd, c->dco);
             HIPCHK(hipGetLastError());
             tstep++;
+        }
+        if (compact && j1 < p->estep) {
+            hipLaunchKernelGGL(k_live_count, dim3(nb256), dim3(256), 0, c->stream,
+                               (const int *)S.esc, n, l_cnt);
+            hipLaunchKernelGGL(k_live_scan, dim3(1), dim3(1024), 0, c->stream,
+                               (const int *)l_cnt, nb256, l_off, l_tot);
+            hipLaunchKernelGGL(k_live_scatter, dim3(nb256), dim3(256), 0, c->stream,
+                               (const int *)S.esc, n, (const int *)l_off, l_in, l_gone);
+            HIPCHK(hipGetLastError());
+            int h = 0;
+            HIPCHK(hipMemcpyAsync(&h, l_tot, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            if (h < 0 || h > n) return TP_E_HIP;
+            nlive = h;
+            use_list = h < n;
         }
         j0 = j1 + 1;
     }

@@ -27,6 +27,8 @@ struct TrackArgs {
     int n, bc, nbins, nt_out, einterval, di, fix_stale, first;
     int j0, j1, ntp0, nd;
     int dstep;           /* step kernels: D row evaluated after step j1 (-1: none) */
+    int ntot;            /* all particles of the run: the stride of st */
+    int pboff;           /* D partial sums go to block rows pboff + blockIdx.x */
     double dt, qm, lo[3], hi[3], logemin, logde;
     double *x, *y, *z, *ux, *uy, *uz, *t, *xs, *ys, *zs;
     const double *x0, *y0, *z0, *ux0, *uy0, *uz0, *g0, *mu0, *th0;
