@@ -120,13 +120,21 @@ __device__ __forceinline__ void dterms_reduce(KArgs *k, const double v[8], bool 
         double r = wave_sum(v[q]);
         if (lane == 0) wsum[wave][q] = r;
     }
-    unsigned long long m = __ballot(live && escaped);
-    if (lane == 0 && m && tstep + 1 < k->nd) atomicAdd(&k->rdec[tstep + 1], __popcll(m));
+    /* the escape count: one atomic per block (per-wave atomics on the one
+     * counter serialise when every lane is escaped) */
+    __shared__ int wesc[4];
+    const unsigned long long m = __ballot(live && escaped);
+    if (lane == 0) wesc[wave] = __popcll(m);
     __syncthreads();
     if (tid < 8) {
         double r = 0.0;
         for (int w = 0; w < nwaves; w++) r += wsum[w][tid];
         k->partials[(size_t)(blockIdx.x + k->pboff) * 8 + tid] = r;
+    }
+    if (tid == 0 && tstep + 1 < k->nd) {
+        int e = 0;
+        for (int w = 0; w < nwaves; w++) e += wesc[w];
+        if (e) atomicAdd(&k->rdec[tstep + 1], e);
     }
 }
 
