@@ -39,6 +39,14 @@ sys.path.insert(0, ROOT)
 METRIC = "particle-steps/s on 1 MI355X, chosen vs plain form, at D_xx err <=1%"
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_PSTEP = 192.0      # 8 corners x 6 FP32 components per field evaluation
+# VALU issue: one wave64 vector instruction per SIMD every 4 cycles (16 lanes
+# per SIMD, the FP64 and unpacked-FP32 rate), 256 CUs x 4 SIMDs at the 2.4 GHz
+# peak engine clock
+VALU_PEAK_GIPS = 256 * 4 * 2.4 / 4.0
+# the FP32 form is chosen only if it passes this validation run outside the
+# timed region (ten D rows, so the gate is the same whatever --steps is)
+VALIDATION_STEPS = 1000
+COUNTERS_JSON = os.path.join(ROOT, "profiles", "k_track_grid_counters.json")
 
 
 def log(msg):
@@ -122,6 +130,9 @@ def main():
     gpu_name, ncu = tr.device_info()
     log(f"[bench] {gpu_name}, {ncu} CUs")
     tr.set_field_grid(e, b, lo, hi, abi.TP_GATHER_REF_COMPAT)
+    gate = None
+    if not args.no_fp32:
+        gate = validate_fp32(tr, ptl, params, VALIDATION_STEPS)
     tr.load(ptl)
     if args.warmup > 0:
         log(f"[bench] warmup {args.warmup} steps")
@@ -152,40 +163,32 @@ def main():
         kms32, launches32 = tr.last_kernel_time()
         d32 = tr.diagnostics(prm32)
         dxx32 = d32["dcoeff"][1, 1:] / np.maximum(d32["remain"][1:], 1)
-        # fewer steps than one D interval leave no D rows to compare
+        # the timed runs' own comparison (no D rows when --steps is below one
+        # D interval); the choice rests on the validation run
         err = float(np.max(np.abs(dxx32 / dxx - 1.0))) if dxx.size else None
         serr = spectrum_err(d32["espect"], diag["espect"])
         v32 = n * K / wall32
         forms["fp32_mixed"] = {"value": v32, "dtype": "f32 push and gather, f64 position "
                                                      "and energy accumulators",
-                               "dxx_max_rel_err_vs_fp64": err,
-                               "spectrum_l1_rel_err_vs_fp64": serr,
+                               "validation": gate,
+                               "timed_run_dxx_max_rel_err_vs_fp64": err,
+                               "timed_run_spectrum_l1_rel_err_vs_fp64": serr,
                                "kernel_ms": kms32, "launches": launches32}
-        log(f"[bench] FP32 form {v32:.4e} particle-steps/s, D_xx err {err}, "
-            f"spectrum L1 err {serr:.2e}")
-        if err is not None and err <= 0.01 and serr <= 0.01 and v32 > value:
+        log(f"[bench] FP32 form {v32:.4e} particle-steps/s; validation over "
+            f"{gate['steps']} steps: D_xx err {gate['dxx_max_rel_err']:.2e}, spectrum L1 err "
+            f"{gate['spectrum_l1_rel_err']:.2e} -> {'passes' if gate['passes'] else 'fails'}")
+        if gate["passes"] and v32 > value:
             chosen, chosen_value = "fp32_mixed", v32
             chosen_dtype = "f32 (f64 position/energy accumulators)"
             kms, launches = kms32, launches32
-    achieved = BYTES_PER_PSTEP * n * K / (kms * 1e-3) / 1e9 if kms > 0 else None
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "k_track_grid_traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get(chosen, {}).get("hbm_bytes_per_launch")
-    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
-                "kernel": "k_track<BORIS, GRID_REF, %s>" % ("FP64" if chosen == "plain_fp64"
-                                                           else "FP32"),
-                "kernel_ms_per_launch": kms / max(launches, 1),
-                "steps_per_launch": K / max(launches, 1)}
+    roofline = headline_roofline(chosen, kms, launches, n, K)
     # what this GPU reaches on a plain 16-byte-per-lane HBM copy (2 GiB each
     # way), beside the published peak
     from test_particle_amd.gpu import measure_peak
     copy_gbs, _ = measure_peak(abi.TP_PEAK_HBM_COPY, 2 << 30, 3)
-    roofline["peak_measured_copy"] = copy_gbs
-    roofline["frac_of_measured_copy"] = achieved / copy_gbs if achieved else None
-    log(f"[bench] HBM copy microbenchmark {copy_gbs:.0f} GB/s")
+    roofline["hbm"]["peak_measured_copy"] = copy_gbs
+    log(f"[bench] HBM copy microbenchmark {copy_gbs:.0f} GB/s; roofline bound "
+        f"{roofline['bound']}, frac {roofline['frac']}")
 
     cpu = None
     if not args.no_cpu:
@@ -221,7 +224,7 @@ def main():
                                f"{K} steps",
                    "particles": n, "grid_nodes": [args.grid + 1] * 3, "pusher": "boris",
                    "gather": "ref_compat", "seed": args.seed, "gpu": gpu_name,
-                   "chosen_form": chosen, "plain_value": value,
+                   "chosen_form": chosen, "fp32_gate": gate, "plain_value": value,
                    "chosen_vs_plain": chosen_value / value,
                    "dxx_last": float(dxx[-1]) if dxx.size else None},
         "roofline": roofline,
@@ -229,6 +232,81 @@ def main():
         "forms": forms,
     }
     print(json.dumps(out), flush=True)
+
+
+def validate_fp32(tr, ptl, params, steps):
+    """The FP32 form against the FP64 form on the same particles over
+    `steps` steps (outside the timed region): largest relative D_xx error
+    over the D rows and largest relative L1 distance of the spectra."""
+    from test_particle_amd import abi
+    out = {}
+    for name, prec in (("f64", abi.TP_FP64), ("f32", abi.TP_FP32_MIXED)):
+        prm = params(steps, prec)
+        tr.load(ptl)
+        tr.run(prm)
+        d = tr.diagnostics(prm)
+        out[name] = (d["dcoeff"][1, 1:] / np.maximum(d["remain"][1:], 1), d["espect"])
+    err = float(np.max(np.abs(out["f32"][0] / out["f64"][0] - 1.0)))
+    serr = spectrum_err(out["f32"][1], out["f64"][1])
+    return {"steps": steps, "dxx_rows": int(out["f64"][0].size), "dxx_max_rel_err": err,
+            "spectrum_l1_rel_err": serr, "passes": bool(err <= 0.01 and serr <= 0.01)}
+
+
+def headline_roofline(form, kms, launches, n, K):
+    """Roofline of the chosen form's step kernel against the resource that
+    binds it.  Per launch, its VALU instructions and HBM bytes come from the
+    rocprofv3 counters in profiles/k_track_grid_counters.json, measured at
+    launch lengths of 20 and 100 steps (fixed part + per-step part; this
+    run's average launch length picks the point on that line); its duration
+    is the live HIP-event time of this run's launches.  VALU: instructions
+    issued per second against one wave64 instruction per SIMD per 4 cycles;
+    HBM: counter bytes per second against 8 TB/s.  The bound is the larger
+    fraction; the algorithmic bytes (192 per particle-step, no reuse credited)
+    are reported beside, with their ratio to the counted traffic."""
+    spl = K / max(launches, 1)
+    t_launch = kms / max(launches, 1) * 1e-3            # seconds
+    alg = BYTES_PER_PSTEP * n * spl                      # algorithmic bytes per launch
+    r = {"bound": None, "achieved": None, "peak": None, "unit": None, "frac": None,
+         "traffic": None,
+         "kernel": "k_track<BORIS, GRID_REF, %s>" % ("FP64" if form == "plain_fp64" else "FP32"),
+         "kernel_ms_per_launch": t_launch * 1e3, "steps_per_launch": spl,
+         "valu": None, "hbm": {"peak": HBM_PEAK_GBS, "unit": "GB/s"},
+         "algorithmic": {"bytes_per_particle_step": BYTES_PER_PSTEP,
+                         "GBps": alg / t_launch / 1e9 if t_launch > 0 else None}}
+    if not os.path.exists(COUNTERS_JSON) or t_launch <= 0:
+        return r
+    with open(COUNTERS_JSON) as f:
+        cj = json.load(f)
+    ent = cj.get(form)
+    if not ent or ent.get("particles") != n:
+        return r
+    pts = sorted((int(k), v) for k, v in ent["per_launch"].items())
+
+    def at(key):
+        (s0, a), (s1, b) = pts[0], pts[-1]
+        if s1 == s0:
+            return a[key]
+        slope = (b[key] - a[key]) / (s1 - s0)
+        return a[key] + slope * (spl - s0)
+
+    valu = at("valu_insts")
+    traffic = at("hbm_bytes")
+    valu_rate = valu / t_launch / 1e9
+    hbm_rate = traffic / t_launch / 1e9
+    fv, fh = valu_rate / VALU_PEAK_GIPS, hbm_rate / HBM_PEAK_GBS
+    r["valu"] = {"achieved": valu_rate, "peak": VALU_PEAK_GIPS,
+                 "unit": "G wave64 instructions/s", "frac": fv,
+                 "insts_per_launch": valu, "insts_per_wave_step": valu / (n / 64.0) / spl}
+    r["hbm"].update({"achieved": hbm_rate, "frac": fh, "bytes_per_launch": traffic})
+    r["algorithmic"]["ratio_traffic_to_algorithmic"] = traffic / alg
+    r["traffic"] = traffic
+    r["source"] = os.path.relpath(COUNTERS_JSON, ROOT)
+    if fv >= fh:
+        r.update(bound="valu", achieved=valu_rate, peak=VALU_PEAK_GIPS,
+                 unit="G wave64 instructions/s", frac=fv)
+    else:
+        r.update(bound="hbm", achieved=hbm_rate, peak=HBM_PEAK_GBS, unit="GB/s", frac=fh)
+    return r
 
 
 def spectrum_err(a, b):
@@ -346,6 +424,16 @@ def forms_bs(args):
         st = tr.run_adaptive(prm)
         w = time.perf_counter() - t0
         fin = tr.store()[0] if not args.no_cpu else None
+        # the rate over horizons of 0.2, 2 and 20 gyroperiods (262,144 particles)
+        flat = {}
+        for tg in (0.2, 2.0, 20.0):
+            tr.load(ptl[:1 << 18])
+            t1 = time.perf_counter()
+            s2 = tr.run_adaptive(abi.make_bs_params(rtol=1e-8, atol=1e-10, t_end_gyro=tg,
+                                                    b_ref=1.0))
+            flat[str(tg)] = s2["accepted"] / (time.perf_counter() - t1)
+        log(f"[bench] BS accepted steps/s over 0.2/2/20 T_g: {flat}")
+        dense = bs_dense_vs_fixed(tr, abi, ptl)
     rate = st["accepted"] / w
     log(f"[bench] BS: {rate:.4e} accepted steps/s, {st['rhs'] / w:.4e} RHS/s, "
         f"rejected {st['rejected']}, failed {st['failed']}")
@@ -353,7 +441,9 @@ def forms_bs(args):
         "value": rate, "unit": "accepted BS steps/s", "dtype": "f64", "particles": n,
         "horizon_gyroperiods": args.bs_tgyro, "rtol": 1e-8, "atol": 1e-10,
         "rhs_per_s": st["rhs"] / w, "accepted": st["accepted"], "rejected": st["rejected"],
-        "failed": st["failed"], "kernel_ms": st["kernel_ms"], "wall_s": w}}
+        "failed": st["failed"], "kernel_ms": st["kernel_ms"], "wall_s": w,
+        "rate_by_horizon_gyroperiods": flat}}
+    out.update(dense)
     if fin is not None:
         # the oracle's sequential BS (OpenMP over particles) on the first
         # particles of the same set, about 10 s; the GPU's end states of
@@ -376,6 +466,57 @@ def forms_bs(args):
         log(f"[bench] BS CPU {ref['accepted'] / ref['seconds']:.3e} accepted steps/s on "
             f"{cores} threads ({m} particles); GPU sample bitwise={same}")
     return out
+
+
+def bs_dense_vs_fixed(tr, abi, ptl):
+    """The adaptive stepper with dense output and the fixed-step loop's
+    diagnostics (tp_gpu_run_adaptive_diag) on the same 1000-mode field and
+    1M particles, against the plain FP64 Boris kernel on the same output
+    grid: dt = 0.1 (0.007-0.016 gyroperiods), 200 steps (t = 20, 1.4-3.2
+    gyroperiods), 10 spectrum frames, D rows every 20 steps, no boundary.
+    D_xx is compared row by row, with the standard error of the last row's
+    difference from the per-particle terms; spectra by relative L1."""
+    n = ptl.shape[0]
+    dt, estep = 0.1, 200
+    prm = abi.make_params(dt=dt, estep=estep, lo=(0, 0, 0), hi=(100, 100, 100),
+                          pusher=abi.TP_PUSH_BORIS, bc_flag=abi.TP_BC_NONE, nbins=102,
+                          nt_out=10, emin=1e-4, emax=1.0, dcoeff_interval=20)
+    tr.load(ptl)
+    t0 = time.perf_counter()
+    tr.run(prm)
+    wf = time.perf_counter() - t0
+    pf, _ = tr.store()
+    df = tr.diagnostics(prm)
+    bs = abi.make_bs_params(rtol=1e-8, atol=1e-10, b_ref=1.0)
+    tr.load(ptl)
+    t0 = time.perf_counter()
+    st = tr.run_adaptive_diag(bs, prm)
+    wa = time.perf_counter() - t0
+    pa, _ = tr.store()
+    da = tr.diagnostics(prm)
+    dxx = lambda d: d["dcoeff"][1, 1:] / np.maximum(d["remain"][1:], 1)  # noqa: E731
+    rel = np.abs(dxx(da) / dxx(df) - 1.0)
+    t_end = estep * dt
+
+    def terms(p):
+        return (ptl[:, 0] - (p[:, 0] + p[:, 8])) ** 2 / (2 * t_end + dt)
+    a, b = terms(pf), terms(pa)
+    se = float(np.hypot(a.std(ddof=1), b.std(ddof=1)) / np.sqrt(n))
+    serr = spectrum_err(da["espect"], df["espect"])
+    log(f"[bench] BS dense + diagnostics: {st['accepted'] / wa:.4e} accepted steps/s; D_xx vs "
+        f"fixed-step FP64 max rel {rel.max():.2e} (last row {rel[-1]:.2e}, "
+        f"{abs(b.mean() - a.mean()) / se:.2f} SE); spectrum L1 {serr:.2e}")
+    return {"bs_dense_diag_modes1000_1M": {
+        "value": st["accepted"] / wa, "unit": "accepted BS steps/s", "dtype": "f64",
+        "particles": n, "output_dt": dt, "output_steps": estep, "rtol": 1e-8, "atol": 1e-10,
+        "rhs_per_s": st["rhs"] / wa, "accepted": st["accepted"], "rejected": st["rejected"],
+        "failed": st["failed"], "kernel_ms": st["kernel_ms"],
+        "vs_fixed_step_fp64": {"fixed_step_value": n * estep / wf,
+                               "dxx_max_rel_diff": float(rel.max()),
+                               "dxx_last_rel_diff": float(rel[-1]),
+                               "dxx_last_diff_in_se": float(abs(b.mean() - a.mean()) / se),
+                               "dxx_last_se_rel": se / float(a.mean()),
+                               "spectrum_l1_rel_diff": serr}}}
 
 
 def forms_surrogate(args):

@@ -72,7 +72,12 @@ typedef struct tp_particle {
 #define TP_FLAG_FIX_STALE_DCOEFF 1  /* zero the per-particle D terms each step
                                        (the reference keeps stale dxx/dyy/dzz/
                                        daa/dee, src/tracking.c:334-336) */
-#define TP_FLAG_CHECK_FINITE     2  /* return TP_E_NONFINITE on NaN/Inf state */
+#define TP_FLAG_CHECK_FINITE     2  /* after the run, scan the particles' positions
+                                       and momenta on the GPU and return
+                                       TP_E_NONFINITE if any is NaN or infinite
+                                       (the reference lets them propagate: B = 0
+                                       makes iBtot infinite, src/tracking.c:782);
+                                       the outputs are still written */
 
 typedef struct tp_track_params {
     int pusher;            /* TP_PUSH_* */
@@ -139,6 +144,26 @@ int tp_gpu_set_field_grid2(tp_ctx *ctx, int nx, int ny, int nz, const double lo[
                            const double hi[3], const float *efield_next,
                            const float *bfield_next, const float *efield_prev,
                            const float *bfield_prev, double grid_dt, int gather_mode);
+
+/* system 3: MHD fields, AoS float3 velocity and B on the same nodes, x
+ * fastest (getemf_mhd_test_particle, src/emfields.c:107-120): v and B are
+ * gathered as the reference's float gathers do (get_float_velocity_at_point,
+ * src/velocity_field.c:593-622, get_float_bfield_at_point,
+ * src/magnetic_field.c:593-622, reference weights and per-corner float
+ * rounding) and E = B x v is formed in double.  One time slice (the
+ * reference's fixed-step loop does not track system 3 with two,
+ * src/tracking.c:151-157); the cell-major copy applies as for PIC grids. */
+int tp_gpu_set_field_mhd(tp_ctx *ctx, int nx, int ny, int nz, const double lo[3],
+                         const double hi[3], const float *vfield, const float *bfield);
+/* PIC grids of doubles, AoS double3 E and B, x fastest: the reference's
+ * double gathers get_double_efield_at_point and get_double_bfield_at_point
+ * (src/electric_field.c:370-406, src/magnetic_field.c:555-591; its weights,
+ * double accumulators), one time slice.  The reference defines them but
+ * never reaches them: its setup loads systems 3 and 4 as float grids
+ * (src/domain.c:357-400) and get_emf calls the float gathers. */
+int tp_gpu_set_field_grid_double(tp_ctx *ctx, int nx, int ny, int nz, const double lo[3],
+                                 const double hi[3], const double *efield,
+                                 const double *bfield);
 
 /* Cell-major copy of a PIC grid.  Besides the nodes, the library can keep
  * every cell's eight corners (the reference gather's corners, six floats
@@ -305,6 +330,28 @@ typedef struct tp_bs_stats {
 /* Resident particles (tp_gpu_load_particles) are advanced to their end
  * times; tp_gpu_store_particles returns them, its nsteps = accepted steps. */
 int tp_gpu_run_adaptive(tp_ctx *ctx, const tp_bs_params *p, tp_bs_stats *stats);
+/* The adaptive loop of the reference's tracking_method 1 (the commented
+ * particle_tracking_adaptive, src/tracking.c:518-630) with the fixed-step
+ * loop's diagnostics.  Resident particles (tp_gpu_load_particles) are
+ * integrated from their time t0 to t0 + estep*dt of q (sstep must be 1,
+ * precision TP_FP64; t_end and t_end_gyro of p are not used) by the
+ * stepper above with NR3's dense output (src/unused/StepperBS.c:478-644:
+ * nseq = 4k+2, a step is redone when its interpolation error exceeds 10
+ * and the next one is capped by it), particle_bc (src/tracking.c:
+ * 1172-1221) after every accepted step.  At the output times of the
+ * fixed-step loop, t0 + j*dt, the state comes from the dense-output
+ * polynomial of the step that contains the time (the final state at the
+ * end time): the spectrum at j % einterval_t == 0, the D terms at j %
+ * dcoeff_interval == 0.  An escaping particle (bc 1) is binned into the
+ * escape spectrum at the frame of the next output time and stops; its
+ * later D rows carry its stale terms, and the remaining counts count it as
+ * tp_gpu_run_fixed does.  D terms are kept per particle and row (8 * nd *
+ * n doubles, TP_E_NOMEM if that and the dense scratch exceed half the free
+ * memory) and summed in a fixed order.  tp_gpu_read_diagnostics returns
+ * the results, tp_gpu_store_particles the particles (nsteps = accepted
+ * steps). */
+int tp_gpu_run_adaptive_diag(tp_ctx *ctx, const tp_bs_params *p, const tp_track_params *q,
+                             tp_bs_stats *stats);
 /* load + run + store in one call (nsteps may be NULL) */
 int tp_gpu_track_adaptive(tp_ctx *ctx, const tp_bs_params *p, int n, tp_particle *ptl,
                           int *nsteps, tp_bs_stats *stats);
@@ -427,6 +474,9 @@ int tp_gpu_train_surrogate(tp_ctx *ctx, const tp_mlp_train_params *p,
 int tp_gpu_set_field_mlp(tp_ctx *ctx, double tol, int check_interval);
 /* Particles that switched to the exact field during the last run. */
 int tp_gpu_mlp_fallback_count(tp_ctx *ctx, long long *count);
+/* Per resident particle (n = the count loaded last): 1 if it used the exact
+ * field at the end of the last surrogate run, else 0. */
+int tp_gpu_mlp_fallback_flags(tp_ctx *ctx, int n, int *flags);
 
 #ifdef __cplusplus
 }

@@ -31,10 +31,19 @@
  *                            src/magnetic_field.c:632-692); OUT: the loaded
  *                            efc_float then magc_float ("-": none); prints
  *                            the readers' wall time
- *   hybrid  SPEC PTL         particle_tracking_hybrid (src/tracking.c:659) with
+ *   hybrid  SPEC PTL [OUT]   particle_tracking_hybrid (src/tracking.c:659) with
  *                            the t=0 spectrum of src/diagnostics.c:50; writes
  *                            data/espectrum.dat and data/diff_coeffs.dat and
- *                            prints the call's wall time
+ *                            prints the call's wall time; OUT: the particles
+ *                            after the call
+ *   dfields SPEC PTS OUT     the reference's double gathers
+ *                            get_double_bfield_at_point /
+ *                            get_double_efield_at_point
+ *                            (src/magnetic_field.c:555-591,
+ *                            src/electric_field.c:370-406) on the double grid
+ *                            of the spec's dgrid; OUT as for fields.  (Its
+ *                            get_emf never calls them: systems 3 and 4 are
+ *                            set up with float grids, src/domain.c:357-400.)
  *   diff    SPEC PTL0 PTL DT OUT  calc_diff_coeff (src/diagnostics.c:659)
  *   ellint  SPEC KS OUT      ellint (src/wlcs.c:460); KS: n f64 moduli k,
  *                            OUT: n*(E, K)
@@ -137,6 +146,13 @@ extern f3 *efc_float;                           /* src/electric_field.c:35 */
 extern f3 *maga_float, *magb_float;             /* two time slices */
 extern f3 *efa_float, *efb_float;
 extern d3 *vc_double;                           /* src/velocity_field.c:34 */
+extern f3 *vc_float;                            /* src/velocity_field.c:35 */
+extern d3 *magc_double;                         /* src/magnetic_field.c:34 */
+extern d3 *efc_double;                          /* src/electric_field.c:34 */
+void get_double_bfield_at_point(double x, double y, double z, double t, double *bx,
+                                double *by, double *bz);
+void get_double_efield_at_point(double x, double y, double z, double t, double *ex,
+                                double *ey, double *ez);
 extern wlcs *config;                            /* src/wlcs.c:12 */
 extern int nwlcs, iBessel;                      /* src/wlcs.c:13 */
 extern double normI;                            /* src/wlcs.c:14 */
@@ -144,10 +160,11 @@ extern double emin, emax, logemin, logemax, logde; /* src/diagnostics.c:32 */
 
 typedef struct {
     int system, bc, nbins, nt_out, nx, ny, nz, nwlcs, ibessel, nsteps_output, mt, grid_nt;
+    int method;                 /* tracking_method of particle_tracking_hybrid */
     double lo[3], hi[3], tmax, qm, dt, pmass, emin, emax, ff_b0, ff_lambda, grid_dt;
     double gda_b0, gda_e0;
     double wlcs_omega, wlcs_normi;
-    char grid[512], vgrid[512], wlcs[512];
+    char grid[512], vgrid[512], wlcs[512], dgrid[512];
 } spec_t;
 
 static void die(const char *m) { fprintf(stderr, "ref_harness: %s\n", m); exit(2); }
@@ -188,10 +205,10 @@ static void read_spec(const char *path, spec_t *s)
 #define D(k) if (!strcmp(key, #k)) { s->k = strtod(val, NULL); continue; }
 #define S(k) if (!strcmp(key, #k)) { strcpy(s->k, val); continue; }
         I(system) I(bc) I(nbins) I(nt_out) I(nx) I(ny) I(nz) I(nwlcs) I(ibessel)
-        I(nsteps_output) I(mt) I(grid_nt)
+        I(nsteps_output) I(mt) I(grid_nt) I(method)
         D(grid_dt) D(gda_b0) D(gda_e0) D(tmax) D(qm) D(dt) D(pmass) D(emin) D(emax) D(ff_b0) D(ff_lambda)
         D(wlcs_omega) D(wlcs_normi)
-        S(grid) S(vgrid) S(wlcs)
+        S(grid) S(vgrid) S(wlcs) S(dgrid)
         if (!strcmp(key, "xmin")) { s->lo[0] = strtod(val, NULL); continue; }
         if (!strcmp(key, "ymin")) { s->lo[1] = strtod(val, NULL); continue; }
         if (!strcmp(key, "zmin")) { s->lo[2] = strtod(val, NULL); continue; }
@@ -275,6 +292,19 @@ static void setup(const spec_t *s)
             memcpy(magb_float, (char *)v + 3 * nn * sizeof(f3), nn * sizeof(f3));
         }
         free(v);
+    } else if (s->system == 3) {
+        /* MHD fields (src/domain.c:357-371): float velocity and magnetic
+         * nodes, one time slice; the file holds v (float3 AoS) then B */
+        set_variables_velocity(&g, &dom, 1.0, mt, (int)sizeof(float));
+        set_variables_bfield(&g, &dom, 1.0, mt, (int)sizeof(float));
+        initialize_vfield();
+        initialize_bfield();
+        if (mt) die("system 3 with two slices is not tracked by the reference");
+        float *v = slurp(s->grid, &n);
+        if (n != 2 * nn * sizeof(f3)) die("grid size");
+        memcpy(vc_float, v, nn * sizeof(f3));
+        memcpy(magc_float, (char *)v + nn * sizeof(f3), nn * sizeof(f3));
+        free(v);
     } else if (s->system == 1) {
         /* same table read_wlcs (src/wlcs.c:53-222) builds */
         double *v = slurp(s->wlcs, &n);
@@ -297,6 +327,18 @@ static void setup(const spec_t *s)
             config[i].cur_wr *= normI;
             config[i].cur_lp *= normI;
         }
+        free(v);
+    }
+    if (s->dgrid[0]) {
+        /* double E and B nodes (d3 AoS, E then B) for mode dfields */
+        set_variables_bfield(&g, &dom, 1.0, 0, (int)sizeof(double));
+        set_variables_efield(&g, &dom, 1.0, 0, (int)sizeof(double));
+        initialize_bfield();
+        initialize_efield();
+        double *v = slurp(s->dgrid, &n);
+        if (n != 2 * nn * sizeof(d3)) die("dgrid size");
+        memcpy(efc_double, v, nn * sizeof(d3));
+        memcpy(magc_double, (char *)v + nn * sizeof(d3), nn * sizeof(d3));
         free(v);
     }
     /* spectrum parameters as src/diagnostics.c:629-631 derives them */
@@ -322,6 +364,19 @@ int main(int argc, char **argv)
         emfields *out = malloc(np * sizeof(emfields) + 1);
         for (size_t i = 0; i < np; i++)
             get_emf(pts[4 * i], pts[4 * i + 1], pts[4 * i + 2], pts[4 * i + 3], &out[i]);
+        spit(argv[4], out, np * sizeof(emfields));
+    } else if (!strcmp(mode, "dfields")) {
+        if (!s.dgrid[0]) die("dfields needs dgrid");
+        double *pts = slurp(argv[3], &n);
+        size_t np = n / (4 * sizeof(double));
+        emfields *out = malloc(np * sizeof(emfields) + 1);
+        for (size_t i = 0; i < np; i++) {
+            const double *q = pts + 4 * i;
+            get_double_bfield_at_point(q[0], q[1], q[2], q[3], &out[i].Bx, &out[i].By,
+                                       &out[i].Bz);
+            get_double_efield_at_point(q[0], q[1], q[2], q[3], &out[i].Ex, &out[i].Ey,
+                                       &out[i].Ez);
+        }
         spit(argv[4], out, np * sizeof(emfields));
     } else if (!strcmp(mode, "wirz")) {
         particles *p = slurp(argv[3], &n);
@@ -497,9 +552,10 @@ int main(int argc, char **argv)
         calc_energy_spectrum(0, np, p, s.nbins, s.nt_out, s.pmass, s.bc);
         const double t_start = omp_get_wtime();
         particle_tracking_hybrid(0, np, s.dt, s.nbins, s.nt_out, s.bc,
-            s.nsteps_output, s.pmass, NULL, 0, 0, p0, nsteps, p, NULL);
+            s.nsteps_output, s.pmass, NULL, s.method, 0, p0, nsteps, p, NULL);
         printf("hybrid_seconds %.6f\nhybrid_threads %d\n", omp_get_wtime() - t_start,
                omp_get_max_threads());
+        if (argc > 4) spit(argv[4], p, n);
     } else if (!strcmp(mode, "pmap")) {
         /* the tracing loop of src/poincare_map.c:83-185 around the
          * reference's next_location, one MPI rank, lines over OpenMP threads

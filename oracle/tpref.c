@@ -469,6 +469,35 @@ static void get_emf(const tpref_field *f, double x, double y, double z,
         }
         break;
     }
+    case TPREF_FIELD_MHD: {
+        /* getemf_mhd_test_particle, src/emfields.c:107-120: the float
+         * velocity and magnetic gathers (get_float_velocity_at_point,
+         * src/velocity_field.c:593-622; get_float_bfield_at_point,
+         * src/magnetic_field.c:593-622), E = B x v in double */
+        long c[8];
+        double w[8];
+        float v[3], b[3];
+        trilinear_ref(f->nx, f->ny, f->nz, f->lo, f->hi, x, y, z, c, w);
+        gather_float_ref(f->egrid, c, w, v);
+        gather_float_ref(f->bgrid, c, w, b);
+        o->Bx = b[0]; o->By = b[1]; o->Bz = b[2];
+        o->Ex = o->By * v[2] - o->Bz * v[1];
+        o->Ey = o->Bz * v[0] - o->Bx * v[2];
+        o->Ez = o->Bx * v[1] - o->By * v[0];
+        break;
+    }
+    case TPREF_FIELD_GRIDD: {
+        /* get_double_efield_at_point / get_double_bfield_at_point
+         * (src/electric_field.c:370-406, src/magnetic_field.c:555-591) */
+        long c[8];
+        double w[8], e[3], b[3];
+        trilinear_ref(f->nx, f->ny, f->nz, f->lo, f->hi, x, y, z, c, w);
+        gather_double(NULL, f->egrid_d, c, w, e);
+        gather_double(NULL, f->bgrid_d, c, w, b);
+        o->Ex = e[0]; o->Ey = e[1]; o->Ez = e[2];
+        o->Bx = b[0]; o->By = b[1]; o->Bz = b[2];
+        break;
+    }
     case TPREF_FIELD_MODES: {
         /* B = b0z z + sum_n a_n cos(k_n.x + phi_n); the phase is built
          * as fma(kz, z, fma(ky, y, fma(kx, x, phi))) and the sum uses fma */
@@ -843,8 +872,8 @@ int tpref_track_fixed(const tpref_field *f, const tp_track_params *q,
 
 
 /* ---------------------------------------------------------------------- */
-/* adaptive Bulirsch-Stoer (algorithm of src/unused/StepperBS.c:103-435,   */
-/* NR3 ch. 17, without dense output), per-particle controller state,       */
+/* adaptive Bulirsch-Stoer (algorithm of src/unused/StepperBS.c:103-644,   */
+/* NR3 ch. 17, dense output optional), per-particle controller state,      */
 /* ODE in (x, u) with du/dt = qm (E + u/gamma x B).                        */
 /* ---------------------------------------------------------------------- */
 #define BS_N 6
@@ -856,16 +885,22 @@ int tpref_track_fixed(const tpref_field *f, const tp_track_params *q,
 #define BS_KFAC2 0.9
 
 typedef struct {
-    int kmax, ktarg0;
-    int nseq[9], cost[9];
-    double coeff[9][9];
+    int kmax, ktarg0, dense;
+    int nseq[10], cost[10], ipoint[11];
+    double coeff[10][10];
+    double errfac[19];
 } bs_consts;
 
-/* StepperBase, src/unused/StepperBS.c:103-131 (non-dense sequence) */
-static void bs_setup(int kmax, double rtol, bs_consts *c)
+/* StepperBase, src/unused/StepperBS.c:103-149: the step-size sequence
+ * (dense output: nseq = 4k+2, else 2(k+1)), costs, k_targ from rtol, the
+ * extrapolation coefficients, the dense interpolation error factors and
+ * ipoint, where each sequence's derivatives start in fsave */
+static void bs_setup(int kmax, double rtol, int dense, bs_consts *c)
 {
+    memset(c, 0, sizeof(*c));
     c->kmax = kmax;
-    for (int i = 0; i <= kmax; i++) c->nseq[i] = 2 * (i + 1);
+    c->dense = dense;
+    for (int i = 0; i <= kmax; i++) c->nseq[i] = dense ? 4 * i + 2 : 2 * (i + 1);
     c->cost[0] = c->nseq[0] + 1;
     for (int k = 0; k < kmax; k++) c->cost[k + 1] = c->cost[k] + c->nseq[k + 1];
     double logfact = -log10(rtol > 1.0e-12 ? rtol : 1.0e-12) * 0.6 + 0.5;
@@ -878,6 +913,18 @@ static void bs_setup(int kmax, double rtol, bs_consts *c)
             double ratio = (double)c->nseq[k] / c->nseq[l];
             c->coeff[k][l] = 1.0 / (ratio * ratio - 1.0);
         }
+    for (int i = 0; i < 2 * (kmax + 1) + 1; i++) {
+        int ip5 = i + 5;
+        c->errfac[i] = 1.0 / (ip5 * ip5);
+        double e = 0.5 * sqrt(((double)(i + 1)) / ip5);
+        for (int j = 0; j <= i; j++) c->errfac[i] *= e / (j + 1);
+    }
+    c->ipoint[0] = 0;
+    for (int i = 1; i <= kmax + 1; i++) {
+        int njadd = 4 * i - 2;
+        if (c->nseq[i - 1] > njadd) njadd++;
+        c->ipoint[i] = c->ipoint[i - 1] + njadd;
+    }
 }
 
 static void bs_rhs(const tpref_field *f, double qm, double t, const double *y, double *dy)
@@ -895,6 +942,16 @@ static void bs_rhs(const tpref_field *f, double qm, double t, const double *y, d
     dy[5] = qm * (e.Ez + (vx * e.By - vy * e.Bx));
 }
 
+/* dense-output storage of one particle (StepperBS's ysave, fsave, dens) */
+#define BS_FSAVE 162            /* ipoint[9] = 2 * 9^2 for kmax = 8 */
+typedef struct {
+    double ysave[9][BS_N];
+    double fsave[BS_FSAVE][BS_N];
+    double dens[(2 * 9 + 5) * BS_N];
+    int mu;
+    double xold, h;             /* the step the polynomial covers */
+} bs_dense;
+
 typedef struct {
     double t, hnext;
     double dydx[BS_N];
@@ -903,10 +960,14 @@ typedef struct {
     int acc, rej, attempts;
 } bs_state;
 
-/* updatey, src/unused/StepperBS.c:364-414 (non-dense part) */
+/* updatey, src/unused/StepperBS.c:364-414: the modified midpoint rule over
+ * nstep substeps.  With dense output the state at substep nstep/2 goes to
+ * ysave[k] and every substep's derivative (substeps 1 .. nstep) to fsave,
+ * consecutive rows from *ipt on (the reference passes ipt by value, so its
+ * rows would all start at 0; NR3's dy passes it by reference, as here). */
 static void bs_midpoint(const tpref_field *f, double qm, double x, const double *y,
-                        const double *dydx, double htot, int nstep, double *yend,
-                        long long *rhs)
+                        const double *dydx, double htot, int k, int nstep, double *yend,
+                        long long *rhs, bs_dense *d, int *ipt)
 {
     double ym[BS_N], yn[BS_N];
     double h = htot / nstep;
@@ -919,6 +980,12 @@ static void bs_midpoint(const tpref_field *f, double qm, double x, const double 
     (*rhs)++;
     double h2 = 2.0 * h;
     for (int nn = 1; nn < nstep; nn++) {
+        if (d && nn == nstep / 2)
+            for (int i = 0; i < BS_N; i++) d->ysave[k][i] = yn[i];
+        if (d && abs(nn - nstep / 2) <= 2 * k + 1) {
+            (*ipt)++;
+            for (int i = 0; i < BS_N; i++) d->fsave[*ipt][i] = yend[i];
+        }
         for (int i = 0; i < BS_N; i++) {
             double swap = ym[i] + h2 * yend[i];
             ym[i] = yn[i];
@@ -928,17 +995,163 @@ static void bs_midpoint(const tpref_field *f, double qm, double x, const double 
         bs_rhs(f, qm, xnew, yn, yend);
         (*rhs)++;
     }
+    if (d && nstep / 2 <= 2 * k + 1) {
+        (*ipt)++;
+        for (int i = 0; i < BS_N; i++) d->fsave[*ipt][i] = yend[i];
+    }
     for (int i = 0; i < BS_N; i++) yend[i] = 0.5 * (ym[i] + yn[i] + h * yend[i]);
 }
 
-/* step, src/unused/StepperBS.c:159-354 (non-dense).  Returns 0, or 1 when
- * the attempt cap was reached. */
-static int bs_step(const tpref_field *f, const tp_bs_params *prm, const bs_consts *c,
-                   bs_state *st, double *y, double htry, int cap)
+/* dense_interp, src/unused/StepperBS.c:478-524: the coefficients of the
+ * interpolating polynomial from the values prepare_dense stored in y */
+static void bs_dense_interp(double *y, int imit)
 {
-    double table[8][BS_N], hopt[9], work[9];
-    double ysav[BS_N], yseq[BS_N];
-    double h = htry, hnew, err = 0.0;
+    double a[31];
+    for (int i = 0; i < BS_N; i++) {
+        const int n = BS_N;
+        double y0 = y[i], y1 = y[2 * n + i], yp0 = y[n + i], yp1 = y[3 * n + i];
+        double ydiff = y1 - y0;
+        double aspl = -yp1 + ydiff;
+        double bspl = yp0 - ydiff;
+        y[n + i] = ydiff;
+        y[2 * n + i] = aspl;
+        y[3 * n + i] = bspl;
+        if (imit < 0) continue;
+        double ph0 = (y0 + y1) * 0.5 + 0.125 * (aspl + bspl);
+        double ph1 = ydiff + (aspl - bspl) * 0.25;
+        double ph2 = -(yp0 - yp1);
+        double ph3 = 6.0 * (bspl - aspl);
+        if (imit >= 1) {
+            a[1] = 16.0 * (y[5 * n + i] - ph1);
+            if (imit >= 3) {
+                a[3] = 16.0 * (y[7 * n + i] - ph3 + 3 * a[1]);
+                for (int im = 5; im <= imit; im += 2) {
+                    double fac1 = im * (im - 1) / 2.0;
+                    double fac2 = fac1 * (im - 2) * (im - 3) * 2.0;
+                    a[im] = 16.0 * (y[(im + 4) * n + i] + fac1 * a[im - 2] - fac2 * a[im - 4]);
+                }
+            }
+        }
+        a[0] = (y[4 * n + i] - ph0) * 16.0;
+        if (imit >= 2) {
+            a[2] = (y[n * 6 + i] - ph2 + a[0]) * 16.0;
+            for (int im = 4; im <= imit; im += 2) {
+                double fac1 = im * (im - 1) / 2.0;
+                double fac2 = (double)(im * (im - 1) * (im - 2) * (im - 3));
+                a[im] = 16.0 * (y[(im + 4) * n + i] + fac1 * a[im - 2] - fac2 * a[im - 4]);
+            }
+        }
+        for (int im = 0; im <= imit; im++) y[n * (im + 4) + i] = a[im];
+    }
+}
+
+/* prepare_dense, src/unused/StepperBS.c:526-615: the dense-output values at
+ * the step of size h (ysav at its start, y and dydxnew at its end, dydx the
+ * derivative at the start), extrapolated from the midpoint values and
+ * derivative differences of the first k+1 sequences; returns the
+ * interpolation error estimate */
+static double bs_prepare_dense(const bs_consts *c, double h, const double *dydxnew,
+                               const double *ysav, const double *scale, int k,
+                               const double *y, const double *dydx, bs_dense *d)
+{
+    const int n = BS_N;
+    const int mu = 2 * k - 1;
+    double *dens = d->dens;
+    for (int i = 0; i < n; i++) {
+        dens[i] = ysav[i];
+        dens[n + i] = h * dydx[i];
+        dens[2 * n + i] = y[i];
+        dens[3 * n + i] = dydxnew[i] * h;
+    }
+    for (int j = 1; j <= k; j++) {
+        double dblenj = c->nseq[j];
+        for (int l = j; l >= 1; l--) {
+            double r = dblenj / c->nseq[l - 1];
+            double factor = r * r - 1.0;
+            for (int i = 0; i < n; i++)
+                d->ysave[l - 1][i] = d->ysave[l][i] + (d->ysave[l][i] - d->ysave[l - 1][i]) / factor;
+        }
+    }
+    for (int i = 0; i < n; i++) dens[4 * n + i] = d->ysave[0][i];
+    for (int kmi = 1; kmi <= mu; kmi++) {
+        int kbeg = (kmi - 1) / 2;
+        for (int kk = kbeg; kk <= k; kk++) {
+            double facnj = tp_powi(c->nseq[kk] / 2.0, kmi - 1);
+            int ipt = c->ipoint[kk + 1] - 2 * kk + kmi - 3;
+            for (int i = 0; i < n; i++) d->ysave[kk][i] = d->fsave[ipt][i] * facnj;
+        }
+        for (int j = kbeg + 1; j <= k; j++) {
+            double dblenj = c->nseq[j];
+            for (int l = j; l >= kbeg + 1; l--) {
+                double r = dblenj / c->nseq[l - 1];
+                double factor = r * r - 1.0;
+                for (int i = 0; i < n; i++)
+                    d->ysave[l - 1][i] =
+                        d->ysave[l][i] + (d->ysave[l][i] - d->ysave[l - 1][i]) / factor;
+            }
+        }
+        for (int i = 0; i < n; i++) dens[(kmi + 4) * n + i] = d->ysave[kbeg][i] * h;
+        if (kmi == mu) continue;
+        for (int kk = kmi / 2; kk <= k; kk++) {
+            int lbeg = c->ipoint[kk + 1] - 1;
+            int lend = c->ipoint[kk] + kmi;
+            if (kmi == 1) lend += 2;
+            for (int l = lbeg; l >= lend; l -= 2)
+                for (int i = 0; i < n; i++) d->fsave[l][i] = d->fsave[l][i] - d->fsave[l - 2][i];
+            if (kmi == 1) {
+                int l = lend - 2;
+                for (int i = 0; i < n; i++) d->fsave[l][i] = d->fsave[l][i] - dydx[i];
+            }
+        }
+        for (int kk = kmi / 2; kk <= k; kk++) {
+            int lbeg = c->ipoint[kk + 1] - 2;
+            int lend = c->ipoint[kk] + kmi + 1;
+            for (int l = lbeg; l >= lend; l -= 2)
+                for (int i = 0; i < n; i++) d->fsave[l][i] = d->fsave[l][i] - d->fsave[l - 2][i];
+        }
+    }
+    bs_dense_interp(dens, mu);
+    d->mu = mu;
+    double error = 0.0;
+    if (mu >= 1) {
+        for (int i = 0; i < n; i++) {
+            double q = dens[(mu + 4) * n + i] / scale[i];
+            error += q * q;
+        }
+        error = sqrt(error / n) * c->errfac[mu - 1];
+    }
+    return error;
+}
+
+/* dense_out, src/unused/StepperBS.c:617-644: component i at time x in the
+ * step [xold, xold + h], with the polynomial of the last prepare_dense */
+static double bs_dense_out(const bs_dense *d, int i, double x)
+{
+    const int n = BS_N, mu = d->mu;
+    const double *dens = d->dens;
+    double theta = (x - d->xold) / d->h;
+    double theta1 = 1.0 - theta;
+    double yinterp = dens[i] + theta * (dens[n + i] + theta1 * (dens[2 * n + i] * theta +
+                                                                dens[3 * n + i] * theta1));
+    if (mu < 0) return yinterp;
+    double theta05 = theta - 0.5;
+    double tt = theta * theta1;
+    double t4 = tt * tt;
+    double cc = dens[n * (mu + 4) + i];
+    for (int j = mu; j > 0; j--) cc = dens[n * (j + 3) + i] + cc * theta05 / j;
+    return yinterp + t4 * cc;
+}
+
+/* step, src/unused/StepperBS.c:159-354.  With d (dense output) the accepted
+ * step's interpolation error is estimated by prepare_dense; above 10 the
+ * step is redone with the step size it suggests, and the next step size is
+ * capped by it.  Returns 0, or 1 when the attempt cap was reached. */
+static int bs_step(const tpref_field *f, const tp_bs_params *prm, const bs_consts *c,
+                   bs_state *st, double *y, double htry, int cap, bs_dense *d)
+{
+    double table[9][BS_N], hopt[10], work[10];
+    double ysav[BS_N], yseq[BS_N], scale[BS_N];
+    double h = htry, hnew, err = 0.0, hopt_int = 0.0;
     int k = 0;
     work[0] = 0.0;
     for (int i = 0; i < BS_N; i++) ysav[i] = y[i];
@@ -950,6 +1163,8 @@ static int bs_step(const tpref_field *f, const tp_bs_params *prm, const bs_const
     st->reject = 0;
     int firstk = 1;
     hnew = fabs(h);
+    double dnew[BS_N];
+    for (;;) {
     while (firstk || st->reject) {
         h = hnew;
         firstk = 0;
@@ -958,8 +1173,10 @@ static int bs_step(const tpref_field *f, const tp_bs_params *prm, const bs_const
             for (int i = 0; i < BS_N; i++) y[i] = ysav[i];
             return 1;
         }
+        int ipt = -1;
         for (k = 0; k <= st->ktarg + 1; k++) {
-            bs_midpoint(f, prm->qm, st->t, ysav, st->dydx, h, c->nseq[k], yseq, &st->rhs);
+            bs_midpoint(f, prm->qm, st->t, ysav, st->dydx, h, k, c->nseq[k], yseq, &st->rhs,
+                        d, &ipt);
             if (k == 0) {
                 for (int i = 0; i < BS_N; i++) y[i] = yseq[i];
                 continue;
@@ -974,8 +1191,8 @@ static int bs_step(const tpref_field *f, const tp_bs_params *prm, const bs_const
             err = 0.0;
             for (int i = 0; i < BS_N; i++) {
                 double a = fabs(ysav[i]), b = fabs(y[i]);
-                double scale = prm->atol + prm->rtol * (a > b ? a : b);
-                double q = (y[i] - table[0][i]) / scale;
+                scale[i] = prm->atol + prm->rtol * (a > b ? a : b);
+                double q = (y[i] - table[0][i]) / scale[i];
                 err += q * q;
             }
             err = sqrt(err / BS_N);
@@ -1030,10 +1247,24 @@ static int bs_step(const tpref_field *f, const tp_bs_params *prm, const bs_const
             st->rej++;
         }
     }
-    double dnew[BS_N];
     bs_rhs(f, prm->qm, st->t + h, y, dnew);
     st->rhs++;
+    if (!d) break;
+    double err_int = bs_prepare_dense(c, h, dnew, ysav, scale, k, y, st->dydx, d);
+    double pe = tp_pow(err_int, 1.0 / (2 * k + 3));
+    hopt_int = h / (pe > 0.01 ? pe : 0.01);
+    if (err_int <= 10.0) break;
+    /* interpolation error too large: the step again, with hopt_int */
+    hnew = fabs(hopt_int);
+    st->reject = 1;
+    st->prev_reject = 1;
+    st->rej++;
+    }
     for (int i = 0; i < BS_N; i++) st->dydx[i] = dnew[i];
+    if (d) {
+        d->xold = st->t;
+        d->h = h;
+    }
     st->t += h;
     st->first = 0;
     st->acc++;
@@ -1066,6 +1297,10 @@ static int bs_step(const tpref_field *f, const tp_bs_params *prm, const bs_const
         }
         st->ktarg = kopt;
     }
+    if (d) {
+        double hi = fabs(hopt_int);
+        hnew = hnew < hi ? hnew : hi;
+    }
     st->hnext = hnew;
     return 0;
 }
@@ -1084,7 +1319,7 @@ int tpref_bs_track(const tpref_field *f, const tp_bs_params *prm, int n, tp_part
 {
     if (n < 0 || prm->kmax < 3 || prm->kmax > 8 || !(prm->rtol > 0.0)) return TP_E_ARG;
     bs_consts c;
-    bs_setup(prm->kmax, prm->rtol, &c);
+    bs_setup(prm->kmax, prm->rtol, 0, &c);
     int cap = prm->max_attempts > 0 ? prm->max_attempts : 10000000;
     if (nthreads <= 0) nthreads = omp_get_max_threads();
     long long acc = 0, rej = 0, rhs = 0, failed = 0;
@@ -1107,7 +1342,7 @@ int tpref_bs_track(const tpref_field *f, const tp_bs_params *prm, int n, tp_part
         while (st.t < tend - 1e-13 * fabs(tend)) {
             double h = st.hnext;
             if (st.t + h > tend) h = tend - st.t;
-            if (bs_step(f, prm, &c, &st, y, h, cap)) {
+            if (bs_step(f, prm, &c, &st, y, h, cap, NULL)) {
                 fail = 1;
                 break;
             }
@@ -1122,6 +1357,184 @@ int tpref_bs_track(const tpref_field *f, const tp_bs_params *prm, int n, tp_part
         failed += fail;
     }
     if (seconds) *seconds = omp_get_wtime() - t1;
+    if (totals) {
+        totals[0] = acc; totals[1] = rej; totals[2] = rhs; totals[3] = failed;
+    }
+    return TP_OK;
+}
+
+/* The adaptive loop with the fixed-step loop's diagnostics: every particle
+ * from its time t0 to t0 + estep*dt with dense output, particle_bc after
+ * each accepted step (the commented particle_tracking_adaptive,
+ * src/tracking.c:518-630), and at the output times of the fixed-step loop
+ * (t0 + j*dt for the steps j that record, src/tracking.c:333-372) the
+ * spectrum (j % einterval_t == 0) and the D terms (j % dcoeff_interval ==
+ * 0) of the state the dense-output polynomial gives there.  Output times
+ * strictly inside a step use the polynomial of that step; at the end time
+ * the final state is used.  Escapes (bc 1) are binned into the escape
+ * spectrum at the frame of the next output time and the particle stops;
+ * its later D rows carry stale terms and it counts against the remaining
+ * particles as the fixed-step loop counts it.  D terms are kept per
+ * particle and row and summed in particle order. */
+static void bs_dterms(const tpref_field *f, const tp_particle *p0, const tp_particle *p,
+                      double dt, int escaped, int fix_stale, double stale[5], double o[8])
+{
+    if (escaped) {
+        o[0] = 0.0; o[4] = 0.0; o[5] = 0.0;
+        if (fix_stale) stale[0] = stale[1] = stale[2] = stale[3] = stale[4] = 0.0;
+        o[1] = stale[0]; o[2] = stale[1]; o[3] = stale[2]; o[6] = stale[3]; o[7] = stale[4];
+        return;
+    }
+    tpref_diff_coeff(f, p0, p, dt, o);
+    stale[0] = o[1]; stale[1] = o[2]; stale[2] = o[3]; stale[3] = o[6]; stale[4] = o[7];
+}
+
+int tpref_bs_track_diag(const tpref_field *f, const tp_bs_params *bs, const tp_track_params *prm,
+                        int n, tp_particle *ptl, const tp_particle *ptl_init, int *nsteps,
+                        double *espect, double *espect_escape, double *dcoeff, int *nptl_remain,
+                        long long *totals, int nthreads, double *seconds)
+{
+    if (n < 0 || bs->kmax < 3 || bs->kmax > 8 || !(bs->rtol > 0.0) || prm->sstep != 1 ||
+        prm->estep < 1 || prm->einterval_t <= 0 || prm->dcoeff_interval <= 0 ||
+        prm->nbins < 1 || prm->nt_out < 1 || !(prm->dt > 0.0))
+        return TP_E_ARG;
+    bs_consts c;
+    bs_setup(bs->kmax, bs->rtol, 1, &c);
+    const int cap = bs->max_attempts > 0 ? bs->max_attempts : 10000000;
+    const int nb = prm->nbins, nt = prm->nt_out, ei = prm->einterval_t, di = prm->dcoeff_interval;
+    const int nd = prm->estep / di + 1;
+    const int fix_stale = (prm->flags & TP_FLAG_FIX_STALE_DCOEFF) != 0;
+    /* frames that exist: steps (f+1)*ei <= estep */
+    int nframes = prm->estep / ei;
+    if (nframes > nt) nframes = nt;
+    double *terms = calloc((size_t)n * nd * 8 + 1, sizeof(double));
+    int *out_from = malloc(sizeof(int) * (n + 1));   /* first row counting it out */
+    if (!terms || !out_from) { free(terms); free(out_from); return TP_E_NOMEM; }
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    long long acc = 0, rej = 0, rhs = 0, failed = 0;
+    unsigned *cnt = calloc((size_t)nthreads * nb * nt * 2 + 1, sizeof(unsigned));
+    if (!cnt) { free(terms); free(out_from); return TP_E_NOMEM; }
+    double t1 = omp_get_wtime();
+#pragma omp parallel num_threads(nthreads) reduction(+ : acc, rej, rhs, failed)
+    {
+        unsigned *hsp = cnt + (size_t)omp_get_thread_num() * nb * nt * 2;
+        unsigned *hes = hsp + (size_t)nb * nt;
+        bs_dense *d = malloc(sizeof(bs_dense));
+#pragma omp for schedule(dynamic, 16)
+        for (int i = 0; i < n; i++) {
+            tp_particle q = ptl[i];
+            const tp_particle *p0 = &ptl_init[i];
+            double *ti = terms + (size_t)i * nd * 8;
+            double stale[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+            const double t0 = q.t;
+            const double tend = t0 + prm->estep * prm->dt;
+            int esc = 0, nf = 0, nr = 1, fail = 0;
+            bs_state st;
+            memset(&st, 0, sizeof(st));
+            tpref_particle_bc(prm, &q, &esc);          /* src/tracking.c:314-332 */
+            const int init_esc = esc;
+            if (!esc) {
+                double y[BS_N] = {q.x, q.y, q.z, q.ux, q.uy, q.uz};
+                double g0 = sqrt(1.0 + q.ux * q.ux + q.uy * q.uy + q.uz * q.uz);
+                double T = 2.0 * M_PI * g0 / (fabs(bs->qm) * (bs->b_ref > 0.0 ? bs->b_ref : 1.0));
+                st.t = t0;
+                st.hnext = bs->h0 > 0.0 ? bs->h0 : 0.01 * T;
+                st.ktarg = c.ktarg0;
+                st.first = 1;
+                bs_rhs(f, bs->qm, st.t, y, st.dydx);
+                st.rhs = 1;
+                while (st.t < tend - 1e-13 * fabs(tend)) {
+                    double h = st.hnext;
+                    if (st.t + h > tend) h = tend - st.t;
+                    if (bs_step(f, bs, &c, &st, y, h, cap, d)) { fail = 1; break; }
+                    /* output times strictly inside the step, from the polynomial
+                     * (the state relative to the step's start; shifts as then) */
+                    tp_particle e = q;
+                    for (; nf < nframes; nf++) {
+                        const double tf = t0 + (double)((nf + 1) * ei) * prm->dt;
+                        if (!(tf < st.t)) break;
+                        e.ux = bs_dense_out(d, 3, tf);
+                        e.uy = bs_dense_out(d, 4, tf);
+                        e.uz = bs_dense_out(d, 5, tf);
+                        hsp[nf * nb + tpref_energy_bin(&e, nb, prm->logemin, prm->logde)]++;
+                    }
+                    for (; nr < nd; nr++) {
+                        const double tr = t0 + (double)(nr * di) * prm->dt;
+                        if (!(tr < st.t)) break;
+                        double yd[BS_N];
+                        for (int m = 0; m < BS_N; m++) yd[m] = bs_dense_out(d, m, tr);
+                        e.x = yd[0]; e.y = yd[1]; e.z = yd[2];
+                        e.ux = yd[3]; e.uy = yd[4]; e.uz = yd[5]; e.t = tr;
+                        bs_dterms(f, p0, &e, prm->dt, 0, fix_stale, stale, ti + 8 * nr);
+                    }
+                    /* the step's end: boundary, escape */
+                    q.x = y[0]; q.y = y[1]; q.z = y[2];
+                    q.ux = y[3]; q.uy = y[4]; q.uz = y[5]; q.t = st.t;
+                    tpref_particle_bc(prm, &q, &esc);
+                    y[0] = q.x; y[1] = q.y; y[2] = q.z;
+                    if (esc) {
+                        if (nf < nt)
+                            hes[nf * nb + tpref_energy_bin(&q, nb, prm->logemin, prm->logde)]++;
+                        break;
+                    }
+                }
+                if (!esc && !fail) {
+                    /* the remaining output times are the end time: the final state */
+                    q.t = st.t;
+                    for (; nf < nframes; nf++)
+                        hsp[nf * nb + tpref_energy_bin(&q, nb, prm->logemin, prm->logde)]++;
+                    for (; nr < nd; nr++) {
+                        tp_particle e = q;
+                        e.t = t0 + (double)(nr * di) * prm->dt;
+                        bs_dterms(f, p0, &e, prm->dt, 0, fix_stale, stale, ti + 8 * nr);
+                    }
+                }
+            }
+            /* rows from nr on: the particle has left (or failed); as the
+             * fixed-step loop counts it, an initial escape is out from row
+             * 1, an escape before D row r from row r + 1 */
+            out_from[i] = !esc ? nd : (init_esc ? 1 : nr + 1);
+            for (int r = nr; r < nd; r++) bs_dterms(f, p0, &q, prm->dt, 1, fix_stale, stale, ti + 8 * r);
+            ptl[i] = q;
+            if (nsteps) nsteps[i] = st.acc;
+            acc += st.acc;
+            rej += st.rej;
+            rhs += st.rhs;
+            failed += fail;
+        }
+        free(d);
+    }
+    if (seconds) *seconds = omp_get_wtime() - t1;
+    /* spectra: integer counts over the threads */
+    for (size_t k = 0; k < (size_t)nb * nt; k++) {
+        unsigned long long a = 0, b = 0;
+        for (int th = 0; th < nthreads; th++) {
+            a += cnt[(size_t)th * nb * nt * 2 + k];
+            b += cnt[(size_t)th * nb * nt * 2 + (size_t)nb * nt + k];
+        }
+        if (espect) espect[k] = (double)a;
+        if (espect_escape) espect_escape[k] = (double)b;
+    }
+    /* D sums in particle order; row 0 stays 0 */
+    if (dcoeff) {
+        memset(dcoeff, 0, sizeof(double) * 8 * nd);
+        for (int r = 1; r < nd; r++)
+            for (int q = 0; q < 8; q++) {
+                double s = 0.0;
+                for (int i = 0; i < n; i++) s += terms[((size_t)i * nd + r) * 8 + q];
+                dcoeff[(size_t)q * nd + r] = s;
+            }
+    }
+    if (nptl_remain) {
+        for (int k = 0; k < nd; k++) {
+            int out = 0;
+            for (int i = 0; i < n; i++) out += out_from[i] <= k;
+            nptl_remain[k] = n - out;
+        }
+    }
+    free(terms);
+    free(out_from);
+    free(cnt);
     if (totals) {
         totals[0] = acc; totals[1] = rej; totals[2] = rhs; totals[3] = failed;
     }

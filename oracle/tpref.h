@@ -21,6 +21,8 @@ extern "C" {
 #define TPREF_FIELD_ABC   2
 #define TPREF_FIELD_GRID  4
 #define TPREF_FIELD_MODES 5
+#define TPREF_FIELD_MHD   3   /* system 3: float v (egrid) and B (bgrid), E = B x v */
+#define TPREF_FIELD_GRIDD 6   /* PIC grid of double E and B (egrid_d, bgrid_d) */
 
 typedef struct tpref_field {
     int kind;
@@ -50,6 +52,8 @@ typedef struct tpref_field {
      * one (efb/magb), grid_dt the time between slices */
     const float *egrid_b, *bgrid_b;
     double grid_dt;
+    /* double E and B nodes (TPREF_FIELD_GRIDD), double3 AoS, x fastest */
+    const double *egrid_d, *bgrid_d;
 } tpref_field;
 
 /* 1: take sin/cos from libm (as the reference does); 0: tp_sincos, the
@@ -82,6 +86,14 @@ int  tpref_track_fixed(const tpref_field *f, const tp_track_params *prm,
 int  tpref_bs_track(const tpref_field *f, const tp_bs_params *prm, int n,
                     tp_particle *ptl, int *nsteps, long long *totals,
                     int nthreads, double *seconds);
+/* the same with NR3 dense output, particle_bc after each accepted step and
+ * the fixed-step loop's diagnostics at its output times (sstep must be 1;
+ * the end time is t + estep*dt) */
+int  tpref_bs_track_diag(const tpref_field *f, const tp_bs_params *bs,
+                         const tp_track_params *prm, int n, tp_particle *ptl,
+                         const tp_particle *ptl_init, int *nsteps, double *espect,
+                         double *espect_escape, double *dcoeff, int *nptl_remain,
+                         long long *totals, int nthreads, double *seconds);
 /* Poincare map of the field at t = 0 (src/poincare_map.c:39-258); arguments
  * as for tp_gpu_poincare_map.  Returns 0 or TP_E_ARG. */
 int  tpref_poincare_map(const tpref_field *f, const tp_pmap_params *prm, int n,

@@ -16,6 +16,7 @@ LIB = os.path.join(HERE, "libtpref.so")
 REF_DIR = os.path.join(HERE, "_ref")
 
 FIELD_TEST, FIELD_WLCS, FIELD_ABC, FIELD_GRID, FIELD_MODES = 0, 1, 2, 4, 5
+FIELD_MHD, FIELD_GRIDD = 3, 6
 
 
 class TprefField(ctypes.Structure):
@@ -42,6 +43,8 @@ class TprefField(ctypes.Structure):
         ("egrid_b", ctypes.POINTER(ctypes.c_float)),
         ("bgrid_b", ctypes.POINTER(ctypes.c_float)),
         ("grid_dt", ctypes.c_double),
+        ("egrid_d", ctypes.POINTER(ctypes.c_double)),
+        ("bgrid_d", ctypes.POINTER(ctypes.c_double)),
     ]
 
 
@@ -75,6 +78,10 @@ def lib():
         L.tpref_bs_track.argtypes = [P(TprefField), P(abi.TpBsParams), i, P(d), P(i),
                                      P(ctypes.c_longlong), i, P(d)]
         L.tpref_bs_track.restype = i
+        L.tpref_bs_track_diag.argtypes = [P(TprefField), P(abi.TpBsParams),
+                                          P(abi.TpTrackParams), i, P(d), P(d), P(i), P(d),
+                                          P(d), P(d), P(i), P(ctypes.c_longlong), i, P(d)]
+        L.tpref_bs_track_diag.restype = i
         L.tpref_poincare_map.argtypes = [P(TprefField), P(abi.TpPmapParams), i, P(d), P(d),
                                          P(d), P(d), P(i), P(ctypes.c_longlong), i, P(d)]
         L.tpref_poincare_map.restype = i
@@ -153,6 +160,30 @@ class Field:
         f.s.egrid_b = abi.fptr(eb)
         f.s.bgrid_b = abi.fptr(bb)
         f.s.grid_dt = float(grid_dt)
+        return f
+
+    @classmethod
+    def mhd(cls, v, b, lo, hi):
+        """System 3: v, b (nz, ny, nx, 3) float32 velocity and magnetic nodes;
+        E = B x v (src/emfields.c:107-120)."""
+        f = cls.grid(v, b, lo, hi, abi.TP_GATHER_REF_COMPAT)
+        f.s.kind = FIELD_MHD
+        return f
+
+    @classmethod
+    def grid_double(cls, e, b, lo, hi):
+        """e, b: (nz, ny, nx, 3) float64 nodes; the reference's double gathers
+        (src/electric_field.c:370-406, src/magnetic_field.c:555-591)."""
+        f = cls(FIELD_GRIDD)
+        e = f._hold(e, np.float64)
+        b = f._hold(b, np.float64)
+        f.s.nz, f.s.ny, f.s.nx = e.shape[:3]
+        for k in range(3):
+            f.s.lo[k] = lo[k]
+            f.s.hi[k] = hi[k]
+        f.s.egrid_d = abi.dptr(e)
+        f.s.bgrid_d = abi.dptr(b)
+        f.s.gather_mode = abi.TP_GATHER_REF_COMPAT
         return f
 
     @classmethod
@@ -287,6 +318,32 @@ def poincare_map(field, params, starts, nthreads=1):
     if rc != 0:
         raise ValueError(f"tpref_poincare_map: {rc}")
     return dict(xpos=xp, ypos=yp, zpos=zp, ncrossed=nc, nsteps=nst, seconds=sec.value)
+
+
+def bs_track_diag(field, bs, params, ptl, ptl_init=None, nthreads=1):
+    """Adaptive Bulirsch-Stoer with dense output over estep*dt, with the
+    fixed-step loop's diagnostics at its output times; returns what
+    track_fixed returns (nsteps = accepted steps) and the step totals."""
+    ptl = abi.as_particles(ptl).copy()
+    ptl_init = ptl.copy() if ptl_init is None else abi.as_particles(ptl_init)
+    n = ptl.shape[0]
+    nd = abi.n_dcoeff_rows(params)
+    nsteps = np.zeros(n, dtype=np.int32)
+    esp = np.zeros((params.nt_out, params.nbins))
+    ese = np.zeros((params.nt_out, params.nbins))
+    dco = np.zeros((8, nd))
+    rem = np.zeros(nd, dtype=np.int32)
+    tot = (ctypes.c_longlong * 4)()
+    secs = ctypes.c_double()
+    rc = lib().tpref_bs_track_diag(ctypes.byref(field.s), ctypes.byref(bs), ctypes.byref(params),
+                                   n, abi.dptr(ptl), abi.dptr(ptl_init), abi.iptr(nsteps),
+                                   abi.dptr(esp), abi.dptr(ese), abi.dptr(dco), abi.iptr(rem),
+                                   tot, nthreads, ctypes.byref(secs))
+    if rc != 0:
+        raise RuntimeError(f"tpref_bs_track_diag failed: {rc}")
+    return dict(ptl=ptl, nsteps=nsteps, espect=esp, espect_escape=ese, dcoeff=dco,
+                remain=rem, accepted=tot[0], rejected=tot[1], rhs=tot[2], failed=tot[3],
+                seconds=secs.value)
 
 
 def bs_track(field, params, ptl, nthreads=1):

@@ -42,14 +42,23 @@
 #define BS_KFAC1 0.8
 #define BS_KFAC2 0.9
 
-enum { PH_INIT = 0, PH_MID = 1, PH_ACCEPT = 2 };
+enum { PH_INIT = 0, PH_MID = 1, PH_ACCEPT = 2, PH_DROW = 3 };
 enum { Q_HEAD = 0, Q_ACC = 1, Q_REJ = 2, Q_RHS = 3, Q_FAIL = 4, Q_N = 8 };
 
 struct BsConsts {
-    int kmax, ktarg0;
-    int nseq[9], cost[9];
-    double coeff[9][9];
+    int kmax, ktarg0, dense;
+    int nseq[10], cost[10], ipoint[11];
+    double coeff[10][10];
+    double errfac[19];
 };
+
+/* dense output: per lane, in HBM, ysave (9 x 6), fsave (162 x 6: every
+ * substep derivative of sequences 0..8, ipoint[9] = 162 rows) and the
+ * interpolation coefficients (23 x 6); slot s of lane g at ds[s * nl + g] */
+#define DS_YSAVE 0
+#define DS_FSAVE (9 * BS_N)
+#define DS_DENS (DS_FSAVE + 162 * BS_N)
+#define DS_SLOTS (DS_DENS + (2 * 9 + 5) * BS_N)
 
 struct BsArgs {
     FieldDev f;
@@ -59,14 +68,30 @@ struct BsArgs {
     double *x, *y, *z, *ux, *uy, *uz, *t;
     int *nsteps, *flag;
     unsigned long long *ctr;
+    /* the dense form with the fixed-step loop's diagnostics */
+    double *ds;                  /* dense-output scratch, DS_SLOTS per lane */
+    long nl;                     /* lanes of the launch */
+    double dt;                   /* output grid: times t0 + j dt */
+    int estep, ei, di, nd, nframes, nbins, nt_out, bc, fix_stale;
+    double lo[3], hi[3], logemin, logde;
+    double *xs, *ys, *zs;        /* periodic shifts */
+    const double *x0, *y0, *z0, *ux0, *uy0, *uz0, *g0, *mu0, *th0;
+    double *st;                  /* 5 stale D terms per particle (SoA) */
+    unsigned *espect, *espect_esc;
+    double *dbuf;                /* D terms [row][8][n] */
+    int *outhist;                /* particles counted out from row r, [nd] */
 };
 
-/* StepperBase (src/unused/StepperBS.c:103-131), non-dense sequence */
-static void bs_consts(int kmax, double rtol, BsConsts *c)
+/* StepperBase (src/unused/StepperBS.c:103-149): the step-size sequence
+ * (dense output: nseq = 4k+2, else 2(k+1)), costs, k_targ from rtol, the
+ * extrapolation coefficients, the dense interpolation error factors and
+ * ipoint, where each sequence's derivatives start in fsave */
+static void bs_consts(int kmax, double rtol, int dense, BsConsts *c)
 {
     memset(c, 0, sizeof(*c));
     c->kmax = kmax;
-    for (int i = 0; i <= kmax; i++) c->nseq[i] = 2 * (i + 1);
+    c->dense = dense;
+    for (int i = 0; i <= kmax; i++) c->nseq[i] = dense ? 4 * i + 2 : 2 * (i + 1);
     c->cost[0] = c->nseq[0] + 1;
     for (int k = 0; k < kmax; k++) c->cost[k + 1] = c->cost[k] + c->nseq[k + 1];
     double logfact = -log10(rtol > 1.0e-12 ? rtol : 1.0e-12) * 0.6 + 0.5;
@@ -79,14 +104,23 @@ static void bs_consts(int kmax, double rtol, BsConsts *c)
             double ratio = (double)c->nseq[k] / c->nseq[l];
             c->coeff[k][l] = 1.0 / (ratio * ratio - 1.0);
         }
+    for (int i = 0; i < 2 * (kmax + 1) + 1; i++) {
+        int ip5 = i + 5;
+        c->errfac[i] = 1.0 / (ip5 * ip5);
+        double e = 0.5 * sqrt(((double)(i + 1)) / ip5);
+        for (int j = 0; j <= i; j++) c->errfac[i] *= e / (j + 1);
+    }
+    c->ipoint[0] = 0;
+    for (int i = 1; i <= kmax + 1; i++) {
+        int njadd = 4 * i - 2;
+        if (c->nseq[i - 1] > njadd) njadd++;
+        c->ipoint[i] = c->ipoint[i - 1] + njadd;
+    }
 }
 
-template <int FK>
-__device__ __forceinline__ void bs_rhs(const FieldDev &f, double qm, double t, const double *y,
-                                       double *dy)
+/* the right-hand side from the field e at the point y */
+__device__ __forceinline__ void bs_deriv(const Emf &e, double qm, const double *y, double *dy)
 {
-    Emf e;
-    get_emf64<FK>(f, y[0], y[1], y[2], t, e);
     double g = sqrt(1.0 + y[3] * y[3] + y[4] * y[4] + y[5] * y[5]);
     double ig = 1.0 / g;
     double vx = y[3] * ig, vy = y[4] * ig, vz = y[5] * ig;
@@ -96,6 +130,15 @@ __device__ __forceinline__ void bs_rhs(const FieldDev &f, double qm, double t, c
     dy[3] = qm * (e.ex + (vy * e.bz - vz * e.by));
     dy[4] = qm * (e.ey + (vz * e.bx - vx * e.bz));
     dy[5] = qm * (e.ez + (vx * e.by - vy * e.bx));
+}
+
+template <int FK>
+__device__ __forceinline__ void bs_rhs(const FieldDev &f, double qm, double t, const double *y,
+                                       double *dy)
+{
+    Emf e;
+    get_emf64<FK>(f, y[0], y[1], y[2], t, e);
+    bs_deriv(e, qm, y, dy);
 }
 
 /* per-lane state of one particle's integration */
@@ -109,16 +152,114 @@ struct BsLane {
     double yv[BS_N], ycur[BS_N], dydx[BS_N], ym[BS_N], yn[BS_N];
     double tab[8][BS_N], hopt[9], work[9];
     int rhs;
+    /* dense form: next fsave row, the polynomial's step (start, size), the
+     * step-size cap of the interpolation error, the output-time grid (t0,
+     * next frame, next D row), periodic shifts, the escape and end flags */
+    int ipt, nf, nr, esc, ending;
+    double told, hd, hopt_int, t0;
+    double xs[3];
 };
 
-__device__ __forceinline__ void bs_finish(const BsArgs &a, BsLane &L, int failed)
+/* slot s of this lane's dense-output scratch */
+#define DSV(a, s) ((a).ds[(size_t)(s) * (a).nl + gl])
+
+__device__ __forceinline__ double bs_time(const BsArgs &a, const BsLane &L, long j)
+{
+    return L.t0 + (double)j * a.dt;
+}
+
+/* particle_bc, src/tracking.c:1172-1221, on the lane's state */
+__device__ __forceinline__ void bs_particle_bc(const BsArgs &a, BsLane &L)
+{
+    if (a.bc == 0) {
+        double d0 = a.hi[0] - a.lo[0], d1 = a.hi[1] - a.lo[1], d2 = a.hi[2] - a.lo[2];
+        if (L.yv[0] > a.hi[0]) { L.yv[0] -= d0; L.xs[0] += d0; }
+        if (L.yv[1] > a.hi[1]) { L.yv[1] -= d1; L.xs[1] += d1; }
+        if (L.yv[2] > a.hi[2]) { L.yv[2] -= d2; L.xs[2] += d2; }
+        if (L.yv[0] < a.lo[0]) { L.yv[0] += d0; L.xs[0] -= d0; }
+        if (L.yv[1] < a.lo[1]) { L.yv[1] += d1; L.xs[1] -= d1; }
+        if (L.yv[2] < a.lo[2]) { L.yv[2] += d2; L.xs[2] -= d2; }
+    } else if (a.bc == 1) {
+        if (L.yv[0] > a.hi[0] || L.yv[1] > a.hi[1] || L.yv[2] > a.hi[2] || L.yv[0] < a.lo[0] ||
+            L.yv[1] < a.lo[1] || L.yv[2] < a.lo[2])
+            L.esc = 1;
+    }
+}
+
+/* ptl_energy_fixed's bin, src/diagnostics.c:187-205 */
+__device__ __forceinline__ int bs_energy_bin(const BsArgs &a, double ux, double uy, double uz)
+{
+    double gama = sqrt(1.0 + ux * ux + uy * uy + uz * uz);
+    double v = floor((log10(gama - 1.0) - a.logemin) / a.logde);
+    if (!(v > -2147483649.0 && v < 2147483648.0)) return 0;
+    int ib = (int)v;
+    if (ib < 0) ib = 0;
+    else if (ib > a.nbins - 1) ib = a.nbins - 1;
+    return ib;
+}
+
+/* calc_diff_coeff's eight terms (src/diagnostics.c:659-705) of state y at
+ * time tt with shifts xs and field e into row r, or, escaped, the stale
+ * terms (src/tracking.c:334-336) */
+__device__ __forceinline__ void bs_dterms(const BsArgs &a, long i, int r, int escaped,
+                                          const double *y, double tt, const double *xs,
+                                          const Emf &e)
+{
+    const long n = a.n;
+    double *st = a.st;
+    double dxx = st[i], dyy = st[n + i], dzz = st[2 * n + i];
+    double daa = st[3 * n + i], dee = st[4 * n + i];
+    double drr = 0.0, dpp = 0.0, duu = 0.0;
+    if (escaped) {
+        if (a.fix_stale) dxx = dyy = dzz = daa = dee = 0.0;
+    } else {
+        double t = tt * 2 + a.dt;
+        double q;
+        q = a.x0[i] - (y[0] + xs[0]); dxx = q * q / t;
+        q = a.y0[i] - (y[1] + xs[1]); dyy = q * q / t;
+        q = a.z0[i] - (y[2] + xs[2]); dzz = q * q / t;
+        drr = dxx + dyy + dzz;
+        double ax = a.ux0[i] - y[3], ay = a.uy0[i] - y[4], az = a.uz0[i] - y[5];
+        dpp = (ax * ax + ay * ay + az * az) / t;
+        double g = sqrt(1.0 + y[3] * y[3] + y[4] * y[4] + y[5] * y[5]);
+        q = g - a.g0[i]; dee = q * q / t;
+        double ab = sqrt(e.bx * e.bx + e.by * e.by + e.bz * e.bz);
+        double au = sqrt(y[3] * y[3] + y[4] * y[4] + y[5] * y[5]);
+        double mu = (e.bx * y[3] + e.by * y[4] + e.bz * y[5]) / (ab * au);
+        q = mu - a.mu0[i]; duu = q * q / t;
+        double th = acos(mu) * 180 / TP_PI;
+        q = th - a.th0[i]; daa = q * q / t;
+    }
+    st[i] = dxx; st[n + i] = dyy; st[2 * n + i] = dzz;
+    st[3 * n + i] = daa; st[4 * n + i] = dee;
+    double *o = a.dbuf + (size_t)r * 8 * n + i;
+    o[0] = drr; o[n] = dxx; o[2 * n] = dyy; o[3 * n] = dzz;
+    o[4 * n] = dpp; o[5 * n] = duu; o[6 * n] = daa; o[7 * n] = dee;
+}
+
+template <bool DENSE>
+__device__ __forceinline__ void bs_finish(const BsArgs &a, BsLane &L, int failed, long gl)
 {
     const long i = L.pi;
     a.x[i] = L.yv[0]; a.y[i] = L.yv[1]; a.z[i] = L.yv[2];
     a.ux[i] = L.yv[3]; a.uy[i] = L.yv[4]; a.uz[i] = L.yv[5];
     a.t[i] = L.t;
     a.nsteps[i] = L.acc;
-    a.flag[i] = failed;
+    if (DENSE) {
+        /* rows from nr on: the particle has left (or failed); it is counted
+         * out from row 1 if it started outside, else from the row after its
+         * first stale row (the fixed-step loop's count, src/tracking.c:366-369) */
+        const Emf e = {};
+        for (int r = L.nr; r < a.nd; r++) bs_dterms(a, i, r, 1, L.yv, 0.0, L.xs, e);
+        if (L.esc) {
+            const int from = L.ending < 0 ? 1 : L.nr + 1;
+            if (from < a.nd) atomicAdd(&a.outhist[from], 1);
+        }
+        a.xs[i] = L.xs[0]; a.ys[i] = L.xs[1]; a.zs[i] = L.xs[2];
+        a.flag[i] = L.esc ? 2 : failed;
+    } else {
+        a.flag[i] = failed;
+    }
     atomicAdd(&a.ctr[Q_ACC], (unsigned long long)L.acc);
     atomicAdd(&a.ctr[Q_REJ], (unsigned long long)L.rej);
     atomicAdd(&a.ctr[Q_RHS], (unsigned long long)L.rhs);
@@ -139,23 +280,44 @@ __device__ __forceinline__ void bs_begin_seq(const BsArgs &a, BsLane &L)
     L.phase = PH_MID;
 }
 
-__device__ __forceinline__ void bs_begin_attempt(const BsArgs &a, BsLane &L)
+template <bool DENSE>
+__device__ __forceinline__ void bs_begin_attempt(const BsArgs &a, BsLane &L, long gl)
 {
     L.h = L.hnew;
     L.reject = 0;
     if (++L.attempts > a.cap) {
-        bs_finish(a, L, 1);
+        bs_finish<DENSE>(a, L, 1, gl);
         return;
     }
     L.k = 0;
+    L.ipt = 0;
     bs_begin_seq(a, L);
 }
 
+/* the output times at the end time: the remaining frames from the final
+ * state; the remaining D rows each need the field there (PH_DROW) */
+template <bool DENSE>
+__device__ __forceinline__ void bs_begin_end_outputs(const BsArgs &a, BsLane &L, long gl)
+{
+    for (; L.nf < a.nframes; L.nf++)
+        atomicAdd(&a.espect[L.nf * a.nbins + bs_energy_bin(a, L.yv[3], L.yv[4], L.yv[5])], 1u);
+    if (L.nr < a.nd) {
+        L.ending = 1;
+        for (int i = 0; i < BS_N; i++) L.yn[i] = L.yv[i];
+        L.xe = bs_time(a, L, (long)L.nr * a.di);
+        L.phase = PH_DROW;
+        return;
+    }
+    bs_finish<DENSE>(a, L, 0, gl);
+}
+
 /* top of step(): src/unused/StepperBS.c:173-191 */
-__device__ __forceinline__ void bs_start_step(const BsArgs &a, BsLane &L)
+template <bool DENSE>
+__device__ __forceinline__ void bs_start_step(const BsArgs &a, BsLane &L, long gl)
 {
     if (!(L.t < L.tend - 1e-13 * fabs(L.tend))) {
-        bs_finish(a, L, 0);
+        if (DENSE) bs_begin_end_outputs<DENSE>(a, L, gl);
+        else bs_finish<DENSE>(a, L, 0, gl);
         return;
     }
     double h = L.hnext;
@@ -168,12 +330,14 @@ __device__ __forceinline__ void bs_start_step(const BsArgs &a, BsLane &L)
     }
     L.reject = 0;
     L.hnew = fabs(h);
-    bs_begin_attempt(a, L);
+    bs_begin_attempt<DENSE>(a, L, gl);
 }
 
 /* after row k's midpoint sequence: extrapolation, error and the
  * convergence/rejection tests of src/unused/StepperBS.c:206-272 */
-__device__ __forceinline__ void bs_end_seq(const BsArgs &a, BsLane &L, const double *yseq)
+template <bool DENSE>
+__device__ __forceinline__ void bs_end_seq(const BsArgs &a, BsLane &L, const double *yseq,
+                                           long gl)
 {
     const int k = L.k;
     if (k == 0) {
@@ -255,7 +419,7 @@ __device__ __forceinline__ void bs_end_seq(const BsArgs &a, BsLane &L, const dou
     if (L.reject) {
         L.prev_reject = 1;
         L.rej++;
-        bs_begin_attempt(a, L);
+        bs_begin_attempt<DENSE>(a, L, gl);
         return;
     }
     /* accepted: the derivative at the new point comes next */
@@ -263,9 +427,170 @@ __device__ __forceinline__ void bs_end_seq(const BsArgs &a, BsLane &L, const dou
     L.phase = PH_ACCEPT;
 }
 
-/* after the derivative at the accepted point: order and step-size choice of
- * src/unused/StepperBS.c:296-347 */
-__device__ __forceinline__ void bs_accept(const BsArgs &a, BsLane &L, const double *f)
+/* dense_interp, src/unused/StepperBS.c:478-524, on the lane's dens slots */
+__device__ __forceinline__ void bs_dense_interp(const BsArgs &a, long gl, int imit)
+{
+    const int n = BS_N;
+    double av[16];
+    for (int i = 0; i < n; i++) {
+        double y0 = DSV(a, DS_DENS + i), y1 = DSV(a, DS_DENS + 2 * n + i);
+        double yp0 = DSV(a, DS_DENS + n + i), yp1 = DSV(a, DS_DENS + 3 * n + i);
+        double ydiff = y1 - y0;
+        double aspl = -yp1 + ydiff;
+        double bspl = yp0 - ydiff;
+        DSV(a, DS_DENS + n + i) = ydiff;
+        DSV(a, DS_DENS + 2 * n + i) = aspl;
+        DSV(a, DS_DENS + 3 * n + i) = bspl;
+        if (imit < 0) continue;
+        double ph0 = (y0 + y1) * 0.5 + 0.125 * (aspl + bspl);
+        double ph1 = ydiff + (aspl - bspl) * 0.25;
+        double ph2 = -(yp0 - yp1);
+        double ph3 = 6.0 * (bspl - aspl);
+        if (imit >= 1) {
+            av[1] = 16.0 * (DSV(a, DS_DENS + 5 * n + i) - ph1);
+            if (imit >= 3) {
+                av[3] = 16.0 * (DSV(a, DS_DENS + 7 * n + i) - ph3 + 3 * av[1]);
+                for (int im = 5; im <= imit; im += 2) {
+                    double fac1 = im * (im - 1) / 2.0;
+                    double fac2 = fac1 * (im - 2) * (im - 3) * 2.0;
+                    av[im] = 16.0 * (DSV(a, DS_DENS + (im + 4) * n + i) + fac1 * av[im - 2] -
+                                     fac2 * av[im - 4]);
+                }
+            }
+        }
+        av[0] = (DSV(a, DS_DENS + 4 * n + i) - ph0) * 16.0;
+        if (imit >= 2) {
+            av[2] = (DSV(a, DS_DENS + 6 * n + i) - ph2 + av[0]) * 16.0;
+            for (int im = 4; im <= imit; im += 2) {
+                double fac1 = im * (im - 1) / 2.0;
+                double fac2 = (double)(im * (im - 1) * (im - 2) * (im - 3));
+                av[im] = 16.0 * (DSV(a, DS_DENS + (im + 4) * n + i) + fac1 * av[im - 2] -
+                                 fac2 * av[im - 4]);
+            }
+        }
+        for (int im = 0; im <= imit; im++) DSV(a, DS_DENS + n * (im + 4) + i) = av[im];
+    }
+}
+
+/* prepare_dense, src/unused/StepperBS.c:526-615: the dense-output values of
+ * the accepted step of size h (start yv, end ycur, derivatives dydx and
+ * dydxnew), extrapolated from the midpoint values and derivative differences
+ * of sequences 0..k; returns the interpolation error estimate */
+__device__ __forceinline__ double bs_prepare_dense(const BsArgs &a, const BsLane &L, long gl,
+                                                   const double *dydxnew)
+{
+    const int n = BS_N, k = L.k, mu = 2 * k - 1;
+    const double h = L.h;
+    for (int i = 0; i < n; i++) {
+        DSV(a, DS_DENS + i) = L.yv[i];
+        DSV(a, DS_DENS + n + i) = h * L.dydx[i];
+        DSV(a, DS_DENS + 2 * n + i) = L.ycur[i];
+        DSV(a, DS_DENS + 3 * n + i) = dydxnew[i] * h;
+    }
+#define YS(r, i) DSV(a, DS_YSAVE + (r) * BS_N + (i))
+#define FS(r, i) DSV(a, DS_FSAVE + (r) * BS_N + (i))
+    for (int j = 1; j <= k; j++) {
+        double dblenj = a.c.nseq[j];
+        for (int l = j; l >= 1; l--) {
+            double r = dblenj / a.c.nseq[l - 1];
+            double factor = r * r - 1.0;
+            for (int i = 0; i < n; i++) YS(l - 1, i) = YS(l, i) + (YS(l, i) - YS(l - 1, i)) / factor;
+        }
+    }
+    for (int i = 0; i < n; i++) DSV(a, DS_DENS + 4 * n + i) = YS(0, i);
+    for (int kmi = 1; kmi <= mu; kmi++) {
+        int kbeg = (kmi - 1) / 2;
+        for (int kk = kbeg; kk <= k; kk++) {
+            double facnj = tp_powi(a.c.nseq[kk] / 2.0, kmi - 1);
+            int ipt = a.c.ipoint[kk + 1] - 2 * kk + kmi - 3;
+            for (int i = 0; i < n; i++) YS(kk, i) = FS(ipt, i) * facnj;
+        }
+        for (int j = kbeg + 1; j <= k; j++) {
+            double dblenj = a.c.nseq[j];
+            for (int l = j; l >= kbeg + 1; l--) {
+                double r = dblenj / a.c.nseq[l - 1];
+                double factor = r * r - 1.0;
+                for (int i = 0; i < n; i++)
+                    YS(l - 1, i) = YS(l, i) + (YS(l, i) - YS(l - 1, i)) / factor;
+            }
+        }
+        for (int i = 0; i < n; i++) DSV(a, DS_DENS + (kmi + 4) * n + i) = YS(kbeg, i) * h;
+        if (kmi == mu) continue;
+        for (int kk = kmi / 2; kk <= k; kk++) {
+            int lbeg = a.c.ipoint[kk + 1] - 1;
+            int lend = a.c.ipoint[kk] + kmi;
+            if (kmi == 1) lend += 2;
+            for (int l = lbeg; l >= lend; l -= 2)
+                for (int i = 0; i < n; i++) FS(l, i) = FS(l, i) - FS(l - 2, i);
+            if (kmi == 1) {
+                int l = lend - 2;
+                for (int i = 0; i < n; i++) FS(l, i) = FS(l, i) - L.dydx[i];
+            }
+        }
+        for (int kk = kmi / 2; kk <= k; kk++) {
+            int lbeg = a.c.ipoint[kk + 1] - 2;
+            int lend = a.c.ipoint[kk] + kmi + 1;
+            for (int l = lbeg; l >= lend; l -= 2)
+                for (int i = 0; i < n; i++) FS(l, i) = FS(l, i) - FS(l - 2, i);
+        }
+    }
+#undef YS
+#undef FS
+    bs_dense_interp(a, gl, mu);
+    double error = 0.0;
+    if (mu >= 1) {
+        for (int i = 0; i < n; i++) {
+            double p = fabs(L.yv[i]), q = fabs(L.ycur[i]);
+            double scale = a.atol + a.rtol * (p > q ? p : q);
+            double v = DSV(a, DS_DENS + (mu + 4) * n + i) / scale;
+            error += v * v;
+        }
+        error = sqrt(error / n) * a.c.errfac[mu - 1];
+    }
+    return error;
+}
+
+/* dense_out, src/unused/StepperBS.c:617-644: component i at time x of the
+ * lane's last accepted step */
+__device__ __forceinline__ double bs_dense_out(const BsArgs &a, const BsLane &L, long gl, int i,
+                                               double x)
+{
+    const int n = BS_N, mu = 2 * L.k - 1;
+    double theta = (x - L.told) / L.hd;
+    double theta1 = 1.0 - theta;
+    double yinterp = DSV(a, DS_DENS + i) +
+                     theta * (DSV(a, DS_DENS + n + i) +
+                              theta1 * (DSV(a, DS_DENS + 2 * n + i) * theta +
+                                        DSV(a, DS_DENS + 3 * n + i) * theta1));
+    if (mu < 0) return yinterp;
+    double theta05 = theta - 0.5;
+    double tt = theta * theta1;
+    double t4 = tt * tt;
+    double cc = DSV(a, DS_DENS + n * (mu + 4) + i);
+    for (int j = mu; j > 0; j--) cc = DSV(a, DS_DENS + n * (j + 3) + i) + cc * theta05 / j;
+    return yinterp + t4 * cc;
+}
+
+/* the step's own output times (strictly before its end) next: a D row
+ * needs the field at its point, evaluated in the next iteration */
+template <bool DENSE>
+__device__ __forceinline__ bool bs_next_drow(const BsArgs &a, BsLane &L, long gl)
+{
+    if (L.nr >= a.nd) return false;
+    const double tr = bs_time(a, L, (long)L.nr * a.di);
+    if (!(tr < L.told + L.hd)) return false;
+    for (int i = 0; i < BS_N; i++) L.yn[i] = bs_dense_out(a, L, gl, i, tr);
+    L.xe = tr;
+    L.phase = PH_DROW;
+    return true;
+}
+
+/* the accepted step's bookkeeping after its derivative f at the new point:
+ * order and step-size choice of src/unused/StepperBS.c:296-347, then (dense
+ * form) the boundary and the escape */
+template <bool DENSE>
+__device__ __forceinline__ void bs_accept_finish(const BsArgs &a, BsLane &L, const double *f,
+                                                 long gl)
 {
     for (int i = 0; i < BS_N; i++) {
         L.dydx[i] = f[i];
@@ -304,14 +629,85 @@ __device__ __forceinline__ void bs_accept(const BsArgs &a, BsLane &L, const doub
         }
         L.ktarg = kopt;
     }
+    if (DENSE) {
+        double hi = fabs(L.hopt_int);
+        hnew = hnew < hi ? hnew : hi;
+        L.hnext = hnew;
+        bs_particle_bc(a, L);
+        if (L.esc) {
+            /* escape spectrum at the frame of the next output time */
+            if (L.nf < a.nt_out)
+                atomicAdd(&a.espect_esc[L.nf * a.nbins +
+                                        bs_energy_bin(a, L.yv[3], L.yv[4], L.yv[5])], 1u);
+            bs_finish<DENSE>(a, L, 0, gl);
+            return;
+        }
+        bs_start_step<DENSE>(a, L, gl);
+        return;
+    }
     L.hnext = hnew;
-    bs_start_step(a, L);
+    bs_start_step<DENSE>(a, L, gl);
 }
 
-template <int FK>
+/* after the derivative at the accepted point: the dense form first checks
+ * the interpolation error (above 10 the step is redone with the step size
+ * it suggests), bins the frames inside the step and queues its D rows */
+template <bool DENSE>
+__device__ __forceinline__ void bs_accept(const BsArgs &a, BsLane &L, const double *f, long gl)
+{
+    if (DENSE) {
+        const double err = bs_prepare_dense(a, L, gl, f);
+        const double pe = tp_pow(err, 1.0 / (2 * L.k + 3));
+        L.hopt_int = L.h / (pe > 0.01 ? pe : 0.01);
+        if (err > 10.0) {
+            L.hnew = fabs(L.hopt_int);
+            L.reject = 1;
+            L.prev_reject = 1;
+            L.rej++;
+            bs_begin_attempt<DENSE>(a, L, gl);
+            return;
+        }
+        L.told = L.t;
+        L.hd = L.h;
+        const double tnew = L.t + L.h;
+        for (; L.nf < a.nframes; L.nf++) {
+            const double tf = bs_time(a, L, (long)(L.nf + 1) * a.ei);
+            if (!(tf < tnew)) break;
+            const double ux = bs_dense_out(a, L, gl, 3, tf);
+            const double uy = bs_dense_out(a, L, gl, 4, tf);
+            const double uz = bs_dense_out(a, L, gl, 5, tf);
+            atomicAdd(&a.espect[L.nf * a.nbins + bs_energy_bin(a, ux, uy, uz)], 1u);
+        }
+        for (int i = 0; i < BS_N; i++) L.ym[i] = f[i];   /* dydxnew, while D rows run */
+        if (bs_next_drow<DENSE>(a, L, gl)) return;
+    }
+    bs_accept_finish<DENSE>(a, L, f, gl);
+}
+
+/* after the field e at a D row's point: its terms, then the next row or
+ * the rest of the step (or, at the end time, of the particle) */
+template <bool DENSE>
+__device__ __forceinline__ void bs_drow(const BsArgs &a, BsLane &L, const Emf &e, long gl)
+{
+    bs_dterms(a, L.pi, L.nr, 0, L.yn, L.xe, L.xs, e);
+    L.nr++;
+    if (L.ending) {
+        if (L.nr < a.nd) {
+            L.xe = bs_time(a, L, (long)L.nr * a.di);
+            return;                                    /* same point, next row */
+        }
+        bs_finish<DENSE>(a, L, 0, gl);
+        return;
+    }
+    if (bs_next_drow<DENSE>(a, L, gl)) return;
+    bs_accept_finish<DENSE>(a, L, L.ym, gl);
+}
+
+template <int FK, bool DENSE>
 __global__ void __launch_bounds__(256) k_bs(BsArgs a)
 {
     const int lane = threadIdx.x & 63;
+    const long gl = (long)blockIdx.x * blockDim.x + threadIdx.x;
     BsLane L;
     L.pi = -1;
     while (true) {
@@ -346,29 +742,59 @@ __global__ void __launch_bounds__(256) k_bs(BsArgs a)
                     L.rhs = 0;
                     L.xe = L.t;
                     L.phase = PH_INIT;
+                    if (DENSE) {
+                        L.t0 = L.t;
+                        L.tend = L.t + a.estep * a.dt;
+                        L.nf = 0; L.nr = 1; L.esc = 0; L.ending = 0;
+                        L.xs[0] = a.xs[i]; L.xs[1] = a.ys[i]; L.xs[2] = a.zs[i];
+                        /* initial wrap and escape check, src/tracking.c:314-332 */
+                        bs_particle_bc(a, L);
+                        if (L.esc) {
+                            L.ending = -1;
+                            bs_finish<DENSE>(a, L, 0, gl);
+                        }
+                    }
                 } else {
                     L.pi = -2;
                 }
             }
         }
         if (__ballot(L.pi >= 0) == 0ull) break;
-        /* the one right-hand-side evaluation of this iteration, executed by
-         * the whole wavefront (retired lanes evaluate a dummy point) so the
-         * field sum stays in uniform control flow and its uniform table
-         * reads become scalar loads */
+        /* the one field evaluation of this iteration, executed by the whole
+         * wavefront (retired lanes evaluate a dummy point) so the field sum
+         * stays in uniform control flow and its uniform table reads become
+         * scalar loads */
         const bool act = L.pi >= 0;
         double ze[BS_N];
         for (int i = 0; i < BS_N; i++)
-            ze[i] = !act ? 0.0 : (L.phase == PH_INIT ? L.yv[i] : (L.phase == PH_MID ? L.yn[i] : L.ycur[i]));
-        double f[BS_N];
-        bs_rhs<FK>(a.f, a.qm, act ? L.xe : 0.0, ze, f);
+            ze[i] = !act ? 0.0
+                         : (L.phase == PH_INIT ? L.yv[i]
+                                               : (L.phase == PH_ACCEPT ? L.ycur[i] : L.yn[i]));
+        Emf e;
+        get_emf64<FK>(a.f, ze[0], ze[1], ze[2], act ? L.xe : 0.0, e);
         if (act) {
+            if (DENSE && L.phase == PH_DROW) {
+                bs_drow<DENSE>(a, L, e, gl);
+                continue;
+            }
+            double f[BS_N];
+            bs_deriv(e, a.qm, ze, f);
             L.rhs++;
             if (L.phase == PH_INIT) {
                 for (int i = 0; i < BS_N; i++) L.dydx[i] = f[i];
-                bs_start_step(a, L);
+                bs_start_step<DENSE>(a, L, gl);
             } else if (L.phase == PH_MID) {
                 if (L.nn < L.nstep) {
+                    if (DENSE) {
+                        if (L.nn == L.nstep / 2)
+                            for (int i = 0; i < BS_N; i++)
+                                DSV(a, DS_YSAVE + L.k * BS_N + i) = L.yn[i];
+                        if (abs(L.nn - L.nstep / 2) <= 2 * L.k + 1) {
+                            for (int i = 0; i < BS_N; i++)
+                                DSV(a, DS_FSAVE + L.ipt * BS_N + i) = f[i];
+                            L.ipt++;
+                        }
+                    }
                     double h2 = 2.0 * L.hsub;
                     for (int i = 0; i < BS_N; i++) {
                         double swap = L.ym[i] + h2 * f[i];
@@ -378,13 +804,18 @@ __global__ void __launch_bounds__(256) k_bs(BsArgs a)
                     L.xe += L.hsub;
                     L.nn++;
                 } else {
+                    if (DENSE && L.nstep / 2 <= 2 * L.k + 1) {
+                        for (int i = 0; i < BS_N; i++)
+                            DSV(a, DS_FSAVE + L.ipt * BS_N + i) = f[i];
+                        L.ipt++;
+                    }
                     double yseq[BS_N];
                     for (int i = 0; i < BS_N; i++)
                         yseq[i] = 0.5 * (L.ym[i] + L.yn[i] + L.hsub * f[i]);
-                    bs_end_seq(a, L, yseq);
+                    bs_end_seq<DENSE>(a, L, yseq, gl);
                 }
             } else {
-                bs_accept(a, L, f);
+                bs_accept<DENSE>(a, L, f, gl);
             }
         }
     }
@@ -398,24 +829,68 @@ __global__ void k_bs_reset(unsigned long long *ctr)
 
 typedef void (*bs_fn)(dim3, dim3, hipStream_t, const BsArgs &);
 
-template <int FK>
+template <int FK, bool DENSE>
 static void launch_bs(dim3 g, dim3 b, hipStream_t s, const BsArgs &a)
 {
-    hipLaunchKernelGGL((k_bs<FK>), g, b, 0, s, a);
+    hipLaunchKernelGGL((k_bs<FK, DENSE>), g, b, 0, s, a);
 }
 
-static bs_fn pick_bs(int fk)
+template <bool DENSE>
+static bs_fn pick_bs_t(int fk)
 {
     switch (fk) {
-    case FK_TEST: return launch_bs<FK_TEST>;
-    case FK_ABC: return launch_bs<FK_ABC>;
-    case FK_ABCV: return launch_bs<FK_ABCV>;
-    case FK_GRID_REF: return launch_bs<FK_GRID_REF>;
-    case FK_GRID_TB: return launch_bs<FK_GRID_TB>;
-    case FK_MODES: return launch_bs<FK_MODES>;
-    case FK_WLCS: return launch_bs<FK_WLCS>;
+    case FK_TEST: return launch_bs<FK_TEST, DENSE>;
+    case FK_ABC: return launch_bs<FK_ABC, DENSE>;
+    case FK_ABCV: return launch_bs<FK_ABCV, DENSE>;
+    case FK_GRID_REF: return launch_bs<FK_GRID_REF, DENSE>;
+    case FK_GRID_TB: return launch_bs<FK_GRID_TB, DENSE>;
+    case FK_MODES: return launch_bs<FK_MODES, DENSE>;
+    case FK_WLCS: return launch_bs<FK_WLCS, DENSE>;
+    case FK_MHD_REF: return launch_bs<FK_MHD_REF, DENSE>;
+    case FK_GRIDD_REF: return launch_bs<FK_GRIDD_REF, DENSE>;
     }
     return nullptr;
+}
+
+static bs_fn pick_bs(int fk) { return pick_bs_t<false>(fk); }
+
+/* D sums of the dense form: column (component q, row r) of the per-particle
+ * terms summed over the particles in a fixed order (strided partial sums,
+ * then an LDS tree), one block per column; row 0 stays 0 */
+__global__ void __launch_bounds__(256) k_bs_dsum(const double *dbuf, long n, int nd,
+                                                 double *dco)
+{
+    __shared__ double sh[256];
+    const int q = blockIdx.x % 8, r = blockIdx.x / 8;
+    const double *col = dbuf + ((size_t)r * 8 + q) * n;
+    double s = 0.0;
+    if (r > 0)
+        for (long i = threadIdx.x; i < n; i += 256) s += col[i];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) dco[(size_t)q * nd + r] = sh[0];
+}
+
+/* escapes counted against each row: the particles counted out from row k
+ * or before (outhist[k] counts those counted out from row k) */
+__global__ void k_bs_rdec(const int *outhist, int nd, int *rdec)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int run = 0;
+    for (int k = 0; k < nd; k++) {
+        run += outhist[k];
+        rdec[k] = run;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_bs_zero_u32(unsigned *p, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        p[i] = 0u;
 }
 
 void tp_bs_release(tp_ctx *c)
@@ -427,6 +902,12 @@ void tp_bs_release(tp_ctx *c)
     (void)hipFree(c->bs_ctr);
     c->bs_ctr = nullptr;
     c->bs_graph_key = 0;
+    (void)hipFree(c->bs_ds);
+    (void)hipFree(c->bs_dbuf);
+    (void)hipFree(c->bs_outhist);
+    c->bs_ds = c->bs_dbuf = nullptr;
+    c->bs_outhist = nullptr;
+    c->bs_ds_cap = c->bs_dbuf_cap = c->bs_outhist_cap = 0;
 }
 
 static unsigned long long fnv(unsigned long long h, const void *p, size_t n)
@@ -453,11 +934,11 @@ static int bs_check(const tp_ctx *c, const tp_bs_params *p)
 
 static void bs_fill_args(const tp_ctx *c, const tp_bs_params *p, BsArgs &a, int n, double *x,
                          double *y, double *z, double *ux, double *uy, double *uz, double *t,
-                         int *nsteps, int *flag, unsigned long long *ctr, int cap)
+                         int *nsteps, int *flag, unsigned long long *ctr, int cap, int dense = 0)
 {
     memset(&a, 0, sizeof(a));
     a.f = c->f;
-    bs_consts(p->kmax, p->rtol, &a.c);
+    bs_consts(p->kmax, p->rtol, dense, &a.c);
     a.n = n;
     a.cap = cap;
     a.qm = p->qm; a.rtol = p->rtol; a.atol = p->atol;
@@ -484,6 +965,22 @@ static dim3 bs_grid(const tp_ctx *c, int block, int wpc, int n)
     long blocks = (lanes + block - 1) / block;
     if (blocks < 1) blocks = 1;
     return dim3((unsigned)blocks);
+}
+
+/* grow a device buffer of a context to at least `need` elements */
+template <typename T>
+static int bs_grow(T **buf, size_t *cap, size_t need)
+{
+    if (need <= *cap) return TP_OK;
+    (void)hipFree(*buf);
+    *buf = nullptr;
+    *cap = 0;
+    if (hipMalloc(buf, need * sizeof(T)) != hipSuccess) {
+        *buf = nullptr;
+        return TP_E_NOMEM;
+    }
+    *cap = need;
+    return TP_OK;
 }
 
 extern "C" {
@@ -563,6 +1060,116 @@ int tp_gpu_run_adaptive(tp_ctx *c, const tp_bs_params *p, tp_bs_stats *stats)
             stats->launches = 1;
             stats->kernel_ms = ms;
         }
+    }
+    return TP_OK;
+}
+
+int tp_gpu_run_adaptive_diag(tp_ctx *c, const tp_bs_params *p, const tp_track_params *q,
+                             tp_bs_stats *stats)
+{
+    if (!c || !q) return TP_E_ARG;
+    int rc = bs_check(c, p);
+    if (rc) return rc;
+    if (q->sstep != 1 || q->estep < 1 || q->einterval_t <= 0 || q->dcoeff_interval <= 0 ||
+        q->nbins < 1 || q->nt_out < 1 || !(q->dt > 0.0) || q->bc_flag < 0 || q->bc_flag > 2 ||
+        q->precision != TP_FP64)
+        return TP_E_ARG;
+    if ((size_t)q->nbins * sizeof(unsigned) > 64 * 1024) return TP_E_ARG;
+    if (!c->s.derived) return TP_E_STATE;
+    bs_fn fn = pick_bs_t<true>(c->f.kind);
+    if (!fn) return TP_E_ARG;
+    HIPCHK(hipSetDevice(c->device));
+    if ((rc = bs_ensure_ctr(c))) return rc;
+    const int n = c->s.n;
+    const int nd = q->estep / q->dcoeff_interval + 1;
+    int nframes = q->estep / q->einterval_t;
+    if (nframes > q->nt_out) nframes = q->nt_out;
+    int block = p->block, wpc = p->waves_per_cu;
+    if (block == 0 || wpc == 0) {
+        int tb = 64, tw = 8;
+        if (c->bs_tune_block) { tb = c->bs_tune_block[0]; tw = c->bs_tune_block[1]; }
+        if (block == 0) block = tb;
+        if (wpc == 0) wpc = tw;
+    }
+    const dim3 grid = bs_grid(c, block, wpc, n);
+    const long nl = (long)grid.x * block;
+    /* the run's outputs in the context's diagnostics buffers (read back by
+     * tp_gpu_read_diagnostics), the scratch of the dense form */
+    const size_t need = (size_t)q->nbins * q->nt_out;
+    if (need > c->diag_cap) {
+        hipFree(c->espect); hipFree(c->espect_esc);
+        c->espect = nullptr; c->espect_esc = nullptr; c->diag_cap = 0;
+        if (hipMalloc(&c->espect, need * sizeof(unsigned)) != hipSuccess ||
+            hipMalloc(&c->espect_esc, need * sizeof(unsigned)) != hipSuccess)
+            return TP_E_NOMEM;
+        c->diag_cap = need;
+    }
+    if ((size_t)nd > c->dco_cap) {
+        hipFree(c->dco); hipFree(c->rdec);
+        c->dco = nullptr; c->rdec = nullptr; c->dco_cap = 0;
+        if (hipMalloc(&c->dco, sizeof(double) * 8 * nd) != hipSuccess ||
+            hipMalloc(&c->rdec, sizeof(int) * (nd + 1)) != hipSuccess)
+            return TP_E_NOMEM;
+        c->dco_cap = nd;
+    }
+    const size_t dneed = (size_t)nd * 8 * (n > 0 ? n : 1);
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) == hipSuccess &&
+        (dneed + (size_t)nl * DS_SLOTS) * sizeof(double) > fr / 2 + c->bs_dbuf_cap * sizeof(double))
+        return TP_E_NOMEM;
+    if ((rc = bs_grow(&c->bs_dbuf, &c->bs_dbuf_cap, dneed)) ||
+        (rc = bs_grow(&c->bs_ds, &c->bs_ds_cap, (size_t)nl * DS_SLOTS)) ||
+        (rc = bs_grow(&c->bs_outhist, &c->bs_outhist_cap, (size_t)nd)))
+        return rc;
+    hipLaunchKernelGGL(k_bs_zero_u32, dim3(256), dim3(256), 0, c->stream, c->espect, need);
+    hipLaunchKernelGGL(k_bs_zero_u32, dim3(256), dim3(256), 0, c->stream, c->espect_esc, need);
+    hipLaunchKernelGGL(k_bs_zero_u32, dim3(64), dim3(256), 0, c->stream,
+                       (unsigned *)c->bs_outhist, (size_t)nd);
+    HIPCHK(hipGetLastError());
+    c->nb = q->nbins; c->nt = q->nt_out; c->nd = nd; c->last_n = n;
+    const int cap = p->max_attempts > 0 ? p->max_attempts : 10000000;
+    BsArgs a;
+    State &S = c->s;
+    bs_fill_args(c, p, a, n, S.x, S.y, S.z, S.ux, S.uy, S.uz, S.t, S.nsteps, S.esc, c->bs_ctr,
+                 cap, 1);
+    a.ds = c->bs_ds; a.nl = nl;
+    a.dt = q->dt; a.estep = q->estep; a.ei = q->einterval_t; a.di = q->dcoeff_interval;
+    a.nd = nd; a.nframes = nframes; a.nbins = q->nbins; a.nt_out = q->nt_out;
+    a.bc = q->bc_flag; a.fix_stale = (q->flags & TP_FLAG_FIX_STALE_DCOEFF) ? 1 : 0;
+    a.lo[0] = q->xmin; a.lo[1] = q->ymin; a.lo[2] = q->zmin;
+    a.hi[0] = q->xmax; a.hi[1] = q->ymax; a.hi[2] = q->zmax;
+    a.logemin = q->logemin; a.logde = q->logde;
+    a.xs = S.xs; a.ys = S.ys; a.zs = S.zs;
+    a.x0 = S.x0; a.y0 = S.y0; a.z0 = S.z0; a.ux0 = S.ux0; a.uy0 = S.uy0; a.uz0 = S.uz0;
+    a.g0 = S.g0; a.mu0 = S.mu0; a.th0 = S.th0; a.st = S.st;
+    a.espect = c->espect; a.espect_esc = c->espect_esc;
+    a.dbuf = c->bs_dbuf; a.outhist = c->bs_outhist;
+    hipLaunchKernelGGL(k_bs_reset, dim3(1), dim3(64), 0, c->stream, c->bs_ctr);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    if (n > 0) fn(grid, dim3(block), c->stream, a);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    hipLaunchKernelGGL(k_bs_dsum, dim3(8 * nd), dim3(256), 0, c->stream,
+                       (const double *)c->bs_dbuf, (long)n, nd, c->dco);
+    hipLaunchKernelGGL(k_bs_rdec, dim3(1), dim3(64), 0, c->stream,
+                       (const int *)c->bs_outhist, nd, c->rdec);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    unsigned long long h[Q_N];
+    HIPCHK(hipMemcpy(h, c->bs_ctr, sizeof(h), hipMemcpyDeviceToHost));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    c->last_launches = 1;
+    if (stats) {
+        memset(stats, 0, sizeof(*stats));
+        stats->accepted = (long long)h[Q_ACC];
+        stats->rejected = (long long)h[Q_REJ];
+        stats->rhs_evals = (long long)h[Q_RHS];
+        stats->failed = (int)h[Q_FAIL];
+        stats->launches = 1;
+        stats->kernel_ms = ms;
     }
     return TP_OK;
 }

@@ -37,6 +37,7 @@ struct tp_ctx {
     float4 *cells = nullptr, *cells_b = nullptr;   /* cell-major copies */
     int cells_mode = TP_CELLS_AUTO;
     double *vgrid = nullptr, *modes = nullptr;
+    double *gridd = nullptr;                       /* double PIC nodes */
     float *modes32 = nullptr;
     WlcsSys *wl = nullptr;
     State s;
@@ -60,6 +61,11 @@ struct tp_ctx {
     hipGraphExec_t bs_graph = nullptr;
     unsigned long long bs_graph_key = 0;
     int *bs_tune_block = nullptr;
+    /* adaptive stepper with diagnostics: dense-output scratch per lane, D
+     * terms per particle and row, counted-out histogram */
+    double *bs_ds = nullptr, *bs_dbuf = nullptr;
+    int *bs_outhist = nullptr;
+    size_t bs_ds_cap = 0, bs_dbuf_cap = 0, bs_outhist_cap = 0;
     /* learned surrogate: network, optimiser state and training buffers */
     struct MlpState *mlp = nullptr;
     /* trajectory diagnostics armed for the next run (traj_nout > 0) */

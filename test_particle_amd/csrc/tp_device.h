@@ -28,6 +28,8 @@ enum {
     FK_MLP = 7,       /* learned surrogate of the modes, exact modes as fallback */
     FK_GRID2_REF = 8, /* PIC grid, two time slices, reference gather */
     FK_GRID2_TB = 9,  /* PIC grid, two time slices, textbook gather */
+    FK_MHD_REF = 10,  /* MHD grids (system 3): float v and B, E = B x v */
+    FK_GRIDD_REF = 11,/* PIC grid of double E and B, reference weights */
 };
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -79,6 +81,9 @@ struct FieldDev {
      * float4 per cell, cells indexed like the nodes.  A cell change reads one
      * contiguous 192-byte record instead of eight nodes. */
     const float4 *cells, *cells_b;
+    /* PIC grid of doubles (FK_GRIDD_REF): nodes of 8 doubles,
+     * ex ey ez bx by bz 0 0, x fastest */
+    const double *gridd;
     /* modes: rows kx ky kz ax ay az phi */
     int nmodes;
     const double *modes;
@@ -701,6 +706,26 @@ __device__ __noinline__ void emf_wlcs(const FieldDev &f, double x, double y,
     o.ex = Ex; o.ey = Ey; o.ez = Ez;
 }
 
+/* getemf_mhd_test_particle, src/emfields.c:107-120: the velocity is
+ * gathered like E (float accumulators, src/velocity_field.c:593-622) and
+ * held in the E slots; E = B x v is formed in double from the float B and v */
+__device__ __forceinline__ void mhd_e64(Emf &o)
+{
+    const double vx = o.ex, vy = o.ey, vz = o.ez;
+    o.ex = o.by * vz - o.bz * vy;
+    o.ey = o.bz * vx - o.bx * vz;
+    o.ez = o.bx * vy - o.by * vx;
+}
+
+/* the same in the FP32 form */
+__device__ __forceinline__ void mhd_e32(EmfF &o)
+{
+    const float vx = o.ex, vy = o.ey, vz = o.ez;
+    o.ex = o.by * vz - o.bz * vy;
+    o.ey = o.bz * vx - o.bx * vz;
+    o.ez = o.bx * vy - o.by * vx;
+}
+
 /* ------------------------------------------------------------------ */
 /* get_emf, FP64 (src/emfields.c:32-60)                                */
 /* ------------------------------------------------------------------ */
@@ -843,6 +868,30 @@ __device__ __forceinline__ void get_emf64(const FieldDev &f, double x, double y,
         o.ex = 0.0; o.ey = 0.0; o.ez = 0.0;
     } else if (FK == FK_WLCS) {
         emf_wlcs(f, x, y, z, t, o);
+    } else if (FK == FK_MHD_REF) {
+        get_emf64<FK_GRID_REF>(f, x, y, z, t, o);
+        mhd_e64(o);
+    } else if (FK == FK_GRIDD_REF) {
+        /* get_double_efield_at_point / get_double_bfield_at_point
+         * (src/electric_field.c:370-406, src/magnetic_field.c:555-591):
+         * the reference's weights, double accumulators */
+        unsigned c[8];
+        double w[8];
+        trilinear_ref(f.nx, f.ny, f.nz, f.lo, f.hi, f.d, x, y, z, c, w);
+        double ax = 0.0, ay = 0.0, az = 0.0, bx = 0.0, by = 0.0, bz = 0.0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const double2 *nd = (const double2 *)(f.gridd + 8 * (size_t)c[k]);
+            const double2 p = nd[0], q = nd[1], r = nd[2];
+            ax = ax + p.x * w[k];
+            ay = ay + p.y * w[k];
+            az = az + q.x * w[k];
+            bx = bx + q.y * w[k];
+            by = by + r.x * w[k];
+            bz = bz + r.y * w[k];
+        }
+        o.ex = ax; o.ey = ay; o.ez = az;
+        o.bx = bx; o.by = by; o.bz = bz;
     }
 }
 
@@ -991,6 +1040,9 @@ __device__ __forceinline__ void get_emf32(const FieldDev &f, double x, double y,
         modes_sum32(f.modes32, f.nmodes, (float)x, (float)y, (float)z, bx, by, bz);
         o.bx = bx; o.by = by; o.bz = (float)f.b0z + bz;
         o.ex = 0.f; o.ey = 0.f; o.ez = 0.f;
+    } else if (FK == FK_MHD_REF) {
+        get_emf32<FK_GRID_REF>(f, x, y, z, t, o);
+        mhd_e32(o);
     } else {
         Emf e;
         get_emf64<FK>(f, x, y, z, t, e);
@@ -1004,12 +1056,16 @@ template <int FK, class GC>
 __device__ __forceinline__ void emf64_step(const FieldDev &f, GC &gc, double x,
                                            double y, double z, double t, Emf &o)
 {
-    if constexpr (FK == FK_GRID_REF || FK == FK_GRID_TB)
+    if constexpr (FK == FK_GRID_REF || FK == FK_GRID_TB) {
         grid_emf64_cached<FK>(f, gc, x, y, z, o);
-    else if constexpr (FK == FK_GRID2_REF || FK == FK_GRID2_TB)
+    } else if constexpr (FK == FK_GRID2_REF || FK == FK_GRID2_TB) {
         grid2_emf64_cached<FK>(f, gc, x, y, z, t, o);
-    else
+    } else if constexpr (FK == FK_MHD_REF) {
+        grid_emf64_cached<FK_GRID_REF>(f, gc, x, y, z, o);
+        mhd_e64(o);
+    } else {
         get_emf64<FK>(f, x, y, z, t, o);
+    }
 }
 
 template <int FK, class GC>
@@ -1020,6 +1076,9 @@ __device__ __forceinline__ void emf32_step(const FieldDev &f, GC &gc, double x,
         grid_emf32_cached<FK>(f, gc, x, y, z, o);
     } else if constexpr (FK == FK_GRID2_REF || FK == FK_GRID2_TB) {
         grid2_emf32_cached<FK>(f, gc, x, y, z, t, o);
+    } else if constexpr (FK == FK_MHD_REF) {
+        grid_emf32_cached<FK_GRID_REF>(f, gc, x, y, z, o);
+        mhd_e32(o);
     } else {
         get_emf32<FK>(f, x, y, z, t, o);
     }

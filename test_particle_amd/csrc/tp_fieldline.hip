@@ -542,6 +542,10 @@ pmap_fn pick_pmap(int fk, int prec)
     case FK_WLCS: return pick_prec<FK_WLCS>(prec);
     case FK_GRID2_REF: return pick_prec<FK_GRID2_REF>(prec);
     case FK_GRID2_TB: return pick_prec<FK_GRID2_TB>(prec);
+    /* system 3: the nodes' B slots hold B as for PIC grids (get_bfield_mhd_
+     * test_particle, src/emfields.c:131-143, gathers only B) */
+    case FK_MHD_REF: return pick_prec<FK_GRID_REF>(prec);
+    case FK_GRIDD_REF: return pick_prec<FK_GRIDD_REF>(prec);
     default: return nullptr;
     }
 }

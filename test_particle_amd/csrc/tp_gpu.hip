@@ -20,6 +20,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <mutex>
 #include <new>
 
 #include "../../include/tp_gpu.h"
@@ -244,6 +245,10 @@ __device__ __forceinline__ void track_body(const TrackArgs &a)
                     emf64_step<FK>(a.f, gc, p.x, p.y, p.z, p.t, e);
                 else if constexpr (FK == FK_GRID_REF || FK == FK_GRID2_REF)
                     ref_emf64_via_cache<FK>(a.f, gc, p.x, p.y, p.z, p.t, e);
+                else if constexpr (FK == FK_MHD_REF) {
+                    ref_emf64_via_cache<FK_GRID_REF>(a.f, gc, p.x, p.y, p.z, p.t, e);
+                    mhd_e64(e);
+                }
                 else
                     get_emf64<FK>(a.f, p.x, p.y, p.z, p.t, e);
             }
@@ -473,7 +478,7 @@ static void launch_track(dim3 g, dim3 b, size_t sh, hipStream_t s, const TrackAr
      * path) and the Wirz push through the reference gather (a few spills;
      * measured 4 % faster than the compiler's 2 waves) */
     if constexpr ((PREC == PREC32 && (FK == FK_GRID2_REF || FK == FK_GRID2_TB)) ||
-                  (PUSH == TP_PUSH_WIRZ && FK == FK_GRID_REF))
+                  (PUSH == TP_PUSH_WIRZ && (FK == FK_GRID_REF || FK == FK_MHD_REF)))
         hipLaunchKernelGGL((k_track_w3<PUSH, FK, PREC, TRAJ>), g, b, sh, s, a);
     else
         hipLaunchKernelGGL((k_track<PUSH, FK, PREC, TRAJ>), g, b, sh, s, a);
@@ -510,6 +515,8 @@ static track_fn pick_track(int fk, int push, int prec, bool traj)
     case FK_WLCS: return pick_track_fk<FK_WLCS>(push, prec, traj);
     case FK_GRID2_REF: return pick_track_fk<FK_GRID2_REF>(push, prec, traj);
     case FK_GRID2_TB: return pick_track_fk<FK_GRID2_TB>(push, prec, traj);
+    case FK_MHD_REF: return pick_track_fk<FK_MHD_REF>(push, prec, traj);
+    case FK_GRIDD_REF: return pick_track_fk<FK_GRIDD_REF>(push, prec, traj);
     }
     return nullptr;
 }
@@ -534,6 +541,8 @@ static void dispatch_dcoeff(int fk, dim3 g, dim3 b, hipStream_t s, const TrackAr
     case FK_MLP: launch_dcoeff<FK_MODES>(g, b, s, a, tstep); break;
     case FK_GRID2_REF: launch_dcoeff<FK_GRID2_REF>(g, b, s, a, tstep); break;
     case FK_GRID2_TB: launch_dcoeff<FK_GRID2_TB>(g, b, s, a, tstep); break;
+    case FK_MHD_REF: launch_dcoeff<FK_MHD_REF>(g, b, s, a, tstep); break;
+    case FK_GRIDD_REF: launch_dcoeff<FK_GRIDD_REF>(g, b, s, a, tstep); break;
     }
 }
 
@@ -561,6 +570,8 @@ static void dispatch_load(int fk, dim3 g, dim3 b, hipStream_t s, const FieldDev 
     case FK_MLP: launch_load<FK_MODES>(g, b, s, f, n, cur, ini, S); break;
     case FK_GRID2_REF: launch_load<FK_GRID2_REF>(g, b, s, f, n, cur, ini, S); break;
     case FK_GRID2_TB: launch_load<FK_GRID2_TB>(g, b, s, f, n, cur, ini, S); break;
+    case FK_MHD_REF: launch_load<FK_MHD_REF>(g, b, s, f, n, cur, ini, S); break;
+    case FK_GRIDD_REF: launch_load<FK_GRIDD_REF>(g, b, s, f, n, cur, ini, S); break;
     }
 }
 
@@ -586,6 +597,8 @@ static void dispatch_eval(int fk, dim3 g, dim3 b, hipStream_t s, const FieldDev 
     case FK_MLP: launch_eval<FK_MODES>(g, b, s, f, n, pts, out, prec); break;
     case FK_GRID2_REF: launch_eval<FK_GRID2_REF>(g, b, s, f, n, pts, out, prec); break;
     case FK_GRID2_TB: launch_eval<FK_GRID2_TB>(g, b, s, f, n, pts, out, prec); break;
+    case FK_MHD_REF: launch_eval<FK_MHD_REF>(g, b, s, f, n, pts, out, prec); break;
+    case FK_GRIDD_REF: launch_eval<FK_GRIDD_REF>(g, b, s, f, n, pts, out, prec); break;
     }
 }
 
@@ -641,6 +654,7 @@ static void free_field(tp_ctx *c)
     hipFree(c->modes); c->modes = nullptr;
     hipFree(c->modes32); c->modes32 = nullptr;
     hipFree(c->wl); c->wl = nullptr;
+    hipFree(c->gridd); c->gridd = nullptr;
     c->have_field = false;
     c->s.derived = false;
 }
@@ -1079,6 +1093,59 @@ int tp_gpu_set_field_grid(tp_ctx *c, int nx, int ny, int nz, const double lo[3],
                      &h, 0);
 }
 
+int tp_gpu_set_field_mhd(tp_ctx *c, int nx, int ny, int nz, const double lo[3],
+                         const double hi[3], const float *vf, const float *bf)
+{
+    /* the velocity takes the E slots of the nodes (and cell records); the
+     * kernels form E = B x v after the gather */
+    int rc = tp_gpu_set_field_grid(c, nx, ny, nz, lo, hi, vf, bf, TP_GATHER_REF_COMPAT);
+    if (rc != TP_OK) return rc;
+    c->f.kind = FK_MHD_REF;
+    return TP_OK;
+}
+
+int tp_gpu_set_field_grid_double(tp_ctx *c, int nx, int ny, int nz, const double lo[3],
+                                 const double hi[3], const double *ef, const double *bf)
+{
+    if (!c || !ef || !bf || !lo || !hi || nx < 2 || ny < 2 || nz < 2) return TP_E_ARG;
+    const size_t nn = (size_t)nx * ny * nz;
+    if (nn >= 0xFFFFFFFFull) return TP_E_ARG;    /* 32-bit corner indices */
+    HIPCHK(hipSetDevice(c->device));
+    free_field(c);
+    memset(&c->f, 0, sizeof(c->f));
+    if (hipMalloc(&c->gridd, nn * 8 * sizeof(double)) != hipSuccess) {
+        c->gridd = nullptr;
+        return TP_E_NOMEM;
+    }
+    /* interleaved on the host in slabs of at most 2^20 nodes */
+    const size_t slab = nn < ((size_t)1 << 20) ? nn : ((size_t)1 << 20);
+    double *h = (double *)malloc(slab * 8 * sizeof(double));
+    if (!h) { free_field(c); return TP_E_NOMEM; }
+    int rc = TP_OK;
+    for (size_t n0 = 0; n0 < nn && rc == TP_OK; n0 += slab) {
+        const size_t cnt = nn - n0 < slab ? nn - n0 : slab;
+        for (size_t k = 0; k < cnt; k++) {
+            const double *e = ef + 3 * (n0 + k), *b = bf + 3 * (n0 + k);
+            double *o = h + 8 * k;
+            o[0] = e[0]; o[1] = e[1]; o[2] = e[2];
+            o[3] = b[0]; o[4] = b[1]; o[5] = b[2];
+            o[6] = 0.0; o[7] = 0.0;
+        }
+        if (hipMemcpy(c->gridd + 8 * n0, h, cnt * 8 * sizeof(double), hipMemcpyHostToDevice) !=
+            hipSuccess)
+            rc = TP_E_HIP;
+    }
+    free(h);
+    if (rc != TP_OK) { free_field(c); return rc; }
+    c->f.nx = nx; c->f.ny = ny; c->f.nz = nz;
+    set_box(c->f.lo, c->f.hi, c->f.d, lo, hi, nx, ny, nz);
+    for (int a = 0; a < 3; a++) c->f.inv_d[a] = 1.0 / c->f.d[a];
+    c->f.gridd = c->gridd;
+    c->f.kind = FK_GRIDD_REF;
+    c->have_field = true;
+    return TP_OK;
+}
+
 int tp_gpu_set_field_grid2(tp_ctx *c, int nx, int ny, int nz, const double lo[3],
                            const double hi[3], const float *ef_next, const float *bf_next,
                            const float *ef_prev, const float *bf_prev, double grid_dt,
@@ -1386,7 +1453,65 @@ static int block_size_for(const tp_ctx *c, int n)
     return 64;
 }
 
-int tp_gpu_run_fixed(tp_ctx *c, const tp_track_params *p)
+/* Buffers of one fixed-step run (the context's own for tp_gpu_run_fixed). */
+struct RunBufs {
+    unsigned *espect, *espect_esc;  /* [nt_out][nbins] counts */
+    double *dco;                    /* [8][nd] D sums */
+    int *rdec;                      /* [nd] escapes counted against each D row */
+    double *partials;               /* block rows of 8 partial D sums */
+    size_t prows;
+    int *live;                      /* open boundaries: the index lists */
+    size_t live_ints;
+    int *flag;                      /* non-finite state found (1 int) */
+    bool per_launch_events;         /* c->evs around every launch, else ev0/ev1
+                                       around the run */
+};
+
+__global__ void __launch_bounds__(256) k_fill_u32(unsigned *a, size_t n, unsigned v)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        a[i] = v;
+}
+
+__global__ void __launch_bounds__(256) k_fill_f64(double *a, size_t n, double v)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        a[i] = v;
+}
+
+/* flag = 1 if any particle's position or momentum is NaN or infinite (the
+ * reference lets them propagate: B = 0 makes iBtot infinite in
+ * tracking_wirz, src/tracking.c:782) */
+__global__ void __launch_bounds__(256) k_nonfinite(int n, const double *x, const double *y,
+                                                   const double *z, const double *ux,
+                                                   const double *uy, const double *uz,
+                                                   int *flag)
+{
+    bool bad = false;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
+        bad |= !isfinite(x[i]) || !isfinite(y[i]) || !isfinite(z[i]) || !isfinite(ux[i]) ||
+               !isfinite(uy[i]) || !isfinite(uz[i]);
+    if (__ballot(bad) != 0ull && (threadIdx.x & 63) == 0) atomicOr(flag, 1);
+}
+
+static int fill_u32(hipStream_t s, unsigned *a, size_t n)
+{
+    if (n == 0) return TP_OK;
+    const size_t nbk = (n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)nbk), dim3(256), 0, s, a, n, 0u);
+    return hipGetLastError() == hipSuccess ? TP_OK : TP_E_HIP;
+}
+
+static int fill_f64(hipStream_t s, double *a, size_t n)
+{
+    if (n == 0) return TP_OK;
+    const size_t nbk = (n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096;
+    hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)nbk), dim3(256), 0, s, a, n, 0.0);
+    return hipGetLastError() == hipSuccess ? TP_OK : TP_E_HIP;
+}
+
+/* the checks of a run's parameters; the step kernel that runs it */
+static int run_check(const tp_ctx *c, const tp_track_params *p, track_fn *fn, bool *mlp)
 {
     if (!c || !p) return TP_E_ARG;
     if (!c->have_field || !c->s.derived) return TP_E_STATE;
@@ -1396,58 +1521,64 @@ int tp_gpu_run_fixed(tp_ctx *c, const tp_track_params *p)
         return TP_E_ARG;
     if (p->pusher != TP_PUSH_WIRZ && p->pusher != TP_PUSH_BORIS) return TP_E_ARG;
     if (p->precision != TP_FP64 && p->precision != TP_FP32_MIXED) return TP_E_ARG;
-    const bool mlp = field_kind(c) == FK_MLP;
-    if (mlp && (p->pusher != TP_PUSH_BORIS || p->precision != TP_FP32_MIXED)) return TP_E_ARG;
-    if (mlp && c->traj_nout > 0) return TP_E_ARG;
+    *mlp = field_kind(c) == FK_MLP;
+    if (*mlp && (p->pusher != TP_PUSH_BORIS || p->precision != TP_FP32_MIXED)) return TP_E_ARG;
+    if (*mlp && c->traj_nout > 0) return TP_E_ARG;
     /* surrogate field: the exact-field kernel runs the particles that fell back */
-    track_fn fn = pick_track(mlp ? FK_MODES : field_kind(c), p->pusher, p->precision,
-                             c->traj_nout > 0);
-    if (!fn) return TP_E_ARG;
-    HIPCHK(hipSetDevice(c->device));
+    *fn = pick_track(*mlp ? FK_MODES : field_kind(c), p->pusher, p->precision,
+                     c->traj_nout > 0);
+    return *fn ? TP_OK : TP_E_ARG;
+}
+
+/* the sizes a run over n particles needs */
+struct RunSizes {
+    int nd, bs, nblocks, nb256;
+    size_t prows, live_ints;
+    long nlaunch;
+};
+
+static RunSizes run_sizes(const tp_ctx *c, const tp_track_params *p, int n)
+{
+    RunSizes r;
+    const int di = p->dcoeff_interval;
+    r.nd = p->estep / di + 1;
+    r.bs = block_size_for(c, n);
+    r.nblocks = n > 0 ? (n + r.bs - 1) / r.bs : 0;
+    r.nb256 = (n + 255) / 256;
+    /* partial-sum rows: the step kernel's blocks, or with open boundaries a
+     * launch over the particles inside plus k_dstale's blocks of 256 */
+    r.prows = (size_t)(r.nblocks > 0 ? r.nblocks : 1) + r.nb256 + 2;
+    /* open boundaries: inside[n], gone[n], block counts and offsets, total */
+    r.live_ints = 2 * (size_t)n + 2 * (size_t)r.nb256 + 1;
+    /* launches: each ends at the next D step (a multiple of di) or estep */
+    r.nlaunch = 0;
+    for (long j0 = p->sstep; j0 <= p->estep; r.nlaunch++) {
+        long j1 = (j0 / di + 1) * (long)di;
+        if (j0 % di == 0) j1 = j0;
+        if (j1 > p->estep) j1 = p->estep;
+        j0 = j1 + 1;
+    }
+    return r;
+}
+
+/* the launches of a fixed-step run over the resident particles into the
+ * buffers B (cleared here) */
+static int run_fixed_impl(tp_ctx *c, const tp_track_params *p, track_fn fn, bool mlp,
+                          const RunSizes &R, const RunBufs &B)
+{
     const int n = c->s.n;
-    if (mlp) {
-        int rc = tp_mlp_prepare_run(c, n);
-        if (rc) return rc;
-    }
     const int nb = p->nbins, nt = p->nt_out, di = p->dcoeff_interval;
-    const int nd = p->estep / di + 1;
-    /* diagnostics buffers */
-    size_t need = (size_t)nb * nt;
-    if (need > c->diag_cap) {
-        hipFree(c->espect); hipFree(c->espect_esc);
-        c->espect = nullptr; c->espect_esc = nullptr; c->diag_cap = 0;
-        if (hipMalloc(&c->espect, need * sizeof(unsigned)) != hipSuccess ||
-            hipMalloc(&c->espect_esc, need * sizeof(unsigned)) != hipSuccess)
-            return TP_E_NOMEM;
-        c->diag_cap = need;
-    }
-    if ((size_t)nd > c->dco_cap) {
-        hipFree(c->dco); hipFree(c->rdec);
-        c->dco = nullptr; c->rdec = nullptr; c->dco_cap = 0;
-        if (hipMalloc(&c->dco, sizeof(double) * 8 * nd) != hipSuccess ||
-            hipMalloc(&c->rdec, sizeof(int) * (nd + 1)) != hipSuccess)
-            return TP_E_NOMEM;
-        c->dco_cap = nd;
-    }
-    HIPCHK(hipMemsetAsync(c->espect, 0, need * sizeof(unsigned), c->stream));
-    HIPCHK(hipMemsetAsync(c->espect_esc, 0, need * sizeof(unsigned), c->stream));
-    HIPCHK(hipMemsetAsync(c->dco, 0, sizeof(double) * 8 * nd, c->stream));
-    HIPCHK(hipMemsetAsync(c->rdec, 0, sizeof(int) * (nd + 1), c->stream));
-    c->nb = nb; c->nt = nt; c->nd = nd; c->last_n = n;
+    const int nd = R.nd, bs = R.bs, nblocks = R.nblocks, nb256 = R.nb256;
+    const size_t need = (size_t)nb * nt;
+    int rc;
+    if ((rc = fill_u32(c->stream, B.espect, need)) ||
+        (rc = fill_u32(c->stream, B.espect_esc, need)) ||
+        (rc = fill_f64(c->stream, B.dco, (size_t)8 * nd)) ||
+        (rc = fill_u32(c->stream, (unsigned *)B.rdec, (size_t)nd)))
+        return rc;
     c->last_ms = 0.0;
     c->last_launches = 0;
 
-    const int bs = block_size_for(c, n);
-    const int nblocks = n > 0 ? (n + bs - 1) / bs : 0;
-    /* partial-sum rows: the step kernel's blocks, or with open boundaries a
-     * launch over the particles inside plus k_dstale's blocks of 256 */
-    size_t pneed = ((size_t)(nblocks > 0 ? nblocks : 1) + (n + 255) / 256 + 2) * 8;
-    if (pneed > c->partials_cap) {
-        hipFree(c->partials);
-        c->partials = nullptr; c->partials_cap = 0;
-        if (hipMalloc(&c->partials, pneed * sizeof(double)) != hipSuccess) return TP_E_NOMEM;
-        c->partials_cap = pneed;
-    }
     size_t shmem = sizeof(unsigned) * nb;
     if (shmem > 64 * 1024) return TP_E_ARG;
     /* open boundaries: after each launch the particles are split (stably)
@@ -1456,39 +1587,10 @@ int tp_gpu_run_fixed(tp_ctx *c, const tp_track_params *p)
      * particles do not idle through the remaining steps.  Its D terms are
      * formed in the step kernel for the particles inside and by k_dstale
      * (stale terms, escape count) for the others; per-particle results and
-     * spectra are unchanged, the D sums are summed in that block order.
-     * Buffer: inside[n], gone[n], block counts and offsets, total. */
+     * spectra are unchanged, the D sums are summed in that block order. */
     const bool compact = p->bc_flag == 1 && !mlp && n > 0;
-    const int nb256 = (n + 255) / 256;
-    const size_t live_need = 2 * (size_t)n + 2 * (size_t)nb256 + 1;
-    if (compact && live_need > c->live_cap) {
-        hipFree(c->live_idx);
-        c->live_idx = nullptr; c->live_cap = 0;
-        if (hipMalloc(&c->live_idx, live_need * sizeof(int)) != hipSuccess) return TP_E_NOMEM;
-        c->live_cap = live_need;
-    }
-    int *l_in = c->live_idx, *l_gone = l_in + n, *l_cnt = l_gone + n;
+    int *l_in = B.live, *l_gone = l_in + n, *l_cnt = l_gone + n;
     int *l_off = l_cnt + nb256, *l_tot = l_off + nb256;
-    /* launches: each ends at the next D step (a multiple of di) or estep */
-    long nlaunch = 0;
-    for (long j0 = p->sstep; j0 <= p->estep; nlaunch++) {
-        long j1 = (j0 / di + 1) * (long)di;
-        if (j0 % di == 0) j1 = j0;
-        if (j1 > p->estep) j1 = p->estep;
-        j0 = j1 + 1;
-    }
-    if (2 * nlaunch > c->nev) {      /* two events per launch */
-        for (int k = 0; k < c->nev; k++) hipEventDestroy(c->evs[k]);
-        free(c->evs);
-        c->nev = 0;
-        c->evs = (hipEvent_t *)malloc(sizeof(hipEvent_t) * 2 * nlaunch);
-        if (!c->evs) return TP_E_NOMEM;
-        for (long k = 0; k < 2 * nlaunch; k++) {
-            HIPCHK(hipEventCreate(&c->evs[k]));
-            c->nev = (int)(k + 1);
-        }
-        c->nev = (int)(2 * nlaunch);
-    }
 
     TrackArgs a;
     memset(&a, 0, sizeof(a));
@@ -1506,8 +1608,8 @@ int tp_gpu_run_fixed(tp_ctx *c, const tp_track_params *p)
     a.x0 = S.x0; a.y0 = S.y0; a.z0 = S.z0; a.ux0 = S.ux0; a.uy0 = S.uy0; a.uz0 = S.uz0;
     a.g0 = S.g0; a.mu0 = S.mu0; a.th0 = S.th0;
     a.st = S.st; a.esc = S.esc; a.nsteps = S.nsteps;
-    a.espect = c->espect; a.espect_esc = c->espect_esc;
-    a.partials = c->partials; a.rdec = c->rdec;
+    a.espect = B.espect; a.espect_esc = B.espect_esc;
+    a.partials = B.partials; a.rdec = B.rdec;
     if (c->traj_nout > 0) {
         a.traj = c->traj; a.traj_accum = c->traj_accum; a.traj_pts = c->traj_pts;
         a.traj_nout = c->traj_nout; a.stage = S.stage;
@@ -1520,6 +1622,7 @@ int tp_gpu_run_fixed(tp_ctx *c, const tp_track_params *p)
     bool use_list = false;      /* the launch runs over l_in[0 .. nlive) */
     int nlive = n;
     int row_blocks = nblocks;   /* partial-sum rows of the launch's D row */
+    if (!B.per_launch_events) HIPCHK(hipEventRecord(c->ev0, c->stream));
     for (long j0 = p->sstep; j0 <= p->estep && nblocks > 0;) {
         long j1 = (j0 % di == 0) ? j0 : (j0 / di + 1) * (long)di;
         if (j1 > p->estep) j1 = p->estep;
@@ -1530,7 +1633,7 @@ int tp_gpu_run_fixed(tp_ctx *c, const tp_track_params *p)
         const bool dstep = j1 % di == 0;
         a.dstep = (dstep && !mlp) ? tstep : -1;
         row_blocks = nblocks;
-        HIPCHK(hipEventRecord(c->evs[2 * launches], c->stream));
+        if (B.per_launch_events) HIPCHK(hipEventRecord(c->evs[2 * launches], c->stream));
         if (use_list) {
             const int nbl = (nlive + bs - 1) / bs, nesc = n - nlive;
             if (nlive > 0) {
@@ -1555,13 +1658,13 @@ int tp_gpu_run_fixed(tp_ctx *c, const tp_track_params *p)
              * start after a multiple of the check interval */
             const int ci = c->f.mlp.check;
             const int chk = a.first || (ci > 0 && (j0 - 1) % ci == 0);
-            int rc = tp_mlp_launch(c, a, shmem, chk, bs, fn);
+            rc = tp_mlp_launch(c, a, shmem, chk, bs, fn);
             if (rc) return rc;
         } else {
             fn(dim3(nblocks), dim3(bs), shmem, c->stream, a);
         }
         HIPCHK(hipGetLastError());
-        HIPCHK(hipEventRecord(c->evs[2 * launches + 1], c->stream));
+        if (B.per_launch_events) HIPCHK(hipEventRecord(c->evs[2 * launches + 1], c->stream));
         launches++;
         for (long j = j0; j <= j1; j++)
             if (j % p->einterval_t == 0) ntp++;
@@ -1570,8 +1673,8 @@ int tp_gpu_run_fixed(tp_ctx *c, const tp_track_params *p)
                 dispatch_dcoeff(field_kind(c), dim3(nblocks), dim3(bs), c->stream, a, tstep);
                 HIPCHK(hipGetLastError());
             }
-            hipLaunchKernelGGL(k_reduce_row, dim3(8), dim3(256), 0, c->stream, c->partials,
-                               row_blocks, tstep, nd, c->dco);
+            hipLaunchKernelGGL(k_reduce_row, dim3(8), dim3(256), 0, c->stream, B.partials,
+                               row_blocks, tstep, nd, B.dco);
             HIPCHK(hipGetLastError());
             tstep++;
         }
@@ -1592,16 +1695,104 @@ int tp_gpu_run_fixed(tp_ctx *c, const tp_track_params *p)
         }
         j0 = j1 + 1;
     }
+    if (!B.per_launch_events) HIPCHK(hipEventRecord(c->ev1, c->stream));
+    int bad = 0;
+    if ((p->flags & TP_FLAG_CHECK_FINITE) && n > 0) {
+        if ((rc = fill_u32(c->stream, (unsigned *)B.flag, 1))) return rc;
+        const int nbk = nb256 < 2048 ? nb256 : 2048;
+        hipLaunchKernelGGL(k_nonfinite, dim3(nbk), dim3(256), 0, c->stream, n,
+                           (const double *)S.x, (const double *)S.y, (const double *)S.z,
+                           (const double *)S.ux, (const double *)S.uy, (const double *)S.uz,
+                           B.flag);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(&bad, B.flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    }
     HIPCHK(hipStreamSynchronize(c->stream));
     double ms_total = 0.0;
-    for (int k = 0; k < launches; k++) {
+    if (B.per_launch_events) {
+        for (int k = 0; k < launches; k++) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, c->evs[2 * k], c->evs[2 * k + 1]));
+            ms_total += ms;
+        }
+    } else if (launches > 0) {
         float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, c->evs[2 * k], c->evs[2 * k + 1]));
-        ms_total += ms;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        ms_total = ms;
     }
     c->last_ms = ms_total;
     c->last_launches = launches;
-    return TP_OK;
+    return bad ? TP_E_NONFINITE : TP_OK;
+}
+
+int tp_gpu_run_fixed(tp_ctx *c, const tp_track_params *p)
+{
+    track_fn fn = nullptr;
+    bool mlp = false;
+    int rc = run_check(c, p, &fn, &mlp);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(c->device));
+    const int n = c->s.n;
+    if (mlp) {
+        rc = tp_mlp_prepare_run(c, n);
+        if (rc) return rc;
+    }
+    const RunSizes R = run_sizes(c, p, n);
+    const int nd = R.nd;
+    /* diagnostics buffers */
+    size_t need = (size_t)p->nbins * p->nt_out;
+    if (need > c->diag_cap) {
+        hipFree(c->espect); hipFree(c->espect_esc);
+        c->espect = nullptr; c->espect_esc = nullptr; c->diag_cap = 0;
+        if (hipMalloc(&c->espect, need * sizeof(unsigned)) != hipSuccess ||
+            hipMalloc(&c->espect_esc, need * sizeof(unsigned)) != hipSuccess)
+            return TP_E_NOMEM;
+        c->diag_cap = need;
+    }
+    if ((size_t)nd > c->dco_cap) {
+        hipFree(c->dco); hipFree(c->rdec);
+        c->dco = nullptr; c->rdec = nullptr; c->dco_cap = 0;
+        if (hipMalloc(&c->dco, sizeof(double) * 8 * nd) != hipSuccess ||
+            hipMalloc(&c->rdec, sizeof(int) * (nd + 1)) != hipSuccess)
+            return TP_E_NOMEM;
+        c->dco_cap = nd;
+    }
+    /* partial-sum rows and the non-finite flag after them */
+    size_t pneed = R.prows * 8 + 1;
+    if (pneed > c->partials_cap) {
+        hipFree(c->partials);
+        c->partials = nullptr; c->partials_cap = 0;
+        if (hipMalloc(&c->partials, pneed * sizeof(double)) != hipSuccess) return TP_E_NOMEM;
+        c->partials_cap = pneed;
+    }
+    const bool compact = p->bc_flag == 1 && !mlp && n > 0;
+    if (compact && R.live_ints > c->live_cap) {
+        hipFree(c->live_idx);
+        c->live_idx = nullptr; c->live_cap = 0;
+        if (hipMalloc(&c->live_idx, R.live_ints * sizeof(int)) != hipSuccess) return TP_E_NOMEM;
+        c->live_cap = R.live_ints;
+    }
+    if (2 * R.nlaunch > c->nev) {      /* two events per launch */
+        for (int k = 0; k < c->nev; k++) hipEventDestroy(c->evs[k]);
+        free(c->evs);
+        c->nev = 0;
+        c->evs = (hipEvent_t *)malloc(sizeof(hipEvent_t) * 2 * R.nlaunch);
+        if (!c->evs) return TP_E_NOMEM;
+        for (long k = 0; k < 2 * R.nlaunch; k++) {
+            HIPCHK(hipEventCreate(&c->evs[k]));
+            c->nev = (int)(k + 1);
+        }
+        c->nev = (int)(2 * R.nlaunch);
+    }
+    c->nb = p->nbins; c->nt = p->nt_out; c->nd = nd; c->last_n = n;
+    RunBufs B;
+    B.espect = c->espect; B.espect_esc = c->espect_esc;
+    B.dco = c->dco; B.rdec = c->rdec;
+    B.partials = c->partials; B.prows = R.prows;
+    B.live = c->live_idx; B.live_ints = c->live_cap;
+    B.flag = (int *)(c->partials + R.prows * 8);
+    B.per_launch_events = true;
+    return run_fixed_impl(c, p, fn, mlp, R, B);
 }
 
 int tp_gpu_last_kernel_time(tp_ctx *c, double *ms, int *launches)
@@ -1713,7 +1904,7 @@ int tp_gpu_track_fixed_traj(tp_ctx *c, const tp_track_params *p, int n, tp_parti
     int rc = tp_gpu_track_fixed(c, p, n, ptl, ptl_init, nsteps, espect, espect_escape, dcoeff,
                                 nptl_remain);
     c->traj_nout = 0;
-    if (rc) return rc;
+    if (rc != TP_OK && rc != TP_E_NONFINITE) return rc;
     if (n > 0) {
         HIPCHK(hipMemcpyAsync(ptl_time, c->traj, ntraj * sizeof(tp_particle),
                               hipMemcpyDeviceToHost, c->stream));
@@ -1722,7 +1913,7 @@ int tp_gpu_track_fixed_traj(tp_ctx *c, const tp_track_params *p, int n, tp_parti
                                   c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
     }
-    return TP_OK;
+    return rc;
 }
 
 int tp_gpu_track_fixed(tp_ctx *c, const tp_track_params *p, int n, tp_particle *ptl,
@@ -1732,21 +1923,15 @@ int tp_gpu_track_fixed(tp_ctx *c, const tp_track_params *p, int n, tp_particle *
     if (!c || !p || n < 0 || (n > 0 && (!ptl || !ptl_init))) return TP_E_ARG;
     int rc = tp_gpu_load_particles(c, n, ptl, ptl_init);
     if (rc) return rc;
-    rc = tp_gpu_run_fixed(c, p);
-    if (rc) return rc;
+    /* TP_E_NONFINITE (TP_FLAG_CHECK_FINITE) still returns the particles and
+     * diagnostics */
+    const int run = tp_gpu_run_fixed(c, p);
+    if (run != TP_OK && run != TP_E_NONFINITE) return run;
     rc = tp_gpu_store_particles(c, ptl, nsteps);
     if (rc) return rc;
     rc = tp_gpu_read_diagnostics(c, espect, espect_escape, dcoeff, nptl_remain);
     if (rc) return rc;
-    if (p->flags & TP_FLAG_CHECK_FINITE) {
-        for (int i = 0; i < n; i++) {
-            const tp_particle &q = ptl[i];
-            if (!isfinite(q.x) || !isfinite(q.y) || !isfinite(q.z) || !isfinite(q.ux) ||
-                !isfinite(q.uy) || !isfinite(q.uz))
-                return TP_E_NONFINITE;
-        }
-    }
-    return TP_OK;
+    return run;
 }
 
 } /* extern "C" */

@@ -201,4 +201,12 @@ TP_HD double tp_pow(double x, double y)
     return tp_exp2(y * tp_log2(x));
 }
 
+/* x^e for an integer e >= 0 by repeated multiplication, left to right */
+TP_HD double tp_powi(double x, int e)
+{
+    double r = 1.0;
+    for (int i = 0; i < e; i++) r = r * x;
+    return r;
+}
+
 #endif /* TP_MATH_H */

@@ -1162,6 +1162,19 @@ int tp_gpu_mlp_fallback_count(tp_ctx *c, long long *count)
     return TP_OK;
 }
 
+int tp_gpu_mlp_fallback_flags(tp_ctx *c, int n, int *flags)
+{
+    if (!c || n < 0 || (n > 0 && !flags)) return TP_E_ARG;
+    if (!c->mlp || !c->mlp->fb) return TP_E_STATE;
+    if (n != c->s.n || n > c->mlp->fb_cap) return TP_E_ARG;
+    if (n == 0) return TP_OK;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(flags, c->mlp->fb, sizeof(int) * n, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; i++) flags[i] = flags[i] != 0;
+    return TP_OK;
+}
+
 }  /* extern "C" */
 
 /* run_fixed's hooks for the surrogate field: per-particle flags and index

@@ -44,6 +44,9 @@ def lib():
                                              P(ctypes.c_float), P(ctypes.c_float),
                                              P(ctypes.c_float), d, i]
         L.tp_gpu_set_field_modes.argtypes = [vp, i, P(d), d]
+        L.tp_gpu_set_field_mhd.argtypes = [vp, i, i, i, P(d), P(d), P(ctypes.c_float),
+                                           P(ctypes.c_float)]
+        L.tp_gpu_set_field_grid_double.argtypes = [vp, i, i, i, P(d), P(d), P(d), P(d)]
         L.tp_gpu_load_field_grid.argtypes = [vp, i, i, i, P(d), P(d), i, i, d, d, d, i,
                                              PLANE_READER, vp, ctypes.c_size_t]
         L.tp_gpu_set_field_grid_planes.argtypes = [vp, i, i, i, P(d), P(d), i, d, d,
@@ -64,6 +67,8 @@ def lib():
         L.tp_gpu_track_adaptive.argtypes = [vp, P(abi.TpBsParams), i, P(d), P(i),
                                             P(abi.TpBsStats)]
         L.tp_gpu_tune_adaptive.argtypes = [vp, P(abi.TpBsParams), ctypes.c_char_p, P(i), P(i)]
+        L.tp_gpu_run_adaptive_diag.argtypes = [vp, P(abi.TpBsParams), P(abi.TpTrackParams),
+                                               P(abi.TpBsStats)]
         f = ctypes.c_float
         L.tp_gpu_mlp_set.argtypes = [vp, P(f), P(f), d]
         L.tp_gpu_mlp_get.argtypes = [vp, P(f)]
@@ -73,6 +78,7 @@ def lib():
                                              P(abi.TpMlpTrainStats)]
         L.tp_gpu_set_field_mlp.argtypes = [vp, d, i]
         L.tp_gpu_mlp_fallback_count.argtypes = [vp, P(ctypes.c_longlong)]
+        L.tp_gpu_mlp_fallback_flags.argtypes = [vp, i, P(i)]
         L.tp_gpu_poincare_map.argtypes = [vp, P(abi.TpPmapParams), i, P(d), P(d), P(d), P(d),
                                           P(i), P(ctypes.c_longlong)]
         cs = ctypes.c_char_p
@@ -169,6 +175,28 @@ class Tracker:
         nz, ny, nx = e.shape[:3]
         _check(lib().tp_gpu_set_field_grid(self._ctx, nx, ny, nz, _vec3(lo), _vec3(hi),
                                            fptr(e), fptr(b), mode), "set_field_grid")
+
+    def set_field_mhd(self, v, b, lo, hi):
+        """System 3: float velocity and magnetic nodes (nz, ny, nx, 3); E = B x v
+        (tp_gpu_set_field_mhd)."""
+        v = np.ascontiguousarray(v, dtype=np.float32)
+        b = np.ascontiguousarray(b, dtype=np.float32)
+        if v.shape != b.shape:
+            raise ValueError("v and b must have the same shape")
+        nz, ny, nx = v.shape[:3]
+        _check(lib().tp_gpu_set_field_mhd(self._ctx, nx, ny, nz, _vec3(lo), _vec3(hi),
+                                          fptr(v), fptr(b)), "set_field_mhd")
+
+    def set_field_grid_double(self, e, b, lo, hi):
+        """PIC grid of doubles (nz, ny, nx, 3) (tp_gpu_set_field_grid_double)."""
+        e = np.ascontiguousarray(e, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        if e.shape != b.shape:
+            raise ValueError("e and b must have the same shape")
+        nz, ny, nx = e.shape[:3]
+        _check(lib().tp_gpu_set_field_grid_double(self._ctx, nx, ny, nz, _vec3(lo),
+                                                  _vec3(hi), dptr(e), dptr(b)),
+               "set_field_grid_double")
 
     def set_field_grid2(self, e_next, b_next, e_prev, b_prev, lo, hi, grid_dt,
                         mode=abi.TP_GATHER_REF_COMPAT):
@@ -292,6 +320,7 @@ class Tracker:
         _check(lib().tp_gpu_track_fixed(self._ctx, ctypes.byref(params), n, dptr(ptl),
                                         dptr(ptl_init), iptr(nsteps), dptr(esp), dptr(ese),
                                         dptr(dco), iptr(rem)), "track_fixed")
+        self._n = n
         return dict(ptl=ptl, nsteps=nsteps, espect=esp, espect_escape=ese, dcoeff=dco,
                     remain=rem)
 
@@ -356,6 +385,25 @@ class Tracker:
         return dict(accepted=st.accepted, rejected=st.rejected, rhs=st.rhs_evals,
                     failed=st.failed, kernel_ms=st.kernel_ms)
 
+    def run_adaptive_diag(self, bs, params):
+        """tp_gpu_run_adaptive_diag on the resident particles: the adaptive
+        stepper with dense output over estep*dt and the fixed-step loop's
+        diagnostics (read with diagnostics(params))."""
+        st = abi.TpBsStats()
+        _check(lib().tp_gpu_run_adaptive_diag(self._ctx, ctypes.byref(bs), ctypes.byref(params),
+                                              ctypes.byref(st)), "run_adaptive_diag")
+        return dict(accepted=st.accepted, rejected=st.rejected, rhs=st.rhs_evals,
+                    failed=st.failed, kernel_ms=st.kernel_ms)
+
+    def track_adaptive_diag(self, bs, params, ptl, ptl_init=None):
+        """load + run_adaptive_diag + store + diagnostics; returns what
+        track_fixed returns (nsteps = accepted steps) and the step totals."""
+        self.load(ptl, ptl_init)
+        r = self.run_adaptive_diag(bs, params)
+        p, ns = self.store()
+        r.update(ptl=p, nsteps=ns, **self.diagnostics(params))
+        return r
+
     def track_adaptive(self, params, ptl):
         ptl = abi.as_particles(ptl).copy()
         nsteps = np.zeros(ptl.shape[0], dtype=np.int32)
@@ -419,6 +467,14 @@ class Tracker:
         _check(lib().tp_gpu_mlp_fallback_count(self._ctx, ctypes.byref(v)),
                "mlp_fallback_count")
         return v.value
+
+    def mlp_fallback_flags(self):
+        """Per resident particle: 1 if it used the exact field at the end of
+        the last surrogate run."""
+        f = np.zeros(self._n, dtype=np.int32)
+        _check(lib().tp_gpu_mlp_fallback_flags(self._ctx, self._n, iptr(f)),
+               "mlp_fallback_flags")
+        return f
 
     def poincare_map(self, params, starts):
         """tp_gpu_poincare_map: returns dict(xpos, ypos, zpos) of shape

@@ -171,6 +171,7 @@ int poincare_map(int mpi_rank, int mpi_size, char *config_file_name, domain *sdo
             fprintf(stderr, "poincare_map: GPU path failed: %s\n", tp_gpu_strerror(rc));
             MPI_Abort(MPI_COMM_WORLD, 1);
         }
+        fprintf(stderr, "tp_gpu: poincare_map on the GPU (%d field lines)\n", npoints);
         for (int i = 0; i < npoints; i++)
             if (nc[i] < ncross)   /* src/poincare_map.c:179-182 */
                 printf("Reached maximum length for field-line tracing before %d crosses\n",

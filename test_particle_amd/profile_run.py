@@ -88,6 +88,10 @@ def main():
     ap.add_argument("--two", action="store_true", help="grid: two time slices (dt 20)")
     ap.add_argument("--cells", default="auto", choices=["auto", "off", "on"],
                     help="grid: cell-major copy of the grid")
+    ap.add_argument("--dinterval", type=int, default=100,
+                    help="grid/abc: steps between diffusion rows (a launch ends at each)")
+    ap.add_argument("--reps", type=int, default=1,
+                    help="grid/abc: runs of --steps steps, each from the initial particles")
     a = ap.parse_args()
     if a.what == "copy":
         import torch
@@ -194,15 +198,18 @@ def main():
         tr.set_field_abc(1.0, 1.0, 1.0)
         dt = 0.01 * gyroperiod(0.7, abc_b_ref())
     ptl = init_particles(a.particles, lo, hi, 0.7)
-    tr.load(ptl)
     prm = abi.make_params(dt=dt, estep=a.steps, lo=lo, hi=hi, pusher=pusher, precision=prec,
-                          nbins=102, nt_out=max(1, a.steps // 100), emin=1e-4, emax=1.0)
-    t0 = time.perf_counter()
-    tr.run(prm)
-    wall = time.perf_counter() - t0
-    ms, k = tr.last_kernel_time()
-    print(f"{a.what}: {a.particles} x {a.steps} steps, {a.particles * a.steps / wall:.4e} "
-          f"particle-steps/s, kernels {ms:.2f} ms / {k} launches", flush=True)
+                          nbins=102, nt_out=max(1, a.steps // 100), emin=1e-4, emax=1.0,
+                          dcoeff_interval=a.dinterval)
+    for rep in range(a.reps):
+        tr.load(ptl)
+        t0 = time.perf_counter()
+        tr.run(prm)
+        wall = time.perf_counter() - t0
+        ms, k = tr.last_kernel_time()
+        print(f"{a.what} #{rep}: {a.particles} x {a.steps} steps, "
+              f"{a.particles * a.steps / wall:.4e} particle-steps/s, kernels {ms:.2f} ms / "
+              f"{k} launches", flush=True)
     tr.close()
     sys.stdout.flush()
 

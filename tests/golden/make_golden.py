@@ -7,6 +7,7 @@ reference's outputs; nothing of the reference's source is stored.
 
     python tests/golden/make_golden.py      # in the build container only
     python tests/golden/make_golden.py h5   # only the PIC HDF5 input fixture
+    python tests/golden/make_golden.py mhd  # only the system-3 and double-grid fixtures
 """
 import json
 import os
@@ -161,11 +162,56 @@ def h5_case(tmp):
              planes=planes, gname="Timestep_3", e=out[0], b=out[1])
 
 
+def mhd_cases(tmp):
+    """System 3 (MHD: float velocity and magnetic grids, E = B x v,
+    src/emfields.c:107-120) through get_emf, tracking_wirz and the
+    fixed-step loop; and the reference's double gathers
+    (src/electric_field.c:370-406, src/magnetic_field.c:555-591) at points."""
+    os.makedirs(os.path.join(tmp, "data"), exist_ok=True)
+    rng = np.random.default_rng(313)
+    glo, ghi = (0.0, -1.0, 2.0), (5.0, 3.0, 5.0)
+    nz, ny, nx = 4, 5, 6
+    vg = rng.normal(0, 0.2, (nz, ny, nx, 3)).astype(np.float32)
+    bg = rng.normal(0, 1, (nz, ny, nx, 3)).astype(np.float32)
+    bg[..., 2] += 2.0
+    np.concatenate([vg.ravel(), bg.ravel()]).tofile(os.path.join(tmp, "mgrid.bin"))
+    mspec = dict(system=3, nx=nx, ny=ny, nz=nz, tmax=1.0, dt=0.01,
+                 grid=os.path.join(tmp, "mgrid.bin"), **box(glo, ghi))
+    mp = np.column_stack([rng.uniform(-0.3, 5.3, 400), rng.uniform(-1.2, 3.2, 400),
+                          rng.uniform(1.8, 5.2, 400), np.zeros(400)])
+    fields_case("fields_mhd", "ref_harness", mspec, mp,
+                extra=dict(vgrid=vg, bgrid=bg, lo=glo, hi=ghi), tmp=tmp)
+    mptl = init_particles(32, glo, ghi, 0.3, seed=314)
+    wirz_case("wirz_mhd", "ref_harness", dict(mspec, bc=0), mptl, [1, 50, 200],
+              extra=dict(dt=0.01, vgrid=vg, bgrid=bg, lo=glo, hi=ghi), tmp=tmp)
+    track_case("track_mhd", "ref_harness",
+               dict(mspec, tmax=300 * 0.01 + 1e-9, bc=0, nbins=50, nt_out=3,
+                    emin=1e-3, emax=1.0), mptl,
+               extra=dict(vgrid=vg, bgrid=bg, lo=glo, hi=ghi, dt=0.01), tmp=tmp)
+    # double grids: the gathers alone (the reference's get_emf never calls them)
+    eg = rng.normal(0, 1, (nz, ny, nx, 3))
+    bd = rng.normal(0, 1, (nz, ny, nx, 3))
+    np.concatenate([eg.ravel(), bd.ravel()]).tofile(os.path.join(tmp, "dgrid.bin"))
+    dspec = dict(system=0, nx=nx, ny=ny, nz=nz, tmax=1.0, dt=0.01,
+                 dgrid=os.path.join(tmp, "dgrid.bin"), **box(glo, ghi))
+    sp = os.path.join(tmp, "spec.txt")
+    write_spec(sp, **dspec)
+    mp.tofile(os.path.join(tmp, "pts.bin"))
+    run("ref_harness", "dfields", sp, "pts.bin", "out.bin", cwd=tmp)
+    out = np.fromfile(os.path.join(tmp, "out.bin")).reshape(-1, 6)
+    np.savez(os.path.join(HERE, "fields_gridd.npz"), pts=mp, out=out, egrid=eg, bgrid=bd,
+             lo=glo, hi=ghi)
+
+
 def main():
     build_ref()
     if sys.argv[1:] == ["h5"]:
         with tempfile.TemporaryDirectory() as tmp:
             h5_case(tmp)
+        return
+    if sys.argv[1:] == ["mhd"]:
+        with tempfile.TemporaryDirectory() as tmp:
+            mhd_cases(tmp)
         return
     rng = np.random.default_rng(7)
     lo2pi, hi2pi = (0.0, 0.0, 0.0), (TWO_PI, TWO_PI, TWO_PI)

@@ -6,7 +6,11 @@ calc_energy_spectrum + particle_tracking_hybrid and must write the same
 data/espectrum.dat and data/diff_coeffs.dat.
 
 The binaries are built in the build container (`make -C oracle ref ref-gpu`)
-and travel with the tree; without them the test is skipped."""
+and travel with the tree; without them the test is skipped.  The drop-in runs
+with TP_SHIM_REQUIRE_GPU=1, so a call the GPU path does not take aborts the
+program instead of running the reference's CPU loop, and each test checks the
+shim's stderr line that the GPU tracked; without libtpgpu.so the drop-in
+binary does not start and the test fails."""
 import os
 import subprocess
 
@@ -34,6 +38,25 @@ def _spec(path, **kw):
             f.write(f"{k} {float(v)!r}\n" if isinstance(v, float) else f"{k} {v}\n")
 
 
+GPU_MARK = "tp_gpu: particle_tracking_hybrid on the GPU"
+PMAP_MARK = "tp_gpu: poincare_map on the GPU"
+
+
+def _run(binary, args, cwd, mark=GPU_MARK, **env):
+    """Runs an oracle/_ref binary; for a drop-in binary (name with _gpu) the
+    GPU path is required and its stderr line checked."""
+    gpu = "_gpu" in binary
+    e = dict(os.environ, OMP_NUM_THREADS="1", **env)
+    if gpu:
+        e["TP_SHIM_REQUIRE_GPU"] = "1"
+    r = subprocess.run([os.path.join(REF, binary), *args], cwd=cwd, timeout=120,
+                       stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, text=True, env=e)
+    assert r.returncode == 0, f"{binary} {args[0]} exited {r.returncode}: {r.stderr[-2000:]}"
+    if gpu:
+        assert mark in r.stderr, f"{binary}: the GPU path did not run: {r.stderr[-2000:]}"
+    return r
+
+
 def _run_both(tmp, cpu_bin, gpu_bin, spec, ptl):
     outs = []
     for b in (cpu_bin, gpu_bin):
@@ -42,9 +65,7 @@ def _run_both(tmp, cpu_bin, gpu_bin, spec, ptl):
         _spec(os.path.join(d, "spec.txt"), **spec)
         ptl.tofile(os.path.join(d, "p.bin"))
         print(f"[dropin] {b} {spec.get('system')}", flush=True)
-        subprocess.run([os.path.join(REF, b), "hybrid", "spec.txt", "p.bin"], cwd=d,
-                       check=True, timeout=120, stdout=subprocess.DEVNULL,
-                       env=dict(os.environ, OMP_NUM_THREADS="1"))
+        _run(b, ["hybrid", "spec.txt", "p.bin"], d)
         outs.append({f: open(os.path.join(d, "data", f)).read()
                      for f in sorted(os.listdir(os.path.join(d, "data")))})
     return outs
@@ -63,6 +84,28 @@ def test_dropin_pic_grid(tmp_path):
     spec = dict(system=4, nx=nx, ny=ny, nz=nz, tmax=800 * 0.02 + 1e-9, dt=0.02, bc=0,
                 nbins=60, nt_out=8, emin=1e-3, emax=1.0,
                 grid=os.path.join(tmp_path, "grid.bin"),
+                xmin=lo[0], ymin=lo[1], zmin=lo[2], xmax=hi[0], ymax=hi[1], zmax=hi[2])
+    cpu, gpu = _run_both(str(tmp_path), "ref_harness", "ref_harness_gpu", spec, ptl)
+    assert set(cpu) == {"diff_coeffs.dat", "espectrum.dat"}
+    assert cpu == gpu
+
+
+def test_dropin_mhd_grid(tmp_path):
+    """System 3 (MHD velocity and magnetic grids, E = B x v): the reference
+    program with the GPU shim writes the reference's espectrum.dat and
+    diff_coeffs.dat."""
+    _need("ref_harness", "ref_harness_gpu")
+    rng = np.random.default_rng(61)
+    nz, ny, nx = 9, 8, 10
+    lo, hi = (0.0, 0.0, 0.0), (9.0, 7.0, 8.0)
+    v = rng.normal(0, 0.2, (nz, ny, nx, 3)).astype(np.float32)
+    b = rng.normal(0, 1, (nz, ny, nx, 3)).astype(np.float32)
+    b[..., 2] += 2.0
+    np.concatenate([v.ravel(), b.ravel()]).tofile(os.path.join(tmp_path, "mgrid.bin"))
+    ptl = init_particles(500, lo, hi, 0.3, seed=62)
+    spec = dict(system=3, nx=nx, ny=ny, nz=nz, tmax=800 * 0.02 + 1e-9, dt=0.02, bc=0,
+                nbins=60, nt_out=8, emin=1e-3, emax=1.0,
+                grid=os.path.join(tmp_path, "mgrid.bin"),
                 xmin=lo[0], ymin=lo[1], zmin=lo[2], xmax=hi[0], ymax=hi[1], zmax=hi[2])
     cpu, gpu = _run_both(str(tmp_path), "ref_harness", "ref_harness_gpu", spec, ptl)
     assert set(cpu) == {"diff_coeffs.dat", "espectrum.dat"}
@@ -134,9 +177,7 @@ def test_dropin_trajectory_diagnostics(tmp_path):
         _spec(os.path.join(d, "spec.txt"), **spec)
         ptl.tofile(os.path.join(d, "p.bin"))
         print(f"[dropin] {b} trajh", flush=True)
-        subprocess.run([os.path.join(REF, b), "trajh", "spec.txt", "p.bin", "o"], cwd=d,
-                       check=True, timeout=120, stdout=subprocess.DEVNULL,
-                       env=dict(os.environ, OMP_NUM_THREADS="1"))
+        _run(b, ["trajh", "spec.txt", "p.bin", "o"], d)
         outs.append((np.fromfile(os.path.join(d, "o", "traj.bin")).reshape(100, -1, 11),
                      np.fromfile(os.path.join(d, "o", "ptl.bin")).reshape(-1, 11)))
     (tc, pc), (tg, pg) = outs
@@ -176,8 +217,7 @@ def _pmap_run(tmp, binary, mode, spec, cfg):
     _spec(os.path.join(d, "spec.txt"), **spec)
     write_config(os.path.join(d, "init.dat"), cfg)
     print(f"[dropin] {binary} {mode}", flush=True)
-    subprocess.run([os.path.join(REF, binary), mode, "spec.txt", "init.dat", "out.bin"], cwd=d,
-                   check=True, timeout=120, stdout=subprocess.DEVNULL)
+    _run(binary, [mode, "spec.txt", "init.dat", "out.bin"], d, mark=PMAP_MARK)
     np_ = 1 if cfg["single"] else cfg["nh"] * cfg["nv"]
     return np.fromfile(os.path.join(d, "out.bin")).reshape(3, np_, cfg["ncross"] + 1)
 
