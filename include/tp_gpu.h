@@ -399,7 +399,14 @@ typedef struct tp_pmap_params {
     double step;            /* field-line step, > 0 */
     double length_max;      /* traced length at which a line stops */
     double lo[3], hi[3];    /* periodic box (simul_domain) */
+    int lanes;              /* lanes per field line: 0 automatic (one; four,
+                               one per component, for the launches over at
+                               most TP_PMAP_QUAD_LINES running lines, on
+                               reference-gather PIC grids in TP_FP64), 1 always
+                               one, 4 four whenever the field allows; the
+                               records are the same bits either way */
 } tp_pmap_params;
+#define TP_PMAP_QUAD_LINES 32768
 
 /* Traces npoints field lines from starts[3*i] (host arrays).  xpos, ypos,
  * zpos have npoints*(ncross+1) entries each (the reference's datasets,
@@ -419,7 +426,10 @@ int tp_gpu_poincare_map(tp_ctx *ctx, const tp_pmap_params *p, int npoints,
  *   f   = [sin 2 pi (q_j . x), cos 2 pi (q_j . x)]   j < 128 (256 features)
  *   h1  = gelu(f W1 + b1), h2 = gelu(h1 W2 + b2), h3 = gelu(h2 W3 + b3)  (128 each)
  *   B   = b0z z-hat + scale (h3 W4 + b4)
- * with gelu(z) = z / (1 + exp(-2 sqrt(2/pi) (z + 0.044715 z^3))).  Layers
+ * with gelu the tanh form z / (1 + exp(-2 sqrt(2/pi) (z + 0.044715 z^3)))
+ * as the kernels evaluate it: max(z, 0) minus a degree-8 polynomial for its
+ * tail, in packed FP16 (within 2^-10 |z| + 1.1e-3 of the tanh form; its
+ * derivative in training is the tanh form's).  Layers
  * 1-3 run on the matrix cores in bfloat16 with FP32 accumulation; layer 4,
  * the biases and the frequencies are FP32.  The frequencies q_j = n_j / L
  * with integer n_j make the network periodic in the box L.

@@ -29,9 +29,11 @@
  *   emulate = 1: the GPU kernels' rounding points -- positions and phases in
  *                float, features, h1, h2, W1..W3 and the back-propagated
  *                dz1..dz3 rounded to bfloat16 (round to nearest even), h3 and
- *                layer 4 in float -- with FP64 accumulation in between, so
- *                that the GPU result differs only by its FP32 summation order
- *                and its hardware sin/cos/exp.
+ *                layer 4 in float -- with FP64 accumulation in between, and
+ *                the kernels' activation (gelu_gpu: the FP16 polynomial form
+ *                of gelu, FP16 rounding emulated step by step; gelu' stays
+ *                the tanh form's), so that the GPU result differs only by
+ *                its FP32 summation order and its hardware sin/cos/exp.
  */
 #include <math.h>
 #include <stdint.h>
@@ -81,6 +83,51 @@ static double gelu_d(double z)
     double a = GELU_C1 * (z + GELU_C2 * z * z * z);
     double s = 1.0 / (1.0 + exp(-a));
     return s + z * s * (1.0 - s) * GELU_C1 * (1.0 + 3.0 * GELU_C2 * z * z);
+}
+
+/* The activation of the GPU kernels (tp_mlp.hip, gelu_pk): max(z, 0) minus
+ * a degree-8 polynomial for the tail G(u) = u - gelu(u) in
+ * s = min(|z|, 4) / 2 - 1, evaluated in FP16 on the matrix-core side:
+ * z converted from float with round toward zero, every FMA and the final
+ * difference rounded to nearest even in FP16 (the packed FP16 instructions
+ * of the kernels).  The coefficients are FP16 values. */
+static const double gelu_tail[9] = {
+    0.045379638671875, -0.1727294921875, 0.21728515625, 0.0158538818359375,
+    -0.3017578125, 0.2413330078125, 0.018310546875, -0.08465576171875,
+    0.020843505859375};
+
+/* v rounded to FP16 (11 significant bits, subnormals down to 2^-24, largest
+ * finite 65504): to nearest even, or toward zero (which saturates at 65504) */
+static double f16r(double v, int rtz)
+{
+    if (v == 0.0 || isnan(v)) return v;
+    if (isinf(v)) return rtz ? copysign(65504.0, v) : v;
+    int e;
+    const double a = fabs(v);
+    frexp(a, &e);                 /* a = m 2^e, m in [0.5, 1) */
+    int qe = e - 11;
+    if (qe < -24) qe = -24;
+    const double q = ldexp(1.0, qe);
+    double r = a / q;             /* exact: q is a power of two */
+    r = rtz ? trunc(r) : nearbyint(r);
+    double w = r * q;
+    if (w > 65504.0) w = rtz ? 65504.0 : INFINITY;
+    return copysign(w, v);
+}
+
+static double gelu_gpu(double z)
+{
+    const double zh = f16r((double)(float)z, 1);
+    const double uc = fmin(fabs(zh), 4.0);
+    const double s = f16r(fma(uc, 0.5, -1.0), 0);
+    double p = gelu_tail[8];
+    for (int k = 7; k >= 0; k--) p = f16r(fma(p, s, gelu_tail[k]), 0);
+    return f16r(fmax(zh, 0.0) - p, 0);
+}
+
+void tpref_gelu_gpu(int n, const double *z, double *g)
+{
+    for (int i = 0; i < n; i++) g[i] = gelu_gpu(z[i]);
 }
 
 void tpref_gelu(int n, const double *z, double *g, double *gd)
@@ -143,11 +190,11 @@ static void forward(const double *p, const double *q, const double *x, int em, A
 {
     features(q, x, em, a->f);
     dense(p, P_W1, P_B1, NIN, NH, a->f, a->z1, em, 1);
-    for (int n = 0; n < NH; n++) a->h1[n] = em ? bf16r((float)gelu(a->z1[n])) : gelu(a->z1[n]);
+    for (int n = 0; n < NH; n++) a->h1[n] = em ? bf16r((float)gelu_gpu(a->z1[n])) : gelu(a->z1[n]);
     dense(p, P_W2, P_B2, NH, NH, a->h1, a->z2, em, 2);
-    for (int n = 0; n < NH; n++) a->h2[n] = em ? bf16r((float)gelu(a->z2[n])) : gelu(a->z2[n]);
+    for (int n = 0; n < NH; n++) a->h2[n] = em ? bf16r((float)gelu_gpu(a->z2[n])) : gelu(a->z2[n]);
     dense(p, P_W3, P_B3, NH, NH, a->h2, a->z3, em, 3);
-    for (int n = 0; n < NH; n++) a->h3[n] = em ? (double)(float)gelu(a->z3[n]) : gelu(a->z3[n]);
+    for (int n = 0; n < NH; n++) a->h3[n] = em ? gelu_gpu(a->z3[n]) : gelu(a->z3[n]);
     dense(p, P_W4, P_B4, NH, NOUT, a->h3, a->out, em, 4);
 }
 

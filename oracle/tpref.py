@@ -93,6 +93,7 @@ def lib():
         L.tpref_bf16_round.argtypes = [d]
         L.tpref_bf16_round.restype = d
         L.tpref_gelu.argtypes = [i, P(d), P(d), P(d)]
+        L.tpref_gelu_gpu.argtypes = [i, P(d), P(d)]
         L.tpref_mlp_forward.argtypes = [P(d), P(d), i, P(d), P(d), i]
         L.tpref_mlp_loss_grad.argtypes = [P(d), P(d), i, P(d), P(d), P(d), P(d), i]
         _lib = L
@@ -388,6 +389,14 @@ def mlp_nparams():
 
 def bf16_round(v):
     return np.array([lib().tpref_bf16_round(float(x)) for x in np.ravel(v)])
+
+
+def gelu_gpu(z):
+    """The GPU kernels' activation (FP16 polynomial form), emulated."""
+    z = np.ascontiguousarray(z, dtype=np.float64).ravel()
+    g = np.empty_like(z)
+    lib().tpref_gelu_gpu(z.size, abi.dptr(z), abi.dptr(g))
+    return g
 
 
 def gelu(z):

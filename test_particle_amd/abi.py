@@ -152,17 +152,18 @@ class TpPmapParams(ctypes.Structure):
         ("length_max", ctypes.c_double),
         ("lo", ctypes.c_double * 3),
         ("hi", ctypes.c_double * 3),
+        ("lanes", ctypes.c_int),
     ]
 
 
 def make_pmap_params(*, face, ncross, step, length_max, lo, hi, mode=TP_PMAP_REF_COMPAT,
-                     chunk=0, precision=TP_FP64):
+                     chunk=0, precision=TP_FP64, lanes=0):
     """tp_pmap_params of a Poincare map (the reference's "Poincare map
     configuration" block: face, step, maximum length, crossing count; lo/hi
     the simulation domain)."""
     p = TpPmapParams()
     p.face, p.ncross, p.mode, p.chunk = int(face), int(ncross), int(mode), int(chunk)
-    p.precision = int(precision)
+    p.precision, p.lanes = int(precision), int(lanes)
     p.step, p.length_max = float(step), float(length_max)
     for d in range(3):
         p.lo[d] = float(lo[d])

@@ -70,6 +70,10 @@ struct FieldDev {
     int nx, ny, nz;
     double lo[3], hi[3], d[3];
     double inv_d[3];                /* 1 / d (the FP32 forms' cell index) */
+    /* every d is a power of two (normal, with a normal reciprocal): then
+     * v / d and v * inv_d are the same double for every v, and the FP64
+     * forms multiply instead of dividing */
+    int exact_inv;
     const float4 *grid;
     /* two time slices: grid = next slice, grid_b = previous slice, gdt the
      * time between slices */
@@ -285,27 +289,65 @@ __device__ __forceinline__ void gc_load(const FieldDev &f, GridCacheN<NS> &gc, c
     }
 }
 
-/* grid_indices / get_trilinar_parameters along one axis
- * (src/interpolation.c:36-105): lower and upper corner and the fraction; a
- * point beyond either end of the box uses the wrap cell n-1 -> 0.  FAST
- * (the FP32 forms only) multiplies by 1/d instead of dividing by d. */
+/* grid_indices / get_trilinar_parameters (src/interpolation.c:36-105):
+ * the point in cell units, (v - lo) / d on each axis.  FAST (the FP32 forms
+ * only) multiplies by 1/d; the FP64 forms divide as the reference does,
+ * except on a grid whose spacings are powers of two, where the product with
+ * the exact reciprocal is the same double.  One uniform branch for the
+ * three axes keeps their divisions independent of each other. */
 template <bool FAST>
-__device__ __forceinline__ void ref_axis_in(double x, double lo, double d, double inv_d, int &i1,
-                                            int &i2, double &fr)
+__device__ __forceinline__ void ref_cell_coords(const FieldDev &f, double x, double y, double z,
+                                                double *q)
 {
-    const double q = FAST ? (x - lo) * inv_d : (x - lo) / d;
+    if (FAST || f.exact_inv) {
+        q[0] = (x - f.lo[0]) * f.inv_d[0];
+        q[1] = (y - f.lo[1]) * f.inv_d[1];
+        q[2] = (z - f.lo[2]) * f.inv_d[2];
+    } else {
+        q[0] = (x - f.lo[0]) / f.d[0];
+        q[1] = (y - f.lo[1]) / f.d[1];
+        q[2] = (z - f.lo[2]) / f.d[2];
+    }
+}
+
+/* along one axis: lower and upper corner and the fraction of the point at
+ * q cell units */
+__device__ __forceinline__ void ref_axis_in(double q, int &i1, int &i2, double &fr)
+{
     i1 = tp_floor_to_int(floor(q));
     i2 = i1 + 1;
     fr = q - i1;
 }
 
-/* the wrap cell, for a point outside [lo, hi] */
+/* the wrap cell n-1 -> 0, for a point beyond either end of [lo, hi] */
 template <bool FAST>
-__device__ __forceinline__ void ref_axis_out(double x, double lo, double hi, double d,
-                                             double inv_d, int n, int &i1, int &i2, double &fr)
+__device__ __forceinline__ void ref_axis_out(const FieldDev &f, int a, double x, int n, int &i1,
+                                             int &i2, double &fr)
 {
-    if (x > hi) { i1 = n - 1; i2 = 0; fr = (FAST ? x * inv_d : x / d) - (n - 1); }
-    if (x < lo) { i1 = n - 1; i2 = 0; fr = (FAST ? x * inv_d : x / d) + 1.0; }
+    const bool mul = FAST || f.exact_inv;
+    if (x > f.hi[a]) { i1 = n - 1; i2 = 0; fr = (mul ? x * f.inv_d[a] : x / f.d[a]) - (n - 1); }
+    if (x < f.lo[a]) { i1 = n - 1; i2 = 0; fr = (mul ? x * f.inv_d[a] : x / f.d[a]) + 1.0; }
+}
+
+/* the reference's corner indices and fractions of the point (x, y, z):
+ * the branch for points outside the box is skipped by a wave none of whose
+ * points is outside (the usual case) */
+template <bool FAST>
+__device__ __forceinline__ void ref_cell_index(const FieldDev &f, double x, double y, double z,
+                                               int *i1, int *i2, double &fx, double &fy,
+                                               double &fz)
+{
+    double q[3];
+    ref_cell_coords<FAST>(f, x, y, z, q);
+    ref_axis_in(q[0], i1[0], i2[0], fx);
+    ref_axis_in(q[1], i1[1], i2[1], fy);
+    ref_axis_in(q[2], i1[2], i2[2], fz);
+    if (x > f.hi[0] || x < f.lo[0] || y > f.hi[1] || y < f.lo[1] || z > f.hi[2] ||
+        z < f.lo[2]) {
+        ref_axis_out<FAST>(f, 0, x, f.nx, i1[0], i2[0], fx);
+        ref_axis_out<FAST>(f, 1, y, f.ny, i1[1], i2[1], fy);
+        ref_axis_out<FAST>(f, 2, z, f.nz, i1[2], i2[2], fz);
+    }
 }
 
 /* The reference gather's parameters (the values of trilinear_ref: corner
@@ -320,17 +362,7 @@ __device__ __forceinline__ void ref_cell_cached(const FieldDev &f, GridCacheN<NS
                                                 double &fy, double &fz)
 {
     int i1[3], i2[3];
-    ref_axis_in<FAST>(x, f.lo[0], f.d[0], f.inv_d[0], i1[0], i2[0], fx);
-    ref_axis_in<FAST>(y, f.lo[1], f.d[1], f.inv_d[1], i1[1], i2[1], fy);
-    ref_axis_in<FAST>(z, f.lo[2], f.d[2], f.inv_d[2], i1[2], i2[2], fz);
-    /* points outside the box take the wrap cell; the branch is skipped by
-     * a wave none of whose points is outside (the usual case) */
-    if (x > f.hi[0] || x < f.lo[0] || y > f.hi[1] || y < f.lo[1] || z > f.hi[2] ||
-        z < f.lo[2]) {
-        ref_axis_out<FAST>(x, f.lo[0], f.hi[0], f.d[0], f.inv_d[0], f.nx, i1[0], i2[0], fx);
-        ref_axis_out<FAST>(y, f.lo[1], f.hi[1], f.d[1], f.inv_d[1], f.ny, i1[1], i2[1], fy);
-        ref_axis_out<FAST>(z, f.lo[2], f.hi[2], f.d[2], f.inv_d[2], f.nz, i1[2], i2[2], fz);
-    }
+    ref_cell_index<FAST>(f, x, y, z, i1, i2, fx, fy, fz);
     const bool same = gc.valid && i1[0] == gc.key[0] && i2[0] == gc.key[1] &&
                       i1[1] == gc.key[2] && i2[1] == gc.key[3] && i1[2] == gc.key[4] &&
                       i2[2] == gc.key[5];
@@ -946,15 +978,7 @@ __device__ __forceinline__ void ref_emf64_via_cache(const FieldDev &f, const Gri
 {
     int i1[3], i2[3];
     double fx, fy, fz;
-    ref_axis_in<false>(x, f.lo[0], f.d[0], f.inv_d[0], i1[0], i2[0], fx);
-    ref_axis_in<false>(y, f.lo[1], f.d[1], f.inv_d[1], i1[1], i2[1], fy);
-    ref_axis_in<false>(z, f.lo[2], f.d[2], f.inv_d[2], i1[2], i2[2], fz);
-    if (x > f.hi[0] || x < f.lo[0] || y > f.hi[1] || y < f.lo[1] || z > f.hi[2] ||
-        z < f.lo[2]) {
-        ref_axis_out<false>(x, f.lo[0], f.hi[0], f.d[0], f.inv_d[0], f.nx, i1[0], i2[0], fx);
-        ref_axis_out<false>(y, f.lo[1], f.hi[1], f.d[1], f.inv_d[1], f.ny, i1[1], i2[1], fy);
-        ref_axis_out<false>(z, f.lo[2], f.hi[2], f.d[2], f.inv_d[2], f.nz, i1[2], i2[2], fz);
-    }
+    ref_cell_index<false>(f, x, y, z, i1, i2, fx, fy, fz);
     const bool same = gc.valid && i1[0] == gc.key[0] && i2[0] == gc.key[1] &&
                       i1[1] == gc.key[2] && i2[1] == gc.key[3] && i1[2] == gc.key[4] &&
                       i2[2] == gc.key[5];

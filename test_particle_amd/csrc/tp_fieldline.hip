@@ -75,22 +75,6 @@ __device__ __forceinline__ void bc_fill(const float4 *g, BCache &bc, const unsig
     }
 }
 
-/* grid_indices / get_trilinar_parameters along one axis
- * (src/interpolation.c:36-105): lower and upper corner and the fraction; a
- * point beyond either end of the box uses the wrap cell n-1 -> 0 */
-template <bool FAST>
-__device__ __forceinline__ void pm_axis(double x, double lo, double hi, double d, double inv_d,
-                                        int n, int &i1, int &i2, double &fr)
-{
-    /* FAST (the FP32 form only): multiplication by 1/d instead of division */
-    const double q = FAST ? (x - lo) * inv_d : (x - lo) / d;
-    i1 = tp_floor_to_int(floor(q));
-    i2 = i1 + 1;
-    fr = q - i1;
-    if (x > hi) { i1 = n - 1; i2 = 0; fr = (FAST ? x * inv_d : x / d) - (n - 1); }
-    if (x < lo) { i1 = n - 1; i2 = 0; fr = (FAST ? x * inv_d : x / d) + 1.0; }
-}
-
 /* The reference's trilinear parameters (corner order and x/z-exchanged
  * weights of src/interpolation.c:141-181, the same values as trilinear_ref)
  * with the cell's B values in bc.  Flat node indices are formed only when
@@ -98,14 +82,11 @@ __device__ __forceinline__ void pm_axis(double x, double lo, double hi, double d
  * grid, otherwise in 64 bits with the reference's clamp into the array. */
 template <bool FAST>
 __device__ __forceinline__ void pm_ref_cell(const FieldDev &f, const float4 *g, BCache &bc,
-                                            const double *inv_d, double x, double y, double z,
-                                            double *w)
+                                            double x, double y, double z, double *w)
 {
     int i1[3], i2[3];
     double fx, fy, fz;
-    pm_axis<FAST>(x, f.lo[0], f.hi[0], f.d[0], inv_d[0], f.nx, i1[0], i2[0], fx);
-    pm_axis<FAST>(y, f.lo[1], f.hi[1], f.d[1], inv_d[1], f.ny, i1[1], i2[1], fy);
-    pm_axis<FAST>(z, f.lo[2], f.hi[2], f.d[2], inv_d[2], f.nz, i1[2], i2[2], fz);
+    ref_cell_index<FAST>(f, x, y, z, i1, i2, fx, fy, fz);
     const bool same = bc.valid && i1[0] == bc.key[0] && i2[0] == bc.key[1] &&
                       i1[1] == bc.key[2] && i2[1] == bc.key[3] && i1[2] == bc.key[4] &&
                       i2[2] == bc.key[5];
@@ -163,7 +144,7 @@ __device__ __forceinline__ void pm_bfield(const FieldDev &f, BCache &bc, double 
 {
     if (FK == FK_GRID_REF || FK == FK_GRID2_REF) {
         double w[8];
-        pm_ref_cell<false>(f, FK == FK_GRID_REF ? f.grid : f.grid_b, bc, nullptr, x, y, z, w);
+        pm_ref_cell<false>(f, FK == FK_GRID_REF ? f.grid : f.grid_b, bc, x, y, z, w);
         if (FK == FK_GRID_REF) {
             /* get_float_bfield_at_point, one slice (src/magnetic_field.c:
              * 603-608): float accumulators rounded after every corner */
@@ -223,16 +204,15 @@ __device__ __forceinline__ void pm_unit(const FieldDev &f, BCache &bc, double x,
  * the tracker's FP32 field evaluators otherwise) and the unit vector by the
  * hardware reciprocal square root. */
 template <int FK>
-__device__ __forceinline__ void pm_unit32(const FieldDev &f, BCache &bc, const double *inv_d,
-                                          double x, double y, double z, float &ux, float &uy,
-                                          float &uz)
+__device__ __forceinline__ void pm_unit32(const FieldDev &f, BCache &bc, double x, double y,
+                                          double z, float &ux, float &uy, float &uz)
 {
     float bx, by, bz;
     if (FK == FK_GRID_REF || FK == FK_GRID2_REF || FK == FK_GRID_TB || FK == FK_GRID2_TB) {
         double w[8];
         const float4 *g = (FK == FK_GRID_REF || FK == FK_GRID_TB) ? f.grid : f.grid_b;
         if (FK == FK_GRID_REF || FK == FK_GRID2_REF) {
-            pm_ref_cell<true>(f, g, bc, inv_d, x, y, z, w);
+            pm_ref_cell<true>(f, g, bc, x, y, z, w);
         } else {
             unsigned c[8];
             trilinear_tb(f.nx, f.ny, f.nz, f.lo, f.hi, f.d, x, y, z, c, w);
@@ -442,7 +422,7 @@ __global__ void __launch_bounds__(256) k_pmap(PmapArgs a, int nact, const int *a
                         ty = pm_fold(y + cs * (double)uy, a.lo[1], a.hi[1], a.L[1]);
                         tz = pm_fold(z + cs * (double)uz, a.lo[2], a.hi[2], a.L[2]);
                     }
-                    pm_unit32<FK>(a.f, bc, a.inv_d, tx, ty, tz, ux, uy, uz);
+                    pm_unit32<FK>(a.f, bc, tx, ty, tz, ux, uy, uz);
                     if (s == 0) {
                         sx = ux; sy = uy; sz = uz;
                     } else {
@@ -487,6 +467,253 @@ __global__ void __launch_bounds__(256) k_pmap(PmapArgs a, int nact, const int *a
     /* lines still running go to the next active list (one atomic per
      * wavefront) */
     const bool alive = live && k > 0;
+    const int lane = threadIdx.x & 63;
+    const unsigned long long m = __ballot(alive);
+    const unsigned pre = (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+    unsigned base = 0u;
+    if (lane == 0 && m != 0ull) base = atomicAdd(cnt, (unsigned)__popcll(m));
+    base = __shfl(base, 0);
+    if (alive) act_next[base + pre] = i;
+}
+
+/* ------------------------------------------------------------------ */
+/* the tail: four lanes per line                                        */
+/* ------------------------------------------------------------------ */
+/* Once most lines have finished, the few still running leave a handful of
+ * wavefronts, each of which advances at the latency of its own instruction
+ * stream.  The tail kernel gives every line a quad of lanes: lane c < 3 of
+ * the quad carries component c (position, stage point, cell index along
+ * axis c, the gather of B_c over the eight corners, the RK4 sum), lane 3
+ * repeats component 2 and writes nothing.  What needs the other axes (the
+ * corner weights, the cell test and node indices, |B|) reads them from the
+ * quad's lanes through DPP quad permutations and computes it in every lane
+ * in the same order as the one-lane kernel, so each component goes through
+ * the same IEEE operations and the crossing records are the same bits.  A
+ * line's lanes take every branch together (their inputs are equal), so a
+ * quad is always whole when it exchanges values. */
+
+/* value of lane K of the quad (quad_perm [K, K, K, K]) */
+template <int K>
+__device__ __forceinline__ int quad_bcast(int v)
+{
+    return __builtin_amdgcn_mov_dpp(v, K | (K << 2) | (K << 4) | (K << 6), 0xF, 0xF, false);
+}
+
+template <int K>
+__device__ __forceinline__ double quad_bcast(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = quad_bcast<K>((int)(unsigned)(b & 0xFFFFFFFFll));
+    const int hi = quad_bcast<K>((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+template <int K>
+__device__ __forceinline__ float quad_bcast(float v)
+{
+    return __int_as_float(quad_bcast<K>(__float_as_int(v)));
+}
+
+/* value of lane k (0..2, the same in every lane) of the quad */
+__device__ __forceinline__ double quad_pick(double v, int k)
+{
+    return k == 0 ? quad_bcast<0>(v) : (k == 1 ? quad_bcast<1>(v) : quad_bcast<2>(v));
+}
+
+/* this lane's component of the cell cache: B_c at the eight corners, as
+ * doubles, and the six corner indices of the cell */
+struct BCacheQ {
+    int key[6];
+    bool valid;
+    double v[8];
+};
+
+/* the grid along this lane's axis, in registers */
+struct QAxis {
+    double lo, hi, d, inv_d;
+    int n;
+};
+
+__device__ __forceinline__ QAxis pmq_axis(const FieldDev &f, int c)
+{
+    QAxis g;
+    g.lo = c == 0 ? f.lo[0] : (c == 1 ? f.lo[1] : f.lo[2]);
+    g.hi = c == 0 ? f.hi[0] : (c == 1 ? f.hi[1] : f.hi[2]);
+    g.d = c == 0 ? f.d[0] : (c == 1 ? f.d[1] : f.d[2]);
+    g.inv_d = c == 0 ? f.inv_d[0] : (c == 1 ? f.inv_d[1] : f.inv_d[2]);
+    g.n = c == 0 ? f.nx : (c == 1 ? f.ny : f.nz);
+    return g;
+}
+
+/* B_c at a stage point of the reference gather (pm_ref_cell + pm_bfield
+ * for FK_GRID_REF, one slice), with t the point's coordinate along this
+ * lane's axis c */
+__device__ __forceinline__ float pmq_bcomp(const FieldDev &f, const QAxis &g, BCacheQ &bc, int c,
+                                           double t)
+{
+    /* ref_cell_index along axis c */
+    const double q = f.exact_inv ? (t - g.lo) * g.inv_d : (t - g.lo) / g.d;
+    int ia, ib;
+    double fr;
+    ref_axis_in(q, ia, ib, fr);
+    if (t > g.hi || t < g.lo) {
+        /* ref_axis_out: the wrap cell n-1 -> 0 */
+        const double v = f.exact_inv ? t * g.inv_d : t / g.d;
+        if (t > g.hi) { ia = g.n - 1; ib = 0; fr = v - (g.n - 1); }
+        if (t < g.lo) { ia = g.n - 1; ib = 0; fr = v + 1.0; }
+    }
+    const double fx = quad_bcast<0>(fr), fy = quad_bcast<1>(fr), fz = quad_bcast<2>(fr);
+    int i1[3], i2[3];
+    i1[0] = quad_bcast<0>(ia); i2[0] = quad_bcast<0>(ib);
+    i1[1] = quad_bcast<1>(ia); i2[1] = quad_bcast<1>(ib);
+    i1[2] = quad_bcast<2>(ia); i2[2] = quad_bcast<2>(ib);
+    const bool same = bc.valid && i1[0] == bc.key[0] && i2[0] == bc.key[1] &&
+                      i1[1] == bc.key[2] && i2[1] == bc.key[3] && i1[2] == bc.key[4] &&
+                      i2[2] == bc.key[5];
+    if (!same) {
+        unsigned cn[8];
+        bool cl = false;
+        const bool inside = (unsigned)i1[0] < (unsigned)f.nx && (unsigned)i2[0] < (unsigned)f.nx &&
+                            (unsigned)i1[1] < (unsigned)f.ny && (unsigned)i2[1] < (unsigned)f.ny &&
+                            (unsigned)i1[2] < (unsigned)f.nz && (unsigned)i2[2] < (unsigned)f.nz;
+        if (inside) {
+            const unsigned sx = (unsigned)f.nx, sxy = (unsigned)f.nx * (unsigned)f.ny;
+            const unsigned X1 = i1[0], X2 = i2[0];
+            const unsigned Y1 = sx * i1[1], Y2 = sx * i2[1];
+            const unsigned Z1 = sxy * i1[2], Z2 = sxy * i2[2];
+            cn[0] = X1 + Y1 + Z1; cn[1] = X1 + Y1 + Z2; cn[2] = X1 + Y2 + Z1;
+            cn[3] = X2 + Y1 + Z1; cn[4] = X2 + Y2 + Z1; cn[5] = X2 + Y1 + Z2;
+            cn[6] = X1 + Y2 + Z2; cn[7] = X2 + Y2 + Z2;
+        } else {
+            const long total = (long)f.nx * f.ny * f.nz;
+            const long sx = f.nx, sxy = (long)f.nx * f.ny;
+            long a[8];
+            a[0] = i1[0] + sx * i1[1] + sxy * i1[2];
+            a[1] = i1[0] + sx * i1[1] + sxy * i2[2];
+            a[2] = i1[0] + sx * i2[1] + sxy * i1[2];
+            a[3] = i2[0] + sx * i1[1] + sxy * i1[2];
+            a[4] = i2[0] + sx * i2[1] + sxy * i1[2];
+            a[5] = i2[0] + sx * i1[1] + sxy * i2[2];
+            a[6] = i1[0] + sx * i2[1] + sxy * i2[2];
+            a[7] = i2[0] + sx * i2[1] + sxy * i2[2];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                cl |= (a[k] < 0) | (a[k] >= total);
+                cn[k] = (unsigned)(a[k] < 0 ? 0 : (a[k] >= total ? total - 1 : a[k]));
+            }
+        }
+        /* node = ex ey ez bx | by bz 0 0: B_c is float 3 + c */
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            bc.v[k] = (double)((const float *)(f.grid + 2 * (size_t)cn[k]))[3 + c];
+        bc.key[0] = i1[0]; bc.key[1] = i2[0]; bc.key[2] = i1[1];
+        bc.key[3] = i2[1]; bc.key[4] = i1[2]; bc.key[5] = i2[2];
+        bc.valid = !cl;
+    }
+    const double gx = 1.0 - fx, gy = 1.0 - fy, gz = 1.0 - fz;
+    double w[8];
+    w[0] = gx * gy * gz;
+    w[1] = fx * gy * gz;
+    w[2] = gx * fy * gz;
+    w[3] = gx * gy * fz;
+    w[4] = gx * fy * fz;
+    w[5] = fx * gy * fz;
+    w[6] = fx * fy * gz;
+    w[7] = fx * fy * fz;
+    /* get_float_bfield_at_point (src/magnetic_field.c:603-608): a float
+     * accumulator rounded after every corner */
+    float acc = 0.f;
+#pragma unroll
+    for (int k = 0; k < 8; k++) acc = (float)((double)acc + bc.v[k] * w[k]);
+    return acc;
+}
+
+/* the unit vector's component c at a stage point */
+__device__ __forceinline__ double pmq_unit(const FieldDev &f, const QAxis &g, BCacheQ &bc, int c,
+                                           double t)
+{
+    const float bcomp = pmq_bcomp(f, g, bc, c, t);
+    const double bx = quad_bcast<0>(bcomp), by = quad_bcast<1>(bcomp), bz = quad_bcast<2>(bcomp);
+    const double ib = 1.0 / sqrt(bx * bx + by * by + bz * bz);
+    return (double)bcomp * ib;
+}
+
+/* Quad gi / 4 traces line act[gi / 4] for up to a.chunk RK4 steps (the
+ * reference gather of one PIC slice, FP64); the same line state, records
+ * and active-list update as k_pmap. */
+__global__ void __launch_bounds__(256) k_pmap_quad(PmapArgs a, int nact, const int *act,
+                                                   int *act_next, unsigned *cnt)
+{
+    const int gq = (int)(((unsigned)blockIdx.x * 256u + threadIdx.x) >> 2);
+    const int sub = threadIdx.x & 3;
+    const int c = sub < 3 ? sub : 2;
+    const bool live = gq < nact;
+    const int i = live ? act[gq] : 0;
+    int k = live ? a.k[i] : -1;
+    if (k > 0) {
+        const double lo = c == 0 ? a.lo[0] : (c == 1 ? a.lo[1] : a.lo[2]);
+        const double hi = c == 0 ? a.hi[0] : (c == 1 ? a.hi[1] : a.hi[2]);
+        const double L = c == 0 ? a.L[0] : (c == 1 ? a.L[1] : a.L[2]);
+        const QAxis g = pmq_axis(a.f, c);
+        double p = c == 0 ? a.px[i] : (c == 1 ? a.py[i] : a.pz[i]);
+        double len = a.len[i];
+        long long ns = a.nst[i];
+        const double c0 = a.c0[i];
+        const double h = 0.5 * a.step, s6 = a.step / 6;
+        BCacheQ bc;
+        bc.valid = false;
+#pragma unroll
+        for (int j = 0; j < 6; j++) bc.key[j] = 0;
+        for (int it = 0; it < a.chunk; it++) {
+            /* next_location (src/poincare_map.c:268-320), component c */
+            const double pre = quad_pick(p, a.face);
+            p = pm_fold(p, lo, hi, L);
+            const double pre_folded = quad_pick(p, a.face);
+            double u = 0.0, sum = 0.0;
+#pragma unroll 1
+            for (int st = 0; st < 4; st++) {
+                double t = p;
+                if (st > 0) t = pm_fold(p + (st == 3 ? a.step : h) * u, lo, hi, L);
+                u = pmq_unit(a.f, g, bc, c, t);
+                if (st == 0) sum = u;
+                else if (st < 3) sum = sum + 2 * u;
+                else sum = sum + u;
+            }
+            p = p + s6 * sum;
+            len = len + a.step;
+            ns++;
+            const double post = quad_pick(p, a.face);
+            bool hit;
+            if (a.mode == TP_PMAP_REF_COMPAT) {
+                hit = !((pre - c0) * (post - c0) > 0);
+            } else {
+                const double Lf = a.L[a.face];
+                const double d0 = pm_offset(pre_folded - c0, Lf), d1 = pm_offset(post - c0, Lf);
+                hit = (d0 * d1 < 0 && fabs(d0 - d1) < 0.5 * Lf) || (d1 == 0 && d0 != 0);
+            }
+            if (!(len < a.lmax)) {
+                k = -k;
+                break;
+            }
+            if (hit) {
+                const size_t o = (size_t)i * a.stride + k;
+                if (sub < 3) (c == 0 ? a.xpos : (c == 1 ? a.ypos : a.zpos))[o] = p;
+                if (++k > a.ncross) {
+                    k = -k;
+                    break;
+                }
+            }
+        }
+        if (sub < 3) (c == 0 ? a.px : (c == 1 ? a.py : a.pz))[i] = p;
+        if (sub == 0) {
+            a.len[i] = len;
+            a.nst[i] = ns;
+            a.k[i] = k;
+        }
+    }
+    /* lines still running go to the next active list: lane 0 of each quad
+     * (one atomic per wavefront) */
+    const bool alive = live && k > 0 && sub == 0;
     const int lane = threadIdx.x & 63;
     const unsigned long long m = __ballot(alive);
     const unsigned pre = (unsigned)__popcll(m & ((1ull << lane) - 1ull));
@@ -561,6 +788,7 @@ extern "C" int tp_gpu_poincare_map(tp_ctx *c, const tp_pmap_params *p, int n,
         return TP_E_ARG;
     if (p->mode != TP_PMAP_REF_COMPAT && p->mode != TP_PMAP_PERIODIC) return TP_E_ARG;
     if (p->precision != TP_FP64 && p->precision != TP_FP32_MIXED) return TP_E_ARG;
+    if (p->lanes != 0 && p->lanes != 1 && p->lanes != 4) return TP_E_ARG;
     /* the traced length grows by step per RK4 step: a finite length_max
      * bounds the steps of every line, and length_max / step < 2^50 keeps
      * len + step > len */
@@ -573,6 +801,11 @@ extern "C" int tp_gpu_poincare_map(tp_ctx *c, const tp_pmap_params *p, int n,
     if (!c->have_field) return TP_E_STATE;
     const pmap_fn fn = pick_pmap(field_kind(c), p->precision);
     if (!fn) return TP_E_STATE;
+    /* four lanes per line: the reference gather of one PIC slice in FP64
+     * (system 3 gathers B the same way) */
+    const bool quad_ok = p->precision == TP_FP64 &&
+                         (field_kind(c) == FK_GRID_REF || field_kind(c) == FK_MHD_REF);
+    const int quad_from = !quad_ok || p->lanes == 1 ? 0 : (p->lanes == 4 ? (n < (1 << 28) ? n : (1 << 28)) : TP_PMAP_QUAD_LINES);
     c->last_ms = 0.0;
     c->last_launches = 0;
     if (n == 0) return TP_OK;
@@ -641,8 +874,12 @@ extern "C" int tp_gpu_poincare_map(tp_ctx *c, const tp_pmap_params *p, int n,
         unsigned *cn = cnt + (cur ^ 1);
         hipLaunchKernelGGL(k_pmap_zero, dim3(1), dim3(1), 0, c->stream, cn);
         if (hipEventRecord(c->ev0, c->stream) != hipSuccess) { rc = TP_E_HIP; break; }
-        hipLaunchKernelGGL(fn, dim3((nact + 255) / 256), dim3(256), 0, c->stream, a, nact,
-                           (const int *)act[cur], act[cur ^ 1], cn);
+        if (nact <= quad_from)
+            hipLaunchKernelGGL(k_pmap_quad, dim3((4 * nact + 255) / 256), dim3(256), 0,
+                               c->stream, a, nact, (const int *)act[cur], act[cur ^ 1], cn);
+        else
+            hipLaunchKernelGGL(fn, dim3((nact + 255) / 256), dim3(256), 0, c->stream, a, nact,
+                               (const int *)act[cur], act[cur ^ 1], cn);
         if (hipGetLastError() != hipSuccess) { rc = TP_E_HIP; break; }
         unsigned h = 0;
         if (hipEventRecord(c->ev1, c->stream) != hipSuccess ||

@@ -717,6 +717,20 @@ static void set_box(double *dst_lo, double *dst_hi, double *dst_d, const double 
     }
 }
 
+/* 1 / d per axis, and whether every d is a power of two with a normal
+ * reciprocal: then v / d and v * (1 / d) round the same real number, so the
+ * FP64 gathers may multiply */
+static void set_inv_d(FieldDev &f)
+{
+    f.exact_inv = 1;
+    for (int a = 0; a < 3; a++) {
+        f.inv_d[a] = 1.0 / f.d[a];
+        int e = 0;
+        const double m = frexp(f.d[a], &e);
+        if (!(m == 0.5 && isnormal(f.d[a]) && isnormal(f.inv_d[a]))) f.exact_inv = 0;
+    }
+}
+
 int tp_gpu_set_field_test(tp_ctx *c)
 {
     if (!c) return TP_E_ARG;
@@ -1139,7 +1153,7 @@ int tp_gpu_set_field_grid_double(tp_ctx *c, int nx, int ny, int nz, const double
     if (rc != TP_OK) { free_field(c); return rc; }
     c->f.nx = nx; c->f.ny = ny; c->f.nz = nz;
     set_box(c->f.lo, c->f.hi, c->f.d, lo, hi, nx, ny, nz);
-    for (int a = 0; a < 3; a++) c->f.inv_d[a] = 1.0 / c->f.d[a];
+    set_inv_d(c->f);
     c->f.gridd = c->gridd;
     c->f.kind = FK_GRIDD_REF;
     c->have_field = true;
@@ -1210,7 +1224,7 @@ int load_grid(tp_ctx *c, int nx, int ny, int nz, const double lo[3], const doubl
     }
     c->f.nx = nx; c->f.ny = ny; c->f.nz = nz;
     set_box(c->f.lo, c->f.hi, c->f.d, lo, hi, nx, ny, nz);
-    for (int a = 0; a < 3; a++) c->f.inv_d[a] = 1.0 / c->f.d[a];
+    set_inv_d(c->f);
     c->f.grid = c->grid;
     if (two_slices) {
         c->f.grid_b = c->grid_b;

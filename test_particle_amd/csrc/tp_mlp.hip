@@ -113,6 +113,56 @@ __device__ __forceinline__ float gelu_f(float z, float &gd)
     return z * s;
 }
 
+/* The network's activation: gelu(z) = max(z, 0) - G(|z|), with the tail
+ * G(u) = u - gelu(u) (= u Phi(-u) for the tanh-form Phi) replaced by a
+ * degree-8 polynomial in s = min(u, 4) / 2 - 1, evaluated on pairs of
+ * values in packed FP16: the FP32 accumulators are converted with round
+ * toward zero (v_cvt_pkrtz), then min, FMA, Horner, max and the difference
+ * are v_pk_* instructions -- no transcendental.  Within 2^-10 |z| + 1.1e-3
+ * of the tanh form for |z| < 65504 (6.3e-4 for z <= 0; 2^-10 |z| is the
+ * FP16 rounding of the input), below the bfloat16 rounding the hidden
+ * activations get anyway; the coefficients (least squares on the
+ * Chebyshev basis of [0, 4], then rounded to FP16) are exact FP16 values. */
+typedef _Float16 h16x2 __attribute__((ext_vector_type(2)));
+#define H2(v) ((h16x2){(_Float16)(v), (_Float16)(v)})
+#define GELU_T0 0.045379638671875f
+#define GELU_T1 -0.1727294921875f
+#define GELU_T2 0.21728515625f
+#define GELU_T3 0.0158538818359375f
+#define GELU_T4 -0.3017578125f
+#define GELU_T5 0.2413330078125f
+#define GELU_T6 0.018310546875f
+#define GELU_T7 -0.08465576171875f
+#define GELU_T8 0.020843505859375f
+
+__device__ __forceinline__ void gelu_pk(float za, float zb, float &ga, float &gb)
+{
+    const h16x2 z = __builtin_bit_cast(h16x2, __builtin_amdgcn_cvt_pkrtz(za, zb));
+    const h16x2 uc = __builtin_elementwise_min(__builtin_elementwise_abs(z), H2(4.0f));
+    const h16x2 sv = __builtin_elementwise_fma(uc, H2(0.5f), H2(-1.0f));
+    h16x2 p = H2(GELU_T8);
+    p = __builtin_elementwise_fma(p, sv, H2(GELU_T7));
+    p = __builtin_elementwise_fma(p, sv, H2(GELU_T6));
+    p = __builtin_elementwise_fma(p, sv, H2(GELU_T5));
+    p = __builtin_elementwise_fma(p, sv, H2(GELU_T4));
+    p = __builtin_elementwise_fma(p, sv, H2(GELU_T3));
+    p = __builtin_elementwise_fma(p, sv, H2(GELU_T2));
+    p = __builtin_elementwise_fma(p, sv, H2(GELU_T1));
+    p = __builtin_elementwise_fma(p, sv, H2(GELU_T0));
+    const h16x2 g = __builtin_elementwise_max(z, H2(0.0f)) - p;
+    ga = (float)g.x;
+    gb = (float)g.y;
+}
+
+/* gelu'(z) of the tanh form in FP32 (the hardware exp2 and reciprocal),
+ * for the backward pass */
+__device__ __forceinline__ float gelu_deriv(float z)
+{
+    float d = 0.f;
+    (void)gelu_f<true>(z, d);
+    return d;
+}
+
 __device__ __forceinline__ f32x4 mfma(const bf16x8 &a, const bf16x8 &b, const f32x4 &c)
 {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
@@ -133,11 +183,13 @@ __device__ __forceinline__ void hidden_epilogue(f32x4 (&acc)[8][4], __bf16 *tile
         for (int mb = 0; mb < 4; mb++) {
             const int m = mb * 16 + i16;
             f32x4 h, gd;
+            float h0, h1, h2, h3;
+            gelu_pk(acc[nb][mb][0], acc[nb][mb][1], h0, h1);
+            gelu_pk(acc[nb][mb][2], acc[nb][mb][3], h2, h3);
+            h[0] = h0; h[1] = h1; h[2] = h2; h[3] = h3;
+            if (MODE == MLP_TRAIN) {
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                float d = 0.f;
-                h[r] = gelu_f<MODE == MLP_TRAIN>(acc[nb][mb][r], d);
-                gd[r] = d;
+                for (int r = 0; r < 4; r++) gd[r] = gelu_deriv(acc[nb][mb][r]);
             }
             if (LAYER < 3) {
                 bf16x4 hb;

@@ -154,6 +154,41 @@ def test_open_boundaries_sample(tracker, workload, oracle_tp):
 
 
 @pytest.mark.parametrize("pusher", [abi.TP_PUSH_BORIS, abi.TP_PUSH_WIRZ])
+def test_open_boundaries_multiblock_vs_oracle(pusher, oracle_tp):
+    """Open boundaries with the escapes and the particles still inside both
+    spanning many blocks: 6,000 particles in the ABC field over [0, 12 pi]^3,
+    3,000 steps, D every 100 steps.  After the first escapes the launches
+    run over the compacted list of particles inside, the escaped particles'
+    D terms come from their last state (src/tracking.c:347-369) in per-block
+    partial rows; everything is compared whole with the oracle: particles,
+    step counts, spectra, escape spectra and remaining counts bit for bit, D
+    sums to 1e-12."""
+    from test_particle_amd.gpu import Tracker
+    two_pi = 2.0 * np.pi
+    lo, hi = (0.0, 0.0, 0.0), (6 * two_pi,) * 3
+    n = 6000
+    dt = 0.01 * two_pi * (1.0 / np.sqrt(1.0 - 0.49)) / np.sqrt(3.0)
+    ptl = init_particles(n, lo, hi, 0.7, seed=SEED + 7)
+    prm = abi.make_params(dt=dt, estep=3000, lo=lo, hi=hi, pusher=pusher,
+                          bc_flag=abi.TP_BC_OPEN, nbins=102, nt_out=30, emin=1e-3, emax=10.0)
+    print(f"[fullsize] ABC open boundaries {n} x 3000 pusher={pusher}", flush=True)
+    with Tracker(0) as t:
+        t.set_field_abc(1.0, 1.0, 1.0)
+        r = t.track_fixed(prm, ptl)
+    ref = oracle_tp.track_fixed(oracle_tp.Field.abc(1.0, 1.0, 1.0), prm, ptl,
+                                nthreads=min(16, os.cpu_count() or 1))
+    for k in ("ptl", "nsteps", "espect", "espect_escape", "remain"):
+        _same(r[k], ref[k], f"open, pusher {pusher}: {k}")
+    np.testing.assert_allclose(r["dcoeff"], ref["dcoeff"], rtol=1e-12, atol=1e-15)
+    rem = r["remain"]
+    left = n - int(rem[-1])
+    print(f"[fullsize] escaped {left}, remaining {int(rem[-1])}", flush=True)
+    # several 256-particle blocks on both sides of the compaction
+    assert left > 4 * 256 and rem[-1] > 256
+    assert ((rem > 4 * 256) & (n - rem > 4 * 256)).sum() >= 10
+
+
+@pytest.mark.parametrize("pusher", [abi.TP_PUSH_BORIS, abi.TP_PUSH_WIRZ])
 def test_abc_16k_10000_steps_bitwise(pusher, oracle_tp):
     """ABC A = B = C = 1 over [0, 2 pi]^3, |v| = 0.7c isotropic, dt = 0.01 T_g
     with B_ref = sqrt(3), 10,000 steps, spectrum every 100 steps (102 bins,

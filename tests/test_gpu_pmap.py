@@ -2,7 +2,9 @@
 
 * the reference's own crossing points (golden vectors of
   src/poincare_map.c's RK4 step and config parser on PIC grids) bit for
-  bit, through any chunking of the steps into launches;
+  bit, through any chunking of the steps into launches, with one lane per
+  line and with four (one per component, the form of the launches over few
+  running lines);
 * the CPU oracle bit for bit for the analytic fields (ABC, multi-mode,
   with the shared sin/cos) and the textbook PIC gather, in both crossing
   modes, on lines that end at different times (the active-list compaction
@@ -71,12 +73,13 @@ def _golden_case(gpu, name):
 
 
 @pytest.mark.parametrize("name", ["pmap_grid", "pmap_single", "pmap_grid2"])
-@pytest.mark.parametrize("chunk", [0, 37])
-def test_reference_golden_maps(gpu, name, chunk):
-    case(f"poincare golden {name}, chunk {chunk}")
+@pytest.mark.parametrize("chunk,lanes", [(0, 0), (37, 0), (0, 1), (37, 1), (37, 4)])
+def test_reference_golden_maps(gpu, name, chunk, lanes):
+    case(f"poincare golden {name}, chunk {chunk}, lanes {lanes}")
     g, c, st = _golden_case(gpu, name)
     prm = abi.make_pmap_params(face=c["face"], ncross=c["ncross"], step=c["step"],
-                               length_max=c["length_max"], lo=g["lo"], hi=g["hi"], chunk=chunk)
+                               length_max=c["length_max"], lo=g["lo"], hi=g["hi"], chunk=chunk,
+                               lanes=lanes)
     r = gpu.poincare_map(prm, st)
     ref = dict(xpos=g["xpos"], ypos=g["ypos"], zpos=g["zpos"],
                ncrossed=(g["xpos"] != 2 * g["hi"][0]).sum(axis=1) - 1)
@@ -115,9 +118,11 @@ def _set(gpu, oracle, kind, seed=0):
     return oracle.Field.grid(e, b, lo, hi, mode), lo, hi
 
 
-@pytest.mark.parametrize("kind", FIELDS)
+@pytest.mark.parametrize("kind", FIELDS + ["grid_ref_1lane"])
 @pytest.mark.parametrize("mode", [abi.TP_PMAP_REF_COMPAT, abi.TP_PMAP_PERIODIC])
 def test_matches_oracle(gpu, oracle_tp, kind, mode):
+    lanes = 1 if kind.endswith("_1lane") else 0
+    kind = kind.replace("_1lane", "")
     f, lo, hi = _set(gpu, oracle_tp, kind)
     face = 2 if kind.startswith("grid") else 1
     n = 300
@@ -126,7 +131,7 @@ def test_matches_oracle(gpu, oracle_tp, kind, mode):
     step = 0.03
     case(f"poincare {kind} mode {mode}: {n} lines, ncross {ncross}, length {lmax}")
     prm = abi.make_pmap_params(face=face, ncross=ncross, step=step, length_max=lmax, lo=lo,
-                               hi=hi, mode=mode, chunk=200)
+                               hi=hi, mode=mode, chunk=200, lanes=lanes)
     r = gpu.poincare_map(prm, st)
     ref = oracle_tp.poincare_map(f, prm, st, nthreads=8)
     same_map(r, ref, f"{kind} mode {mode}")
@@ -303,7 +308,7 @@ def test_errors(gpu):
     st = np.array([[0.1, 0.2, 0.3]])
     gpu.set_field_abc()
     for kw in (dict(face=-1), dict(step=-0.1), dict(length_max=np.nan), dict(mode=5),
-               dict(chunk=-2), dict(hi=(0.0, 1.0, 1.0)), dict(precision=3)):
+               dict(chunk=-2), dict(hi=(0.0, 1.0, 1.0)), dict(precision=3), dict(lanes=2)):
         args = dict(face=0, ncross=2, step=0.1, length_max=5.0, lo=(0.0,) * 3, hi=(1.0,) * 3)
         args.update(kw)
         with pytest.raises(TPError) as ei:

@@ -17,6 +17,28 @@ from test_particle_amd import surrogate
 L_BOX = 129.73
 
 
+# the GPU kernels' activation, restated with numpy's IEEE half type: the
+# FP32 input converted toward zero, every FMA (exact in FP64 for half
+# operands) and the final difference rounded to nearest even in FP16
+GELU_TAIL = [0.045379638671875, -0.1727294921875, 0.21728515625, 0.0158538818359375,
+             -0.3017578125, 0.2413330078125, 0.018310546875, -0.08465576171875,
+             0.020843505859375]
+
+
+def _gelu_fp16(z):
+    zf = np.asarray(z, np.float64).astype(np.float32)
+    with np.errstate(over="ignore"):
+        h = zf.astype(np.float16)
+    away = np.abs(h.astype(np.float64)) > np.abs(zf.astype(np.float64))
+    h = np.where(away, np.nextafter(h, np.float16(0)), h)
+    s = (np.minimum(np.abs(h), np.float16(4)).astype(np.float64) * 0.5 - 1.0).astype(np.float16)
+    p = np.full(h.shape, np.float16(GELU_TAIL[8]))
+    for c in reversed(GELU_TAIL[:8]):
+        p = (p.astype(np.float64) * s.astype(np.float64) + c).astype(np.float16)
+    g = np.maximum(h, np.float16(0)).astype(np.float64) - p.astype(np.float64)
+    return g.astype(np.float16).astype(np.float64)
+
+
 def _torch_net(params, q, emulate=False):
     W1, b1, W2, b2, W3, b3, W4, b4 = [torch.tensor(a, dtype=torch.float64)
                                       for a in surrogate.split_params(params)]
@@ -29,6 +51,8 @@ def _torch_net(params, q, emulate=False):
         return t.to(torch.float32).to(torch.float64) if emulate else t
 
     def gelu(z):
+        if emulate:
+            return torch.tensor(_gelu_fp16(z.detach().numpy()).reshape(z.shape))
         return torch.nn.functional.gelu(z, approximate="tanh")
 
     def net(x):
@@ -86,6 +110,30 @@ def test_gelu_is_the_tanh_form_and_close_to_erf_gelu():
     fd = (tpref.gelu(z + h)[0] - tpref.gelu(z - h)[0]) / (2 * h)
     assert np.allclose(gd, fd, rtol=1e-7, atol=1e-8)
     assert np.all(np.isfinite(tpref.gelu(np.array([-1e4, 1e4]))[0]))
+
+
+def test_gpu_activation_is_the_fp16_tail_polynomial():
+    """The kernels' gelu (oracle emulation with frexp-based FP16 rounding)
+    equals numpy's half-precision restatement bit for bit, special values
+    included, and stays within 2^-10 |z| + 1.1e-3 of the tanh form for
+    |z| < 65504 (6.3e-4 for z <= 0); beyond, the FP16 input saturates at
+    65504."""
+    rng = np.random.default_rng(1)
+    z = np.concatenate([np.linspace(-10.0, 10.0, 200001), rng.normal(0.0, 3.0, 100000),
+                        rng.normal(0.0, 1e-4, 10000), np.linspace(-3000.0, 3000.0, 20001),
+                        [0.0, -0.0, 4.0, -4.0, 65504.0, 65519.0, 65520.0, 1e5, -1e5, 6e-8,
+                         -6e-8, 1e-9, 3e-5]])
+    g = tpref.gelu_gpu(z)
+    assert np.array_equal(g, _gelu_fp16(z))
+    exact = tpref.gelu(z)[0]
+    inside = np.abs(z) < 65504.0
+    err = np.abs(g - exact)
+    assert np.all(err[inside] <= 2.0 ** -10 * np.abs(z[inside]) + 1.1e-3)
+    assert np.all(err[z <= 0] <= 6.3e-4)
+    assert tpref.gelu_gpu([1e5])[0] == 65504.0 and abs(tpref.gelu_gpu([-1e5])[0]) <= 6.3e-4
+    # a dropped tail term or a wrong coefficient shows at the 1e-2 level
+    assert np.max(err[np.abs(z) < 4]) > 1e-4           # it is an approximation
+    assert np.max(np.abs(g - np.maximum(z, 0))[np.abs(z) < 2]) > 0.1
 
 
 def test_forward_matches_torch_fp64():
