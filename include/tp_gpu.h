@@ -290,6 +290,25 @@ int tp_gpu_store_particles(tp_ctx *ctx, tp_particle *ptl, int *nsteps);
 int tp_gpu_read_diagnostics(tp_ctx *ctx, double *espect,
                             double *espect_escape, double *dcoeff,
                             int *nptl_remain);
+/* The same run as tp_gpu_run_fixed with no device or host allocation, for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  No counterpart in the reference.
+ * The outputs are DEVICE buffers the caller owns, all required:
+ *   espect, espect_escape  [nt_out][nbins] particle counts (unsigned)
+ *   dcoeff                 [8][nd] D sums, as tp_gpu_track_fixed
+ *   nptl_remain            [nd] remaining particles
+ * Every buffer is cleared by the call (with kernels, no memset).  The
+ * library's own scratch (D partial-sum rows, open-boundary index lists, the
+ * non-finite flag) is a static device array of about 36 MB that every GPU
+ * the library runs on holds until the process ends; it bounds the run to
+ * TP_NOALLOC_MAX_PARTICLES resident particles (TP_E_ARG beyond).  Calls on
+ * one GPU share that array, so calls that overlap on one GPU (from two
+ * contexts or threads) race.  Not available with the surrogate field or
+ * trajectory recording (TP_E_ARG).  Returns when the run has finished;
+ * tp_gpu_last_kernel_time gives its device time as one launch span. */
+#define TP_NOALLOC_MAX_PARTICLES 4194304
+int tp_gpu_run_fixed_noalloc(tp_ctx *ctx, const tp_track_params *p, unsigned *espect,
+                             unsigned *espect_escape, double *dcoeff, int *nptl_remain);
 /* Time of the tracking kernels of the last run, measured with HIP events on
  * the stream they run on: total milliseconds and number of launches. */
 int tp_gpu_last_kernel_time(tp_ctx *ctx, double *ms, int *launches);

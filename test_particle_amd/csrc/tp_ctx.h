@@ -80,6 +80,9 @@ struct tp_ctx {
     float *stage_dev[2] = {nullptr, nullptr};
     hipEvent_t stage_ev[2] = {nullptr, nullptr};
     size_t stage_floats = 0;
+    /* device address of the static scratch array on this context's GPU
+     * (tp_gpu_run_fixed_noalloc); null if the lookup failed */
+    double *scratch = nullptr;
 };
 
 void tp_bs_release(tp_ctx *c);

@@ -621,6 +621,8 @@ const char *tp_gpu_strerror(int code)
     return "unknown error";
 }
 
+static void *noalloc_scratch(int dev);
+
 int tp_gpu_init(int device, tp_ctx **out)
 {
     if (!out) return TP_E_ARG;
@@ -640,6 +642,8 @@ int tp_gpu_init(int device, tp_ctx **out)
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cu = prop.multiProcessorCount;
     memset(&c->f, 0, sizeof(c->f));
+    /* the scratch of the allocation-free run, looked up with the device current */
+    c->scratch = (double *)noalloc_scratch(device);
     *out = c;
     return TP_OK;
 }
@@ -1467,7 +1471,9 @@ static int block_size_for(const tp_ctx *c, int n)
     return 64;
 }
 
-/* Buffers of one fixed-step run (the context's own for tp_gpu_run_fixed). */
+/* Buffers of one fixed-step run: the context's own (tp_gpu_run_fixed) or the
+ * caller's device outputs and the static scratch array
+ * (tp_gpu_run_fixed_noalloc). */
 struct RunBufs {
     unsigned *espect, *espect_esc;  /* [nt_out][nbins] counts */
     double *dco;                    /* [8][nd] D sums */
@@ -1491,6 +1497,13 @@ __global__ void __launch_bounds__(256) k_fill_f64(double *a, size_t n, double v)
 {
     for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
         a[i] = v;
+}
+
+/* remaining counts from the escapes counted against each D row, in place */
+__global__ void __launch_bounds__(256) k_remain(int *rdec, int nd, int n)
+{
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k < nd) rdec[k] = n - rdec[k];
 }
 
 /* flag = 1 if any particle's position or momentum is NaN or infinite (the
@@ -1807,6 +1820,88 @@ int tp_gpu_run_fixed(tp_ctx *c, const tp_track_params *p)
     B.flag = (int *)(c->partials + R.prows * 8);
     B.per_launch_events = true;
     return run_fixed_impl(c, p, fn, mlp, R, B);
+}
+
+/* ------------------------------------------------------------------ */
+/* the allocation-free run: scratch in a static device array           */
+/* ------------------------------------------------------------------ */
+/* Scratch of tp_gpu_run_fixed_noalloc: the D partial-sum rows, the
+ * open-boundary index lists and the non-finite flag of runs over at most
+ * TP_NOALLOC_MAX_PARTICLES particles.  Every GPU the library runs on holds
+ * one copy (about 36 MB) until the process ends; host code reaches it only
+ * through the device address kept per device by noalloc_scratch(). */
+#define NOALLOC_BLK_ROWS (2 * ((TP_NOALLOC_MAX_PARTICLES + 255) / 256) + 2)
+#define NOALLOC_LIVE_INTS (2 * (size_t)TP_NOALLOC_MAX_PARTICLES + \
+                           2 * ((TP_NOALLOC_MAX_PARTICLES + 255) / 256) + 1)
+#define NOALLOC_DOUBLES ((size_t)NOALLOC_BLK_ROWS * 8 + (NOALLOC_LIVE_INTS + 1) / 2 + 1)
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+#define NOALLOC_MAX_DEV 64
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[NOALLOC_MAX_DEV];
+static int g_scratch_rc[NOALLOC_MAX_DEV];   /* 0 not looked up, 1 ok, -1 failed */
+
+/* The device address of scratch on the current device `dev`, looked up once
+ * per device (each GPU holds its own copy at its own address). */
+static void *noalloc_scratch(int dev)
+{
+    if (dev < 0 || dev >= NOALLOC_MAX_DEV) return nullptr;
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (g_scratch_rc[dev] == 0) {
+        void *ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            fprintf(stderr, "tp_gpu: hipGetSymbolAddress(scratch) on device %d failed: %s\n",
+                    dev, hipGetErrorString(e));
+            g_scratch_rc[dev] = -1;
+        } else {
+            g_scratch_ptr[dev] = ptr;
+            g_scratch_rc[dev] = 1;
+        }
+    }
+    return g_scratch_rc[dev] == 1 ? g_scratch_ptr[dev] : nullptr;
+}
+
+int tp_gpu_run_fixed_noalloc(tp_ctx *c, const tp_track_params *p, unsigned *espect,
+                             unsigned *espect_escape, double *dcoeff, int *nptl_remain)
+{
+    track_fn fn = nullptr;
+    bool mlp = false;
+    int rc = run_check(c, p, &fn, &mlp);
+    if (rc) return rc;
+    if (mlp || c->traj_nout > 0) return TP_E_ARG;   /* these paths allocate */
+    if (!espect || !espect_escape || !dcoeff || !nptl_remain) return TP_E_ARG;
+    const int n = c->s.n;
+    if (n > TP_NOALLOC_MAX_PARTICLES) {
+        fprintf(stderr, "tp_gpu_run_fixed_noalloc: %d particles, at most %d\n", n,
+                TP_NOALLOC_MAX_PARTICLES);
+        return TP_E_ARG;
+    }
+    const RunSizes R = run_sizes(c, p, n);
+    if (R.prows > (size_t)NOALLOC_BLK_ROWS || R.live_ints > NOALLOC_LIVE_INTS) {
+        fprintf(stderr, "tp_gpu_run_fixed_noalloc: the run needs more scratch than the "
+                        "library holds\n");
+        return TP_E_ARG;
+    }
+    HIPCHK(hipSetDevice(c->device));
+    double *sp = c->scratch;
+    if (!sp) return TP_E_HIP;
+    RunBufs B;
+    B.espect = espect; B.espect_esc = espect_escape;
+    B.dco = dcoeff; B.rdec = nptl_remain;
+    B.partials = sp; B.prows = R.prows;
+    B.live = (int *)(sp + (size_t)NOALLOC_BLK_ROWS * 8);
+    B.live_ints = NOALLOC_LIVE_INTS;
+    B.flag = (int *)(sp + NOALLOC_DOUBLES - 1);
+    B.per_launch_events = false;
+    rc = run_fixed_impl(c, p, fn, false, R, B);
+    if (rc != TP_OK && rc != TP_E_NONFINITE) return rc;
+    /* the escapes counted against each row become the remaining counts */
+    hipLaunchKernelGGL(k_remain, dim3((R.nd + 255) / 256), dim3(256), 0, c->stream,
+                       nptl_remain, R.nd, n);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return rc;
 }
 
 int tp_gpu_last_kernel_time(tp_ctx *c, double *ms, int *launches)

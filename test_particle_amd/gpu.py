@@ -63,6 +63,7 @@ def lib():
         L.tp_gpu_store_particles.argtypes = [vp, P(d), P(i)]
         L.tp_gpu_read_diagnostics.argtypes = [vp, P(d), P(d), P(d), P(i)]
         L.tp_gpu_last_kernel_time.argtypes = [vp, P(d), P(i)]
+        L.tp_gpu_run_fixed_noalloc.argtypes = [vp, P(abi.TpTrackParams), vp, vp, vp, vp]
         L.tp_gpu_run_adaptive.argtypes = [vp, P(abi.TpBsParams), P(abi.TpBsStats)]
         L.tp_gpu_track_adaptive.argtypes = [vp, P(abi.TpBsParams), i, P(d), P(i),
                                             P(abi.TpBsStats)]
@@ -359,6 +360,16 @@ class Tracker:
 
     def run(self, params):
         _check(lib().tp_gpu_run_fixed(self._ctx, ctypes.byref(params)), "run_fixed")
+
+    def run_noalloc(self, params, espect, espect_escape, dcoeff, remain):
+        """tp_gpu_run_fixed_noalloc into caller-owned DEVICE buffers given as
+        integer addresses: counts [nt_out][nbins] uint32 (twice), D sums [8][nd]
+        float64, remaining counts [nd] int32."""
+        rc = lib().tp_gpu_run_fixed_noalloc(self._ctx, ctypes.byref(params),
+                                            ctypes.c_void_p(espect),
+                                            ctypes.c_void_p(espect_escape),
+                                            ctypes.c_void_p(dcoeff), ctypes.c_void_p(remain))
+        _check(rc, "run_fixed_noalloc")
 
     def store(self):
         ptl = np.zeros((self._n, abi.NFIELD))
