@@ -1499,6 +1499,23 @@ __global__ void __launch_bounds__(256) k_fill_f64(double *a, size_t n, double v)
         a[i] = v;
 }
 
+/* the outputs a fixed-step run accumulates into, cleared in one launch
+ * (one dispatch instead of four ahead of the step kernel): counts
+ * [0, na) and [na, 2 na), D sums [2 na, 2 na + 8 nd), escape counts per D
+ * row after that */
+__global__ void __launch_bounds__(256) k_clear_run(unsigned *espect, unsigned *espect_esc,
+                                                   size_t na, double *dco, int *rdec, size_t nd)
+{
+    const size_t total = 2 * na + 9 * nd;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total;
+         i += (size_t)gridDim.x * 256) {
+        if (i < na) espect[i] = 0u;
+        else if (i < 2 * na) espect_esc[i - na] = 0u;
+        else if (i < 2 * na + 8 * nd) dco[i - 2 * na] = 0.0;
+        else rdec[i - 2 * na - 8 * nd] = 0;
+    }
+}
+
 /* remaining counts from the escapes counted against each D row, in place */
 __global__ void __launch_bounds__(256) k_remain(int *rdec, int nd, int n)
 {
@@ -1598,11 +1615,13 @@ static int run_fixed_impl(tp_ctx *c, const tp_track_params *p, track_fn fn, bool
     const int nd = R.nd, bs = R.bs, nblocks = R.nblocks, nb256 = R.nb256;
     const size_t need = (size_t)nb * nt;
     int rc;
-    if ((rc = fill_u32(c->stream, B.espect, need)) ||
-        (rc = fill_u32(c->stream, B.espect_esc, need)) ||
-        (rc = fill_f64(c->stream, B.dco, (size_t)8 * nd)) ||
-        (rc = fill_u32(c->stream, (unsigned *)B.rdec, (size_t)nd)))
-        return rc;
+    {
+        const size_t total = 2 * need + 9 * (size_t)nd;
+        const size_t nbk = total ? (total + 255) / 256 < 1024 ? (total + 255) / 256 : 1024 : 1;
+        hipLaunchKernelGGL(k_clear_run, dim3((unsigned)nbk), dim3(256), 0, c->stream, B.espect,
+                           B.espect_esc, need, B.dco, B.rdec, (size_t)nd);
+        HIPCHK(hipGetLastError());
+    }
     c->last_ms = 0.0;
     c->last_launches = 0;
 
