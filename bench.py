@@ -424,15 +424,20 @@ def forms_bs(args):
         st = tr.run_adaptive(prm)
         w = time.perf_counter() - t0
         fin = tr.store()[0] if not args.no_cpu else None
-        # the rate over horizons of 0.2, 2 and 20 gyroperiods (262,144 particles)
-        flat = {}
+        # the rate over horizons of 0.2, 2 and 20 gyroperiods, all particles
+        # (the persistent kernel's lanes are refilled from the same queue)
+        flat, flat_rhs = {}, {}
         for tg in (0.2, 2.0, 20.0):
-            tr.load(ptl[:1 << 18])
+            if tg == args.bs_tgyro:
+                flat[str(tg)], flat_rhs[str(tg)] = st["accepted"] / w, st["rhs"] / w
+                continue
+            tr.load(ptl)
             t1 = time.perf_counter()
             s2 = tr.run_adaptive(abi.make_bs_params(rtol=1e-8, atol=1e-10, t_end_gyro=tg,
                                                     b_ref=1.0))
-            flat[str(tg)] = s2["accepted"] / (time.perf_counter() - t1)
-        log(f"[bench] BS accepted steps/s over 0.2/2/20 T_g: {flat}")
+            w2 = time.perf_counter() - t1
+            flat[str(tg)], flat_rhs[str(tg)] = s2["accepted"] / w2, s2["rhs"] / w2
+        log(f"[bench] BS accepted steps/s over 0.2/2/20 T_g: {flat}; RHS/s {flat_rhs}")
         dense = bs_dense_vs_fixed(tr, abi, ptl)
     rate = st["accepted"] / w
     log(f"[bench] BS: {rate:.4e} accepted steps/s, {st['rhs'] / w:.4e} RHS/s, "
@@ -442,7 +447,7 @@ def forms_bs(args):
         "horizon_gyroperiods": args.bs_tgyro, "rtol": 1e-8, "atol": 1e-10,
         "rhs_per_s": st["rhs"] / w, "accepted": st["accepted"], "rejected": st["rejected"],
         "failed": st["failed"], "kernel_ms": st["kernel_ms"], "wall_s": w,
-        "rate_by_horizon_gyroperiods": flat}}
+        "rate_by_horizon_gyroperiods": flat, "rhs_rate_by_horizon_gyroperiods": flat_rhs}}
     out.update(dense)
     if fin is not None:
         # the oracle's sequential BS (OpenMP over particles) on the first

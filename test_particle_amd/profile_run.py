@@ -76,6 +76,8 @@ def main():
     ap.add_argument("--nmodes", type=int, default=1000)
     ap.add_argument("--tgyro", type=float, default=0.05, help="adaptive horizon (gyroperiods)")
     ap.add_argument("--tune", action="store_true")
+    ap.add_argument("--dense", action="store_true",
+                    help="modes: dense output with the fixed-step diagnostics (--steps outputs)")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--wpc", type=int, default=0)
     ap.add_argument("--tol", type=float, default=1e-3, help="surrogate residual tolerance")
@@ -126,6 +128,21 @@ def main():
             b, w = tr.tune_adaptive(prm)
             print(f"tuned: block {b}, waves/CU {w} ({time.perf_counter() - t0:.1f}s)", flush=True)
             tr.load(ptl)
+        if a.dense:
+            # dense output with the fixed-step loop's diagnostics: output
+            # step 0.1 over 200 steps, D rows every 20, no boundary
+            dp = abi.make_params(dt=0.1, estep=a.steps, lo=(0, 0, 0), hi=(100, 100, 100),
+                                 pusher=abi.TP_PUSH_BORIS, bc_flag=abi.TP_BC_NONE, nbins=102,
+                                 nt_out=10, emin=1e-4, emax=1.0, dcoeff_interval=20)
+            t0 = time.perf_counter()
+            st = tr.run_adaptive_diag(prm, dp)
+            wall = time.perf_counter() - t0
+            print(f"modes({a.nmodes}) dense: {a.particles} particles, {a.steps} output steps "
+                  f"of 0.1: {st['accepted'] / wall:.4e} accepted steps/s, "
+                  f"{st['rhs'] / wall:.4e} RHS/s, rejected {st['rejected']}, kernel "
+                  f"{st['kernel_ms']:.1f} ms, wall {wall:.2f}s", flush=True)
+            tr.close()
+            return
         t0 = time.perf_counter()
         st = tr.run_adaptive(prm)
         wall = time.perf_counter() - t0

@@ -1,5 +1,5 @@
 """D_xx of the configured ABC workload on the GPU against the reference's
-own tracking loop, judged statistically over three seeds.
+own tracking loop, compared statistically over three seeds.
 
 Workload: 16,384 protons at |v| = 0.7c, ABC field A = B = C = 1, periodic
 [0, 2 pi]^3, dt = 0.01 T_g (T_g = 2 pi gamma0 / sqrt(3)), 10,000 FP64 Wirz
