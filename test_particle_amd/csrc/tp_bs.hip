@@ -42,7 +42,17 @@
 #define BS_KFAC1 0.8
 #define BS_KFAC2 0.9
 
-enum { PH_INIT = 0, PH_MID = 1, PH_ACCEPT = 2, PH_DROW = 3 };
+enum { PH_INIT = 0, PH_MID = 1, PH_ACCEPT = 2, PH_DROW = 3, PH_PREP = 4 };
+
+/* Dense form: a lane whose step was accepted waits (PH_PREP, evaluating a
+ * dummy point like an idle lane) until this many lanes of its wavefront
+ * wait, or no other lane is busy, and then they form their interpolants
+ * together.  prepare_dense is a long chain of per-lane scratch accesses; run
+ * for one lane at a time it stalled the whole wavefront at almost every
+ * iteration.  Each lane's arithmetic is unchanged (same results). */
+#ifndef BS_PREP_BATCH
+#define BS_PREP_BATCH 16
+#endif
 enum { Q_HEAD = 0, Q_ACC = 1, Q_REJ = 2, Q_RHS = 3, Q_FAIL = 4, Q_N = 8 };
 
 struct BsConsts {
@@ -58,7 +68,11 @@ struct BsConsts {
 #define DS_YSAVE 0
 #define DS_FSAVE (9 * BS_N)
 #define DS_DENS (DS_FSAVE + 162 * BS_N)
-#define DS_SLOTS (DS_DENS + (2 * 9 + 5) * BS_N)
+#define DS_YV (DS_DENS + (2 * 9 + 5) * BS_N)
+/* largest kmax of the dense form; YV: the values each interpolation order
+ * takes from each sequence, for one component at a time ([order][sequence]) */
+#define BS_KMAX 8
+#define DS_SLOTS (DS_YV + (2 * BS_KMAX - 1) * (BS_KMAX + 1))
 
 struct BsArgs {
     FieldDev f;
@@ -427,11 +441,12 @@ __device__ __forceinline__ void bs_end_seq(const BsArgs &a, BsLane &L, const dou
     L.phase = PH_ACCEPT;
 }
 
-/* dense_interp, src/unused/StepperBS.c:478-524, on the lane's dens slots */
+/* dense_interp, src/unused/StepperBS.c:478-524, on the lane's dens slots;
+ * the coefficient array is indexed by compile-time orders (unrolled up to
+ * 2 BS_KMAX - 1) so that it stays in registers */
 __device__ __forceinline__ void bs_dense_interp(const BsArgs &a, long gl, int imit)
 {
     const int n = BS_N;
-    double av[16];
     for (int i = 0; i < n; i++) {
         double y0 = DSV(a, DS_DENS + i), y1 = DSV(a, DS_DENS + 2 * n + i);
         double yp0 = DSV(a, DS_DENS + n + i), yp1 = DSV(a, DS_DENS + 3 * n + i);
@@ -446,11 +461,14 @@ __device__ __forceinline__ void bs_dense_interp(const BsArgs &a, long gl, int im
         double ph1 = ydiff + (aspl - bspl) * 0.25;
         double ph2 = -(yp0 - yp1);
         double ph3 = 6.0 * (bspl - aspl);
+        double av[2 * BS_KMAX];
         if (imit >= 1) {
             av[1] = 16.0 * (DSV(a, DS_DENS + 5 * n + i) - ph1);
             if (imit >= 3) {
                 av[3] = 16.0 * (DSV(a, DS_DENS + 7 * n + i) - ph3 + 3 * av[1]);
-                for (int im = 5; im <= imit; im += 2) {
+#pragma unroll
+                for (int im = 5; im <= 2 * BS_KMAX - 1; im += 2) {
+                    if (im > imit) break;
                     double fac1 = im * (im - 1) / 2.0;
                     double fac2 = fac1 * (im - 2) * (im - 3) * 2.0;
                     av[im] = 16.0 * (DSV(a, DS_DENS + (im + 4) * n + i) + fac1 * av[im - 2] -
@@ -461,21 +479,130 @@ __device__ __forceinline__ void bs_dense_interp(const BsArgs &a, long gl, int im
         av[0] = (DSV(a, DS_DENS + 4 * n + i) - ph0) * 16.0;
         if (imit >= 2) {
             av[2] = (DSV(a, DS_DENS + 6 * n + i) - ph2 + av[0]) * 16.0;
-            for (int im = 4; im <= imit; im += 2) {
+#pragma unroll
+            for (int im = 4; im <= 2 * BS_KMAX - 1; im += 2) {
+                if (im > imit) break;
                 double fac1 = im * (im - 1) / 2.0;
                 double fac2 = (double)(im * (im - 1) * (im - 2) * (im - 3));
                 av[im] = 16.0 * (DSV(a, DS_DENS + (im + 4) * n + i) + fac1 * av[im - 2] -
                                  fac2 * av[im - 4]);
             }
         }
-        for (int im = 0; im <= imit; im++) DSV(a, DS_DENS + n * (im + 4) + i) = av[im];
+#pragma unroll
+        for (int im = 0; im <= 2 * BS_KMAX - 1; im++) {
+            if (im > imit) break;
+            DSV(a, DS_DENS + n * (im + 4) + i) = av[im];
+        }
+    }
+}
+
+/* prepare_dense's polynomial extrapolation over sequences KB..k of one
+ * component held in y: y[l-1] = y[l] + (y[l] - y[l-1]) / (r^2 - 1), r =
+ * n_j / n_{l-1}, for j = KB+1..k and l = j..KB+1 (src/unused/StepperBS.c:
+ * 545-553 and 564-572); returns y[KB] */
+template <int KB>
+__device__ __forceinline__ double bs_dense_extrap(const BsArgs &a, double (&y)[BS_KMAX + 1],
+                                                  int k)
+{
+#pragma unroll
+    for (int j = KB + 1; j <= BS_KMAX; j++) {
+        if (j > k) break;
+        double dblenj = a.c.nseq[j];
+#pragma unroll
+        for (int l = j; l >= KB + 1; l--) {
+            double r = dblenj / a.c.nseq[l - 1];
+            double factor = r * r - 1.0;
+            y[l - 1] = y[l] + (y[l] - y[l - 1]) / factor;
+        }
+    }
+    return y[KB];
+}
+
+/* sequence KK's derivative rows of component i (rows 2 KK^2 .. 2 (KK+1)^2 - 1
+ * of fsave: ipoint[kk] = 2 kk^2 for nseq = 4k + 2) in registers, order by
+ * order: the value order KMI takes from them (times (n_KK/2)^(KMI-1)) goes to
+ * YV, then the differences prepare_dense takes between orders
+ * (src/unused/StepperBS.c:558-601), element by element in its order.  Each
+ * difference pairs rows two apart, so the rows at odd and at even offsets
+ * in the sequence evolve separately: P selects one half (offset o is held
+ * in row[o / 2]), which halves the registers held. */
+template <int KK, int P, int KMI>
+__device__ __forceinline__ void bs_dense_orders(const BsArgs &a, long gl, int mu, double dydx_i,
+                                                double (&row)[2 * KK + 1])
+{
+    if constexpr (KMI <= 2 * BS_KMAX - 1) {
+        constexpr int R = 4 * KK + 2;
+        if (KMI > mu) return;
+        if constexpr (KK >= (KMI - 1) / 2 && (2 * KK + KMI - 1) % 2 == P)
+            DSV(a, DS_YV + (KMI - 1) * (BS_KMAX + 1) + KK) =
+                row[(2 * KK + KMI - 1) / 2] * tp_powi(a.c.nseq[KK] / 2.0, KMI - 1);
+        if (KMI == mu) return;
+        if constexpr (KK >= KMI / 2) {
+            if constexpr (P == 1) {
+#pragma unroll
+                for (int o = R - 1; o >= (KMI == 1 ? 3 : KMI); o -= 2)
+                    row[o / 2] = row[o / 2] - row[(o - 2) / 2];
+                if constexpr (KMI == 1) row[0] = row[0] - dydx_i;    /* offset 1 */
+            } else {
+#pragma unroll
+                for (int o = R - 2; o >= KMI + 1; o -= 2)
+                    row[o / 2] = row[o / 2] - row[(o - 2) / 2];
+            }
+        }
+        bs_dense_orders<KK, P, KMI + 1>(a, gl, mu, dydx_i, row);
+    }
+}
+
+template <int KK>
+__device__ __forceinline__ void bs_dense_seq(const BsArgs &a, long gl, int i, int k, int mu,
+                                             double dydx_i)
+{
+    if constexpr (KK <= BS_KMAX) {
+        if (KK <= k) {
+            constexpr int R0 = 2 * KK * KK;
+#pragma unroll
+            for (int P = 0; P < 2; P++) {
+                double row[2 * KK + 1];
+#pragma unroll
+                for (int q = 0; q < 2 * KK + 1; q++)
+                    row[q] = DSV(a, DS_FSAVE + (R0 + 2 * q + P) * BS_N + i);
+                if (P == 0) bs_dense_orders<KK, 0, 1>(a, gl, mu, dydx_i, row);
+                else bs_dense_orders<KK, 1, 1>(a, gl, mu, dydx_i, row);
+            }
+        }
+        bs_dense_seq<KK + 1>(a, gl, i, k, mu, dydx_i);
+    }
+}
+
+/* order KMI's dens coefficient of component i: the extrapolation of its
+ * values over sequences (KMI-1)/2 .. k */
+template <int KMI>
+__device__ __forceinline__ void bs_dense_coeffs(const BsArgs &a, long gl, int i, int k, int mu,
+                                                double h)
+{
+    if constexpr (KMI <= 2 * BS_KMAX - 1) {
+        if (KMI > mu) return;
+        constexpr int KB = (KMI - 1) / 2;
+        double y[BS_KMAX + 1];
+#pragma unroll
+        for (int kk = KB; kk <= BS_KMAX; kk++) {
+            if (kk > k) break;
+            y[kk] = DSV(a, DS_YV + (KMI - 1) * (BS_KMAX + 1) + kk);
+        }
+        DSV(a, DS_DENS + (KMI + 4) * BS_N + i) = bs_dense_extrap<KB>(a, y, k) * h;
+        bs_dense_coeffs<KMI + 1>(a, gl, i, k, mu, h);
     }
 }
 
 /* prepare_dense, src/unused/StepperBS.c:526-615: the dense-output values of
  * the accepted step of size h (start yv, end ycur, derivatives dydx and
  * dydxnew), extrapolated from the midpoint values and derivative differences
- * of sequences 0..k; returns the interpolation error estimate */
+ * of sequences 0..k; returns the interpolation error estimate.  The
+ * reference updates its ysave and fsave arrays in place order by order;
+ * here each component's values go through the same operations in
+ * registers, a sequence at a time, and the values each order needs pass
+ * through the YV slots: the same results with a few hundred scratch
+ * accesses per step instead of thousands of dependent ones. */
 __device__ __forceinline__ double bs_prepare_dense(const BsArgs &a, const BsLane &L, long gl,
                                                    const double *dydxnew)
 {
@@ -487,55 +614,19 @@ __device__ __forceinline__ double bs_prepare_dense(const BsArgs &a, const BsLane
         DSV(a, DS_DENS + 2 * n + i) = L.ycur[i];
         DSV(a, DS_DENS + 3 * n + i) = dydxnew[i] * h;
     }
-#define YS(r, i) DSV(a, DS_YSAVE + (r) * BS_N + (i))
-#define FS(r, i) DSV(a, DS_FSAVE + (r) * BS_N + (i))
-    for (int j = 1; j <= k; j++) {
-        double dblenj = a.c.nseq[j];
-        for (int l = j; l >= 1; l--) {
-            double r = dblenj / a.c.nseq[l - 1];
-            double factor = r * r - 1.0;
-            for (int i = 0; i < n; i++) YS(l - 1, i) = YS(l, i) + (YS(l, i) - YS(l - 1, i)) / factor;
+    for (int i = 0; i < n; i++) {
+        /* the midpoint values, extrapolated */
+        double y[BS_KMAX + 1];
+#pragma unroll
+        for (int kk = 0; kk <= BS_KMAX; kk++) {
+            if (kk > k) break;
+            y[kk] = DSV(a, DS_YSAVE + kk * BS_N + i);
         }
+        DSV(a, DS_DENS + 4 * n + i) = bs_dense_extrap<0>(a, y, k);
+        /* the derivative differences, order by order */
+        bs_dense_seq<0>(a, gl, i, k, mu, L.dydx[i]);
+        bs_dense_coeffs<1>(a, gl, i, k, mu, h);
     }
-    for (int i = 0; i < n; i++) DSV(a, DS_DENS + 4 * n + i) = YS(0, i);
-    for (int kmi = 1; kmi <= mu; kmi++) {
-        int kbeg = (kmi - 1) / 2;
-        for (int kk = kbeg; kk <= k; kk++) {
-            double facnj = tp_powi(a.c.nseq[kk] / 2.0, kmi - 1);
-            int ipt = a.c.ipoint[kk + 1] - 2 * kk + kmi - 3;
-            for (int i = 0; i < n; i++) YS(kk, i) = FS(ipt, i) * facnj;
-        }
-        for (int j = kbeg + 1; j <= k; j++) {
-            double dblenj = a.c.nseq[j];
-            for (int l = j; l >= kbeg + 1; l--) {
-                double r = dblenj / a.c.nseq[l - 1];
-                double factor = r * r - 1.0;
-                for (int i = 0; i < n; i++)
-                    YS(l - 1, i) = YS(l, i) + (YS(l, i) - YS(l - 1, i)) / factor;
-            }
-        }
-        for (int i = 0; i < n; i++) DSV(a, DS_DENS + (kmi + 4) * n + i) = YS(kbeg, i) * h;
-        if (kmi == mu) continue;
-        for (int kk = kmi / 2; kk <= k; kk++) {
-            int lbeg = a.c.ipoint[kk + 1] - 1;
-            int lend = a.c.ipoint[kk] + kmi;
-            if (kmi == 1) lend += 2;
-            for (int l = lbeg; l >= lend; l -= 2)
-                for (int i = 0; i < n; i++) FS(l, i) = FS(l, i) - FS(l - 2, i);
-            if (kmi == 1) {
-                int l = lend - 2;
-                for (int i = 0; i < n; i++) FS(l, i) = FS(l, i) - L.dydx[i];
-            }
-        }
-        for (int kk = kmi / 2; kk <= k; kk++) {
-            int lbeg = a.c.ipoint[kk + 1] - 2;
-            int lend = a.c.ipoint[kk] + kmi + 1;
-            for (int l = lbeg; l >= lend; l -= 2)
-                for (int i = 0; i < n; i++) FS(l, i) = FS(l, i) - FS(l - 2, i);
-        }
-    }
-#undef YS
-#undef FS
     bs_dense_interp(a, gl, mu);
     double error = 0.0;
     if (mu >= 1) {
@@ -656,6 +747,20 @@ template <bool DENSE>
 __device__ __forceinline__ void bs_accept(const BsArgs &a, BsLane &L, const double *f, long gl)
 {
     if (DENSE) {
+        /* dydxnew waits in ym (free after the step) for the lane's turn */
+        for (int i = 0; i < BS_N; i++) L.ym[i] = f[i];
+        L.phase = PH_PREP;
+        return;
+    }
+    bs_accept_finish<DENSE>(a, L, f, gl);
+}
+
+/* the dense form's rest of the accepted step, with dydxnew in ym */
+template <bool DENSE>
+__device__ __forceinline__ void bs_prep(const BsArgs &a, BsLane &L, long gl)
+{
+    const double *f = L.ym;
+    {
         const double err = bs_prepare_dense(a, L, gl, f);
         const double pe = tp_pow(err, 1.0 / (2 * L.k + 3));
         L.hopt_int = L.h / (pe > 0.01 ? pe : 0.01);
@@ -678,7 +783,7 @@ __device__ __forceinline__ void bs_accept(const BsArgs &a, BsLane &L, const doub
             const double uz = bs_dense_out(a, L, gl, 5, tf);
             atomicAdd(&a.espect[L.nf * a.nbins + bs_energy_bin(a, ux, uy, uz)], 1u);
         }
-        for (int i = 0; i < BS_N; i++) L.ym[i] = f[i];   /* dydxnew, while D rows run */
+        /* ym keeps dydxnew while the D rows run */
         if (bs_next_drow<DENSE>(a, L, gl)) return;
     }
     bs_accept_finish<DENSE>(a, L, f, gl);
@@ -764,7 +869,7 @@ __global__ void __launch_bounds__(256) k_bs(BsArgs a)
          * wavefront (retired lanes evaluate a dummy point) so the field sum
          * stays in uniform control flow and its uniform table reads become
          * scalar loads */
-        const bool act = L.pi >= 0;
+        const bool act = L.pi >= 0 && !(DENSE && L.phase == PH_PREP);
         double ze[BS_N];
         for (int i = 0; i < BS_N; i++)
             ze[i] = !act ? 0.0
@@ -772,11 +877,9 @@ __global__ void __launch_bounds__(256) k_bs(BsArgs a)
                                                : (L.phase == PH_ACCEPT ? L.ycur[i] : L.yn[i]));
         Emf e;
         get_emf64<FK>(a.f, ze[0], ze[1], ze[2], act ? L.xe : 0.0, e);
-        if (act) {
-            if (DENSE && L.phase == PH_DROW) {
-                bs_drow<DENSE>(a, L, e, gl);
-                continue;
-            }
+        if (act && DENSE && L.phase == PH_DROW) {
+            bs_drow<DENSE>(a, L, e, gl);
+        } else if (act) {
             double f[BS_N];
             bs_deriv(e, a.qm, ze, f);
             L.rhs++;
@@ -816,6 +919,15 @@ __global__ void __launch_bounds__(256) k_bs(BsArgs a)
                 }
             } else {
                 bs_accept<DENSE>(a, L, f, gl);
+            }
+        }
+        if (DENSE) {
+            const unsigned long long pend = __ballot(L.pi >= 0 && L.phase == PH_PREP);
+            if (pend) {
+                const unsigned long long busy = __ballot(L.pi >= 0 && L.phase != PH_PREP);
+                if ((__popcll(pend) >= BS_PREP_BATCH || busy == 0ull) && L.pi >= 0 &&
+                    L.phase == PH_PREP)
+                    bs_prep<DENSE>(a, L, gl);
             }
         }
     }

@@ -139,6 +139,39 @@ __device__ __forceinline__ void dterms_reduce(KArgs *k, const double v[8], bool 
     }
 }
 
+/* The particle's position shifts (xs, ys, zs) are read only when a wrap
+ * first changes them, or when the D terms or a trajectory record need
+ * them, and written back only when a wrap changed them: most particles
+ * cross no box face during a launch, so this saves 48 of the 176 bytes of
+ * state a launch reads and writes per particle.  The wraps themselves are
+ * applied to the loaded values in the same order (bit-identical shifts). */
+__device__ __forceinline__ void load_shifts(int i, double &xs, double &ys, double &zs, bool &have)
+{
+    if (!have) {
+        KArgs *k = kargs_again();
+        xs = k->xs[i]; ys = k->ys[i]; zs = k->zs[i];
+        have = true;
+    }
+}
+
+/* particle_bc (src/tracking.c:1172-1221) with the shifts read on the first
+ * wrap */
+__device__ __forceinline__ void particle_bc_lazy(const TrackArgs &a, Ptl &p, int i, double &xs,
+                                                 double &ys, double &zs, bool &have,
+                                                 bool &wrapped, int &esc)
+{
+    if (a.bc == 0) {
+        if (p.x > a.hi[0] || p.y > a.hi[1] || p.z > a.hi[2] || p.x < a.lo[0] ||
+            p.y < a.lo[1] || p.z < a.lo[2]) {
+            load_shifts(i, xs, ys, zs, have);
+            wrapped = true;
+            particle_bc(a, p, xs, ys, zs, esc);
+        }
+    } else {
+        particle_bc(a, p, xs, ys, zs, esc);
+    }
+}
+
 template <int PUSH, int FK, int PREC, bool TRAJ>
 __device__ __forceinline__ void track_body(const TrackArgs &a)
 {
@@ -151,18 +184,20 @@ __device__ __forceinline__ void track_body(const TrackArgs &a)
 
     Ptl p;
     double xs = 0.0, ys = 0.0, zs = 0.0;
-    int esc = 1, nst = 0;
+    bool have_shift = false, wrapped = false;
+    int esc = 1, esc0 = 1, nst = 0;
     if (live) {
         p.x = a.x[i]; p.y = a.y[i]; p.z = a.z[i];
         p.ux = a.ux[i]; p.uy = a.uy[i]; p.uz = a.uz[i]; p.t = a.t[i];
-        xs = a.xs[i]; ys = a.ys[i]; zs = a.zs[i];
+        if (TRAJ) load_shifts(i, xs, ys, zs, have_shift);
         esc = a.esc[i];
+        esc0 = esc;
         nst = a.nsteps[i];
         if (a.first) {
             /* initial wrap and escape check, src/tracking.c:314-332 */
             esc = 0;
             nst = 1;
-            particle_bc(a, p, xs, ys, zs, esc);
+            particle_bc_lazy(a, p, i, xs, ys, zs, have_shift, wrapped, esc);
             if (esc && 1 < a.nd) atomicAdd(&a.rdec[1], 1);
         }
     } else {
@@ -189,7 +224,7 @@ __device__ __forceinline__ void track_body(const TrackArgs &a)
                 boris_step32<FK>(a.f, gc, p, fux, fuy, fuz, a.dt, dtf, hf);
                 p.ux = fux; p.uy = fuy; p.uz = fuz;
             }
-            particle_bc(a, p, xs, ys, zs, esc_after);
+            particle_bc_lazy(a, p, i, xs, ys, zs, have_shift, wrapped, esc_after);
             nst++;
             /* src/tracking.c:440-447: the particle after the step, every
              * nsteps_output steps; slots past the particle's range are
@@ -239,6 +274,7 @@ __device__ __forceinline__ void track_body(const TrackArgs &a)
         KArgs *k = kargs_again();
         double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         if (live) {
+            load_shifts(i, xs, ys, zs, have_shift);
             Emf e = {};
             if (esc == 0) {
                 if constexpr (PREC == PREC64)
@@ -260,8 +296,8 @@ __device__ __forceinline__ void track_body(const TrackArgs &a)
         KArgs *k = kargs_again();
         k->x[i] = p.x; k->y[i] = p.y; k->z[i] = p.z;
         k->ux[i] = p.ux; k->uy[i] = p.uy; k->uz[i] = p.uz; k->t[i] = p.t;
-        k->xs[i] = xs; k->ys[i] = ys; k->zs[i] = zs;
-        k->esc[i] = esc;
+        if (wrapped) { k->xs[i] = xs; k->ys[i] = ys; k->zs[i] = zs; }
+        if (esc != esc0) k->esc[i] = esc;
         k->nsteps[i] = nst;
         if (TRAJ) k->traj_pts[i] = tpts;
     }
