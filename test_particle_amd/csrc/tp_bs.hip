@@ -1209,7 +1209,7 @@ int tp_gpu_run_adaptive_diag(tp_ctx *c, const tp_bs_params *p, const tp_track_pa
      * tp_gpu_read_diagnostics), the scratch of the dense form */
     const size_t need = (size_t)q->nbins * q->nt_out;
     if (need > c->diag_cap) {
-        hipFree(c->espect); hipFree(c->espect_esc);
+        (void)hipFree(c->espect); (void)hipFree(c->espect_esc);
         c->espect = nullptr; c->espect_esc = nullptr; c->diag_cap = 0;
         if (hipMalloc(&c->espect, need * sizeof(unsigned)) != hipSuccess ||
             hipMalloc(&c->espect_esc, need * sizeof(unsigned)) != hipSuccess)
@@ -1217,7 +1217,7 @@ int tp_gpu_run_adaptive_diag(tp_ctx *c, const tp_bs_params *p, const tp_track_pa
         c->diag_cap = need;
     }
     if ((size_t)nd > c->dco_cap) {
-        hipFree(c->dco); hipFree(c->rdec);
+        (void)hipFree(c->dco); (void)hipFree(c->rdec);
         c->dco = nullptr; c->rdec = nullptr; c->dco_cap = 0;
         if (hipMalloc(&c->dco, sizeof(double) * 8 * nd) != hipSuccess ||
             hipMalloc(&c->rdec, sizeof(int) * (nd + 1)) != hipSuccess)

@@ -686,15 +686,15 @@ int tp_gpu_init(int device, tp_ctx **out)
 
 static void free_field(tp_ctx *c)
 {
-    hipFree(c->grid); c->grid = nullptr;
-    hipFree(c->grid_b); c->grid_b = nullptr;
-    hipFree(c->cells); c->cells = nullptr;
-    hipFree(c->cells_b); c->cells_b = nullptr;
-    hipFree(c->vgrid); c->vgrid = nullptr;
-    hipFree(c->modes); c->modes = nullptr;
-    hipFree(c->modes32); c->modes32 = nullptr;
-    hipFree(c->wl); c->wl = nullptr;
-    hipFree(c->gridd); c->gridd = nullptr;
+    (void)hipFree(c->grid); c->grid = nullptr;
+    (void)hipFree(c->grid_b); c->grid_b = nullptr;
+    (void)hipFree(c->cells); c->cells = nullptr;
+    (void)hipFree(c->cells_b); c->cells_b = nullptr;
+    (void)hipFree(c->vgrid); c->vgrid = nullptr;
+    (void)hipFree(c->modes); c->modes = nullptr;
+    (void)hipFree(c->modes32); c->modes32 = nullptr;
+    (void)hipFree(c->wl); c->wl = nullptr;
+    (void)hipFree(c->gridd); c->gridd = nullptr;
     c->have_field = false;
     c->s.derived = false;
 }
@@ -706,30 +706,30 @@ void staging_release(tp_ctx *c);
 void tp_gpu_finalize(tp_ctx *c)
 {
     if (!c) return;
-    hipSetDevice(c->device);
-    if (c->stream) hipStreamSynchronize(c->stream);
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
     free_field(c);
-    hipFree(c->s.buf);
-    hipFree(c->s.ibuf);
-    hipFree(c->s.stage);
-    hipFree(c->espect);
-    hipFree(c->espect_esc);
-    hipFree(c->dco);
-    hipFree(c->partials);
-    hipFree(c->rdec);
-    hipFree(c->live_idx);
-    for (int k = 0; k < c->nev; k++) hipEventDestroy(c->evs[k]);
+    (void)hipFree(c->s.buf);
+    (void)hipFree(c->s.ibuf);
+    (void)hipFree(c->s.stage);
+    (void)hipFree(c->espect);
+    (void)hipFree(c->espect_esc);
+    (void)hipFree(c->dco);
+    (void)hipFree(c->partials);
+    (void)hipFree(c->rdec);
+    (void)hipFree(c->live_idx);
+    for (int k = 0; k < c->nev; k++) (void)hipEventDestroy(c->evs[k]);
     free(c->evs);
     tp_bs_release(c);
     tp_mlp_release(c);
-    hipFree(c->traj);
-    hipFree(c->traj_accum);
-    hipFree(c->traj_pts);
+    (void)hipFree(c->traj);
+    (void)hipFree(c->traj_accum);
+    (void)hipFree(c->traj_pts);
     free(c->bs_tune_block);
     staging_release(c);
-    if (c->ev0) hipEventDestroy(c->ev0);
-    if (c->ev1) hipEventDestroy(c->ev1);
-    if (c->stream) hipStreamDestroy(c->stream);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -813,7 +813,7 @@ int tp_gpu_set_velocity_grid(tp_ctx *c, int nx, int ny, int nz, const double lo[
 {
     if (!c) return TP_E_ARG;
     HIPCHK(hipSetDevice(c->device));
-    hipFree(c->vgrid);
+    (void)hipFree(c->vgrid);
     c->vgrid = nullptr;
     c->f.vgrid = nullptr;
     if (c->f.kind == FK_ABCV) c->f.kind = FK_ABC;
@@ -971,7 +971,7 @@ static int build_cells(tp_ctx *c)
         float4 **dst = s == 0 ? &c->cells : &c->cells_b;
         if (hipMalloc((void **)dst, bytes) != hipSuccess) {
             *dst = nullptr;
-            hipFree(c->cells); c->cells = nullptr;
+            (void)hipFree(c->cells); c->cells = nullptr;
             return c->cells_mode == TP_CELLS_ON ? TP_E_NOMEM : TP_OK;
         }
         const size_t want = (nn + 255) / 256;
@@ -1007,9 +1007,9 @@ struct Staging {
 void staging_release(tp_ctx *c)
 {
     for (int b = 0; b < 2; b++) {
-        if (c->stage_pin[b]) hipHostFree(c->stage_pin[b]);
-        hipFree(c->stage_dev[b]);
-        if (c->stage_ev[b]) hipEventDestroy(c->stage_ev[b]);
+        if (c->stage_pin[b]) (void)hipHostFree(c->stage_pin[b]);
+        (void)hipFree(c->stage_dev[b]);
+        if (c->stage_ev[b]) (void)hipEventDestroy(c->stage_ev[b]);
         c->stage_pin[b] = c->stage_dev[b] = nullptr;
         c->stage_ev[b] = nullptr;
     }
@@ -1436,7 +1436,7 @@ int tp_gpu_eval_field(tp_ctx *c, int n, const double *pts, double *out, int prec
     HIPCHK(hipSetDevice(c->device));
     double *dp = nullptr, *dout = nullptr;
     if (hipMalloc(&dp, sizeof(double) * 4 * n) != hipSuccess) return TP_E_NOMEM;
-    if (hipMalloc(&dout, sizeof(double) * 6 * n) != hipSuccess) { hipFree(dp); return TP_E_NOMEM; }
+    if (hipMalloc(&dout, sizeof(double) * 6 * n) != hipSuccess) { (void)hipFree(dp); return TP_E_NOMEM; }
     int rc = TP_OK;
     if (hipMemcpy(dp, pts, sizeof(double) * 4 * n, hipMemcpyHostToDevice) != hipSuccess) rc = TP_E_HIP;
     if (rc == TP_OK) {
@@ -1448,8 +1448,8 @@ int tp_gpu_eval_field(tp_ctx *c, int n, const double *pts, double *out, int prec
     if (rc == TP_OK &&
         hipMemcpy(out, dout, sizeof(double) * 6 * n, hipMemcpyDeviceToHost) != hipSuccess)
         rc = TP_E_HIP;
-    hipFree(dp);
-    hipFree(dout);
+    (void)hipFree(dp);
+    (void)hipFree(dout);
     return rc;
 }
 
@@ -1457,7 +1457,7 @@ static int ensure_state(tp_ctx *c, int n)
 {
     State &S = c->s;
     if (n <= S.cap) { S.n = n; return TP_OK; }
-    hipFree(S.buf); hipFree(S.ibuf); hipFree(S.stage);
+    (void)hipFree(S.buf); (void)hipFree(S.ibuf); (void)hipFree(S.stage);
     S.buf = nullptr; S.ibuf = nullptr; S.stage = nullptr; S.cap = 0;
     size_t cap = (size_t)n;
     if (hipMalloc(&S.buf, sizeof(double) * 25 * cap) != hipSuccess) { S.buf = nullptr; return TP_E_NOMEM; }
@@ -1824,7 +1824,7 @@ int tp_gpu_run_fixed(tp_ctx *c, const tp_track_params *p)
     /* diagnostics buffers */
     size_t need = (size_t)p->nbins * p->nt_out;
     if (need > c->diag_cap) {
-        hipFree(c->espect); hipFree(c->espect_esc);
+        (void)hipFree(c->espect); (void)hipFree(c->espect_esc);
         c->espect = nullptr; c->espect_esc = nullptr; c->diag_cap = 0;
         if (hipMalloc(&c->espect, need * sizeof(unsigned)) != hipSuccess ||
             hipMalloc(&c->espect_esc, need * sizeof(unsigned)) != hipSuccess)
@@ -1832,7 +1832,7 @@ int tp_gpu_run_fixed(tp_ctx *c, const tp_track_params *p)
         c->diag_cap = need;
     }
     if ((size_t)nd > c->dco_cap) {
-        hipFree(c->dco); hipFree(c->rdec);
+        (void)hipFree(c->dco); (void)hipFree(c->rdec);
         c->dco = nullptr; c->rdec = nullptr; c->dco_cap = 0;
         if (hipMalloc(&c->dco, sizeof(double) * 8 * nd) != hipSuccess ||
             hipMalloc(&c->rdec, sizeof(int) * (nd + 1)) != hipSuccess)
@@ -1842,20 +1842,20 @@ int tp_gpu_run_fixed(tp_ctx *c, const tp_track_params *p)
     /* partial-sum rows and the non-finite flag after them */
     size_t pneed = R.prows * 8 + 1;
     if (pneed > c->partials_cap) {
-        hipFree(c->partials);
+        (void)hipFree(c->partials);
         c->partials = nullptr; c->partials_cap = 0;
         if (hipMalloc(&c->partials, pneed * sizeof(double)) != hipSuccess) return TP_E_NOMEM;
         c->partials_cap = pneed;
     }
     const bool compact = p->bc_flag == 1 && !mlp && n > 0;
     if (compact && R.live_ints > c->live_cap) {
-        hipFree(c->live_idx);
+        (void)hipFree(c->live_idx);
         c->live_idx = nullptr; c->live_cap = 0;
         if (hipMalloc(&c->live_idx, R.live_ints * sizeof(int)) != hipSuccess) return TP_E_NOMEM;
         c->live_cap = R.live_ints;
     }
     if (2 * R.nlaunch > c->nev) {      /* two events per launch */
-        for (int k = 0; k < c->nev; k++) hipEventDestroy(c->evs[k]);
+        for (int k = 0; k < c->nev; k++) (void)hipEventDestroy(c->evs[k]);
         free(c->evs);
         c->nev = 0;
         c->evs = (hipEvent_t *)malloc(sizeof(hipEvent_t) * 2 * R.nlaunch);
@@ -2044,7 +2044,7 @@ int tp_gpu_track_fixed_traj(tp_ctx *c, const tp_track_params *p, int n, tp_parti
     const size_t ntraj = n > 0 ? (size_t)ntraj_accum[n - 1] : 0;
     HIPCHK(hipSetDevice(c->device));
     if (ntraj > c->traj_cap || n > c->traj_ncap) {
-        hipFree(c->traj); hipFree(c->traj_accum); hipFree(c->traj_pts);
+        (void)hipFree(c->traj); (void)hipFree(c->traj_accum); (void)hipFree(c->traj_pts);
         c->traj = nullptr; c->traj_accum = nullptr; c->traj_pts = nullptr;
         c->traj_cap = 0; c->traj_ncap = 0;
         const size_t t = ntraj > 0 ? ntraj : 1, k = n > 0 ? (size_t)n : 1;

@@ -890,7 +890,7 @@ void tp_mlp_release(tp_ctx *c)
                     m->h0, m->h1,
                     m->h2, m->dz1, m->dz2, m->dz3, m->g1, m->g2, m->g3, m->h3, m->dout,
                     m->lossp, m->lacc, m->part, m->grad, m->dsx, m->dsy};
-    for (void *p : ptrs) hipFree(p);
+    for (void *p : ptrs) (void)hipFree(p);
     delete m;
     c->mlp = nullptr;
 }
@@ -932,7 +932,7 @@ static int ensure_rows(MlpState *m, long rows)
     if (rows <= m->rows_cap) return TP_OK;
     void *old[] = {m->h0, m->h1, m->h2, m->dz1, m->dz2, m->dz3, m->g1, m->g2, m->g3,
                    m->h3, m->dout, m->lossp, m->part};
-    for (void *p : old) hipFree(p);
+    for (void *p : old) (void)hipFree(p);
     m->rows_cap = 0;
     const size_t r = (size_t)rows;
     int rc = TP_OK;
@@ -957,7 +957,7 @@ static int ensure_rows(MlpState *m, long rows)
 static int ensure_ds(MlpState *m, long n)
 {
     if (n <= m->ds_cap) return TP_OK;
-    hipFree(m->dsx); hipFree(m->dsy);
+    (void)hipFree(m->dsx); (void)hipFree(m->dsy);
     m->ds_cap = 0;
     if (dalloc((void **)&m->dsx, (size_t)n * 12) || dalloc((void **)&m->dsy, (size_t)n * 12))
         return TP_E_NOMEM;
@@ -1063,7 +1063,7 @@ int tp_gpu_mlp_forward(tp_ctx *c, int n, const double *x, double *out)
     HIPCHK(hipSetDevice(c->device));
     double *dx = nullptr, *dout = nullptr;
     if (dalloc((void **)&dx, (size_t)n * 24) || dalloc((void **)&dout, (size_t)n * 24)) {
-        hipFree(dx);
+        (void)hipFree(dx);
         return TP_E_NOMEM;
     }
     int rc = TP_OK;
@@ -1078,8 +1078,8 @@ int tp_gpu_mlp_forward(tp_ctx *c, int n, const double *x, double *out)
             hipStreamSynchronize(c->stream) != hipSuccess)
             rc = TP_E_HIP;
     }
-    hipFree(dx);
-    hipFree(dout);
+    (void)hipFree(dx);
+    (void)hipFree(dout);
     return rc;
 }
 
@@ -1237,8 +1237,8 @@ int tp_mlp_prepare_run(tp_ctx *c, int n)
     MlpState *m = c->mlp;
     if (!m || !m->set) return TP_E_STATE;
     if (n > m->fb_cap) {
-        hipFree(m->fb);
-        hipFree(m->idx);
+        (void)hipFree(m->fb);
+        (void)hipFree(m->idx);
         m->fb = nullptr;
         m->idx = nullptr;
         m->fb_cap = 0;
