@@ -23,6 +23,13 @@
  *   tp_gpu: particle_tracking_hybrid runs the CPU loop: <reason>
  * With TP_SHIM_REQUIRE_GPU=1 in the environment a call the GPU path does not
  * take aborts the program (MPI_Abort) instead of running the CPU loop.
+ *
+ * The optional "GPU tracking" section of init.dat (tp_shim_config.h) picks
+ * the pusher (Wirz or Boris), the precision (FP64 or the FP32 form), the
+ * non-finite check, the stale-D fix and the adaptive stepper's tolerances,
+ * first step and depth; it is read once, at the first call.  A malformed
+ * section, or a setting other than the reference's Wirz/FP64 loop when the
+ * CPU loop would have to run, aborts the program.
  */
 #include <math.h>
 #include <stdio.h>
@@ -41,6 +48,7 @@
 #include "velocity_field.h"
 #include "wlcs.h"
 #include "tp_gpu.h"
+#include "tp_shim_config.h"
 
 void particle_tracking_hybrid_cpu(int mpi_rank, int nptl, double dt, int nbins,
         int nt_out, int bc_flag, int nsteps_output, double pmass,
@@ -68,6 +76,27 @@ extern double emin, logemin, logde;
 
 static tp_ctx *tp_shim_ctx;
 static int tp_shim_init_rc = 1;   /* 1: not tried yet */
+static tp_shim_config tp_shim_cfg;
+static int tp_shim_cfg_read;
+
+/* the GPU tracking section of init.dat (src/main.c:30 reads the same file),
+ * once; a malformed section stops the program as the reference's parser
+ * does (src/domain.c:93-96) */
+static void tp_shim_read_config(void)
+{
+    char err[256];
+    if (tp_shim_cfg_read) return;
+    tp_shim_cfg_read = 1;
+    if (tp_shim_config_read("init.dat", &tp_shim_cfg, err, sizeof(err)) != 0) {
+        fprintf(stderr, "tp_gpu: %s\n", err);
+        MPI_Abort(MPI_COMM_WORLD, 1);
+    }
+}
+
+static const char *tp_shim_pusher_name(int pusher)
+{
+    return pusher == TP_PUSH_BORIS ? "boris" : "wirz";
+}
 
 /* NULL on success; otherwise why the GPU path does not take this setup */
 static const char *tp_shim_set_field(tp_ctx *c)
@@ -144,6 +173,8 @@ void particle_tracking_hybrid(int mpi_rank, int nptl, double dt, int nbins,
 {
     int rc;
     const char *why = NULL;
+    tp_shim_read_config();
+    const tp_shim_config *cfg = &tp_shim_cfg;
     if (tp_shim_init_rc == 1) tp_shim_init_rc = tp_gpu_init(0, &tp_shim_ctx);
     if (tp_shim_init_rc != TP_OK)
         why = tp_gpu_strerror(tp_shim_init_rc);
@@ -161,6 +192,17 @@ void particle_tracking_hybrid(int mpi_rank, int nptl, double dt, int nbins,
             fprintf(stderr, "tp_gpu: TP_SHIM_REQUIRE_GPU is set; aborting\n");
             MPI_Abort(MPI_COMM_WORLD, 1);
         }
+        /* the reference's loop is the Wirz pusher in FP64 with its stale D
+         * terms (src/tracking.c:429); it cannot honour other settings */
+        if (tracking_method == 0 && (cfg->pusher != TP_PUSH_WIRZ || cfg->precision != TP_FP64 ||
+                                     (cfg->flags & TP_FLAG_FIX_STALE_DCOEFF))) {
+            fprintf(stderr, "tp_gpu: the GPU tracking section of init.dat asks for what the "
+                    "CPU loop does not do (pusher %s, precision %s, stale-D fix %d); "
+                    "aborting\n", tp_shim_pusher_name(cfg->pusher),
+                    cfg->precision == TP_FP64 ? "fp64" : "fp32",
+                    (cfg->flags & TP_FLAG_FIX_STALE_DCOEFF) ? 1 : 0);
+            MPI_Abort(MPI_COMM_WORLD, 1);
+        }
         particle_tracking_hybrid_cpu(mpi_rank, nptl, dt, nbins, nt_out, bc_flag,
                 nsteps_output, pmass, ntraj_accum, tracking_method, traj_diagnose,
                 ptl_init, nsteps_ptl_tracking, ptl, ptl_time);
@@ -169,12 +211,20 @@ void particle_tracking_hybrid(int mpi_rank, int nptl, double dt, int nbins,
     if (mpi_rank == 0) {
         char dev[128] = "";
         tp_gpu_device_info(tp_shim_ctx, dev, (int)sizeof(dev), NULL);
-        fprintf(stderr, "tp_gpu: particle_tracking_hybrid on the GPU (%s; system %d, "
-                "%d particles, traj_diagnose %d)\n", dev, system_type, nptl, traj_diagnose);
+        if (tracking_method == 1)
+            fprintf(stderr, "tp_gpu: particle_tracking_hybrid on the GPU (%s; system %d, "
+                    "%d particles, adaptive, rtol %g, atol %g, first step %g, kmax %d)\n",
+                    dev, system_type, nptl, cfg->bs_rtol, cfg->bs_atol, cfg->bs_h0,
+                    cfg->bs_kmax);
+        else
+            fprintf(stderr, "tp_gpu: particle_tracking_hybrid on the GPU (%s; system %d, "
+                    "%d particles, traj_diagnose %d, %s, %s, flags %d)\n", dev, system_type,
+                    nptl, traj_diagnose, tp_shim_pusher_name(cfg->pusher),
+                    cfg->precision == TP_FP64 ? "fp64" : "fp32", cfg->flags);
     }
     tp_track_params p;
-    p.pusher = TP_PUSH_WIRZ;
-    p.precision = TP_FP64;
+    p.pusher = cfg->pusher;
+    p.precision = cfg->precision;
     p.bc_flag = bc_flag;
     p.nbins = nbins;
     p.nt_out = nt_out;
@@ -182,7 +232,7 @@ void particle_tracking_hybrid(int mpi_rank, int nptl, double dt, int nbins,
     p.estep = simul_domain.tmax / dt;              /* src/tracking.c:103 */
     p.einterval_t = p.estep / nt_out;              /* src/tracking.c:104 */
     p.dcoeff_interval = 100;                       /* src/tracking.c:34 */
-    p.flags = 0;
+    p.flags = cfg->flags;
     p.dt = dt;
     p.qm = charge_mass;
     p.xmin = simul_domain.xmin; p.xmax = simul_domain.xmax;
@@ -200,19 +250,22 @@ void particle_tracking_hybrid(int mpi_rank, int nptl, double dt, int nbins,
     if (tracking_method == 1) {
         /* the adaptive loop (particle_tracking_adaptive, commented out in
          * src/tracking.c:518-630; its branch at :680-683 is empty): the
-         * Bulirsch-Stoer stepper with dense output and the reference's
-         * constants (KMAXX 8, atol 1e-5, rtol 1e-14, include/StepperBS.h:
-         * 18-22; first step 1e-5, src/tracking.c:566), spectra at the
-         * fixed-step loop's output times; like the reference's adaptive
-         * branch it writes no diffusion file */
+         * Bulirsch-Stoer stepper with dense output, by default with the
+         * reference's constants (KMAXX 8, atol 1e-5, rtol 1e-14,
+         * include/StepperBS.h:18-22; first step 1e-5, src/tracking.c:566),
+         * spectra at the fixed-step loop's output times; like the
+         * reference's adaptive branch it writes no diffusion file.  It
+         * integrates the equations of motion in FP64; the pusher and
+         * precision settings do not apply to it. */
         tp_bs_params b;
         memset(&b, 0, sizeof(b));
-        b.rtol = 1.0e-14;
-        b.atol = 1.0e-5;
+        b.rtol = cfg->bs_rtol;
+        b.atol = cfg->bs_atol;
         b.qm = charge_mass;
         b.b_ref = 1.0;
-        b.h0 = 1.0e-5;
-        b.kmax = 8;
+        b.h0 = cfg->bs_h0;
+        b.kmax = cfg->bs_kmax;
+        p.precision = TP_FP64;
         rc = tp_gpu_load_particles(tp_shim_ctx, nptl, (const tp_particle *)ptl,
                                    (const tp_particle *)ptl_init);
         if (rc == TP_OK) rc = tp_gpu_run_adaptive_diag(tp_shim_ctx, &b, &p, NULL);

@@ -17,6 +17,7 @@ import subprocess
 import numpy as np
 import pytest
 
+from test_particle_amd import abi
 from test_particle_amd.particles import init_particles
 
 pytestmark = pytest.mark.gpu
@@ -88,6 +89,67 @@ def test_dropin_pic_grid(tmp_path):
     cpu, gpu = _run_both(str(tmp_path), "ref_harness", "ref_harness_gpu", spec, ptl)
     assert set(cpu) == {"diff_coeffs.dat", "espectrum.dat"}
     assert cpu == gpu
+
+
+GPU_SECTION = "------------------------------- GPU tracking ---------------------------------\n"
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_dropin_gpu_section_boris(tmp_path, oracle_tp, precision):
+    """init.dat's GPU tracking section picks the Boris pusher (and the FP32
+    form) for the reference program's tracking loop: the drop-in's particles
+    are, bit for bit, the library's run with those settings (FP64: also the
+    oracle's Boris run), so the section reaches the kernels; the FP32 form's
+    differ from the FP64 form's."""
+    from test_particle_amd.gpu import Tracker
+    _need("ref_harness_gpu")
+    rng = np.random.default_rng(71)
+    nz, ny, nx = 9, 8, 10
+    lo, hi = (0.0, 0.0, 0.0), (9.0, 7.0, 8.0)
+    g = rng.normal(0, 1, (2, nz, ny, nx, 3)).astype(np.float32)
+    g[1, ..., 2] += 2.0
+    g.tofile(os.path.join(tmp_path, "grid.bin"))
+    ptl = init_particles(300, lo, hi, 0.3, seed=72)
+    steps, dt = 400, 0.02
+    spec = dict(system=4, nx=nx, ny=ny, nz=nz, tmax=steps * dt + 1e-9, dt=dt, bc=0,
+                nbins=60, nt_out=4, emin=1e-3, emax=1.0,
+                grid=os.path.join(tmp_path, "grid.bin"),
+                xmin=lo[0], ymin=lo[1], zmin=lo[2], xmax=hi[0], ymax=hi[1], zmax=hi[2])
+    d = os.path.join(str(tmp_path), "run")
+    os.makedirs(os.path.join(d, "data"))
+    _spec(os.path.join(d, "spec.txt"), **spec)
+    ptl.tofile(os.path.join(d, "p.bin"))
+    with open(os.path.join(d, "init.dat"), "w") as f:
+        f.write("Run type (1: test particle simulation 2: Poincare map): 1\n" + GPU_SECTION +
+                f"Pusher (wirz or boris): boris\nPrecision (fp64 or fp32): {precision}\n" +
+                "-" * 78 + "\n")
+    print(f"[dropin] ref_harness_gpu boris {precision}", flush=True)
+    r = _run("ref_harness_gpu", ["hybrid", "spec.txt", "p.bin", "out.bin"], d)
+    assert f"boris, {precision}, flags 0)" in r.stderr, r.stderr[-2000:]
+    out = np.fromfile(os.path.join(d, "out.bin")).reshape(-1, 11)
+
+    def params(prec):
+        return abi.make_params(dt=dt, estep=steps, lo=lo, hi=hi, pusher=abi.TP_PUSH_BORIS,
+                               precision=prec, nbins=60, nt_out=4, emin=1e-3, emax=1.0)
+
+    tr = Tracker(0)
+    try:
+        tr.set_field_grid(g[0], g[1], lo, hi, abi.TP_GATHER_REF_COMPAT)
+        runs = {}
+        for name, prec in (("fp64", abi.TP_FP64), ("fp32", abi.TP_FP32_MIXED)):
+            tr.load(ptl)
+            tr.run(params(prec))
+            runs[name] = tr.store()[0]
+    finally:
+        tr.close()
+    assert np.array_equal(out, runs[precision])
+    if precision == "fp64":
+        ref = oracle_tp.track_fixed(oracle_tp.Field.grid(g[0], g[1], lo, hi), params(abi.TP_FP64),
+                                    ptl)
+        assert np.array_equal(out, ref["ptl"])
+    else:
+        assert not np.array_equal(out, runs["fp64"])
+        assert np.median(np.abs(out[:, :3] - runs["fp64"][:, :3])) < 1e-3
 
 
 def test_dropin_mhd_grid(tmp_path):

@@ -34,9 +34,12 @@ def _spec(d):
             f.write(f"{k} {v!r}\n" if isinstance(v, float) else f"{k} {v}\n")
 
 
-def _run(binary, d, **env):
+def _run(binary, d, init=None, **env):
     os.makedirs(os.path.join(d, "data"), exist_ok=True)
     _spec(d)
+    if init is not None:
+        with open(os.path.join(d, "init.dat"), "w") as f:
+            f.write(init)
     return subprocess.run([os.path.join(REF, binary), "hybrid", "spec.txt", "p.bin"], cwd=d,
                           timeout=120, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE,
                           text=True, env=dict(os.environ, OMP_NUM_THREADS="1", **env))
@@ -58,5 +61,43 @@ def test_shim_aborts_when_gpu_required(tmp_path):
     r = _run("ref_harness_gpu", str(tmp_path / "shim"), TP_SHIM_REQUIRE_GPU="1")
     assert r.returncode != 0
     assert "TP_SHIM_REQUIRE_GPU is set; aborting" in r.stderr
+
+
+SECTION = "------------------------------- GPU tracking ---------------------------------\n"
+RULE = "-" * 78 + "\n"
+
+
+def test_shim_gpu_section_with_the_reference_settings_falls_back(tmp_path):
+    """init.dat's GPU tracking section asking for what the reference's loop
+    does (Wirz, FP64): without a GPU the CPU loop runs as before."""
+    init = (SECTION + "Pusher (wirz or boris): wirz\nPrecision (fp64 or fp32): fp64\n"
+            "Adaptive relative tolerance: 1e-12\n" + RULE)
+    ref = _run("ref_harness", str(tmp_path / "cpu"))
+    shim = _run("ref_harness_gpu", str(tmp_path / "shim"), init=init)
+    assert ref.returncode == 0 and shim.returncode == 0, shim.stderr[-2000:]
+    assert "runs the CPU loop: no usable GPU" in shim.stderr
+    for f in ("espectrum.dat", "diff_coeffs.dat"):
+        assert (open(tmp_path / "cpu" / "data" / f).read() ==
+                open(tmp_path / "shim" / "data" / f).read()), f
+
+
+def test_shim_gpu_section_boris_does_not_fall_back(tmp_path):
+    """The reference's loop has no Boris pusher: asked for one, the drop-in
+    without a GPU stops instead of running the Wirz loop."""
+    r = _run("ref_harness_gpu", str(tmp_path / "shim"),
+             init=SECTION + "Pusher (wirz or boris): boris\n" + RULE)
+    assert r.returncode != 0
+    assert "asks for what the CPU loop does not do (pusher boris" in r.stderr
+    assert not os.path.exists(tmp_path / "shim" / "data" / "diff_coeffs.dat")
+
+
+def test_shim_gpu_section_malformed_stops(tmp_path):
+    """A malformed GPU tracking section stops the program with the line and
+    the reason, as the reference's own parser stops on a bad line
+    (src/domain.c:93-96)."""
+    r = _run("ref_harness_gpu", str(tmp_path / "shim"),
+             init="Run type: 1\n" + SECTION + "Precision (fp64 or fp32): fp16\n" + RULE)
+    assert r.returncode != 0
+    assert 'tp_gpu: init.dat line 3: precision "fp16" is neither fp64 nor fp32' in r.stderr
     # the CPU loop did not run: it would have written the diffusion file
     assert not os.path.exists(tmp_path / "shim" / "data" / "diff_coeffs.dat")
