@@ -74,6 +74,10 @@ struct FieldDev {
      * v / d and v * inv_d are the same double for every v, and the FP64
      * forms multiply instead of dividing */
     int exact_inv;
+    /* set by a fixed-step run, not by the field: periodic particles whose
+     * box is the grid's own [lo, hi], so particle_bc keeps every position
+     * in the box between steps (the FP32 forms' cell index uses it) */
+    int in_box;
     const float4 *grid;
     /* two time slices: grid = next slice, grid_b = previous slice, gdt the
      * time between slices */
@@ -356,16 +360,41 @@ __device__ __forceinline__ void ref_cell_index(const FieldDev &f, double x, doub
  * indices, and flat node indices are formed only when it changes: in 32
  * bits when every corner index lies inside the grid, otherwise in 64 bits
  * with the clamp into the array (a clamped cell is never reused). */
+/* One axis of a point that particle_bc has folded into [lo, hi]: the lower
+ * corner is the integer part of the cell coordinate q = (v - lo) / d and the
+ * fraction q - floor(q), which is exact (v_fract); q is clamped into
+ * [0, n - 1] first so that the conversion is defined for any v (a NaN
+ * gives corner 0 and a NaN fraction).  For 0 <= q <= n - 1 these are the
+ * values ref_axis_in gives, and no point in the box needs the wrap cell. */
+__device__ __forceinline__ void ref_axis_box(double v, double lo, double inv_d, int n, int &i1,
+                                             double &fr)
+{
+    const double q = (v - lo) * inv_d;
+    fr = __builtin_amdgcn_fract(q);
+    i1 = (int)fmin(fmax(q, 0.0), (double)(n - 1));
+}
+
 template <bool FAST, int NS>
 __device__ __forceinline__ void ref_cell_cached(const FieldDev &f, GridCacheN<NS> &gc,
                                                 double x, double y, double z, double &fx,
                                                 double &fy, double &fz)
 {
     int i1[3], i2[3];
-    ref_cell_index<FAST>(f, x, y, z, i1, i2, fx, fy, fz);
-    const bool same = gc.valid && i1[0] == gc.key[0] && i2[0] == gc.key[1] &&
-                      i1[1] == gc.key[2] && i2[1] == gc.key[3] && i1[2] == gc.key[4] &&
-                      i2[2] == gc.key[5];
+    bool same;
+    if (FAST && f.in_box) {
+        /* the FP32 forms with periodic particles in the grid's box: the
+         * corners i2 = i1 + 1 follow from i1, which alone keys the cache
+         * (the launch's cache never holds a wrap cell) */
+        ref_axis_box(x, f.lo[0], f.inv_d[0], f.nx, i1[0], fx);
+        ref_axis_box(y, f.lo[1], f.inv_d[1], f.ny, i1[1], fy);
+        ref_axis_box(z, f.lo[2], f.inv_d[2], f.nz, i1[2], fz);
+        i2[0] = i1[0] + 1; i2[1] = i1[1] + 1; i2[2] = i1[2] + 1;
+        same = gc.valid && i1[0] == gc.key[0] && i1[1] == gc.key[2] && i1[2] == gc.key[4];
+    } else {
+        ref_cell_index<FAST>(f, x, y, z, i1, i2, fx, fy, fz);
+        same = gc.valid && i1[0] == gc.key[0] && i2[0] == gc.key[1] && i1[1] == gc.key[2] &&
+               i2[1] == gc.key[3] && i1[2] == gc.key[4] && i2[2] == gc.key[5];
+    }
     if (!same) {
         unsigned c[8];
         bool cl = false;

@@ -1677,6 +1677,10 @@ static int run_fixed_impl(tp_ctx *c, const tp_track_params *p, track_fn fn, bool
     TrackArgs a;
     memset(&a, 0, sizeof(a));
     a.f = c->f;
+    /* periodic particles in the grid's own box stay in it between steps */
+    a.f.in_box = p->bc_flag == 0 && c->f.lo[0] == p->xmin && c->f.hi[0] == p->xmax &&
+                 c->f.lo[1] == p->ymin && c->f.hi[1] == p->ymax && c->f.lo[2] == p->zmin &&
+                 c->f.hi[2] == p->zmax;
     a.n = n; a.ntot = n; a.pboff = 0; a.bc = p->bc_flag; a.nbins = nb; a.nt_out = nt; a.einterval = p->einterval_t;
     a.di = di; a.fix_stale = (p->flags & TP_FLAG_FIX_STALE_DCOEFF) ? 1 : 0;
     a.nd = nd;
