@@ -5,6 +5,8 @@
     python -m test_particle_amd.profile_run copy  [--mib 1024]     # plain copy kernel
     python -m test_particle_amd.profile_run pmap  [--side 512]     # Poincare map, PIC grid
     python -m test_particle_amd.profile_run fieldload [--side 384] [--h5 DIR]  # PIC field input
+    python -m test_particle_amd.profile_run sursweep [--epochs 4]  # surrogate residual vs
+                                                                # its Fourier frequencies
 
 ``copy`` launches one device-to-device copy kernel (torch's) of a known size;
 its FETCH_SIZE / WRITE_SIZE readings calibrate those counters against bytes.
@@ -64,10 +66,47 @@ def field_load(tr, n, h5dir):
         assert np.array_equal(e2, e) and np.array_equal(b2, b)
 
 
+def sur_sweep(tr, epochs, seed=20261202):
+    """The surrogate of the 1000-mode turbulence (the bench's training: 2^24
+    samples, batch 16384) trained from the same initial weights with
+    Fourier frequencies of several spreads -- Gaussian lattice vectors of
+    standard deviation s, and radii log-uniform up to rmax -- and its
+    residual |B_net - B| / |B| at 65,536 uniform points."""
+    from . import abi, surrogate
+    rows, period = surrogate.surrogate_field(1000, seed=seed)
+    scale = surrogate.output_scale(rows)
+    for cut in (8, 12, 16, 32, 64):
+        print(f"field beyond |n| = {cut}: rms {surrogate.spectral_tail(rows, period, cut):.2e} "
+              f"of |B|", flush=True)
+    xs = np.random.default_rng(seed + 1).uniform(0.0, period, (1 << 16, 3))
+    pts = np.zeros((xs.shape[0], 4))
+    pts[:, :3] = xs
+    tr.set_field_modes(rows, 1.0)
+    exact = tr.eval_field(pts)[:, :3]
+    cases = [(f"gauss s={s:g}", surrogate.fourier_frequencies(period, sigma=s, seed=seed))
+             for s in (2.0, 3.0, 4.0, 6.0, 8.0, 12.0)]
+    cases += [(f"logr rmax={r:g}",
+               surrogate.fourier_frequencies_logradial(period, r, seed=seed))
+              for r in (8.0, 16.0, 32.0, 64.0)]
+    for name, q in cases:
+        reach = np.max(np.linalg.norm(q.astype(np.float64) * period, axis=1))
+        tr.mlp_set(q, surrogate.init_params(seed), scale)
+        st = tr.train_surrogate(abi.make_mlp_train_params(
+            nsamples=1 << 24, period=period, batch=16384, epochs=epochs, lr=2e-3,
+            lr_min=2e-5, seed=seed))
+        bnet = scale * tr.mlp_forward(xs)
+        bnet[:, 2] += 1.0
+        res = np.linalg.norm(bnet - exact, axis=1) / np.linalg.norm(exact, axis=1)
+        print(f"epochs {epochs:2d} {name:15s} max |n| {reach:5.1f} loss {st['last_loss']:.3e} "
+              f"median {np.median(res):.2e} p90 {np.quantile(res, .9):.2e} "
+              f"p99 {np.quantile(res, .99):.2e} below 1e-3 {np.mean(res < 1e-3):.3f}",
+              flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("what", choices=["grid", "abc", "copy", "modes", "surrogate", "pmap",
-                                       "fieldload"])
+                                       "fieldload", "sursweep"])
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--particles", type=int, default=1 << 20)
     ap.add_argument("--pusher", default="boris", choices=["boris", "wirz"])
@@ -150,6 +189,10 @@ def main():
               f"{st['accepted'] / wall:.4e} accepted steps/s, {st['rhs'] / wall:.4e} RHS/s, "
               f"rejected {st['rejected']}, failed {st['failed']}, kernel {st['kernel_ms']:.1f} ms, "
               f"wall {wall:.2f}s", flush=True)
+        tr.close()
+        return
+    if a.what == "sursweep":
+        sur_sweep(tr, a.epochs if a.epochs > 0 else 4)
         tr.close()
         return
     if a.what == "surrogate":

@@ -82,6 +82,31 @@ def fourier_frequencies(period, sigma=4.0, nfreq=NFREQ, seed=0):
     return (n / period).astype(np.float32)
 
 
+def fourier_frequencies_logradial(period, rmax, nfreq=NFREQ, seed=0):
+    """q (nfreq, 3) float32: lattice vectors round(r d) with |r| log-uniform
+    in [1, rmax] and d isotropic (zero vectors redrawn), divided by the
+    period -- frequencies spread over the decades the mode table spans."""
+    rng = np.random.default_rng(seed)
+    out = []
+    while len(out) < nfreq:
+        r = np.exp(rng.uniform(0.0, np.log(rmax)))
+        d = rng.normal(size=3)
+        n = np.rint(r * d / np.linalg.norm(d))
+        if np.any(n != 0):
+            out.append(n)
+    return (np.array(out) / period).astype(np.float32)
+
+
+def spectral_tail(rows, period, cut):
+    """rms |dB| of the modes whose lattice radius |n| = |k| L / 2 pi exceeds
+    cut, relative to the rms |B| (guide field 1): the part of the field a
+    network whose features reach |n| <= cut cannot represent linearly."""
+    rows = np.asarray(rows, dtype=np.float64)
+    n = np.linalg.norm(rows[:, :3], axis=1) * period / (2.0 * np.pi)
+    a2 = np.sum(rows[:, 3:6] ** 2, axis=1)
+    return float(np.sqrt(0.5 * a2[n > cut].sum()) / np.sqrt(1.0 + 0.5 * a2.sum()))
+
+
 def init_params(seed=0):
     """Flat float32 parameters, every weight and bias uniform in
     +-1/sqrt(fan_in) (the default of common deep-learning libraries)."""

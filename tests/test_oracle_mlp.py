@@ -242,3 +242,29 @@ def test_fourier_frequencies_on_lattice(sigma):
     assert np.allclose(n, np.rint(n), atol=1e-3)
     assert np.all(np.any(np.rint(n) != 0, axis=1))
     assert 0.5 * sigma < np.std(np.rint(n)) < 1.5 * sigma
+
+
+def test_logradial_frequencies_on_lattice():
+    q = surrogate.fourier_frequencies_logradial(L_BOX, 16.0, seed=1)
+    n = q.astype(np.float64) * L_BOX
+    assert q.shape == (128, 3)
+    assert np.allclose(n, np.rint(n), atol=1e-3)
+    r = np.linalg.norm(np.rint(n), axis=1)
+    assert np.all(r > 0) and r.max() <= 16.0 + 0.5 * np.sqrt(3.0)
+    # log-uniform radii: about as many below 4 as between 4 and 16
+    assert 0.3 < np.mean(r < 4.0) < 0.7
+
+
+def test_spectral_tail_brackets():
+    """The tail of one mode is its amplitude over |B| when the cut is below
+    its lattice radius and zero above; over the whole table it is the rms
+    fluctuation over |B|."""
+    period = 2 * np.pi
+    rows = np.array([[3.0, 0.0, 0.0, 0.0, 0.3, 0.0, 0.0],
+                     [0.0, 0.0, 10.0, 0.2, 0.0, 0.0, 1.0]])
+    tot = np.sqrt(0.5 * (0.09 + 0.04))
+    bmag = np.sqrt(1.0 + 0.5 * (0.09 + 0.04))
+    assert np.isclose(surrogate.spectral_tail(rows, period, 0.5), tot / bmag, rtol=1e-14)
+    assert np.isclose(surrogate.spectral_tail(rows, period, 5.0), np.sqrt(0.02) / bmag,
+                      rtol=1e-14)
+    assert surrogate.spectral_tail(rows, period, 10.5) == 0.0
