@@ -115,6 +115,8 @@ def main():
     ap.add_argument("--nmodes", type=int, default=1000)
     ap.add_argument("--tgyro", type=float, default=0.05, help="adaptive horizon (gyroperiods)")
     ap.add_argument("--tune", action="store_true")
+    ap.add_argument("--segments", type=int, default=1,
+                    help="modes: runs of --tgyro gyroperiods each, one after the other")
     ap.add_argument("--dense", action="store_true",
                     help="modes: dense output with the fixed-step diagnostics (--steps outputs)")
     ap.add_argument("--block", type=int, default=0)
@@ -182,13 +184,33 @@ def main():
                   f"{st['kernel_ms']:.1f} ms, wall {wall:.2f}s", flush=True)
             tr.close()
             return
-        t0 = time.perf_counter()
-        st = tr.run_adaptive(prm)
-        wall = time.perf_counter() - t0
-        print(f"modes({a.nmodes}): {a.particles} particles to {a.tgyro} T_g: "
-              f"{st['accepted'] / wall:.4e} accepted steps/s, {st['rhs'] / wall:.4e} RHS/s, "
-              f"rejected {st['rejected']}, failed {st['failed']}, kernel {st['kernel_ms']:.1f} ms, "
-              f"wall {wall:.2f}s", flush=True)
+        # --segments K: K runs of --tgyro gyroperiods each, every run taking
+        # the resident particles on from where the last one left them (the
+        # controller restarts from its first trial step in each)
+        tot = {"accepted": 0, "rhs": 0, "rejected": 0, "failed": 0, "wall": 0.0}
+        for seg in range(max(1, a.segments)):
+            t0 = time.perf_counter()
+            st = tr.run_adaptive(prm)
+            wall = time.perf_counter() - t0
+            for k in ("accepted", "rhs", "rejected"):
+                tot[k] += st[k]
+            tot["failed"] = max(tot["failed"], st["failed"])
+            tot["wall"] += wall
+            print(f"modes({a.nmodes}): {a.particles} particles, {a.tgyro} T_g"
+                  f"{f' (segment {seg + 1} of {a.segments})' if a.segments > 1 else ''}: "
+                  f"{st['accepted'] / wall:.4e} accepted steps/s, {st['rhs'] / wall:.4e} RHS/s, "
+                  f"rejected {st['rejected']}, failed {st['failed']}, kernel "
+                  f"{st['kernel_ms']:.1f} ms, wall {wall:.2f}s", flush=True)
+        if a.segments > 1:
+            ptl_end, nst = tr.store()
+            g0 = np.sqrt(1.0 + np.sum(ptl[:, 4:7] ** 2, axis=1))        # ux uy uz
+            g1 = np.sqrt(1.0 + np.sum(ptl_end[:, 4:7] ** 2, axis=1))
+            print(f"modes({a.nmodes}): {a.particles} particles to {a.segments * a.tgyro} T_g: "
+                  f"{tot['accepted'] / tot['wall']:.4e} accepted steps/s, "
+                  f"{tot['rhs'] / tot['wall']:.4e} RHS/s, accepted {tot['accepted']}, "
+                  f"rejected {tot['rejected']}, failed {tot['failed']}, wall "
+                  f"{tot['wall']:.1f}s; max |gamma/gamma0 - 1| {np.max(np.abs(g1 / g0 - 1)):.2e}",
+                  flush=True)
         tr.close()
         return
     if a.what == "sursweep":
