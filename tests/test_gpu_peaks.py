@@ -12,7 +12,11 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.parametrize("name", list(KINDS))
 def test_peak_kernels_run(name):
-    v, ms = measure_peak(KINDS[name], 256 << 20, 2)
+    # the HBM sweeps on 1 GiB buffers: a buffer that fits the 256 MB
+    # Infinity Cache is partly served from it on the second pass, faster
+    # than HBM (8.18 TB/s was measured for a 256 MiB copy)
+    nbytes = (1 << 30) if name.startswith("hbm") else (256 << 20)
+    v, ms = measure_peak(KINDS[name], nbytes, 2)
     peak, _ = PUBLISHED[name]
     print(f"[peak] {name}: {v:.1f} ({ms:.3f} ms)", flush=True)
     assert ms > 0 and 0 < v <= 1.02 * peak
