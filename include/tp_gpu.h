@@ -371,6 +371,23 @@ int tp_gpu_run_adaptive(tp_ctx *ctx, const tp_bs_params *p, tp_bs_stats *stats);
  * steps). */
 int tp_gpu_run_adaptive_diag(tp_ctx *ctx, const tp_bs_params *p, const tp_track_params *q,
                              tp_bs_stats *stats);
+/* The adaptive loop with the diagnostics above and the trajectory records
+ * of tp_gpu_track_fixed_traj, the reference's tracking_method = 1 call with
+ * traj_diagnose = 1 (src/diagnostics.c:545-557, the adaptive loop at
+ * src/tracking.c:518-630, recording at src/tracking.c:440-447): load, run,
+ * store and read back in one call.  Particle i's point k is its state at
+ * the output time t0 + k*nsteps_output*dt <= t0 + estep*dt, from the
+ * dense-output polynomial of the accepted step that contains the time (the
+ * final state at the end time), folded into the box with its shifts as
+ * particle_bc folds the fixed-step records (bc 0); an escaping particle
+ * records none after its escape.  Slots, points and the surrogate field
+ * as tp_gpu_track_fixed_traj; nsteps = accepted steps. */
+int tp_gpu_track_adaptive_diag_traj(tp_ctx *ctx, const tp_bs_params *bs,
+                                    const tp_track_params *p, int n, tp_particle *ptl,
+                                    const tp_particle *ptl_init, int *nsteps, double *espect,
+                                    double *espect_escape, double *dcoeff, int *nptl_remain,
+                                    int nsteps_output, const int *ntraj_accum,
+                                    tp_particle *ptl_time, int *points, tp_bs_stats *stats);
 /* load + run + store in one call (nsteps may be NULL) */
 int tp_gpu_track_adaptive(tp_ctx *ctx, const tp_bs_params *p, int n, tp_particle *ptl,
                           int *nsteps, tp_bs_stats *stats);

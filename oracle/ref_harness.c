@@ -487,8 +487,9 @@ int main(int argc, char **argv)
         spit(argv[4], out, nk * 2 * sizeof(double));
     } else if (!strcmp(mode, "trajh")) {
         /* particle_tracking_hybrid with traj_diagnose = 1 (the call of
-         * src/diagnostics.c:555-557), estep/nsteps_output + 1 slots per
-         * particle, slot 0 the initial state */
+         * src/diagnostics.c:555-557) and the spec's tracking_method,
+         * estep/nsteps_output + 1 slots per particle, slot 0 the initial
+         * state */
         particles *p = slurp(argv[3], &n);
         int np = (int)(n / sizeof(particles));
         particles *p0 = malloc(n + 1);
@@ -501,7 +502,7 @@ int main(int argc, char **argv)
         for (int i = 0; i < np; i++) memcpy(&traj[(size_t)i * slots], &p[i], sizeof(particles));
         int *nsteps = malloc(sizeof(int) * (np + 1));
         particle_tracking_hybrid(0, np, s.dt, s.nbins, s.nt_out, s.bc,
-            s.nsteps_output, s.pmass, accum, 0, 1, p0, nsteps, p, traj);
+            s.nsteps_output, s.pmass, accum, s.method, 1, p0, nsteps, p, traj);
         char path[1024];
         snprintf(path, sizeof(path), "%s/ptl.bin", argv[4]);
         spit(path, p, n);

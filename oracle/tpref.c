@@ -1433,6 +1433,12 @@ int tpref_bs_track_diag(const tpref_field *f, const tp_bs_params *bs, const tp_t
             memset(&st, 0, sizeof(st));
             tpref_particle_bc(prm, &q, &esc);          /* src/tracking.c:314-332 */
             const int init_esc = esc;
+            /* trajectory records (armed by tpref_set_trajectory): the
+             * fixed-step loop's rule, the particle at the output steps j =
+             * k nsteps_output <= estep, slot 1 on */
+            const int toff = (g_traj_nout > 0 && i > 0) ? g_traj_accum[i - 1] : 0;
+            const int tcap = g_traj_nout > 0 ? g_traj_accum[i] - toff : 0;
+            int tpts = 1, jt = g_traj_nout;
             if (!esc) {
                 double y[BS_N] = {q.x, q.y, q.z, q.ux, q.uy, q.uz};
                 double g0 = sqrt(1.0 + q.ux * q.ux + q.uy * q.uy + q.uz * q.uz);
@@ -1467,6 +1473,24 @@ int tpref_bs_track_diag(const tpref_field *f, const tp_bs_params *bs, const tp_t
                         e.ux = yd[3]; e.uy = yd[4]; e.uz = yd[5]; e.t = tr;
                         bs_dterms(f, p0, &e, prm->dt, 0, fix_stale, stale, ti + 8 * nr);
                     }
+                    /* trajectory points inside the step: the polynomial's
+                     * state, folded into the box as particle_bc folds the
+                     * fixed-step loop's records */
+                    for (; g_traj_nout > 0 && jt <= prm->estep; jt += g_traj_nout) {
+                        const double tj = t0 + (double)jt * prm->dt;
+                        if (!(tj < st.t)) break;
+                        tp_particle r = q;
+                        r.x = bs_dense_out(d, 0, tj); r.y = bs_dense_out(d, 1, tj);
+                        r.z = bs_dense_out(d, 2, tj); r.ux = bs_dense_out(d, 3, tj);
+                        r.uy = bs_dense_out(d, 4, tj); r.uz = bs_dense_out(d, 5, tj);
+                        r.t = tj;
+                        if (prm->bc_flag == 0) {
+                            int dummy = 0;
+                            tpref_particle_bc(prm, &r, &dummy);
+                        }
+                        if (tpts < tcap) g_traj[toff + tpts] = r;
+                        tpts++;
+                    }
                     /* the step's end: boundary, escape */
                     q.x = y[0]; q.y = y[1]; q.z = y[2];
                     q.ux = y[3]; q.uy = y[4]; q.uz = y[5]; q.t = st.t;
@@ -1488,8 +1512,15 @@ int tpref_bs_track_diag(const tpref_field *f, const tp_bs_params *bs, const tp_t
                         e.t = t0 + (double)(nr * di) * prm->dt;
                         bs_dterms(f, p0, &e, prm->dt, 0, fix_stale, stale, ti + 8 * nr);
                     }
+                    for (; g_traj_nout > 0 && jt <= prm->estep; jt += g_traj_nout) {
+                        tp_particle r = q;
+                        r.t = t0 + (double)jt * prm->dt;
+                        if (tpts < tcap) g_traj[toff + tpts] = r;
+                        tpts++;
+                    }
                 }
             }
+            if (g_traj_nout > 0 && g_traj_points) g_traj_points[i] = tpts;
             /* rows from nr on: the particle has left (or failed); as the
              * fixed-step loop counts it, an initial escape is out from row
              * 1, an escape before D row r from row r + 1 */

@@ -88,7 +88,10 @@ int  tpref_bs_track(const tpref_field *f, const tp_bs_params *prm, int n,
                     int nthreads, double *seconds);
 /* the same with NR3 dense output, particle_bc after each accepted step and
  * the fixed-step loop's diagnostics at its output times (sstep must be 1;
- * the end time is t + estep*dt) */
+ * the end time is t + estep*dt); with trajectory diagnostics armed
+ * (tpref_set_trajectory) the particle at the output steps j = k
+ * nsteps_output, from the dense output inside a step (folded into the box
+ * for periodic boundaries) or the final state at the end time */
 int  tpref_bs_track_diag(const tpref_field *f, const tp_bs_params *bs,
                          const tp_track_params *prm, int n, tp_particle *ptl,
                          const tp_particle *ptl_init, int *nsteps, double *espect,

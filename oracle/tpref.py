@@ -347,6 +347,26 @@ def bs_track_diag(field, bs, params, ptl, ptl_init=None, nthreads=1):
                 seconds=secs.value)
 
 
+def bs_track_diag_traj(field, bs, params, ptl, nsteps_output, ntraj_accum, ptl_time,
+                       ptl_init=None, nthreads=1):
+    """bs_track_diag with trajectory diagnostics (the particle at the
+    output steps j = k nsteps_output); ptl_time holds the caller's initial
+    points and receives the others; returns the bs_track_diag result plus
+    'ptl_time' and 'points'."""
+    ntraj_accum = np.ascontiguousarray(ntraj_accum, dtype=np.int32)
+    ptl_time = np.ascontiguousarray(ptl_time, dtype=np.float64).copy()
+    points = np.zeros(ntraj_accum.size, dtype=np.int32)
+    lib().tpref_set_trajectory(int(nsteps_output), abi.iptr(ntraj_accum), abi.dptr(ptl_time),
+                               abi.iptr(points))
+    try:
+        r = bs_track_diag(field, bs, params, ptl, ptl_init=ptl_init, nthreads=nthreads)
+    finally:
+        lib().tpref_set_trajectory(0, None, None, None)
+    r["ptl_time"] = ptl_time
+    r["points"] = points
+    return r
+
+
 def bs_track(field, params, ptl, nthreads=1):
     """Adaptive Bulirsch-Stoer of every particle to its end time."""
     ptl = abi.as_particles(ptl).copy()

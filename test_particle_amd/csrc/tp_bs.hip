@@ -94,6 +94,14 @@ struct BsArgs {
     unsigned *espect, *espect_esc;
     double *dbuf;                /* D terms [row][8][n] */
     int *outhist;                /* particles counted out from row r, [nd] */
+    /* trajectory diagnostics (traj != null): the particle at the output
+     * steps j = k traj_nout into traj[traj_accum[i-1] + traj_pts[i]++]
+     * while that slot is the particle's; qm from stage[i] */
+    tp_particle *traj;
+    const int *traj_accum;
+    int *traj_pts;
+    int traj_nout;
+    const tp_particle *stage;
 };
 
 /* StepperBase (src/unused/StepperBS.c:103-149): the step-size sequence
@@ -170,6 +178,7 @@ struct BsLane {
      * step-size cap of the interpolation error, the output-time grid (t0,
      * next frame, next D row), periodic shifts, the escape and end flags */
     int ipt, nf, nr, esc, ending;
+    int jt;                     /* next trajectory output step */
     double told, hd, hopt_int, t0;
     double xs[3];
 };
@@ -308,6 +317,34 @@ __device__ __forceinline__ void bs_begin_attempt(const BsArgs &a, BsLane &L, lon
     bs_begin_seq(a, L);
 }
 
+/* one trajectory point of the lane's particle: state y at time t with the
+ * lane's shifts, folded into the box for periodic boundaries as particle_bc
+ * folds the fixed-step loop's records (src/tracking.c:440-447, 1172-1221) */
+__device__ __forceinline__ void bs_traj_record(const BsArgs &a, const BsLane &L, const double *y,
+                                               double t)
+{
+    const long i = L.pi;
+    const int toff = i > 0 ? a.traj_accum[i - 1] : 0;
+    const int k = a.traj_pts[i];
+    if (k < a.traj_accum[i] - toff) {
+        tp_particle r;
+        r.x = y[0]; r.y = y[1]; r.z = y[2]; r.qm = a.stage[i].qm;
+        r.ux = y[3]; r.uy = y[4]; r.uz = y[5]; r.t = t;
+        r.xshift = L.xs[0]; r.yshift = L.xs[1]; r.zshift = L.xs[2];
+        if (a.bc == 0) {
+            const double d0 = a.hi[0] - a.lo[0], d1 = a.hi[1] - a.lo[1], d2 = a.hi[2] - a.lo[2];
+            if (r.x > a.hi[0]) { r.x -= d0; r.xshift += d0; }
+            if (r.y > a.hi[1]) { r.y -= d1; r.yshift += d1; }
+            if (r.z > a.hi[2]) { r.z -= d2; r.zshift += d2; }
+            if (r.x < a.lo[0]) { r.x += d0; r.xshift -= d0; }
+            if (r.y < a.lo[1]) { r.y += d1; r.yshift -= d1; }
+            if (r.z < a.lo[2]) { r.z += d2; r.zshift -= d2; }
+        }
+        a.traj[toff + k] = r;
+    }
+    a.traj_pts[i] = k + 1;
+}
+
 /* the output times at the end time: the remaining frames from the final
  * state; the remaining D rows each need the field there (PH_DROW) */
 template <bool DENSE>
@@ -315,6 +352,8 @@ __device__ __forceinline__ void bs_begin_end_outputs(const BsArgs &a, BsLane &L,
 {
     for (; L.nf < a.nframes; L.nf++)
         atomicAdd(&a.espect[L.nf * a.nbins + bs_energy_bin(a, L.yv[3], L.yv[4], L.yv[5])], 1u);
+    if (a.traj)
+        for (; L.jt <= a.estep; L.jt += a.traj_nout) bs_traj_record(a, L, L.yv, bs_time(a, L, L.jt));
     if (L.nr < a.nd) {
         L.ending = 1;
         for (int i = 0; i < BS_N; i++) L.yn[i] = L.yv[i];
@@ -783,6 +822,16 @@ __device__ __forceinline__ void bs_prep(const BsArgs &a, BsLane &L, long gl)
             const double uz = bs_dense_out(a, L, gl, 5, tf);
             atomicAdd(&a.espect[L.nf * a.nbins + bs_energy_bin(a, ux, uy, uz)], 1u);
         }
+        /* trajectory points strictly inside the step, from the polynomial */
+        if (a.traj) {
+            for (; L.jt <= a.estep; L.jt += a.traj_nout) {
+                const double tj = bs_time(a, L, L.jt);
+                if (!(tj < tnew)) break;
+                double yj[BS_N];
+                for (int m = 0; m < BS_N; m++) yj[m] = bs_dense_out(a, L, gl, m, tj);
+                bs_traj_record(a, L, yj, tj);
+            }
+        }
         /* ym keeps dydxnew while the D rows run */
         if (bs_next_drow<DENSE>(a, L, gl)) return;
     }
@@ -851,6 +900,7 @@ __global__ void __launch_bounds__(256) k_bs(BsArgs a)
                         L.t0 = L.t;
                         L.tend = L.t + a.estep * a.dt;
                         L.nf = 0; L.nr = 1; L.esc = 0; L.ending = 0;
+                        L.jt = a.traj_nout;
                         L.xs[0] = a.xs[i]; L.xs[1] = a.ys[i]; L.xs[2] = a.zs[i];
                         /* initial wrap and escape check, src/tracking.c:314-332 */
                         bs_particle_bc(a, L);
@@ -1256,6 +1306,11 @@ int tp_gpu_run_adaptive_diag(tp_ctx *c, const tp_bs_params *p, const tp_track_pa
     a.g0 = S.g0; a.mu0 = S.mu0; a.th0 = S.th0; a.st = S.st;
     a.espect = c->espect; a.espect_esc = c->espect_esc;
     a.dbuf = c->bs_dbuf; a.outhist = c->bs_outhist;
+    if (c->traj_nout > 0) {
+        /* armed by tp_gpu_track_adaptive_diag_traj */
+        a.traj = c->traj; a.traj_accum = c->traj_accum; a.traj_pts = c->traj_pts;
+        a.traj_nout = c->traj_nout; a.stage = S.stage;
+    }
     hipLaunchKernelGGL(k_bs_reset, dim3(1), dim3(64), 0, c->stream, c->bs_ctr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev0, c->stream));

@@ -2033,14 +2033,12 @@ __global__ void k_fill_int(int *a, int n, int v)
     if (i < n) a[i] = v;
 }
 
-int tp_gpu_track_fixed_traj(tp_ctx *c, const tp_track_params *p, int n, tp_particle *ptl,
-                            const tp_particle *ptl_init, int *nsteps, double *espect,
-                            double *espect_escape, double *dcoeff, int *nptl_remain,
-                            int nsteps_output, const int *ntraj_accum, tp_particle *ptl_time,
-                            int *points)
+/* trajectory buffers of the next run: the caller's slots (ptl_time, slot
+ * ranges ntraj_accum) on the device, every particle's next slot 1 */
+static int traj_arm(tp_ctx *c, int n, int nsteps_output, const int *ntraj_accum,
+                    const tp_particle *ptl_time)
 {
-    if (!c || !p || n < 0 || nsteps_output <= 0 || (n > 0 && (!ntraj_accum || !ptl_time)))
-        return TP_E_ARG;
+    if (n < 0 || nsteps_output <= 0 || (n > 0 && (!ntraj_accum || !ptl_time))) return TP_E_ARG;
     if (field_kind(c) == FK_MLP) return TP_E_ARG;
     /* slot ranges must be non-decreasing from 0 */
     for (int i = 0; i < n; i++)
@@ -2069,11 +2067,16 @@ int tp_gpu_track_fixed_traj(tp_ctx *c, const tp_track_params *p, int n, tp_parti
         HIPCHK(hipGetLastError());
     }
     c->traj_nout = nsteps_output;
-    int rc = tp_gpu_track_fixed(c, p, n, ptl, ptl_init, nsteps, espect, espect_escape, dcoeff,
-                                nptl_remain);
+    return TP_OK;
+}
+
+/* the armed run's records back to the caller (and disarm) */
+static int traj_collect(tp_ctx *c, int n, const int *ntraj_accum, tp_particle *ptl_time,
+                        int *points)
+{
     c->traj_nout = 0;
-    if (rc != TP_OK && rc != TP_E_NONFINITE) return rc;
     if (n > 0) {
+        const size_t ntraj = (size_t)ntraj_accum[n - 1];
         HIPCHK(hipMemcpyAsync(ptl_time, c->traj, ntraj * sizeof(tp_particle),
                               hipMemcpyDeviceToHost, c->stream));
         if (points)
@@ -2081,7 +2084,47 @@ int tp_gpu_track_fixed_traj(tp_ctx *c, const tp_track_params *p, int n, tp_parti
                                   c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
     }
-    return rc;
+    return TP_OK;
+}
+
+int tp_gpu_track_fixed_traj(tp_ctx *c, const tp_track_params *p, int n, tp_particle *ptl,
+                            const tp_particle *ptl_init, int *nsteps, double *espect,
+                            double *espect_escape, double *dcoeff, int *nptl_remain,
+                            int nsteps_output, const int *ntraj_accum, tp_particle *ptl_time,
+                            int *points)
+{
+    if (!c || !p) return TP_E_ARG;
+    int rc = traj_arm(c, n, nsteps_output, ntraj_accum, ptl_time);
+    if (rc) return rc;
+    rc = tp_gpu_track_fixed(c, p, n, ptl, ptl_init, nsteps, espect, espect_escape, dcoeff,
+                            nptl_remain);
+    if (rc != TP_OK && rc != TP_E_NONFINITE) {
+        c->traj_nout = 0;
+        return rc;
+    }
+    const int rc2 = traj_collect(c, n, ntraj_accum, ptl_time, points);
+    return rc2 ? rc2 : rc;
+}
+
+int tp_gpu_track_adaptive_diag_traj(tp_ctx *c, const tp_bs_params *bs, const tp_track_params *p,
+                                    int n, tp_particle *ptl, const tp_particle *ptl_init,
+                                    int *nsteps, double *espect, double *espect_escape,
+                                    double *dcoeff, int *nptl_remain, int nsteps_output,
+                                    const int *ntraj_accum, tp_particle *ptl_time, int *points,
+                                    tp_bs_stats *stats)
+{
+    if (!c || !bs || !p || n < 0 || (n > 0 && (!ptl || !ptl_init))) return TP_E_ARG;
+    int rc = traj_arm(c, n, nsteps_output, ntraj_accum, ptl_time);
+    if (rc) return rc;
+    rc = tp_gpu_load_particles(c, n, ptl, ptl_init);
+    if (!rc) rc = tp_gpu_run_adaptive_diag(c, bs, p, stats);
+    if (!rc) rc = tp_gpu_store_particles(c, ptl, nsteps);
+    if (!rc) rc = tp_gpu_read_diagnostics(c, espect, espect_escape, dcoeff, nptl_remain);
+    if (rc) {
+        c->traj_nout = 0;
+        return rc;
+    }
+    return traj_collect(c, n, ntraj_accum, ptl_time, points);
 }
 
 int tp_gpu_track_fixed(tp_ctx *c, const tp_track_params *p, int n, tp_particle *ptl,

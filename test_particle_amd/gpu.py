@@ -70,6 +70,10 @@ def lib():
         L.tp_gpu_tune_adaptive.argtypes = [vp, P(abi.TpBsParams), ctypes.c_char_p, P(i), P(i)]
         L.tp_gpu_run_adaptive_diag.argtypes = [vp, P(abi.TpBsParams), P(abi.TpTrackParams),
                                                P(abi.TpBsStats)]
+        L.tp_gpu_track_adaptive_diag_traj.argtypes = [vp, P(abi.TpBsParams),
+                                                      P(abi.TpTrackParams), i, P(d), P(d),
+                                                      P(i), P(d), P(d), P(d), P(i), i, P(i),
+                                                      P(d), P(i), P(abi.TpBsStats)]
         f = ctypes.c_float
         L.tp_gpu_mlp_set.argtypes = [vp, P(f), P(f), d]
         L.tp_gpu_mlp_get.argtypes = [vp, P(f)]
@@ -414,6 +418,35 @@ class Tracker:
         p, ns = self.store()
         r.update(ptl=p, nsteps=ns, **self.diagnostics(params))
         return r
+
+    def track_adaptive_diag_traj(self, bs, params, ptl, nsteps_output, ntraj_accum, ptl_time,
+                                 ptl_init=None):
+        """track_adaptive_diag with the reference's trajectory diagnostics
+        (tracking_method = 1, traj_diagnose = 1); ptl_time holds the initial
+        points on entry."""
+        ptl = abi.as_particles(ptl).copy()
+        ptl_init = ptl.copy() if ptl_init is None else abi.as_particles(ptl_init)
+        n = ptl.shape[0]
+        nd = abi.n_dcoeff_rows(params)
+        acc = np.ascontiguousarray(ntraj_accum, dtype=np.int32)
+        pt = abi.as_particles(ptl_time).copy()
+        if acc.size != n or (n and pt.shape[0] < acc[-1]):
+            raise ValueError("ntraj_accum must have n entries and ptl_time ntraj_accum[-1] rows")
+        nsteps = np.zeros(n, dtype=np.int32)
+        points = np.zeros(n, dtype=np.int32)
+        esp = np.zeros((params.nt_out, params.nbins))
+        ese = np.zeros((params.nt_out, params.nbins))
+        dco = np.zeros((8, nd))
+        rem = np.zeros(nd, dtype=np.int32)
+        st = abi.TpBsStats()
+        _check(lib().tp_gpu_track_adaptive_diag_traj(
+            self._ctx, ctypes.byref(bs), ctypes.byref(params), n, dptr(ptl), dptr(ptl_init),
+            iptr(nsteps), dptr(esp), dptr(ese), dptr(dco), iptr(rem), int(nsteps_output),
+            iptr(acc), dptr(pt), iptr(points), ctypes.byref(st)), "track_adaptive_diag_traj")
+        return dict(ptl=ptl, nsteps=nsteps, espect=esp, espect_escape=ese, dcoeff=dco,
+                    remain=rem, ptl_time=pt, points=points, accepted=st.accepted,
+                    rejected=st.rejected, rhs=st.rhs_evals, failed=st.failed,
+                    kernel_ms=st.kernel_ms)
 
     def track_adaptive(self, params, ptl):
         ptl = abi.as_particles(ptl).copy()

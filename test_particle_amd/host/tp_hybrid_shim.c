@@ -9,7 +9,8 @@
  * so the CPU loop stays available for what the GPU path does not cover
  * (see tp_shim_set_field; systems 2 and 3 with several time frames, which
  * the reference's loop does not track either).  Trajectory runs
- * (traj_diagnose = 1) go to tp_gpu_track_fixed_traj, two PIC time slices
+ * (traj_diagnose = 1) go to tp_gpu_track_fixed_traj (tracking_method 0) or
+ * tp_gpu_track_adaptive_diag_traj (tracking_method 1), two PIC time slices
  * to tp_gpu_set_field_grid2, system 3 to tp_gpu_set_field_mhd.
  *
  * The field source is taken from the reference's globals once per call;
@@ -180,8 +181,6 @@ void particle_tracking_hybrid(int mpi_rank, int nptl, double dt, int nbins,
         why = tp_gpu_strerror(tp_shim_init_rc);
     else if (tracking_method != 0 && tracking_method != 1)
         why = "tracking_method is neither 0 nor 1";
-    else if (tracking_method == 1 && traj_diagnose != 0)
-        why = "trajectory diagnostics of the adaptive loop";
     else if (traj_diagnose != 0 && traj_diagnose != 1)
         why = "traj_diagnose is neither 0 nor 1";
     else
@@ -266,6 +265,22 @@ void particle_tracking_hybrid(int mpi_rank, int nptl, double dt, int nbins,
         b.h0 = cfg->bs_h0;
         b.kmax = cfg->bs_kmax;
         p.precision = TP_FP64;
+        if (traj_diagnose == 1) {
+            /* trajectory run: ptl_time filled at the output times, no
+             * spectrum files (src/tracking.c:185) */
+            rc = tp_gpu_track_adaptive_diag_traj(
+                tp_shim_ctx, &b, &p, nptl, (tp_particle *)ptl, (const tp_particle *)ptl_init,
+                nsteps_ptl_tracking, NULL, NULL, NULL, NULL, nsteps_output, ntraj_accum,
+                (tp_particle *)ptl_time, NULL, NULL);
+            if (rc != TP_OK) {
+                fprintf(stderr, "tp_gpu_track_adaptive_diag_traj: %s\n", tp_gpu_strerror(rc));
+                MPI_Abort(MPI_COMM_WORLD, 1);
+            }
+            double t2 = omp_get_wtime();
+            if (mpi_rank == 0) printf("Elapsed time: %lf s\n", t2 - t1);
+            free(espect); free(espect_esc); free(tot); free(d); free(remain);
+            return;
+        }
         rc = tp_gpu_load_particles(tp_shim_ctx, nptl, (const tp_particle *)ptl,
                                    (const tp_particle *)ptl_init);
         if (rc == TP_OK) rc = tp_gpu_run_adaptive_diag(tp_shim_ctx, &b, &p, NULL);

@@ -12,7 +12,10 @@ test_oracle_bs_dense.py).
 * against the plain FP64 fixed-step kernel on the same field and output
   grid: D_xx and the spectra agree statistically (trajectories of the two
   integrators decorrelate in the turbulent field);
-* the reference program's tracking_method 1 through the drop-in shim.
+* trajectory records (tp_gpu_track_adaptive_diag_traj) bitwise against the
+  oracle's, with periodic folds, escapes and slot overflow;
+* the reference program's tracking_method 1 through the drop-in shim, with
+  and without trajectory diagnostics.
 """
 import os
 import subprocess
@@ -121,6 +124,59 @@ def test_dense_pic_grid(gpu, oracle_tp):
     _compare(r, ref, "grid")
 
 
+def _slots(n, nrec, seed):
+    """nrec + 1 slots per particle, every third particle fewer (its
+    later points are counted, not written); slot 0 the initial point, the
+    rest a fill value the records must replace"""
+    slots = np.full(n, nrec + 1)
+    slots[::3] = np.random.default_rng(seed).integers(1, nrec + 1, slots[::3].size)
+    accum = np.cumsum(slots).astype(np.int32)
+    off = np.concatenate([[0], accum[:-1]])
+    return slots, accum, off
+
+
+@pytest.mark.parametrize("name", ["uniform_periodic", "modes_periodic", "modes_open"])
+def test_dense_trajectory_bitwise(gpu, oracle_tp, name):
+    field, ptl, prm, bs = _case(name)
+    f = _set(gpu, oracle_tp, field)
+    n, nout = ptl.shape[0], 30
+    nrec = prm.estep // nout
+    slots, accum, off = _slots(n, nrec, 85)
+    pt = np.full((accum[-1], abi.NFIELD), -7.0)
+    pt[off] = ptl
+    print(f"[dense traj] {name}: {n} particles, output every {nout} steps", flush=True)
+    r = gpu.track_adaptive_diag_traj(bs, prm, ptl, nout, accum, pt)
+    ref = oracle_tp.bs_track_diag_traj(f, bs, prm, ptl, nout, accum, pt)
+    _compare(r, ref, name)
+    _same(r["ptl_time"], ref["ptl_time"], f"{name} ptl_time")
+    _same(r["points"], ref["points"], f"{name} points")
+    # the records replaced every fill value they reached
+    for i in range(n):
+        w = min(int(r["points"][i]), int(slots[i]))
+        assert (r["ptl_time"][off[i]:off[i] + w, 7] != -7.0).all(), i
+    if name == "modes_open":
+        assert (r["points"] < nrec + 1).any()      # escaped particles stop recording
+    else:
+        assert (r["points"] == nrec + 1).all()
+    # the diagnostics are those of the run without records
+    plain = gpu.track_adaptive_diag(bs, prm, ptl)
+    for k in ("ptl", "espect", "espect_escape", "remain"):
+        _same(r[k], plain[k], f"{name} {k} with and without records")
+
+
+def test_dense_trajectory_arguments(gpu):
+    """decreasing slot ranges and nsteps_output <= 0 are refused"""
+    from test_particle_amd.gpu import TPError
+    field, ptl, prm, bs = _case("uniform_periodic")
+    gpu.set_field_test()
+    ptl = ptl[:4]
+    pt = np.zeros((8, abi.NFIELD))
+    with pytest.raises(TPError):
+        gpu.track_adaptive_diag_traj(bs, prm, ptl, 10, [2, 4, 3, 8], pt)
+    with pytest.raises(TPError):
+        gpu.track_adaptive_diag_traj(bs, prm, ptl, 0, [2, 4, 6, 8], pt)
+
+
 def test_dense_vs_fixed_step_statistics(gpu):
     """The same particles, field and output grid through the plain FP64
     Boris kernel (dt = 0.01 T_g) and the adaptive stepper (rtol 1e-8):
@@ -190,3 +246,42 @@ def test_dropin_tracking_method_1(tmp_path, oracle_tp):
     lines = open(tmp_path / "data" / "espectrum.dat").read().splitlines()
     assert len(lines) == 2 + 3
     assert not os.path.exists(tmp_path / "data" / "diff_coeffs.dat")
+
+
+def test_dropin_tracking_method_1_trajectories(tmp_path, oracle_tp):
+    """traj_diagnose = 1 with tracking_method 1 through the drop-in shim
+    (the call of src/diagnostics.c:555-557): the reference program's
+    ptl_time and particles are the oracle's adaptive records bit for bit."""
+    exe = os.path.join(REF, "ref_harness_gpu")
+    if not os.path.exists(exe):
+        pytest.skip("oracle/_ref/ref_harness_gpu not built")
+    lo, hi = (0.0, 0.0, 0.0), (5.0, 5.0, 5.0)
+    n, estep, nout = 60, 300, 20
+    ptl = init_particles(n, lo, hi, 0.5, seed=88)
+    spec = dict(system=0, tmax=estep * 0.05 + 1e-9, dt=0.05, bc=0, nbins=30, nt_out=3,
+                emin=1e-3, emax=1.0, method=1, nsteps_output=nout, xmin=lo[0], ymin=lo[1],
+                zmin=lo[2], xmax=hi[0], ymax=hi[1], zmax=hi[2])
+    os.makedirs(tmp_path / "o")
+    with open(tmp_path / "spec.txt", "w") as f:
+        for k, v in spec.items():
+            f.write(f"{k} {v!r}\n" if isinstance(v, float) else f"{k} {v}\n")
+    ptl.tofile(tmp_path / "p.bin")
+    print("[dropin] tracking_method 1 trajh", flush=True)
+    r = subprocess.run([exe, "trajh", "spec.txt", "p.bin", "o"], cwd=tmp_path, timeout=120,
+                       stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, text=True,
+                       env=dict(os.environ, OMP_NUM_THREADS="1", TP_SHIM_REQUIRE_GPU="1"))
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "particle_tracking_hybrid on the GPU" in r.stderr
+    slots = estep // nout + 1
+    traj = np.fromfile(tmp_path / "o" / "traj.bin").reshape(-1, 11)
+    out = np.fromfile(tmp_path / "o" / "ptl.bin").reshape(-1, 11)
+    prm = abi.make_params(dt=0.05, estep=estep, lo=lo, hi=hi, nbins=30, nt_out=3, emin=1e-3,
+                          emax=1.0)
+    bs = abi.make_bs_params(rtol=1e-14, atol=1e-5, h0=1e-5, kmax=8, b_ref=1.0)
+    accum = (np.arange(1, n + 1) * slots).astype(np.int32)
+    pt = np.zeros((n * slots, 11))
+    pt[::slots] = ptl
+    ref = oracle_tp.bs_track_diag_traj(oracle_tp.Field.test(), bs, prm, ptl, nout, accum, pt)
+    _same(out, ref["ptl"], "drop-in particles")
+    _same(traj, ref["ptl_time"], "drop-in ptl_time")
+    assert (ref["points"] == slots).all()

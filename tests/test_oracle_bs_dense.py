@@ -153,3 +153,79 @@ def test_dense_output_open_boundary_accounting(oracle_tp):
         assert (te + H <= tk1).sum() <= out <= (te <= tk1).sum(), (k, out)
     assert r["remain"][0] == n
     assert (np.diff(r["remain"]) <= 0).all()
+
+
+def test_dense_trajectory_records_follow_the_helix(oracle_tp):
+    """Trajectory diagnostics of the adaptive loop: the particle at the
+    output steps j = k nsteps_output, from the dense output inside a step and
+    the final state at the end time, folded into the periodic box as the
+    fixed-step loop's records are (position + shift = the unwrapped helix).
+    Slot 0 keeps the caller's point; points past a particle's slots are
+    counted, not written."""
+    n, dt, estep, nout = 24, 0.05, 400, 25
+    box = ((0.0, 0.0, 0.0), (5.0, 5.0, 5.0))
+    p = init_particles(n, *box, 0.6, seed=72)
+    prm = abi.make_params(dt=dt, estep=estep, lo=box[0], hi=box[1], nbins=40, nt_out=10,
+                          emin=1e-3, emax=1.0, einterval_t=40, dcoeff_interval=40)
+    bs = abi.make_bs_params(rtol=1e-10, atol=1e-12, b_ref=1.0)
+    nrec = estep // nout                             # 16 records after slot 0
+    slots = np.full(n, nrec + 1)
+    slots[::3] = 5                                   # these overflow
+    accum = np.cumsum(slots).astype(np.int32)
+    pt = np.full((accum[-1], 11), -7.0)
+    off = np.concatenate([[0], accum[:-1]])
+    pt[off] = p
+    fld = oracle_tp.Field.test()
+    r = oracle_tp.bs_track_diag_traj(fld, bs, prm, p, nout, accum, pt)
+    plain = oracle_tp.bs_track_diag(fld, bs, prm, p)
+    # the diagnostics are those of the run without records
+    for k in ("ptl", "espect", "dcoeff", "remain", "nsteps"):
+        assert np.array_equal(r[k], plain[k]), k
+    assert (r["points"] == nrec + 1).all()
+    tp = r["ptl_time"]
+    assert np.array_equal(tp[off], p)
+    for i in range(n):
+        got = tp[off[i] + 1: off[i] + slots[i]]
+        k = np.arange(1, got.shape[0] + 1)
+        t = k * nout * dt
+        x, u = _helix(np.repeat(p[i:i + 1], k.size, axis=0), t)
+        assert np.abs(got[:, 7] - t).max() < 1e-12
+        assert np.abs(got[:, 3] - p[i, 3]).max() == 0.0
+        assert np.abs(got[:, :3] + got[:, 8:11] - x).max() < 1e-8, i
+        assert np.abs(got[:, 4:7] - u).max() < 1e-8, i
+        assert (got[:, :3] >= 0.0).all() and (got[:, :3] <= 5.0).all()
+    # the last record is the final state
+    full = np.flatnonzero(slots == nrec + 1)
+    assert np.allclose(tp[off[full] + nrec][:, :7], r["ptl"][full][:, :7], rtol=0, atol=0)
+
+
+def test_dense_trajectory_records_stop_at_the_escape(oracle_tp):
+    """Open boundaries: a particle is recorded at the output steps before
+    and inside the step in which it leaves, and never after."""
+    n, dt, estep, nout = 16, 0.1, 300, 10
+    rng = np.random.default_rng(75)
+    p = np.zeros((n, 11))
+    p[:, 0:2] = rng.uniform(3, 7, (n, 2))
+    p[:, 2] = rng.uniform(0.5, 9.5, n)
+    p[:, 3] = 1.0
+    p[:, 4] = 0.3
+    p[:, 6] = np.where(np.arange(n) < n // 2, 0.4, 0.0)
+    prm = abi.make_params(dt=dt, estep=estep, lo=(0, 0, 0), hi=(10, 10, 10), bc_flag=1,
+                          nbins=20, nt_out=10, emin=1e-4, emax=1.0, einterval_t=30,
+                          dcoeff_interval=30)
+    bs = abi.make_bs_params(rtol=1e-9, atol=1e-12, b_ref=1.0)
+    accum = np.cumsum(np.full(n, estep // nout + 1)).astype(np.int32)
+    pt = np.zeros((accum[-1], 11))
+    r = oracle_tp.bs_track_diag_traj(oracle_tp.Field.test(), bs, prm, p, nout, accum, pt)
+    g = np.sqrt(1 + (p[:, 4:7] ** 2).sum(1))
+    te = np.where(p[:, 6] > 0, (10 - p[:, 2]) / (np.maximum(p[:, 6], 1e-300) / g), np.inf)
+    H = 3.0                                          # bounds an accepted step
+    for i in range(n):
+        m = r["points"][i] - 1                       # records written
+        if np.isinf(te[i]):
+            assert m == estep // nout, i
+        else:
+            # records at t = k nout dt up to the end of the escaping step
+            lo_k = int(np.floor(te[i] / (nout * dt)))
+            hi_k = int(np.floor((te[i] + H) / (nout * dt)))
+            assert lo_k <= m <= hi_k, (i, m, lo_k, hi_k)
