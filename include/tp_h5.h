@@ -51,6 +51,27 @@ int tp_h5_load_pic(tp_ctx *ctx, const char *fname, const char *gname, int nx, in
 int tp_h5_write_pic_planes(const char *fname, const char *gname, const long long dims[3],
                            const float *planes);
 
+/* Trajectory output, the file trajectory_diagnostics writes
+ * (data/particle_diagnostics.h5, save_particles_fields,
+ * src/diagnostics.c:290-390): group "/particles_fields" holding the
+ * datasets "particles" (x y z qm ux uy uz t, the compound type of
+ * src/particle_info.c:452-488) and "fields" (Bx By Bz Ex Ey Ez,
+ * src/emfields.c:227-257) of n entries each, members big-endian IEEE
+ * doubles at offsets 8*k.  emf[6*i] is the field at ptl[i] in that order.
+ * The whole file from one process (the reference writes each rank's
+ * hyperslab of the same datasets through MPI-IO). */
+int tp_h5_write_particles_fields(const char *fname, int n, const tp_particle *ptl,
+                                 const double *emf);
+/* The same with the field evaluated on the GPU at each point, as the
+ * reference calls get_emf at each (src/diagnostics.c:307-310): the
+ * context's field through tp_gpu_eval_field in FP64. */
+int tp_h5_save_particles_fields(tp_ctx *ctx, const char *fname, int n, const tp_particle *ptl);
+/* Reads such a file back as native doubles: ptl8[8*i] = x y z qm ux uy uz t,
+ * emf[6*i]; cap is the capacity in entries, *n the count in the file
+ * (TP_E_ARG if it exceeds cap or the layout differs from the one
+ * written: member names, big-endian doubles, offsets 8*k). */
+int tp_h5_read_particles_fields(const char *fname, int cap, int *n, double *ptl8, double *emf);
+
 #ifdef __cplusplus
 }
 #endif

@@ -3,6 +3,7 @@
  * functions of include/tp_h5.h).  Plain C over the serial HDF5 library; the
  * GPU load goes through libtpgpu.so's tp_gpu_load_field_grid.
  */
+#include <stddef.h>
 #include <stdlib.h>
 #include <string.h>
 #include <hdf5.h>
@@ -183,6 +184,144 @@ int tp_h5_write_pic_planes(const char *fname, const char *gname, const long long
         if (d >= 0) H5Dclose(d);
     }
     if (sp >= 0) H5Sclose(sp);
+    if (g >= 0) H5Gclose(g);
+    if (f >= 0) H5Fclose(f);
+    quiet_off(q);
+    return rc;
+}
+
+/* --- trajectory output ------------------------------------------------- */
+static const char *const tp_h5_ptl_names[8] = {"x", "y", "z", "qm", "ux", "uy", "uz", "t"};
+static const char *const tp_h5_emf_names[6] = {"Bx", "By", "Bz", "Ex", "Ey", "Ez"};
+
+/* compound of k doubles named names[], at offsets 8*j in the file (big
+ * endian) or from off[j] in a record of rec bytes in memory (native) */
+static hid_t dcompound(int k, const char *const *names, size_t rec, const size_t *off, int file)
+{
+    hid_t t = H5Tcreate(H5T_COMPOUND, file ? (size_t)8 * k : rec);
+    if (t < 0) return -1;
+    for (int j = 0; j < k; j++)
+        if (H5Tinsert(t, names[j], file ? (size_t)8 * j : off[j],
+                      file ? H5T_IEEE_F64BE : H5T_NATIVE_DOUBLE) < 0) {
+            H5Tclose(t);
+            return -1;
+        }
+    return t;
+}
+
+static const size_t tp_h5_ptl_off[8] = {
+    offsetof(tp_particle, x), offsetof(tp_particle, y), offsetof(tp_particle, z),
+    offsetof(tp_particle, qm), offsetof(tp_particle, ux), offsetof(tp_particle, uy),
+    offsetof(tp_particle, uz), offsetof(tp_particle, t)};
+static const size_t tp_h5_seq_off[8] = {0, 8, 16, 24, 32, 40, 48, 56};
+
+static int write_dset(hid_t g, const char *name, int k, const char *const *names, size_t rec,
+                      const size_t *off, hsize_t n, const void *buf)
+{
+    int rc = TP_OK;
+    hid_t ft = dcompound(k, names, rec, off, 1), mt = dcompound(k, names, rec, off, 0);
+    hid_t sp = H5Screate_simple(1, &n, NULL);
+    hid_t d = (ft >= 0 && sp >= 0)
+                  ? H5Dcreate2(g, name, ft, sp, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT) : -1;
+    if (d < 0 || mt < 0 || (n > 0 && H5Dwrite(d, mt, H5S_ALL, H5S_ALL, H5P_DEFAULT, buf) < 0))
+        rc = TP_E_ARG;
+    if (d >= 0) H5Dclose(d);
+    if (sp >= 0) H5Sclose(sp);
+    if (mt >= 0) H5Tclose(mt);
+    if (ft >= 0) H5Tclose(ft);
+    return rc;
+}
+
+int tp_h5_write_particles_fields(const char *fname, int n, const tp_particle *ptl,
+                                 const double *emf)
+{
+    if (!fname || n < 0 || (n > 0 && (!ptl || !emf))) return TP_E_ARG;
+    quiet_t q = quiet_on();
+    int rc = TP_OK;
+    hid_t f = H5Fcreate(fname, H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT);
+    hid_t g = f >= 0 ? H5Gcreate2(f, "/particles_fields", H5P_DEFAULT, H5P_DEFAULT,
+                                  H5P_DEFAULT) : -1;
+    if (g < 0) rc = TP_E_ARG;
+    if (rc == TP_OK)
+        rc = write_dset(g, "particles", 8, tp_h5_ptl_names, sizeof(tp_particle), tp_h5_ptl_off,
+                        (hsize_t)n, ptl);
+    if (rc == TP_OK)
+        rc = write_dset(g, "fields", 6, tp_h5_emf_names, 6 * sizeof(double), tp_h5_seq_off,
+                        (hsize_t)n, emf);
+    if (g >= 0) H5Gclose(g);
+    if (f >= 0 && H5Fclose(f) < 0) rc = TP_E_ARG;
+    quiet_off(q);
+    return rc;
+}
+
+int tp_h5_save_particles_fields(tp_ctx *ctx, const char *fname, int n, const tp_particle *ptl)
+{
+    if (!ctx || !fname || n < 0 || (n > 0 && !ptl)) return TP_E_ARG;
+    double *pts = malloc(sizeof(double) * 4 * (size_t)(n > 0 ? n : 1));
+    double *emf = malloc(sizeof(double) * 6 * (size_t)(n > 0 ? n : 1));
+    int rc = (pts && emf) ? TP_OK : TP_E_NOMEM;
+    for (int i = 0; rc == TP_OK && i < n; i++) {
+        pts[4 * i + 0] = ptl[i].x; pts[4 * i + 1] = ptl[i].y;
+        pts[4 * i + 2] = ptl[i].z; pts[4 * i + 3] = ptl[i].t;
+    }
+    if (rc == TP_OK && n > 0) rc = tp_gpu_eval_field(ctx, n, pts, emf, TP_FP64);
+    if (rc == TP_OK) rc = tp_h5_write_particles_fields(fname, n, ptl, emf);
+    free(pts);
+    free(emf);
+    return rc;
+}
+
+static int read_dset(hid_t g, const char *name, int k, const char *const *names, int cap,
+                     int *n, double *buf)
+{
+    int rc = TP_OK;
+    hid_t d = H5Dopen2(g, name, H5P_DEFAULT);
+    hid_t sp = d >= 0 ? H5Dget_space(d) : -1;
+    hid_t ft = d >= 0 ? H5Dget_type(d) : -1;
+    hsize_t dim = 0;
+    if (sp < 0 || ft < 0 || H5Sget_simple_extent_ndims(sp) != 1 ||
+        H5Tget_class(ft) != H5T_COMPOUND || H5Tget_nmembers(ft) != k) {
+        rc = TP_E_ARG;
+    } else {
+        H5Sget_simple_extent_dims(sp, &dim, NULL);
+        if (dim > (hsize_t)cap || (*n >= 0 && (hsize_t)*n != dim)) rc = TP_E_ARG;
+    }
+    /* the reference's file types: the named members, IEEE doubles (big
+     * endian) at offsets 8*j of a record of 8*k bytes */
+    if (rc == TP_OK && H5Tget_size(ft) != (size_t)8 * k) rc = TP_E_ARG;
+    for (int j = 0; rc == TP_OK && j < k; j++) {
+        char *m = H5Tget_member_name(ft, (unsigned)j);
+        hid_t mt = H5Tget_member_type(ft, (unsigned)j);
+        if (!m || strcmp(m, names[j]) != 0 || mt < 0 || H5Tequal(mt, H5T_IEEE_F64BE) <= 0 ||
+            H5Tget_member_offset(ft, (unsigned)j) != (size_t)8 * j)
+            rc = TP_E_ARG;
+        if (mt >= 0) H5Tclose(mt);
+        if (m) H5free_memory(m);
+    }
+    if (rc == TP_OK) {
+        *n = (int)dim;
+        hid_t mt = dcompound(k, names, (size_t)8 * k, tp_h5_seq_off, 0);
+        if (mt < 0 || (dim > 0 && H5Dread(d, mt, H5S_ALL, H5S_ALL, H5P_DEFAULT, buf) < 0))
+            rc = TP_E_ARG;
+        if (mt >= 0) H5Tclose(mt);
+    }
+    if (ft >= 0) H5Tclose(ft);
+    if (sp >= 0) H5Sclose(sp);
+    if (d >= 0) H5Dclose(d);
+    return rc;
+}
+
+int tp_h5_read_particles_fields(const char *fname, int cap, int *n, double *ptl8, double *emf)
+{
+    if (!fname || !n || cap < 0 || (cap > 0 && (!ptl8 || !emf))) return TP_E_ARG;
+    quiet_t q = quiet_on();
+    int rc = TP_OK, cnt = -1;
+    hid_t f = H5Fopen(fname, H5F_ACC_RDONLY, H5P_DEFAULT);
+    hid_t g = f >= 0 ? H5Gopen2(f, "/particles_fields", H5P_DEFAULT) : -1;
+    if (g < 0) rc = TP_E_ARG;
+    if (rc == TP_OK) rc = read_dset(g, "particles", 8, tp_h5_ptl_names, cap, &cnt, ptl8);
+    if (rc == TP_OK) rc = read_dset(g, "fields", 6, tp_h5_emf_names, cap, &cnt, emf);
+    if (rc == TP_OK) *n = cnt;
     if (g >= 0) H5Gclose(g);
     if (f >= 0) H5Fclose(f);
     quiet_off(q);

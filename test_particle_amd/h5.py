@@ -35,6 +35,9 @@ def lib():
         L.tp_h5_read_pic.argtypes = [cs, cs, i, i, i, d, d, P(f), P(f)]
         L.tp_h5_load_pic.argtypes = [vp, cs, cs, i, i, i, P(d), P(d), d, d, i]
         L.tp_h5_write_pic_planes.argtypes = [cs, cs, P(ctypes.c_longlong), P(f)]
+        L.tp_h5_write_particles_fields.argtypes = [cs, i, P(d), P(d)]
+        L.tp_h5_save_particles_fields.argtypes = [vp, cs, i, P(d)]
+        L.tp_h5_read_particles_fields.argtypes = [cs, i, P(i), P(d), P(d)]
         _lib = L
     return _lib
 
@@ -84,3 +87,34 @@ def write_pic_planes(fname, gname, planes):
     _check(lib().tp_h5_write_pic_planes(fname.encode(), gname.encode(),
                                         dims.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)),
                                         fptr(p)), "h5_write_pic_planes")
+
+
+def write_particles_fields(fname, ptl, emf):
+    """data/particle_diagnostics.h5 as trajectory_diagnostics writes it
+    (tp_h5_write_particles_fields): the trajectory points ptl and the field
+    emf[i] = (Bx, By, Bz, Ex, Ey, Ez) at each."""
+    ptl = abi.as_particles(ptl)
+    emf = np.ascontiguousarray(emf, dtype=np.float64).reshape(-1, 6)
+    if emf.shape[0] != ptl.shape[0]:
+        raise ValueError("one field row per point")
+    _check(lib().tp_h5_write_particles_fields(os.fsencode(fname), ptl.shape[0], abi.dptr(ptl),
+                                              abi.dptr(emf)), "h5_write_particles_fields")
+
+
+def save_particles_fields(tracker, fname, ptl):
+    """The same with the field evaluated on the tracker's GPU at each point
+    (tp_h5_save_particles_fields)."""
+    ptl = abi.as_particles(ptl)
+    _check(lib().tp_h5_save_particles_fields(tracker._ctx, os.fsencode(fname), ptl.shape[0],
+                                             abi.dptr(ptl)), "h5_save_particles_fields")
+
+
+def read_particles_fields(fname, cap=1 << 20):
+    """(points [n, 8] = x y z qm ux uy uz t, fields [n, 6]) of such a file."""
+    p8 = np.zeros((cap, 8))
+    emf = np.zeros((cap, 6))
+    n = ctypes.c_int(0)
+    _check(lib().tp_h5_read_particles_fields(os.fsencode(fname), cap, ctypes.byref(n),
+                                             abi.dptr(p8), abi.dptr(emf)),
+           "h5_read_particles_fields")
+    return p8[:n.value].copy(), emf[:n.value].copy()

@@ -5,6 +5,8 @@ The oracle runs with the shared tp_sincos (the GPU's sin/cos), so FP64
 trajectories, spectra and counts must agree bit for bit; the diffusion sums
 are reduced in a different order (per-block trees vs a sequential sum) and
 agree to 1e-12 relative."""
+import os
+
 import numpy as np
 import pytest
 
@@ -371,6 +373,29 @@ def test_trajectory_fp32_last_record_is_final_state(gpu):
     bitwise(rec[:, 1], r["ptl"], "last record")
     bitwise(rec[:, 0], ptl, "initial slot kept")
     assert not rec[:, 2].any()
+
+
+def test_trajectory_output_file(gpu, oracle_tp, tmp_path):
+    """trajectory_diagnostics' output (save_particles_fields,
+    src/diagnostics.c:290-390): the records of a PIC-grid trajectory run and
+    the field at each, evaluated on the GPU, in data/particle_diagnostics.h5;
+    read back, the points are the records and the fields the oracle's at
+    the same points, bitwise (the reference gather, no sin/cos)."""
+    from test_particle_amd import h5
+    if not os.path.exists(abi.H5_LIB_PATH):
+        pytest.skip("libtph5.so not built")
+    print("[case] trajectory output file", flush=True)
+    g, prm, ptl, nout, accum, ptl_time, slots = _traj_case("traj_grid")
+    gpu.set_field_grid(g["egrid"], g["bgrid"], g["lo"], g["hi"], abi.TP_GATHER_REF_COMPAT)
+    f = oracle_tp.Field.grid(g["egrid"], g["bgrid"], g["lo"], g["hi"])
+    r = gpu.track_fixed_traj(prm, ptl, nout, accum, ptl_time)
+    fname = str(tmp_path / "particle_diagnostics.h5")
+    h5.save_particles_fields(gpu, fname, r["ptl_time"])
+    p8, emf = h5.read_particles_fields(fname)
+    bitwise(p8, r["ptl_time"][:, :8], "file points")
+    pts = r["ptl_time"][:, [0, 1, 2, 7]]
+    bitwise(emf, oracle_tp.eval_field(f, pts).reshape(-1, 6), "file fields")
+    assert np.abs(emf).max() > 0
 
 
 @pytest.mark.parametrize("mode", [abi.TP_GATHER_REF_COMPAT, abi.TP_GATHER_TEXTBOOK])
