@@ -164,6 +164,29 @@ def test_dense_trajectory_bitwise(gpu, oracle_tp, name):
         _same(r[k], plain[k], f"{name} {k} with and without records")
 
 
+def test_dense_trajectory_pic_grid(gpu, oracle_tp):
+    """the records through the reference gather of the PIC grid, with
+    overflowing slots"""
+    g = golden("track_grid")
+    ptl = init_particles(150, g["lo"], g["hi"], 0.3, seed=86)
+    prm = abi.make_params(dt=0.05, estep=200, lo=g["lo"], hi=g["hi"], nbins=30, nt_out=5,
+                          emin=1e-3, emax=1.0, einterval_t=40, dcoeff_interval=50)
+    bs = abi.make_bs_params(rtol=1e-8, atol=1e-10, b_ref=1.0)
+    gpu.set_field_grid(g["egrid"], g["bgrid"], g["lo"], g["hi"])
+    f = oracle_tp.Field.grid(g["egrid"], g["bgrid"], g["lo"], g["hi"])
+    n, nout = ptl.shape[0], 7
+    slots, accum, off = _slots(n, prm.estep // nout, 87)
+    pt = np.zeros((accum[-1], abi.NFIELD))
+    pt[off] = ptl
+    print("[dense traj] PIC grid", flush=True)
+    r = gpu.track_adaptive_diag_traj(bs, prm, ptl, nout, accum, pt)
+    ref = oracle_tp.bs_track_diag_traj(f, bs, prm, ptl, nout, accum, pt)
+    _compare(r, ref, "grid traj")
+    _same(r["ptl_time"], ref["ptl_time"], "grid ptl_time")
+    _same(r["points"], ref["points"], "grid points")
+    assert (r["points"] == prm.estep // nout + 1).all()
+
+
 def test_dense_trajectory_arguments(gpu):
     """decreasing slot ranges and nsteps_output <= 0 are refused"""
     from test_particle_amd.gpu import TPError
