@@ -45,12 +45,12 @@ def run_both(gpu, set_field, prm, ptl):
     return out
 
 
-@pytest.mark.parametrize("precision", [abi.TP_FP64, abi.TP_FP32_MIXED])
-@pytest.mark.parametrize("pusher", [abi.TP_PUSH_BORIS, abi.TP_PUSH_WIRZ])
+# the FP32 form is Boris only
+@pytest.mark.parametrize("precision,pusher", [(abi.TP_FP64, abi.TP_PUSH_BORIS),
+                                              (abi.TP_FP64, abi.TP_PUSH_WIRZ),
+                                              (abi.TP_FP32_MIXED, abi.TP_PUSH_BORIS)])
 @pytest.mark.parametrize("bc", [abi.TP_BC_PERIODIC, abi.TP_BC_OPEN, abi.TP_BC_NONE])
 def test_cells_same_results(gpu, precision, pusher, bc):
-    if precision == abi.TP_FP32_MIXED and pusher == abi.TP_PUSH_WIRZ:
-        pytest.skip("the FP32 form is Boris only")
     case(f"cells prec={precision} pusher={pusher} bc={bc}")
     g = golden("track_grid")
     ptl = init_particles(700, g["lo"], g["hi"], 0.5, seed=71)
