@@ -119,6 +119,9 @@ def main():
                     help="modes: runs of --tgyro gyroperiods each, one after the other")
     ap.add_argument("--dense", action="store_true",
                     help="modes: dense output with the fixed-step diagnostics (--steps outputs)")
+    ap.add_argument("--traj", type=int, default=0,
+                    help="modes --dense: also record every particle every TRAJ output steps "
+                         "(tp_gpu_track_adaptive_diag_traj)")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--wpc", type=int, default=0)
     ap.add_argument("--tol", type=float, default=1e-3, help="surrogate residual tolerance")
@@ -175,6 +178,25 @@ def main():
             dp = abi.make_params(dt=0.1, estep=a.steps, lo=(0, 0, 0), hi=(100, 100, 100),
                                  pusher=abi.TP_PUSH_BORIS, bc_flag=abi.TP_BC_NONE, nbins=102,
                                  nt_out=10, emin=1e-4, emax=1.0, dcoeff_interval=20)
+            if a.traj > 0:
+                # every particle's records, slot 0 the initial point (the
+                # load, store and copies are inside the wall time here)
+                import numpy as np
+                slots = a.steps // a.traj + 1
+                accum = (np.arange(1, a.particles + 1, dtype=np.int64) * slots).astype(np.int32)
+                pt = np.zeros((a.particles * slots, abi.NFIELD))
+                pt[::slots] = ptl
+                t0 = time.perf_counter()
+                st = tr.track_adaptive_diag_traj(prm, dp, ptl, a.traj, accum, pt)
+                wall = time.perf_counter() - t0
+                print(f"modes({a.nmodes}) dense + records every {a.traj} steps: {a.particles} "
+                      f"particles, {a.steps} output steps of 0.1: kernel {st['kernel_ms']:.1f} ms, "
+                      f"{st['accepted'] / (st['kernel_ms'] * 1e-3):.4e} accepted steps/s of "
+                      f"kernel time, points {int(st['points'].sum())} "
+                      f"({int((st['points'] == slots).sum())} particles complete), wall "
+                      f"{wall:.2f}s", flush=True)
+                tr.close()
+                return
             t0 = time.perf_counter()
             st = tr.run_adaptive_diag(prm, dp)
             wall = time.perf_counter() - t0
